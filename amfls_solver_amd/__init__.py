@@ -1,0 +1,17 @@
+"""MI355X-native AMFLS adaptive-width block Golub-Kahan least-squares engine.
+
+The solver runs in libamfls_hip.so (hand-written gfx950 HIP kernels and a
+C++20 host engine behind a C ABI, include/amfls_hip.h); this package is the
+Python mirror of the reference's binding (run_amfls, run_fixed_rbgk,
+run_lsqr, SolveResult) plus the torch.ops.amfls registration.
+"""
+from .binding import (  # noqa: F401
+    Context,
+    DeviceMatrix,
+    SolveResult,
+    run_amfls,
+    run_fixed_rbgk,
+    run_lsqr,
+)
+
+__all__ = ["Context", "DeviceMatrix", "SolveResult", "run_amfls", "run_fixed_rbgk", "run_lsqr"]

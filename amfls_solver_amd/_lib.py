@@ -1,0 +1,142 @@
+"""ctypes loader for libamfls_hip.so (the C ABI declared in include/amfls_hip.h).
+
+torch is imported before the library so that the process uses a single HIP
+runtime (the one torch loaded); the library then resolves its HIP symbols
+against it.  A missing library is an error: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, c_char_p, c_double, c_float, c_int, c_int32, c_int64, c_uint64, c_void_p
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libamfls_hip.so"
+
+F64, F32, BF16 = 0, 1, 2
+PRECISION = {"fp64": 0, "mixed": 1, "fp32": 2}
+CLOCK = {"wall": 0, "synthetic": 1}
+
+ERRORS = {
+    1: ValueError,      # invalid_argument
+    2: RuntimeError,    # logic_error
+    3: OverflowError,   # length_error
+    4: RuntimeError,    # runtime_error
+    5: RuntimeError,    # HIP failure
+    6: RuntimeError,
+}
+
+
+class Options(ctypes.Structure):
+    _fields_ = [
+        ("regularization", c_double),
+        ("tolerance", c_double),
+        ("failure_probability", c_double),
+        ("maximum_epochs", c_int32),
+        ("maximum_basis_size", c_int32),
+        ("seed", c_uint64),
+        ("stream", c_uint64),
+        ("maximum_depth", c_int32),
+        ("maximum_auxiliary_width", c_int32),
+        ("precision", c_int32),
+        ("clock_mode", c_int32),
+        ("synthetic_operator_unit", c_double),
+        ("synthetic_local_unit", c_double),
+        ("maximum_refinements", c_int32),
+        ("inner_tolerance", c_double),
+        ("width_policy", c_int32),
+    ]
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "amfls_last_error": (c_char_p, []),
+    "amfls_version": (c_int, []),
+    "amfls_options_default": (None, [POINTER(Options)]),
+    "amfls_context_create": (c_int, [POINTER(c_void_p)]),
+    "amfls_context_destroy": (c_int, [c_void_p]),
+    "amfls_context_synchronize": (c_int, [c_void_p]),
+    "amfls_csr_create": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_int,
+                                 POINTER(c_void_p)]),
+    "amfls_dense_create": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_int, c_int, c_int64,
+                                   POINTER(c_void_p)]),
+    "amfls_operator_destroy": (c_int, [c_void_p]),
+    "amfls_operator_shape": (c_int, [c_void_p, POINTER(c_int64), POINTER(c_int64)]),
+    "amfls_operator_validation_model": (c_int, [c_void_p, POINTER(c_double), POINTER(c_double),
+                                                POINTER(c_int64), POINTER(c_int64)]),
+    "amfls_operator_relative_block_product_cost": (c_double, [c_void_p, c_int]),
+    "amfls_operator_apply": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p, c_int]),
+    "amfls_operator_apply_rows": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_int64, c_void_p,
+                                          c_int64, c_int]),
+    "amfls_operator_product_bytes": (c_double, [c_void_p, c_int, c_int, c_int]),
+    "amfls_operator_product_flops": (c_double, [c_void_p, c_int]),
+    "amfls_operator_set_variant": (c_int, [c_void_p, c_int, c_int, c_int]),
+    "amfls_operator_profile": (c_int, [c_void_p, c_int]),
+    "amfls_operator_kernel_stats": (c_int, [c_void_p, POINTER(c_int64), POINTER(c_double), POINTER(c_double)]),
+    "amfls_spmm_variant_count": (c_int, []),
+    "amfls_spmm_variant_name": (c_char_p, [c_int]),
+    "amfls_spmm_variant_supports": (c_int, [c_int, c_int]),
+    "amfls_solve": (c_int, [c_void_p, c_void_p, c_void_p, c_int, POINTER(Options), POINTER(c_void_p)]),
+    "amfls_solve_fixed_rbgk": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_double, c_double, c_double,
+                                       c_int32, c_int32, c_int32, c_uint64, c_uint64, c_int32,
+                                       POINTER(c_void_p)]),
+    "amfls_solve_lsqr": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_double, c_double, c_int32, c_int32,
+                                 POINTER(c_void_p)]),
+    "amfls_result_sizes": (c_int, [c_void_p, POINTER(c_int64), POINTER(c_int64)]),
+    "amfls_result_copy": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                  POINTER(c_int64), POINTER(c_int64)]),
+    "amfls_result_trace_extension": (c_int, [c_void_p, c_void_p]),
+    "amfls_result_refinement_steps": (c_int, [c_void_p, POINTER(c_int32)]),
+    "amfls_result_destroy": (c_int, [c_void_p]),
+    "amfls_gaussian_fill": (c_int, [c_void_p, c_void_p, c_int64, c_int32, c_int64, c_int, c_uint64, c_uint64,
+                                    c_uint64]),
+}
+
+_lib = None
+
+
+def declared_symbols() -> list[str]:
+    """Function names declared in include/amfls_hip.h."""
+    import re
+    header = Path(__file__).resolve().parent.parent / "include" / "amfls_hip.h"
+    text = header.read_text()
+    return sorted(set(re.findall(r"\b(amfls_[a-z0-9_]+)\s*\(", text)))
+
+
+def load(path: Path | None = None):
+    """Load the engine library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    try:  # share torch's HIP runtime when torch is present
+        import torch  # noqa: F401
+    except ImportError:  # pragma: no cover - torch is part of this image
+        pass
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(f"{p} is not built; run `python -m amfls_solver_amd.build`")
+    lib = ctypes.CDLL(str(p))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(status: int) -> None:
+    if status != 0:
+        msg = load().amfls_last_error().decode(errors="replace")
+        raise ERRORS.get(status, RuntimeError)(msg)
+
+
+def options(**kw) -> Options:
+    o = Options()
+    load().amfls_options_default(ctypes.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+__all__ = ["load", "check", "options", "Options", "F64", "F32", "BF16", "PRECISION", "CLOCK",
+           "declared_symbols", "c_float"]

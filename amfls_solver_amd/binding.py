@@ -1,0 +1,436 @@
+"""Python mirror of the reference's AMFLS binding on the MI355X engine.
+
+Same entry points, argument meaning and result type as the reference's
+python/amfls/binding.py (run_amfls :259-290, run_fixed_rbgk, run_lsqr,
+SolveResult :9-90, _decode :159-255) and the same 8-tuple layout as
+bindings/torch/result_tensors.hpp:12-156.  Matrices may be CPU or CUDA
+(HIP) tensors, dense strided or sparse CSR; they are moved to the GPU, the
+solve runs in libamfls_hip.so, and results come back as CPU tensors like the
+reference's.
+
+Engine extensions (keyword-only): ``precision`` ("fp64" or "mixed"),
+``clock`` ("wall" or "synthetic"), ``value_dtype`` (storage of A on the
+device: torch.float64, torch.float32 or torch.bfloat16 for dense A).
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, fields
+
+import numpy as np
+import torch
+
+from . import _lib
+
+_UINT64_MODULUS = 1 << 64
+_INT64_SIGN_BIT = 1 << 63
+
+
+@dataclass(frozen=True)
+class SolveResult:
+    solution: torch.Tensor
+    objective: float
+    residual_norm: float
+    augmented_residual_norm: float
+    normal_residual_norm: float
+    solution_norm: float
+    solution_energy_norm: float
+    operator_norm_lower_bound: float
+    augmented_operator_norm_lower_bound: float
+    compatible_backward_error_upper_bound: float
+    least_squares_backward_error_upper_bound: float
+    backward_error_upper_bound: float
+    relative_normal_residual_upper_bound: float
+    energy_error_upper_bound: float
+    relative_energy_error_upper_bound: float
+    objective_gap_upper_bound: float
+    ridge_base_energy_error_upper_bound: float | None
+    ridge_corrected_energy_error_upper_bound: float | None
+    ridge_correction_gamma: float | None
+    ridge_correction_z_h_z: float | None
+    ridge_correction_two_abs_z_t_q: float | None
+    ridge_correction_q_norm_squared: float | None
+    a_columns: int
+    at_columns: int
+    a_block_calls: int
+    at_block_calls: int
+    search_a_columns: int
+    search_at_columns: int
+    validation_a_columns: int
+    validation_at_columns: int
+    iterations: int
+    depth: int
+    auxiliary_width: int
+    basis_rank: int
+    sketch_a_columns: int
+    sketch_at_columns: int
+    sketch_a_block_calls: int
+    sketch_at_block_calls: int
+    iterative_a_columns: int
+    iterative_at_columns: int
+    iterative_a_block_calls: int
+    iterative_at_block_calls: int
+    gaussian_random_columns: int
+    gaussian_random_values: int
+    search_a_block_calls: int
+    search_at_block_calls: int
+    validation_a_block_calls: int
+    validation_at_block_calls: int
+    base_validation_a_columns: int
+    base_validation_at_columns: int
+    base_validation_a_block_calls: int
+    base_validation_at_block_calls: int
+    ridge_correction_a_columns: int
+    ridge_correction_at_columns: int
+    ridge_correction_a_block_calls: int
+    ridge_correction_at_block_calls: int
+    gaussian_random_block_requests: int
+    ridge_correction_disposition: int
+    total_seconds: float
+    a_seconds: float
+    at_seconds: float
+    orthogonalization_seconds: float
+    projected_solve_seconds: float
+    validation_seconds: float
+    other_seconds: float
+    base_validation_seconds: float
+    ridge_correction_validation_seconds: float
+    trace: torch.Tensor
+    trace_float: torch.Tensor
+    trace_int: torch.Tensor
+    status: int
+    stop_reason: int
+    # engine extensions
+    trace_extension: torch.Tensor | None = None
+    refinement_steps: int = 0
+
+    @property
+    def success(self) -> bool:
+        return self.status == 0
+
+    @property
+    def search_vector_work(self) -> int:
+        return self.search_a_columns + self.search_at_columns
+
+
+_METRIC_NAMES = [f.name for f in fields(SolveResult)][1:22]
+_COUNTER_NAMES = [
+    "a_columns", "at_columns", "a_block_calls", "at_block_calls", "search_a_columns", "search_at_columns",
+    "validation_a_columns", "validation_at_columns", "iterations", "depth", "auxiliary_width", "basis_rank",
+    "sketch_a_columns", "sketch_at_columns", "sketch_a_block_calls", "sketch_at_block_calls",
+    "iterative_a_columns", "iterative_at_columns", "iterative_a_block_calls", "iterative_at_block_calls",
+    "gaussian_random_columns", "gaussian_random_values", "search_a_block_calls", "search_at_block_calls",
+    "validation_a_block_calls", "validation_at_block_calls", "base_validation_a_columns",
+    "base_validation_at_columns", "base_validation_a_block_calls", "base_validation_at_block_calls",
+    "ridge_correction_a_columns", "ridge_correction_at_columns", "ridge_correction_a_block_calls",
+    "ridge_correction_at_block_calls", "gaussian_random_block_requests", "ridge_correction_disposition",
+]
+_TIMING_NAMES = ["total_seconds", "a_seconds", "at_seconds", "orthogonalization_seconds",
+                 "projected_solve_seconds", "validation_seconds", "other_seconds", "base_validation_seconds",
+                 "ridge_correction_validation_seconds"]
+
+
+class Context:
+    """One engine context (HIP stream + scratch) on the current device."""
+
+    def __init__(self):
+        self._lib = _lib.load()
+        h = ctypes.c_void_p()
+        _lib.check(self._lib.amfls_context_create(ctypes.byref(h)))
+        self.handle = h
+
+    def synchronize(self) -> None:
+        _lib.check(self._lib.amfls_context_synchronize(self.handle))
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            self._lib.amfls_context_destroy(h)
+            self.handle = None
+
+
+_default_context: Context | None = None
+
+
+def default_context() -> Context:
+    global _default_context
+    if _default_context is None:
+        _default_context = Context()
+    return _default_context
+
+
+def _require_gpu() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("the AMFLS HIP engine needs a ROCm GPU; none is visible")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+_DTYPE_CODE = {torch.float64: _lib.F64, torch.float32: _lib.F32, torch.bfloat16: _lib.BF16}
+
+
+class DeviceMatrix:
+    """A device operator built from a torch tensor (dense strided or sparse CSR).
+
+    The reference builds a RowMajorDenseOperator from contiguous dense tensors
+    and a SparseCsrOperator from CSR tensors (torch_binding.cpp:69-114); this
+    keeps the device copies alive for the operator's lifetime.
+    """
+
+    def __init__(self, matrix: torch.Tensor, *, value_dtype: torch.dtype | None = None,
+                 context: Context | None = None):
+        dev = _require_gpu()
+        self.context = context or default_context()
+        lib = self.context._lib
+        if matrix.ndim != 2 or matrix.shape[0] <= 0 or matrix.shape[1] <= 0:
+            raise ValueError("matrix must have shape (m,n) with m, n > 0")
+        if matrix.shape[0] > 2**31 - 1 or matrix.shape[1] > 2**31 - 1:
+            raise ValueError("matrix exceeds the LP64 dimension limit")
+        self.rows, self.cols = int(matrix.shape[0]), int(matrix.shape[1])
+        h = ctypes.c_void_p()
+        if matrix.layout == torch.sparse_csr:
+            vd = value_dtype or (matrix.dtype if matrix.dtype in (torch.float64, torch.float32) else torch.float64)
+            if vd not in (torch.float64, torch.float32):
+                raise ValueError("CSR values are stored as float64 or float32")
+            self._rp = matrix.crow_indices().to(device=dev, dtype=torch.int64).contiguous()
+            self._ci = matrix.col_indices().to(device=dev, dtype=torch.int32).contiguous()
+            self._val = matrix.values().to(device=dev, dtype=vd).contiguous()
+            if self._rp.numel() != self.rows + 1 or self._ci.numel() != self._val.numel():
+                raise ValueError("CSR row offsets must have length rows + 1")
+            torch.cuda.synchronize()
+            _lib.check(lib.amfls_csr_create(self.context.handle, self.rows, self.cols, self._val.numel(),
+                                            self._rp.data_ptr(), self._ci.data_ptr(), self._val.data_ptr(),
+                                            _DTYPE_CODE[vd], ctypes.byref(h)))
+            self.kind = "csr"
+            self.nnz = int(self._val.numel())
+        elif matrix.layout == torch.strided:
+            vd = value_dtype or (matrix.dtype if matrix.dtype in _DTYPE_CODE else torch.float64)
+            if vd not in _DTYPE_CODE:
+                raise ValueError("dense values are stored as float64, float32 or bfloat16")
+            t = matrix.to(device=dev, dtype=vd)
+            if t.stride(1) == 1 and t.stride(0) >= self.cols:
+                self._val, row_major, ld = t, 1, t.stride(0)
+            elif t.stride(0) == 1 and t.stride(1) >= self.rows:
+                self._val, row_major, ld = t, 0, t.stride(1)
+            else:
+                self._val, row_major, ld = t.contiguous(), 1, self.cols
+            torch.cuda.synchronize()
+            _lib.check(lib.amfls_dense_create(self.context.handle, self.rows, self.cols, self._val.data_ptr(),
+                                              _DTYPE_CODE[vd], row_major, ld, ctypes.byref(h)))
+            self.kind = "dense"
+            self.nnz = self.rows * self.cols
+        else:
+            raise ValueError("matrix must use dense strided or sparse CSR storage")
+        self.handle = h
+        self.value_dtype = vd
+
+    def validation_model(self):
+        lib = self.context._lib
+        up, lo = ctypes.c_double(), ctypes.c_double()
+        sa, st = ctypes.c_int64(), ctypes.c_int64()
+        _lib.check(lib.amfls_operator_validation_model(self.handle, ctypes.byref(up), ctypes.byref(lo),
+                                                       ctypes.byref(sa), ctypes.byref(st)))
+        return up.value, lo.value, sa.value, st.value
+
+    def relative_block_product_cost(self, s: int) -> float:
+        return self.context._lib.amfls_operator_relative_block_product_cost(self.handle, s)
+
+    def apply(self, x: torch.Tensor, transpose: bool = False) -> torch.Tensor:
+        """Column-major block product (MatrixOperator::apply contract)."""
+        if x.ndim == 1:
+            x = x[:, None]
+        if x.dtype not in (torch.float64, torch.float32):
+            raise ValueError("blocks must be float64 or float32")
+        rows_in = self.rows if transpose else self.cols
+        rows_out = self.cols if transpose else self.rows
+        if x.shape[0] != rows_in:
+            raise ValueError("block has the wrong number of rows")
+        s = int(x.shape[1])
+        xin = x.to(device=self._val.device).t().contiguous().t()  # column-major storage
+        out = torch.empty((s, rows_out), dtype=x.dtype, device=xin.device).t()
+        torch.cuda.synchronize()
+        _lib.check(self.context._lib.amfls_operator_apply(self.context.handle, self.handle, int(transpose),
+                                                          xin.data_ptr(), s, out.data_ptr(),
+                                                          _DTYPE_CODE[x.dtype]))
+        return out
+
+    def set_variant(self, s: int, transpose: bool, variant: int) -> None:
+        _lib.check(self.context._lib.amfls_operator_set_variant(self.handle, s, int(transpose), variant))
+
+    def profile(self, enable: bool = True) -> None:
+        _lib.check(self.context._lib.amfls_operator_profile(self.handle, int(enable)))
+
+    def kernel_stats(self):
+        """(launches, seconds, algorithmic bytes) of timed products; resets."""
+        l, sec, by = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
+        _lib.check(self.context._lib.amfls_operator_kernel_stats(self.handle, ctypes.byref(l), ctypes.byref(sec),
+                                                                ctypes.byref(by)))
+        return l.value, sec.value, by.value
+
+    def product_bytes(self, s: int, transpose: bool = False, dtype: torch.dtype = torch.float32) -> float:
+        return self.context._lib.amfls_operator_product_bytes(self.handle, s, int(transpose), _DTYPE_CODE[dtype])
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            self.context._lib.amfls_operator_destroy(h)
+            self.handle = None
+
+
+def _canonical_uint64_to_int64_transport(value: int, name: str) -> int:
+    if isinstance(value, bool) or not isinstance(value, int):
+        raise ValueError(f"{name} must be an integer in [0, 2**64 - 1]")
+    if value < 0 or value >= _UINT64_MODULUS:
+        raise ValueError(f"{name} must be an integer in [0, 2**64 - 1]")
+    return value if value < _INT64_SIGN_BIT else value - _UINT64_MODULUS
+
+
+def _check_inputs(matrix: torch.Tensor, b: torch.Tensor) -> None:
+    """reference binding.py:100-110 (device restriction lifted)."""
+    if matrix.layout not in (torch.strided, torch.sparse_csr):
+        raise ValueError("matrix must use dense strided or sparse CSR storage")
+    if b.layout != torch.strided:
+        raise ValueError("b must use dense strided storage")
+    if matrix.ndim != 2 or b.ndim != 1 or matrix.shape[0] != b.shape[0]:
+        raise ValueError("matrix must have shape (m,n) and b must have shape (m,)")
+    if matrix.dtype not in (torch.float64, torch.float32, torch.bfloat16) or b.dtype != torch.float64:
+        raise ValueError("matrix must be floating point and b must use torch.float64")
+
+
+def _rhs(b: torch.Tensor, dev: torch.device) -> torch.Tensor:
+    return b.to(device=dev, dtype=torch.float64).contiguous()
+
+
+def _result_tuple(ctx: Context, res_handle: ctypes.c_void_p):
+    """Copy an engine result into the reference's 8-tuple (CPU tensors)."""
+    lib = ctx._lib
+    try:
+        cols, rows = ctypes.c_int64(), ctypes.c_int64()
+        _lib.check(lib.amfls_result_sizes(res_handle, ctypes.byref(cols), ctypes.byref(rows)))
+        sol = np.empty(cols.value, dtype=np.float64)
+        met = np.empty(21, dtype=np.float64)
+        cnt = np.empty(36, dtype=np.int64)
+        tim = np.empty(9, dtype=np.float64)
+        tf = np.empty((rows.value, 29), dtype=np.float64)
+        ti = np.empty((rows.value, 4), dtype=np.int64)
+        ext = np.empty((rows.value, 3), dtype=np.float64)
+        st, sr = ctypes.c_int64(), ctypes.c_int64()
+        steps = ctypes.c_int32()
+        ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+        _lib.check(lib.amfls_result_copy(res_handle, ptr(sol), ptr(met), ptr(cnt), ptr(tim), ptr(tf), ptr(ti),
+                                         ctypes.byref(st), ctypes.byref(sr)))
+        _lib.check(lib.amfls_result_trace_extension(res_handle, ptr(ext)))
+        _lib.check(lib.amfls_result_refinement_steps(res_handle, ctypes.byref(steps)))
+    finally:
+        lib.amfls_result_destroy(res_handle)
+    raw = (torch.from_numpy(sol), torch.from_numpy(met), torch.from_numpy(cnt), torch.from_numpy(tim),
+           torch.from_numpy(tf), torch.from_numpy(ti), int(st.value), int(sr.value))
+    return raw, torch.from_numpy(ext), int(steps.value)
+
+
+def _decode(raw, ext=None, steps=0) -> SolveResult:
+    """reference binding.py:159-255."""
+    if not isinstance(raw, tuple) or len(raw) != 8:
+        raise RuntimeError("AMFLS binding result must be an eight-item tuple")
+    solution, metrics, counters, timings, trace_float, trace_int, status, reason = raw
+    if metrics.shape != (21,) or counters.shape != (36,) or timings.shape != (9,):
+        raise RuntimeError("AMFLS result tensors have the wrong widths")
+    if trace_float.ndim != 2 or trace_float.shape[1] != 29 or trace_int.shape != (trace_float.shape[0], 4):
+        raise RuntimeError("AMFLS trace tensors have the wrong widths")
+    if trace_int.shape[0] > 0 and not bool(torch.all((trace_int[:, 2] == 0) | (trace_int[:, 2] == 1))):
+        raise RuntimeError("AMFLS trace success flags must be zero or one")
+    kw = {"solution": solution}
+    for i, name in enumerate(_METRIC_NAMES):
+        kw[name] = float(metrics[i])
+    for i, name in enumerate(_COUNTER_NAMES):
+        kw[name] = int(counters[i])
+    for i, name in enumerate(_TIMING_NAMES):
+        kw[name] = float(timings[i])
+    kw.update(trace=trace_float[:, :18], trace_float=trace_float, trace_int=trace_int, status=int(status),
+              stop_reason=int(reason), trace_extension=ext, refinement_steps=steps)
+    return SolveResult(**kw)
+
+
+def _operator(matrix, value_dtype, context):
+    if isinstance(matrix, DeviceMatrix):
+        return matrix
+    return DeviceMatrix(matrix, value_dtype=value_dtype, context=context)
+
+
+def run_amfls_raw(matrix, b, regularization=0.0, tolerance=1e-8, failure_probability=1e-6, maximum_epochs=0,
+                  maximum_depth=0, maximum_auxiliary_width=0, maximum_basis_size=0, seed=0, stream=0, *,
+                  precision="fp64", clock="wall", value_dtype=None, maximum_refinements=8,
+                  inner_tolerance=0.0, width_policy=0, context=None):
+    """The engine call behind torch.ops.amfls.run_amfls; returns (8-tuple, ext, steps)."""
+    op = _operator(matrix, value_dtype, context)
+    ctx = op.context
+    dev = op._val.device
+    bd = _rhs(b, dev)
+    if bd.numel() != op.rows:
+        raise ValueError("right-hand side shape mismatch")
+    o = _lib.options(regularization=float(regularization), tolerance=float(tolerance),
+                     failure_probability=float(failure_probability), maximum_epochs=int(maximum_epochs),
+                     maximum_depth=int(maximum_depth), maximum_auxiliary_width=int(maximum_auxiliary_width),
+                     maximum_basis_size=int(maximum_basis_size), seed=int(seed) % _UINT64_MODULUS,
+                     stream=int(stream) % _UINT64_MODULUS, precision=_lib.PRECISION[precision],
+                     clock_mode=_lib.CLOCK[clock], maximum_refinements=int(maximum_refinements),
+                     inner_tolerance=float(inner_tolerance), width_policy=int(width_policy))
+    res = ctypes.c_void_p()
+    torch.cuda.synchronize()
+    _lib.check(ctx._lib.amfls_solve(ctx.handle, op.handle, bd.data_ptr(), 1, ctypes.byref(o), ctypes.byref(res)))
+    return _result_tuple(ctx, res)
+
+
+def run_amfls(matrix, b, *, regularization: float = 0.0, tolerance: float = 1e-8,
+              failure_probability: float = 1e-6, maximum_epochs: int = 0, maximum_depth: int = 0,
+              maximum_auxiliary_width: int = 0, maximum_basis_size: int = 0, seed: int = 0, stream: int = 0,
+              precision: str = "fp64", clock: str = "wall", value_dtype=None, maximum_refinements: int = 8,
+              inner_tolerance: float = 0.0, width_policy: int = 0, context: Context | None = None) -> SolveResult:
+    """reference binding.py:259-290 on the HIP engine."""
+    if not isinstance(matrix, DeviceMatrix):
+        _check_inputs(matrix, b)
+    seed_t = _canonical_uint64_to_int64_transport(seed, "seed")
+    stream_t = _canonical_uint64_to_int64_transport(stream, "stream")
+    raw, ext, steps = run_amfls_raw(matrix, b, regularization, tolerance, failure_probability, maximum_epochs,
+                                    maximum_depth, maximum_auxiliary_width, maximum_basis_size, seed_t, stream_t,
+                                    precision=precision, clock=clock, value_dtype=value_dtype,
+                                    maximum_refinements=maximum_refinements, inner_tolerance=inner_tolerance,
+                                    width_policy=width_policy, context=context)
+    return _decode(raw, ext, steps)
+
+
+def run_fixed_rbgk(matrix, b, *, auxiliary_width: int, maximum_depth: int, maximum_basis_size: int = 0,
+                   regularization: float = 0.0, tolerance: float = 1e-8, failure_probability: float = 1e-6,
+                   seed: int = 0, stream: int = 0, clock: str = "wall", value_dtype=None,
+                   context: Context | None = None) -> SolveResult:
+    """reference binding.py run_fixed_rbgk / fixed_rbgk.cpp:18-209 on the HIP engine."""
+    if not isinstance(matrix, DeviceMatrix):
+        _check_inputs(matrix, b)
+    op = _operator(matrix, value_dtype, context)
+    ctx = op.context
+    bd = _rhs(b, op._val.device)
+    res = ctypes.c_void_p()
+    torch.cuda.synchronize()
+    _lib.check(ctx._lib.amfls_solve_fixed_rbgk(ctx.handle, op.handle, bd.data_ptr(), 1, float(regularization),
+                                               float(tolerance), float(failure_probability), int(auxiliary_width),
+                                               int(maximum_depth), int(maximum_basis_size),
+                                               int(seed) % _UINT64_MODULUS, int(stream) % _UINT64_MODULUS,
+                                               _lib.CLOCK[clock], ctypes.byref(res)))
+    raw, ext, steps = _result_tuple(ctx, res)
+    return _decode(raw, ext, steps)
+
+
+def run_lsqr(matrix, b, *, regularization: float = 0.0, tolerance: float = 1e-8, maximum_iterations: int = 0,
+             clock: str = "wall", value_dtype=None, context: Context | None = None) -> SolveResult:
+    """reference binding.py run_lsqr / lsqr.cpp:84-279 on the HIP engine."""
+    if not isinstance(matrix, DeviceMatrix):
+        _check_inputs(matrix, b)
+    op = _operator(matrix, value_dtype, context)
+    ctx = op.context
+    bd = _rhs(b, op._val.device)
+    res = ctypes.c_void_p()
+    torch.cuda.synchronize()
+    _lib.check(ctx._lib.amfls_solve_lsqr(ctx.handle, op.handle, bd.data_ptr(), 1, float(regularization),
+                                         float(tolerance), int(maximum_iterations), _lib.CLOCK[clock],
+                                         ctypes.byref(res)))
+    raw, ext, steps = _result_tuple(ctx, res)
+    return _decode(raw, ext, steps)

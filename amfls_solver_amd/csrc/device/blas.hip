@@ -1,0 +1,962 @@
+// Tall-skinny block kernels for the block Golub-Kahan engine on gfx950:
+// the orthogonalization GEMMs (C = A^T B, Y -= A C, row-local R^{-1}
+// products), dense-operator products, vector reductions for the certified
+// norm intervals, and the counter-addressed Gaussian generator.
+//
+// Everything here is HBM-bound at the widths AMFLS uses (s <= 64): each
+// kernel streams its tall operand once, keeps the small operand in LDS, and
+// accumulates reductions in FP64 with a deterministic two-stage split.
+#include "kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace amflsx {
+
+void Scratch::ensure(size_t need) {
+    if (need <= bytes) return;
+    if (ptr) AMFLS_HIP_CHECK(hipFree(ptr));
+    ptr = nullptr;
+    size_t grow = std::max(need, bytes * 2);
+    AMFLS_HIP_CHECK(hipMalloc(&ptr, grow));
+    bytes = grow;
+}
+
+void Scratch::release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+namespace {
+
+constexpr int kThreads = 256;
+
+template <typename V>
+__device__ __forceinline__ double ld_d(const V* p, int64_t i) { return to_double(p[i]); }
+
+template <typename Acc, typename V>
+__device__ __forceinline__ Acc ld_acc(const V* p, int64_t i) { return load_as<Acc, V>(p[i]); }
+
+template <typename T> struct AccOf { using type = T; };
+
+// ------------------------------------------------------------------ gemm_nn
+// Wide tile: 64 rows x 64 columns per 256-thread block, 4x4 per thread.
+template <typename V, typename TB, typename T, typename Acc, bool AROW>
+__global__ __launch_bounds__(kThreads) void gemm_nn_tile_kernel(
+    int64_t M, int N, int64_t K, const V* __restrict__ A, int64_t rs, int64_t cs,
+    const TB* __restrict__ B, int64_t ldb, T* __restrict__ Y, int64_t ldy, Acc alpha, Acc beta) {
+    constexpr int BM = 64, BN = 64, BK = 32;
+    __shared__ Acc As[BK][BM + 1];
+    __shared__ Acc Bs[BK][BN];
+    const int64_t row0 = static_cast<int64_t>(blockIdx.x) * BM;
+    const int col0 = blockIdx.y * BN;
+    const int t = threadIdx.x;
+    const int tx = t % 16, ty = t / 16;
+    Acc acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = Acc(0);
+    for (int64_t k0 = 0; k0 < K; k0 += BK) {
+#pragma unroll
+        for (int l = 0; l < (BM * BK) / kThreads; ++l) {
+            const int idx = t + l * kThreads;
+            int r, kk;
+            if (AROW) { kk = idx % BK; r = idx / BK; } else { r = idx % BM; kk = idx / BM; }
+            const int64_t gr = row0 + r, gk = k0 + kk;
+            Acc v = Acc(0);
+            if (gr < M && gk < K) v = ld_acc<Acc>(A, gr * rs + gk * cs);
+            As[kk][r] = v;
+        }
+#pragma unroll
+        for (int l = 0; l < (BN * BK) / kThreads; ++l) {
+            const int idx = t + l * kThreads;
+            const int c = idx % BN, kk = idx / BN;
+            const int64_t gk = k0 + kk;
+            const int gc = col0 + c;
+            Bs[kk][c] = (gk < K && gc < N) ? static_cast<Acc>(B[gk * ldb + gc]) : Acc(0);
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int kk = 0; kk < BK; ++kk) {
+            Acc a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = As[kk][ty * 4 + i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = Bs[kk][tx * 4 + j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = fma(a[i], b[j], acc[i][j]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int64_t gr = row0 + ty * 4 + i;
+        if (gr >= M) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int gc = col0 + tx * 4 + j;
+            if (gc >= N) continue;
+            T* out = Y + gr * ldy + gc;
+            Acc v = alpha * acc[i][j];
+            if (beta != Acc(0)) v = fma(beta, static_cast<Acc>(*out), v);
+            *out = static_cast<T>(v);
+        }
+    }
+}
+
+// Narrow (N <= 8) row-major A: one wavefront per output row, lanes across K.
+template <typename V, typename TB, typename T, typename Acc, int NMAX>
+__global__ __launch_bounds__(kThreads) void gemm_nn_narrow_rowmajor_kernel(
+    int64_t M, int N, int64_t K, const V* __restrict__ A, int64_t rs, const TB* __restrict__ B,
+    int64_t ldb, T* __restrict__ Y, int64_t ldy, Acc alpha, Acc beta) {
+    const int lane = threadIdx.x % kWave;
+    const int64_t row = static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + threadIdx.x / kWave;
+    if (row >= M) return;
+    Acc acc[NMAX];
+#pragma unroll
+    for (int j = 0; j < NMAX; ++j) acc[j] = Acc(0);
+    const V* arow = A + row * rs;
+    for (int64_t k = lane; k < K; k += kWave) {
+        const Acc a = ld_acc<Acc>(arow, k);
+#pragma unroll
+        for (int j = 0; j < NMAX; ++j)
+            if (j < N) acc[j] = fma(a, static_cast<Acc>(B[k * ldb + j]), acc[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < NMAX; ++j) {
+        Acc v = acc[j];
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        acc[j] = v;
+    }
+    if (lane < N) {
+        Acc v = Acc(0);
+#pragma unroll
+        for (int j = 0; j < NMAX; ++j)
+            if (j == lane) v = acc[j];
+        T* out = Y + row * ldy + lane;
+        Acc r = alpha * v;
+        if (beta != Acc(0)) r = fma(beta, static_cast<Acc>(*out), r);
+        *out = static_cast<T>(r);
+    }
+}
+
+// Narrow (N <= 8) column-major A: one thread per output row, loop over K
+// (consecutive threads read consecutive rows of each column: coalesced).
+template <typename V, typename TB, typename T, typename Acc, int NMAX>
+__global__ __launch_bounds__(kThreads) void gemm_nn_narrow_colmajor_kernel(
+    int64_t M, int N, int64_t K, const V* __restrict__ A, int64_t cs, const TB* __restrict__ B,
+    int64_t ldb, T* __restrict__ Y, int64_t ldy, Acc alpha, Acc beta) {
+    __shared__ Acc Bs[64][NMAX];
+    const int64_t row = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    Acc acc[NMAX];
+#pragma unroll
+    for (int j = 0; j < NMAX; ++j) acc[j] = Acc(0);
+    for (int64_t k0 = 0; k0 < K; k0 += 64) {
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < 64 * NMAX; idx += kThreads) {
+            const int kk = idx / NMAX, j = idx % NMAX;
+            const int64_t gk = k0 + kk;
+            Bs[kk][j] = (gk < K && j < N) ? static_cast<Acc>(B[gk * ldb + j]) : Acc(0);
+        }
+        __syncthreads();
+        if (row < M) {
+            const int kend = static_cast<int>(std::min<int64_t>(64, K - k0));
+            for (int kk = 0; kk < kend; ++kk) {
+                const Acc a = ld_acc<Acc>(A, row + (k0 + kk) * cs);
+#pragma unroll
+                for (int j = 0; j < NMAX; ++j) acc[j] = fma(a, Bs[kk][j], acc[j]);
+            }
+        }
+    }
+    if (row >= M) return;
+    for (int j = 0; j < N; ++j) {
+        T* out = Y + row * ldy + j;
+        Acc r = alpha * acc[j];
+        if (beta != Acc(0)) r = fma(beta, static_cast<Acc>(*out), r);
+        *out = static_cast<T>(r);
+    }
+}
+
+template <typename V, typename TB, typename T>
+void gemm_nn_dispatch(hipStream_t s, int64_t M, int N, int64_t K, const V* A, int64_t rs,
+                      int64_t cs, const TB* B, int64_t ldb, T* Y, int64_t ldy, double alpha,
+                      double beta) {
+    using Acc = T;
+    if (M <= 0 || N <= 0) return;
+    const bool row_major = (cs == 1);
+    if (N <= 8) {
+        if (row_major) {
+            dim3 grid(static_cast<unsigned>(ceil_div(M, kThreads / kWave)));
+            hipLaunchKernelGGL((gemm_nn_narrow_rowmajor_kernel<V, TB, T, Acc, 8>), grid, dim3(kThreads), 0, s,
+                               M, N, K, A, rs, B, ldb, Y, ldy, Acc(alpha), Acc(beta));
+        } else {
+            dim3 grid(static_cast<unsigned>(ceil_div(M, kThreads)));
+            hipLaunchKernelGGL((gemm_nn_narrow_colmajor_kernel<V, TB, T, Acc, 8>), grid, dim3(kThreads), 0, s,
+                               M, N, K, A, cs, B, ldb, Y, ldy, Acc(alpha), Acc(beta));
+        }
+    } else {
+        dim3 grid(static_cast<unsigned>(ceil_div(M, 64)), static_cast<unsigned>(ceil_div(N, 64)));
+        if (row_major)
+            hipLaunchKernelGGL((gemm_nn_tile_kernel<V, TB, T, Acc, true>), grid, dim3(kThreads), 0, s, M, N,
+                               K, A, rs, cs, B, ldb, Y, ldy, Acc(alpha), Acc(beta));
+        else
+            hipLaunchKernelGGL((gemm_nn_tile_kernel<V, TB, T, Acc, false>), grid, dim3(kThreads), 0, s, M, N,
+                               K, A, rs, cs, B, ldb, Y, ldy, Acc(alpha), Acc(beta));
+    }
+    AMFLS_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------ gemm_tn
+// Partial sums over a chunk of rows; P[chunk][K][N] (double).
+template <typename V, typename TB, bool AROW>
+__global__ __launch_bounds__(kThreads) void gemm_tn_tile_kernel(
+    int64_t M, int64_t K, int N, const V* __restrict__ A, int64_t rs, int64_t cs,
+    const TB* __restrict__ B, int64_t ldb, double* __restrict__ P, int64_t rows_per_chunk) {
+    constexpr int BK = 64, BN = 64, BR = 32;
+    __shared__ double As[BR][BK + 1];
+    __shared__ double Bs[BR][BN];
+    const int64_t k0 = static_cast<int64_t>(blockIdx.x) * BK;
+    const int n0 = blockIdx.y * BN;
+    const int64_t chunk = blockIdx.z;
+    const int64_t r_begin = chunk * rows_per_chunk;
+    const int64_t r_end = std::min<int64_t>(M, r_begin + rows_per_chunk);
+    const int t = threadIdx.x, tx = t % 16, ty = t / 16;
+    double acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+    for (int64_t r0 = r_begin; r0 < r_end; r0 += BR) {
+#pragma unroll
+        for (int l = 0; l < (BR * BK) / kThreads; ++l) {
+            const int idx = t + l * kThreads;
+            int rr, kk;
+            if (AROW) { kk = idx % BK; rr = idx / BK; } else { rr = idx % BR; kk = idx / BR; }
+            const int64_t gr = r0 + rr, gk = k0 + kk;
+            As[rr][kk] = (gr < r_end && gk < K) ? ld_d(A, gr * rs + gk * cs) : 0.0;
+        }
+#pragma unroll
+        for (int l = 0; l < (BR * BN) / kThreads; ++l) {
+            const int idx = t + l * kThreads;
+            const int c = idx % BN, rr = idx / BN;
+            const int64_t gr = r0 + rr;
+            const int gc = n0 + c;
+            Bs[rr][c] = (gr < r_end && gc < N) ? static_cast<double>(B[gr * ldb + gc]) : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int rr = 0; rr < BR; ++rr) {
+            double a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = As[rr][ty * 4 + i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = Bs[rr][tx * 4 + j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = fma(a[i], b[j], acc[i][j]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int64_t gk = k0 + ty * 4 + i;
+        if (gk >= K) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int gn = n0 + tx * 4 + j;
+            if (gn >= N) continue;
+            P[(chunk * K + gk) * N + gn] = acc[i][j];
+        }
+    }
+}
+
+// Narrow N (<= 8), row-major A: thread per k column, loop over the chunk rows.
+template <typename V, typename TB, int NMAX>
+__global__ __launch_bounds__(kThreads) void gemm_tn_narrow_rowmajor_kernel(
+    int64_t M, int64_t K, int N, const V* __restrict__ A, int64_t rs, const TB* __restrict__ B,
+    int64_t ldb, double* __restrict__ P, int64_t rows_per_chunk) {
+    __shared__ double Bs[64][NMAX];
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    const int64_t chunk = blockIdx.y;
+    const int64_t r_begin = chunk * rows_per_chunk;
+    const int64_t r_end = std::min<int64_t>(M, r_begin + rows_per_chunk);
+    double acc[NMAX];
+#pragma unroll
+    for (int j = 0; j < NMAX; ++j) acc[j] = 0.0;
+    for (int64_t r0 = r_begin; r0 < r_end; r0 += 64) {
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < 64 * NMAX; idx += kThreads) {
+            const int rr = idx / NMAX, j = idx % NMAX;
+            const int64_t gr = r0 + rr;
+            Bs[rr][j] = (gr < r_end && j < N) ? static_cast<double>(B[gr * ldb + j]) : 0.0;
+        }
+        __syncthreads();
+        if (k < K) {
+            const int rend = static_cast<int>(std::min<int64_t>(64, r_end - r0));
+            for (int rr = 0; rr < rend; ++rr) {
+                const double a = ld_d(A, (r0 + rr) * rs + k);
+#pragma unroll
+                for (int j = 0; j < NMAX; ++j) acc[j] = fma(a, Bs[rr][j], acc[j]);
+            }
+        }
+    }
+    if (k >= K) return;
+    for (int j = 0; j < N; ++j) P[(chunk * K + k) * N + j] = acc[j];
+}
+
+// Narrow N (<= 8), column-major A: one wavefront per k column, lanes across
+// the chunk rows, wave reduction.
+template <typename V, typename TB, int NMAX>
+__global__ __launch_bounds__(kThreads) void gemm_tn_narrow_colmajor_kernel(
+    int64_t M, int64_t K, int N, const V* __restrict__ A, int64_t cs, const TB* __restrict__ B,
+    int64_t ldb, double* __restrict__ P, int64_t rows_per_chunk) {
+    const int lane = threadIdx.x % kWave;
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + threadIdx.x / kWave;
+    const int64_t chunk = blockIdx.y;
+    if (k >= K) return;
+    const int64_t r_begin = chunk * rows_per_chunk;
+    const int64_t r_end = std::min<int64_t>(M, r_begin + rows_per_chunk);
+    double acc[NMAX];
+#pragma unroll
+    for (int j = 0; j < NMAX; ++j) acc[j] = 0.0;
+    const V* acol = A + k * cs;
+    for (int64_t r = r_begin + lane; r < r_end; r += kWave) {
+        const double a = ld_d(acol, r);
+#pragma unroll
+        for (int j = 0; j < NMAX; ++j)
+            if (j < N) acc[j] = fma(a, static_cast<double>(B[r * ldb + j]), acc[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < NMAX; ++j) {
+        double v = acc[j];
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        acc[j] = v;
+    }
+    if (lane < N) {
+        double v = 0.0;
+#pragma unroll
+        for (int j = 0; j < NMAX; ++j)
+            if (j == lane) v = acc[j];
+        P[(chunk * K + k) * N + lane] = v;
+    }
+}
+
+template <typename TC>
+__global__ void reduce_chunks_kernel(int64_t chunks, int64_t K, int N, const double* __restrict__ P,
+                                     TC* __restrict__ C, int64_t ldc, double alpha) {
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= K * N) return;
+    const int64_t k = idx / N;
+    const int n = static_cast<int>(idx % N);
+    double sum = 0.0;
+    for (int64_t c = 0; c < chunks; ++c) sum += P[(c * K + k) * N + n];
+    C[k * ldc + n] = static_cast<TC>(alpha * sum);
+}
+
+template <typename V, typename TB>
+void gemm_tn_dispatch(hipStream_t s, int64_t M, int64_t K, int N, const V* A, int64_t rs,
+                      int64_t cs, const TB* B, int64_t ldb, void* C, ValueType ctype, int64_t ldc,
+                      double alpha, Scratch& scratch) {
+    if (K <= 0 || N <= 0) return;
+    const bool row_major = (cs == 1);
+    int64_t col_blocks;
+    if (N <= 8) {
+        col_blocks = row_major ? ceil_div(K, kThreads) : ceil_div(K, kThreads / kWave);
+    } else {
+        col_blocks = ceil_div(K, 64) * ceil_div(N, 64);
+    }
+    // Enough chunks along M for >= ~2048 blocks, each chunk >= 256 rows.
+    int64_t chunks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(M, 256), ceil_div(2048, col_blocks)));
+    chunks = std::min<int64_t>(chunks, 65535);
+    const int64_t rows_per_chunk = std::max<int64_t>(1, ceil_div(std::max<int64_t>(M, 1), chunks));
+    chunks = std::max<int64_t>(1, ceil_div(std::max<int64_t>(M, 1), rows_per_chunk));
+    scratch.ensure(static_cast<size_t>(chunks * K * N) * sizeof(double));
+    double* P = static_cast<double*>(scratch.ptr);
+    if (M <= 0) {
+        AMFLS_HIP_CHECK(hipMemsetAsync(P, 0, static_cast<size_t>(K * N) * sizeof(double), s));
+        chunks = 1;
+    } else if (N <= 8) {
+        if (row_major) {
+            dim3 grid(static_cast<unsigned>(ceil_div(K, kThreads)), static_cast<unsigned>(chunks));
+            hipLaunchKernelGGL((gemm_tn_narrow_rowmajor_kernel<V, TB, 8>), grid, dim3(kThreads), 0, s, M, K,
+                               N, A, rs, B, ldb, P, rows_per_chunk);
+        } else {
+            dim3 grid(static_cast<unsigned>(ceil_div(K, kThreads / kWave)), static_cast<unsigned>(chunks));
+            hipLaunchKernelGGL((gemm_tn_narrow_colmajor_kernel<V, TB, 8>), grid, dim3(kThreads), 0, s, M, K,
+                               N, A, cs, B, ldb, P, rows_per_chunk);
+        }
+    } else {
+        dim3 grid(static_cast<unsigned>(ceil_div(K, 64)), static_cast<unsigned>(ceil_div(N, 64)),
+                  static_cast<unsigned>(chunks));
+        if (row_major)
+            hipLaunchKernelGGL((gemm_tn_tile_kernel<V, TB, true>), grid, dim3(kThreads), 0, s, M, K, N, A, rs,
+                               cs, B, ldb, P, rows_per_chunk);
+        else
+            hipLaunchKernelGGL((gemm_tn_tile_kernel<V, TB, false>), grid, dim3(kThreads), 0, s, M, K, N, A,
+                               rs, cs, B, ldb, P, rows_per_chunk);
+    }
+    AMFLS_LAUNCH_CHECK();
+    const int64_t total = K * N;
+    dim3 g2(static_cast<unsigned>(ceil_div(total, kThreads)));
+    if (ctype == ValueType::f64)
+        hipLaunchKernelGGL(reduce_chunks_kernel<double>, g2, dim3(kThreads), 0, s, chunks, K, N, P,
+                           static_cast<double*>(C), ldc, alpha);
+    else
+        hipLaunchKernelGGL(reduce_chunks_kernel<float>, g2, dim3(kThreads), 0, s, chunks, K, N, P,
+                           static_cast<float*>(C), ldc, alpha);
+    AMFLS_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------ rowmul
+// Each block owns 32 rows entirely: loads them to LDS, then overwrites.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void rowmul_kernel(int64_t rows, int w, int r, const T* X,
+                                                          int64_t ldx, const double* __restrict__ M,
+                                                          T* Y, int64_t ldy) {
+    constexpr int BR = 32;
+    __shared__ T Xs[BR][128];
+    const int64_t r0 = static_cast<int64_t>(blockIdx.x) * BR;
+    for (int idx = threadIdx.x; idx < BR * w; idx += kThreads) {
+        const int rr = idx / w, c = idx % w;
+        const int64_t gr = r0 + rr;
+        Xs[rr][c] = gr < rows ? X[gr * ldx + c] : T(0);
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < BR * r; idx += kThreads) {
+        const int rr = idx / r, c = idx % r;
+        const int64_t gr = r0 + rr;
+        if (gr >= rows) continue;
+        double acc = 0.0;
+        for (int j = 0; j < w; ++j) acc = fma(static_cast<double>(Xs[rr][j]), M[j * r + c], acc);
+        Y[gr * ldy + c] = static_cast<T>(acc);
+    }
+}
+
+// ------------------------------------------------------------------ vectors
+template <typename Tin, typename Tout>
+__global__ void copy_block_kernel(int64_t rows, int cols, const Tin* X, int64_t ldx, Tout* Y,
+                                  int64_t ldy, double scale) {
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= rows * cols) return;
+    const int64_t i = idx / cols;
+    const int j = static_cast<int>(idx % cols);
+    const double v = static_cast<double>(X[i * ldx + j]);
+    Y[i * ldy + j] = static_cast<Tout>(scale == 1.0 ? v : scale * v);
+}
+
+template <typename T>
+__global__ void axpy_block_kernel(int64_t rows, int cols, double a, const T* X, int64_t ldx, T* Y,
+                                  int64_t ldy) {
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= rows * cols) return;
+    const int64_t i = idx / cols;
+    const int j = static_cast<int>(idx % cols);
+    T* y = Y + i * ldy + j;
+    *y = static_cast<T>(fma(a, static_cast<double>(X[i * ldx + j]), static_cast<double>(*y)));
+}
+
+template <typename T>
+__global__ void fill_kernel(T* y, int64_t n, T value) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = value;
+}
+
+__global__ void sub_kernel(int64_t n, const double* a, const double* b, double* r) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) r[i] = a[i] - b[i];
+}
+
+__global__ void fma_kernel(int64_t n, double alpha, const double* x, const double* z, double* y) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = fma(alpha, x[i], z[i]);
+}
+
+__global__ void add_kernel(int64_t n, const double* x, double* y) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) y[i] += x[i];
+}
+
+__global__ void scale_kernel(int64_t n, double alpha, const double* x, double* y) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = alpha * x[i];
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+    return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// Pass 1: per-block max |v| over finite values and nonfinite count.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void stats_max_kernel(int64_t rows, int cols, const T* X,
+                                                             int64_t ldx, double* partial) {
+    __shared__ double smax[kThreads / kWave];
+    __shared__ double scnt[kThreads / kWave];
+    const int64_t total = rows * cols;
+    double m = 0.0, bad = 0.0;
+    for (int64_t idx = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; idx < total;
+         idx += static_cast<int64_t>(gridDim.x) * kThreads) {
+        const int64_t i = idx / cols;
+        const int j = static_cast<int>(idx % cols);
+        const double v = fabs(static_cast<double>(X[i * ldx + j]));
+        if (isfinite(v)) m = fmax(m, v); else bad += 1.0;
+    }
+    m = wave_max(m);
+    bad = wave_sum(bad);
+    const int w = threadIdx.x / kWave;
+    if (threadIdx.x % kWave == 0) { smax[w] = m; scnt[w] = bad; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double mm = 0.0, bb = 0.0;
+        for (int k = 0; k < kThreads / kWave; ++k) { mm = fmax(mm, smax[k]); bb += scnt[k]; }
+        partial[2 * blockIdx.x] = mm;
+        partial[2 * blockIdx.x + 1] = bb;
+    }
+}
+
+__global__ void stats_max_final_kernel(int blocks, const double* partial, double* out) {
+    __shared__ double smax[kThreads / kWave];
+    __shared__ double scnt[kThreads / kWave];
+    double m = 0.0, bad = 0.0;
+    for (int b = threadIdx.x; b < blocks; b += kThreads) {
+        m = fmax(m, partial[2 * b]);
+        bad += partial[2 * b + 1];
+    }
+    m = wave_max(m);
+    bad = wave_sum(bad);
+    const int w = threadIdx.x / kWave;
+    if (threadIdx.x % kWave == 0) { smax[w] = m; scnt[w] = bad; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double mm = 0.0, bb = 0.0;
+        for (int k = 0; k < kThreads / kWave; ++k) { mm = fmax(mm, smax[k]); bb += scnt[k]; }
+        out[0] = mm;
+        out[2] = bb;
+    }
+}
+
+// Pass 2: sum of (|v| * 2^-e)^2 where frexp(max) = (f, e).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void stats_sum_kernel(int64_t rows, int cols, const T* X,
+                                                             int64_t ldx, const double* out,
+                                                             double* partial) {
+    __shared__ double ssum[kThreads / kWave];
+    const double mx = out[0];
+    int e = 0;
+    if (mx > 0.0 && isfinite(mx)) (void)frexp(mx, &e);
+    const int64_t total = rows * cols;
+    double acc = 0.0;
+    for (int64_t idx = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; idx < total;
+         idx += static_cast<int64_t>(gridDim.x) * kThreads) {
+        const int64_t i = idx / cols;
+        const int j = static_cast<int>(idx % cols);
+        const double v = fabs(static_cast<double>(X[i * ldx + j]));
+        if (isfinite(v)) {
+            const double sv = ldexp(v, -e);
+            acc = fma(sv, sv, acc);
+        }
+    }
+    acc = wave_sum(acc);
+    const int w = threadIdx.x / kWave;
+    if (threadIdx.x % kWave == 0) ssum[w] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double ss = 0.0;
+        for (int k = 0; k < kThreads / kWave; ++k) ss += ssum[k];
+        partial[blockIdx.x] = ss;
+    }
+}
+
+__global__ void stats_sum_final_kernel(int blocks, const double* partial, double* out) {
+    __shared__ double ssum[kThreads / kWave];
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < blocks; b += kThreads) acc += partial[b];
+    acc = wave_sum(acc);
+    const int w = threadIdx.x / kWave;
+    if (threadIdx.x % kWave == 0) ssum[w] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double ss = 0.0;
+        for (int k = 0; k < kThreads / kWave; ++k) ss += ssum[k];
+        out[1] = ss;
+    }
+}
+
+// ------------------------------------------------------------------ RNG
+__device__ __forceinline__ uint64_t splitmix64(uint64_t v) {
+    v += 0x9e3779b97f4a7c15ULL;
+    v = (v ^ (v >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    v = (v ^ (v >> 27)) * 0x94d049bb133111ebULL;
+    return v ^ (v >> 31);
+}
+
+__device__ __forceinline__ double open_uniform(uint64_t bits) {
+    return static_cast<double>((bits >> 11) + 1ULL) * (1.0 / 9007199254740992.0);
+}
+
+__device__ __forceinline__ double gaussian_at(uint64_t key, uint64_t index) {
+    const double u1 = open_uniform(splitmix64(key ^ (2ULL * index)));
+    const double u2 = open_uniform(splitmix64(key ^ (2ULL * index + 1ULL)));
+    return sqrt(-2.0 * log(u1)) * cos(2.0 * 3.141592653589793 * u2);
+}
+
+template <typename T>
+__global__ void gaussian_block_kernel(int64_t rows, int cols, uint64_t key, uint64_t first_col, T* Y,
+                                      int64_t ldy) {
+    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= rows * cols) return;
+    const int j = static_cast<int>(idx / rows);
+    const int64_t i = idx % rows;
+    const uint64_t index = (first_col + static_cast<uint64_t>(j)) * static_cast<uint64_t>(rows) +
+                           static_cast<uint64_t>(i);
+    Y[i * ldy + j] = static_cast<T>(gaussian_at(key, index));
+}
+
+uint64_t host_splitmix64(uint64_t v) {
+    v += 0x9e3779b97f4a7c15ULL;
+    v = (v ^ (v >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    v = (v ^ (v >> 27)) * 0x94d049bb133111ebULL;
+    return v ^ (v >> 31);
+}
+
+// ------------------------------------------------------------------ dense stats
+template <typename V>
+__global__ __launch_bounds__(kThreads) void dense_row_stats_kernel(int64_t M, int64_t N, const V* A,
+                                                                   int64_t rs, int64_t cs,
+                                                                   double* sum, double* sq,
+                                                                   double* misc_partial) {
+    // one wavefront per row; lanes along the row
+    const int lane = threadIdx.x % kWave;
+    const int64_t row = static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + threadIdx.x / kWave;
+    double s = 0.0, q = 0.0, mx = 0.0, bad = 0.0;
+    if (row < M) {
+        for (int64_t j = lane; j < N; j += kWave) {
+            const double v = fabs(to_double(A[row * rs + j * cs]));
+            if (!isfinite(v)) { bad += 1.0; continue; }
+            s += v;
+            q = fma(v, v, q);
+            mx = fmax(mx, v);
+        }
+    }
+    s = wave_sum(s);
+    q = wave_sum(q);
+    mx = wave_max(mx);
+    bad = wave_sum(bad);
+    if (row < M && lane == 0) {
+        sum[row] = s;
+        sq[row] = q;
+        misc_partial[2 * row] = mx;
+        misc_partial[2 * row + 1] = bad;
+    }
+}
+
+template <typename V>
+__global__ __launch_bounds__(kThreads) void dense_col_stats_kernel(int64_t M, int64_t N, const V* A,
+                                                                   int64_t rs, int64_t cs,
+                                                                   double* sum, double* sq) {
+    const int lane = threadIdx.x % kWave;
+    const int64_t col = static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + threadIdx.x / kWave;
+    if (col >= N) return;
+    double s = 0.0, q = 0.0;
+    for (int64_t i = lane; i < M; i += kWave) {
+        const double v = fabs(to_double(A[i * rs + col * cs]));
+        if (!isfinite(v)) continue;
+        s += v;
+        q = fma(v, v, q);
+    }
+    s = wave_sum(s);
+    q = wave_sum(q);
+    if (lane == 0) {
+        sum[col] = s;
+        sq[col] = q;
+    }
+}
+
+// Column stats for row-major storage: thread per column, rows looped
+// (coalesced along the row).
+template <typename V>
+__global__ __launch_bounds__(kThreads) void dense_col_stats_rowmajor_kernel(
+    int64_t M, int64_t N, const V* A, int64_t rs, double* sum, double* sq, int64_t rows_per_chunk,
+    double* psum, double* psq) {
+    const int64_t col = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    const int64_t chunk = blockIdx.y;
+    if (col >= N) return;
+    const int64_t r0 = chunk * rows_per_chunk;
+    const int64_t r1 = std::min<int64_t>(M, r0 + rows_per_chunk);
+    double s = 0.0, q = 0.0;
+    for (int64_t i = r0; i < r1; ++i) {
+        const double v = fabs(to_double(A[i * rs + col]));
+        if (!isfinite(v)) continue;
+        s += v;
+        q = fma(v, v, q);
+    }
+    psum[chunk * N + col] = s;
+    psq[chunk * N + col] = q;
+}
+
+__global__ void sum_chunks2_kernel(int64_t chunks, int64_t N, const double* psum, const double* psq,
+                                   double* sum, double* sq) {
+    const int64_t col = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (col >= N) return;
+    double s = 0.0, q = 0.0;
+    for (int64_t c = 0; c < chunks; ++c) {
+        s += psum[c * N + col];
+        q += psq[c * N + col];
+    }
+    sum[col] = s;
+    sq[col] = q;
+}
+
+__global__ void misc_final_kernel(int64_t rows, const double* misc_partial, double* misc) {
+    __shared__ double smax[kThreads / kWave];
+    __shared__ double scnt[kThreads / kWave];
+    double m = 0.0, bad = 0.0;
+    for (int64_t r = threadIdx.x; r < rows; r += kThreads) {
+        m = fmax(m, misc_partial[2 * r]);
+        bad += misc_partial[2 * r + 1];
+    }
+    m = wave_max(m);
+    bad = wave_sum(bad);
+    const int w = threadIdx.x / kWave;
+    if (threadIdx.x % kWave == 0) { smax[w] = m; scnt[w] = bad; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double mm = 0.0, bb = 0.0;
+        for (int k = 0; k < kThreads / kWave; ++k) { mm = fmax(mm, smax[k]); bb += scnt[k]; }
+        misc[0] = mm;
+        misc[1] = bb;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- launchers
+template <typename T, typename TB>
+void gemm_nn(hipStream_t s, int64_t M, int N, int64_t K, const Strided& A, const TB* B, int64_t ldb,
+             T* Y, int64_t ldy, double alpha, double beta) {
+    switch (A.type) {
+        case ValueType::f64:
+            gemm_nn_dispatch<double, TB, T>(s, M, N, K, static_cast<const double*>(A.ptr), A.rs, A.cs, B,
+                                            ldb, Y, ldy, alpha, beta);
+            break;
+        case ValueType::f32:
+            gemm_nn_dispatch<float, TB, T>(s, M, N, K, static_cast<const float*>(A.ptr), A.rs, A.cs, B,
+                                           ldb, Y, ldy, alpha, beta);
+            break;
+        case ValueType::bf16:
+            gemm_nn_dispatch<__hip_bfloat16, TB, T>(s, M, N, K,
+                                                    static_cast<const __hip_bfloat16*>(A.ptr), A.rs,
+                                                    A.cs, B, ldb, Y, ldy, alpha, beta);
+            break;
+    }
+}
+
+template <typename TB>
+void gemm_tn(hipStream_t s, int64_t M, int64_t K, int N, const Strided& A, const TB* B, int64_t ldb,
+             void* C, ValueType ctype, int64_t ldc, double alpha, Scratch& scratch) {
+    switch (A.type) {
+        case ValueType::f64:
+            gemm_tn_dispatch<double, TB>(s, M, K, N, static_cast<const double*>(A.ptr), A.rs, A.cs, B,
+                                         ldb, C, ctype, ldc, alpha, scratch);
+            break;
+        case ValueType::f32:
+            gemm_tn_dispatch<float, TB>(s, M, K, N, static_cast<const float*>(A.ptr), A.rs, A.cs, B, ldb,
+                                        C, ctype, ldc, alpha, scratch);
+            break;
+        case ValueType::bf16:
+            gemm_tn_dispatch<__hip_bfloat16, TB>(s, M, K, N, static_cast<const __hip_bfloat16*>(A.ptr),
+                                                 A.rs, A.cs, B, ldb, C, ctype, ldc, alpha, scratch);
+            break;
+    }
+}
+
+template <typename T>
+void rowmul(hipStream_t s, int64_t rows, int w, int r, const T* X, int64_t ldx, const double* M,
+            T* Y, int64_t ldy) {
+    if (rows <= 0 || r <= 0) return;
+    if (w > 128 || r > 128) throw std::invalid_argument("rowmul supports widths up to 128");
+    hipLaunchKernelGGL(rowmul_kernel<T>, dim3(static_cast<unsigned>(ceil_div(rows, 32))), dim3(kThreads), 0,
+                       s, rows, w, r, X, ldx, M, Y, ldy);
+    AMFLS_LAUNCH_CHECK();
+}
+
+template <typename Tin, typename Tout>
+void copy_block(hipStream_t s, int64_t rows, int cols, const Tin* X, int64_t ldx, Tout* Y,
+                int64_t ldy, double scale) {
+    const int64_t total = rows * cols;
+    if (total <= 0) return;
+    hipLaunchKernelGGL((copy_block_kernel<Tin, Tout>), dim3(static_cast<unsigned>(ceil_div(total, kThreads))),
+                       dim3(kThreads), 0, s, rows, cols, X, ldx, Y, ldy, scale);
+    AMFLS_LAUNCH_CHECK();
+}
+
+template <typename T>
+void fill(hipStream_t s, T* y, int64_t n, T value) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(fill_kernel<T>, dim3(static_cast<unsigned>(ceil_div(n, kThreads))), dim3(kThreads), 0, s,
+                       y, n, value);
+    AMFLS_LAUNCH_CHECK();
+}
+
+template <typename T>
+void axpy_block(hipStream_t s, int64_t rows, int cols, double a, const T* X, int64_t ldx, T* Y, int64_t ldy) {
+    const int64_t total = rows * cols;
+    if (total <= 0) return;
+    hipLaunchKernelGGL(axpy_block_kernel<T>, dim3(static_cast<unsigned>(ceil_div(total, kThreads))), dim3(kThreads),
+                       0, s, rows, cols, a, X, ldx, Y, ldy);
+    AMFLS_LAUNCH_CHECK();
+}
+
+void sub_vec(hipStream_t s, int64_t n, const double* a, const double* b, double* r) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(sub_kernel, dim3(static_cast<unsigned>(ceil_div(n, kThreads))), dim3(kThreads), 0, s, n,
+                       a, b, r);
+    AMFLS_LAUNCH_CHECK();
+}
+
+void fma_vec(hipStream_t s, int64_t n, double alpha, const double* x, const double* z, double* y) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(fma_kernel, dim3(static_cast<unsigned>(ceil_div(n, kThreads))), dim3(kThreads), 0, s, n,
+                       alpha, x, z, y);
+    AMFLS_LAUNCH_CHECK();
+}
+
+void add_vec(hipStream_t s, int64_t n, const double* x, double* y) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(add_kernel, dim3(static_cast<unsigned>(ceil_div(n, kThreads))), dim3(kThreads), 0, s, n,
+                       x, y);
+    AMFLS_LAUNCH_CHECK();
+}
+
+void scale_vec(hipStream_t s, int64_t n, double alpha, const double* x, double* y) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(scale_kernel, dim3(static_cast<unsigned>(ceil_div(n, kThreads))), dim3(kThreads), 0, s,
+                       n, alpha, x, y);
+    AMFLS_LAUNCH_CHECK();
+}
+
+template <typename T>
+void vector_stats(hipStream_t s, int64_t rows, int cols, const T* X, int64_t ldx, double* out,
+                  Scratch& scratch) {
+    const int64_t total = rows * cols;
+    const int blocks = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(1024, ceil_div(total, kThreads * 4))));
+    scratch.ensure(static_cast<size_t>(2 * blocks) * sizeof(double));
+    double* partial = static_cast<double*>(scratch.ptr);
+    hipLaunchKernelGGL(stats_max_kernel<T>, dim3(blocks), dim3(kThreads), 0, s, rows, cols, X, ldx, partial);
+    AMFLS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(stats_max_final_kernel, dim3(1), dim3(kThreads), 0, s, blocks, partial, out);
+    AMFLS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(stats_sum_kernel<T>, dim3(blocks), dim3(kThreads), 0, s, rows, cols, X, ldx, out,
+                       partial);
+    AMFLS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(stats_sum_final_kernel, dim3(1), dim3(kThreads), 0, s, blocks, partial, out);
+    AMFLS_LAUNCH_CHECK();
+}
+
+template <typename T>
+void gaussian_block(hipStream_t s, int64_t rows, int cols, uint64_t seed, uint64_t stream,
+                    uint64_t first_col, T* Y, int64_t ldy) {
+    const int64_t total = rows * cols;
+    if (total <= 0) return;
+    const uint64_t key = host_splitmix64(seed) ^ host_splitmix64(stream + 0xd2b74407b1ce6e93ULL);
+    hipLaunchKernelGGL(gaussian_block_kernel<T>, dim3(static_cast<unsigned>(ceil_div(total, kThreads))),
+                       dim3(kThreads), 0, s, rows, cols, key, first_col, Y, ldy);
+    AMFLS_LAUNCH_CHECK();
+}
+
+void dense_stats(hipStream_t s, int64_t M, int64_t N, const Strided& A, double* row_sum,
+                 double* row_sq, double* col_sum, double* col_sq, double* misc, Scratch& scratch) {
+    const bool row_major = A.cs == 1;
+    int64_t chunks = 1, rows_per_chunk = M;
+    if (row_major) {
+        chunks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(M, 64), ceil_div(4096, ceil_div(N, kThreads))));
+        rows_per_chunk = ceil_div(M, chunks);
+        chunks = ceil_div(M, rows_per_chunk);
+    }
+    const size_t misc_bytes = static_cast<size_t>(2 * M) * sizeof(double);
+    const size_t chunk_bytes = row_major ? static_cast<size_t>(2 * chunks * N) * sizeof(double) : 0;
+    scratch.ensure(misc_bytes + chunk_bytes);
+    double* misc_partial = static_cast<double*>(scratch.ptr);
+    double* psum = misc_partial + 2 * M;
+    double* psq = psum + chunks * N;
+    auto run = [&](auto tag) {
+        using V = decltype(tag);
+        const V* p = static_cast<const V*>(A.ptr);
+        hipLaunchKernelGGL(dense_row_stats_kernel<V>, dim3(static_cast<unsigned>(ceil_div(M, kThreads / kWave))),
+                           dim3(kThreads), 0, s, M, N, p, A.rs, A.cs, row_sum, row_sq, misc_partial);
+        AMFLS_LAUNCH_CHECK();
+        if (row_major) {
+            hipLaunchKernelGGL(dense_col_stats_rowmajor_kernel<V>,
+                               dim3(static_cast<unsigned>(ceil_div(N, kThreads)), static_cast<unsigned>(chunks)),
+                               dim3(kThreads), 0, s, M, N, p, A.rs, col_sum, col_sq, rows_per_chunk, psum,
+                               psq);
+            AMFLS_LAUNCH_CHECK();
+            hipLaunchKernelGGL(sum_chunks2_kernel, dim3(static_cast<unsigned>(ceil_div(N, kThreads))),
+                               dim3(kThreads), 0, s, chunks, N, psum, psq, col_sum, col_sq);
+        } else {
+            hipLaunchKernelGGL(dense_col_stats_kernel<V>,
+                               dim3(static_cast<unsigned>(ceil_div(N, kThreads / kWave))), dim3(kThreads), 0, s,
+                               M, N, p, A.rs, A.cs, col_sum, col_sq);
+        }
+        AMFLS_LAUNCH_CHECK();
+    };
+    switch (A.type) {
+        case ValueType::f64: run(double{}); break;
+        case ValueType::f32: run(float{}); break;
+        case ValueType::bf16: run(__hip_bfloat16{}); break;
+    }
+    hipLaunchKernelGGL(misc_final_kernel, dim3(1), dim3(kThreads), 0, s, M, misc_partial, misc);
+    AMFLS_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------- instantiations
+template void gemm_nn<double, double>(hipStream_t, int64_t, int, int64_t, const Strided&, const double*,
+                                      int64_t, double*, int64_t, double, double);
+template void gemm_nn<float, float>(hipStream_t, int64_t, int, int64_t, const Strided&, const float*,
+                                    int64_t, float*, int64_t, double, double);
+template void gemm_nn<float, double>(hipStream_t, int64_t, int, int64_t, const Strided&, const double*,
+                                     int64_t, float*, int64_t, double, double);
+template void gemm_nn<double, float>(hipStream_t, int64_t, int, int64_t, const Strided&, const float*,
+                                     int64_t, double*, int64_t, double, double);
+template void gemm_tn<double>(hipStream_t, int64_t, int64_t, int, const Strided&, const double*, int64_t,
+                              void*, ValueType, int64_t, double, Scratch&);
+template void gemm_tn<float>(hipStream_t, int64_t, int64_t, int, const Strided&, const float*, int64_t,
+                             void*, ValueType, int64_t, double, Scratch&);
+template void rowmul<double>(hipStream_t, int64_t, int, int, const double*, int64_t, const double*,
+                             double*, int64_t);
+template void rowmul<float>(hipStream_t, int64_t, int, int, const float*, int64_t, const double*, float*,
+                            int64_t);
+template void copy_block<double, double>(hipStream_t, int64_t, int, const double*, int64_t, double*,
+                                         int64_t, double);
+template void copy_block<double, float>(hipStream_t, int64_t, int, const double*, int64_t, float*,
+                                        int64_t, double);
+template void copy_block<float, double>(hipStream_t, int64_t, int, const float*, int64_t, double*,
+                                        int64_t, double);
+template void copy_block<float, float>(hipStream_t, int64_t, int, const float*, int64_t, float*, int64_t,
+                                       double);
+template void axpy_block<double>(hipStream_t, int64_t, int, double, const double*, int64_t, double*, int64_t);
+template void axpy_block<float>(hipStream_t, int64_t, int, double, const float*, int64_t, float*, int64_t);
+template void fill<double>(hipStream_t, double*, int64_t, double);
+template void fill<float>(hipStream_t, float*, int64_t, float);
+template void fill<int>(hipStream_t, int*, int64_t, int);
+template void vector_stats<double>(hipStream_t, int64_t, int, const double*, int64_t, double*, Scratch&);
+template void vector_stats<float>(hipStream_t, int64_t, int, const float*, int64_t, double*, Scratch&);
+template void gaussian_block<double>(hipStream_t, int64_t, int, uint64_t, uint64_t, uint64_t, double*,
+                                     int64_t);
+template void gaussian_block<float>(hipStream_t, int64_t, int, uint64_t, uint64_t, uint64_t, float*,
+                                    int64_t);
+
+}  // namespace amflsx

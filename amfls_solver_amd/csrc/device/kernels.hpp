@@ -1,0 +1,117 @@
+// Host-side launchers for the AMFLS gfx950 kernels.  Every launcher enqueues
+// work on `stream` and returns without synchronizing; results that the host
+// needs are copied out by the caller.
+#pragma once
+
+#include "common.hpp"
+
+namespace amflsx {
+
+enum class ValueType : int { f64 = 0, f32 = 1, bf16 = 2 };
+
+inline size_t value_size(ValueType t) {
+    return t == ValueType::f64 ? 8 : (t == ValueType::f32 ? 4 : 2);
+}
+
+// Stride description of a large operand A(i, j) = base[i * rs + j * cs].
+struct Strided {
+    const void* ptr = nullptr;
+    ValueType type = ValueType::f64;
+    int64_t rs = 0;
+    int64_t cs = 0;
+    bool row_major() const { return cs == 1; }
+};
+
+// Device scratch owned by one engine instance (grown on demand).
+struct Scratch {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t need);
+    void release();
+};
+
+// ---------------------------------------------------------------- GEMMs
+// Y(M x N, T row-major) = alpha * A(M x K) * B(K x N, TB row-major) + beta * Y.
+template <typename T, typename TB>
+void gemm_nn(hipStream_t s, int64_t M, int N, int64_t K, const Strided& A, const TB* B, int64_t ldb,
+             T* Y, int64_t ldy, double alpha, double beta);
+
+// C(K x N) = A(M x K)^T * B(M x N, TB row-major); accumulation in double with a
+// deterministic split-M reduction through `scratch`; C written as `ctype`
+// (double or float) row-major with leading dimension ldc, C = alpha * sum.
+template <typename TB>
+void gemm_tn(hipStream_t s, int64_t M, int64_t K, int N, const Strided& A, const TB* B, int64_t ldb,
+             void* C, ValueType ctype, int64_t ldc, double alpha, Scratch& scratch);
+
+// In-place safe row-local product: Y[i, 0:r] = X[i, 0:w] * M (w x r, double
+// row-major, device).  w, r <= 128.
+template <typename T>
+void rowmul(hipStream_t s, int64_t rows, int w, int r, const T* X, int64_t ldx, const double* M,
+            T* Y, int64_t ldy);
+
+// ---------------------------------------------------------------- vectors
+template <typename Tin, typename Tout>
+void copy_block(hipStream_t s, int64_t rows, int cols, const Tin* X, int64_t ldx, Tout* Y,
+                int64_t ldy, double scale);
+template <typename T>
+void fill(hipStream_t s, T* y, int64_t n, T value);
+// Y += a * X on row-major blocks.
+template <typename T>
+void axpy_block(hipStream_t s, int64_t rows, int cols, double a, const T* X, int64_t ldx, T* Y, int64_t ldy);
+void sub_vec(hipStream_t s, int64_t n, const double* a, const double* b, double* r);
+// y = fma(alpha, x, z) elementwise, fused (matches std::fma).
+void fma_vec(hipStream_t s, int64_t n, double alpha, const double* x, const double* z, double* y);
+void add_vec(hipStream_t s, int64_t n, const double* x, double* y);
+void scale_vec(hipStream_t s, int64_t n, double alpha, const double* x, double* y);
+
+// Norm statistics of a block viewed as one vector (rows*cols values):
+// out[0] = max |v| over finite values, out[1] = sum (|v| 2^-e)^2 with e the
+// frexp exponent of out[0], out[2] = number of nonfinite values.
+template <typename T>
+void vector_stats(hipStream_t s, int64_t rows, int cols, const T* X, int64_t ldx, double* out_dev,
+                  Scratch& scratch);
+
+// Counter-addressed Gaussian block, index = (first_col + j) * rows + i.
+template <typename T>
+void gaussian_block(hipStream_t s, int64_t rows, int cols, uint64_t seed, uint64_t stream,
+                    uint64_t first_col, T* Y, int64_t ldy);
+
+// ---------------------------------------------------------------- CSR
+struct CsrView {
+    int64_t rows = 0;
+    int64_t cols = 0;
+    int64_t nnz = 0;
+    const int64_t* row_ptr = nullptr;
+    const int32_t* col_idx = nullptr;
+    const void* values = nullptr;
+    ValueType type = ValueType::f32;
+};
+
+// Y(rows x ncols) = A * X (+ add_scale * Add[0:rows]); X, Y, Add row-major.
+template <typename T>
+void csr_spmm(hipStream_t s, const CsrView& A, int ncols, const T* X, int64_t ldx, T* Y, int64_t ldy,
+              const T* Add, int64_t ld_add, double add_scale, int variant);
+int csr_spmm_variant_count();
+bool csr_spmm_variant_supports(int variant, int ncols);
+const char* csr_spmm_variant_name(int variant);
+int csr_spmm_default_variant(int ncols);
+
+// CSR of A^T, stable (each transposed row lists original rows in increasing
+// order).  Outputs sized cols+1, nnz, nnz (values of A's type).
+void csr_transpose(hipStream_t s, const CsrView& A, int64_t* t_row_ptr, int32_t* t_col_idx,
+                   void* t_values, Scratch& scratch);
+
+// Row statistics of a CSR matrix in double: |a| sums, square sums per row,
+// plus misc[0] = max |a|, misc[1] = nonfinite count.
+void csr_row_stats(hipStream_t s, const CsrView& A, double* out_sum, double* out_sq, double* misc,
+                   Scratch& scratch);
+// Structure checks: out[0] row_ptr monotone and spanning, out[1] column
+// indices in range, out[2] no duplicate column inside a row (1 = ok).
+void csr_validate(hipStream_t s, const CsrView& A, int* out_dev);
+
+// Dense statistics: per-row and per-column |a| sums and square sums
+// (double), misc[0] = max |a|, misc[1] = nonfinite count.
+void dense_stats(hipStream_t s, int64_t M, int64_t N, const Strided& A, double* row_sum,
+                 double* row_sq, double* col_sum, double* col_sq, double* misc, Scratch& scratch);
+
+}  // namespace amflsx

@@ -1,0 +1,364 @@
+// CSR block products for the AMFLS engine on gfx950.
+//
+// Y = A X with X, Y row-major (n x s and m x s).  A group of G = P * LC
+// lanes owns one CSR row: LC lanes split the s block columns (each lane
+// holds CPL columns) and P lanes split the row's entries; the P partial sums
+// are combined with xor-shuffles inside the (wave-aligned) group.  A row-major
+// block makes every nonzero a contiguous s-wide gather from X, so one
+// nonzero costs one coalesced row segment instead of s scattered words.
+//
+// A^T Y uses the same kernel on a transposed CSR that is built once on the
+// device by a stable radix sort of the column indices, so no atomics are
+// needed and the summation order is deterministic.
+#include "kernels.hpp"
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+
+namespace amflsx {
+namespace {
+
+constexpr int kThreads = 256;
+
+template <typename V, typename T, int P, int LC, int CPL>
+__global__ __launch_bounds__(kThreads) void csr_spmm_kernel(
+    int64_t rows, const int64_t* __restrict__ rp, const int32_t* __restrict__ ci,
+    const V* __restrict__ val, int ncols, const T* __restrict__ X, int64_t ldx, T* __restrict__ Y,
+    int64_t ldy, const T* __restrict__ Add, int64_t ld_add, T add_scale) {
+    constexpr int G = P * LC;
+    static_assert(G <= kWave && (kWave % G) == 0, "group must tile a wavefront");
+    const int gid = threadIdx.x / G;
+    const int lig = threadIdx.x % G;
+    const int sub = lig / LC;
+    const int cl = lig % LC;
+    const int64_t row = static_cast<int64_t>(blockIdx.x) * (kThreads / G) + gid;
+    if (row >= rows) return;  // whole group leaves together
+    T acc[CPL];
+#pragma unroll
+    for (int t = 0; t < CPL; ++t) acc[t] = T(0);
+    const int64_t e0 = rp[row], e1 = rp[row + 1];
+    int64_t e = e0 + sub;
+    // two entries per trip to keep more gathers in flight
+    for (; e + P < e1; e += 2 * P) {
+        const int32_t c0 = ci[e], c1 = ci[e + P];
+        const T v0 = load_as<T, V>(val[e]);
+        const T v1 = load_as<T, V>(val[e + P]);
+        const T* x0 = X + static_cast<int64_t>(c0) * ldx;
+        const T* x1 = X + static_cast<int64_t>(c1) * ldx;
+#pragma unroll
+        for (int t = 0; t < CPL; ++t) {
+            const int j = cl + LC * t;
+            if (j < ncols) {
+                acc[t] = fma(v0, x0[j], acc[t]);
+                acc[t] = fma(v1, x1[j], acc[t]);
+            }
+        }
+    }
+    if (e < e1) {
+        const int32_t c0 = ci[e];
+        const T v0 = load_as<T, V>(val[e]);
+        const T* x0 = X + static_cast<int64_t>(c0) * ldx;
+#pragma unroll
+        for (int t = 0; t < CPL; ++t) {
+            const int j = cl + LC * t;
+            if (j < ncols) acc[t] = fma(v0, x0[j], acc[t]);
+        }
+    }
+    if constexpr (P > 1) {
+#pragma unroll
+        for (int off = LC; off < G; off <<= 1) {
+#pragma unroll
+            for (int t = 0; t < CPL; ++t) acc[t] += __shfl_xor(acc[t], off);
+        }
+    }
+    if (sub != 0) return;
+#pragma unroll
+    for (int t = 0; t < CPL; ++t) {
+        const int j = cl + LC * t;
+        if (j >= ncols) continue;
+        T r = acc[t];
+        if (Add != nullptr) r = fma(add_scale, Add[row * ld_add + j], r);
+        Y[row * ldy + j] = r;
+    }
+}
+
+struct VariantInfo {
+    int p, lc, cpl;
+    const char* name;
+};
+
+constexpr VariantInfo kVariants[] = {
+    {8, 1, 1, "p8_c1"},   {16, 1, 1, "p16_c1"}, {4, 2, 1, "p4_c2"},   {8, 2, 1, "p8_c2"},
+    {4, 4, 1, "p4_c4"},   {8, 4, 1, "p8_c4"},   {2, 8, 1, "p2_c8"},   {4, 8, 1, "p4_c8"},
+    {1, 16, 1, "p1_c16"}, {2, 16, 1, "p2_c16"}, {4, 16, 1, "p4_c16"}, {1, 32, 1, "p1_c32"},
+    {2, 32, 1, "p2_c32"}, {1, 64, 1, "p1_c64"}, {1, 64, 2, "p1_c128"}, {1, 64, 4, "p1_c256"},
+};
+constexpr int kVariantCount = sizeof(kVariants) / sizeof(kVariants[0]);
+
+template <typename V, typename T, int P, int LC, int CPL>
+void launch_variant(hipStream_t s, const CsrView& A, int ncols, const T* X, int64_t ldx, T* Y,
+                    int64_t ldy, const T* Add, int64_t ld_add, double add_scale) {
+    constexpr int G = P * LC;
+    const int64_t groups_per_block = kThreads / G;
+    const dim3 grid(static_cast<unsigned>(ceil_div(A.rows, groups_per_block)));
+    hipLaunchKernelGGL((csr_spmm_kernel<V, T, P, LC, CPL>), grid, dim3(kThreads), 0, s, A.rows, A.row_ptr,
+                       A.col_idx, static_cast<const V*>(A.values), ncols, X, ldx, Y, ldy, Add, ld_add,
+                       static_cast<T>(add_scale));
+    AMFLS_LAUNCH_CHECK();
+}
+
+template <typename V, typename T>
+void launch_by_index(int variant, hipStream_t s, const CsrView& A, int ncols, const T* X, int64_t ldx,
+                     T* Y, int64_t ldy, const T* Add, int64_t ld_add, double add_scale) {
+#define AMFLS_CASE(i)                                                                            \
+    case i:                                                                                      \
+        launch_variant<V, T, kVariants[i].p, kVariants[i].lc, kVariants[i].cpl>(                 \
+            s, A, ncols, X, ldx, Y, ldy, Add, ld_add, add_scale);                                \
+        break;
+    switch (variant) {
+        AMFLS_CASE(0) AMFLS_CASE(1) AMFLS_CASE(2) AMFLS_CASE(3) AMFLS_CASE(4) AMFLS_CASE(5)
+        AMFLS_CASE(6) AMFLS_CASE(7) AMFLS_CASE(8) AMFLS_CASE(9) AMFLS_CASE(10) AMFLS_CASE(11)
+        AMFLS_CASE(12) AMFLS_CASE(13) AMFLS_CASE(14) AMFLS_CASE(15)
+        default:
+            throw std::invalid_argument("unknown CSR SpMM variant");
+    }
+#undef AMFLS_CASE
+}
+
+// ------------------------------------------------------------ transpose
+__global__ void row_of_kernel(int64_t rows, const int64_t* rp, int32_t* row_of) {
+    const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    for (int64_t e = rp[r]; e < rp[r + 1]; ++e) row_of[e] = static_cast<int32_t>(r);
+}
+
+__global__ void iota_kernel(int64_t n, int32_t* v) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = static_cast<int32_t>(i);
+}
+
+// t_rp[c] = first position k with sorted_key[k] >= c.
+__global__ void offsets_from_sorted_kernel(int64_t nnz, int64_t cols, const int32_t* keys,
+                                           int64_t* t_rp) {
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k > nnz) return;
+    const int64_t prev = (k == 0) ? -1 : static_cast<int64_t>(keys[k - 1]);
+    const int64_t cur = (k == nnz) ? cols : static_cast<int64_t>(keys[k]);
+    for (int64_t c = prev + 1; c <= cur; ++c) t_rp[c] = k;
+}
+
+template <typename V>
+__global__ void transpose_gather_kernel(int64_t nnz, const int32_t* order, const int32_t* row_of,
+                                        const V* val, int32_t* t_ci, V* t_val) {
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= nnz) return;
+    const int32_t e = order[k];
+    t_ci[k] = row_of[e];
+    t_val[k] = val[e];
+}
+
+// ------------------------------------------------------------ stats
+template <typename V>
+__global__ __launch_bounds__(kThreads) void csr_row_stats_kernel(int64_t rows, const int64_t* rp,
+                                                                 const V* val, double* sum, double* sq,
+                                                                 double* misc_partial) {
+    constexpr int G = 8;
+    const int lig = threadIdx.x % G;
+    const int64_t row = static_cast<int64_t>(blockIdx.x) * (kThreads / G) + threadIdx.x / G;
+    if (row >= rows) return;
+    double s = 0.0, q = 0.0, mx = 0.0, bad = 0.0;
+    for (int64_t e = rp[row] + lig; e < rp[row + 1]; e += G) {
+        const double v = fabs(to_double(val[e]));
+        if (!isfinite(v)) { bad += 1.0; continue; }
+        s += v;
+        q = fma(v, v, q);
+        mx = fmax(mx, v);
+    }
+#pragma unroll
+    for (int off = 1; off < G; off <<= 1) {
+        s += __shfl_xor(s, off);
+        q += __shfl_xor(q, off);
+        mx = fmax(mx, __shfl_xor(mx, off));
+        bad += __shfl_xor(bad, off);
+    }
+    if (lig == 0) {
+        sum[row] = s;
+        sq[row] = q;
+        misc_partial[2 * row] = mx;
+        misc_partial[2 * row + 1] = bad;
+    }
+}
+
+__global__ void misc_reduce_kernel(int64_t rows, const double* misc_partial, double* misc) {
+    __shared__ double smax[kThreads];
+    __shared__ double scnt[kThreads];
+    double m = 0.0, bad = 0.0;
+    for (int64_t r = threadIdx.x; r < rows; r += kThreads) {
+        m = fmax(m, misc_partial[2 * r]);
+        bad += misc_partial[2 * r + 1];
+    }
+    smax[threadIdx.x] = m;
+    scnt[threadIdx.x] = bad;
+    __syncthreads();
+    for (int off = kThreads / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + off]);
+            scnt[threadIdx.x] += scnt[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        misc[0] = smax[0];
+        misc[1] = scnt[0];
+    }
+}
+
+__global__ void csr_validate_kernel(int64_t rows, int64_t cols, int64_t nnz, const int64_t* rp,
+                                    const int32_t* ci, int* flags) {
+    const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r == 0 && (rp[0] != 0 || rp[rows] != nnz)) flags[0] = 0;
+    if (r >= rows) return;
+    const int64_t e0 = rp[r], e1 = rp[r + 1];
+    if (e1 < e0 || e0 < 0 || e1 > nnz) {
+        flags[0] = 0;
+        return;
+    }
+    bool sorted = true;
+    for (int64_t e = e0; e < e1; ++e) {
+        const int32_t c = ci[e];
+        if (c < 0 || c >= cols) flags[1] = 0;
+        if (e + 1 < e1 && ci[e + 1] <= c) sorted = false;
+    }
+    if (!sorted) {
+        for (int64_t e = e0; e < e1; ++e)
+            for (int64_t f = e + 1; f < e1; ++f)
+                if (ci[e] == ci[f]) flags[2] = 0;
+    }
+}
+
+}  // namespace
+
+int csr_spmm_variant_count() { return kVariantCount; }
+
+bool csr_spmm_variant_supports(int variant, int ncols) {
+    if (variant < 0 || variant >= kVariantCount) return false;
+    return ncols <= kVariants[variant].lc * kVariants[variant].cpl;
+}
+
+const char* csr_spmm_variant_name(int variant) {
+    return (variant >= 0 && variant < kVariantCount) ? kVariants[variant].name : "invalid";
+}
+
+int csr_spmm_default_variant(int ncols) {
+    if (ncols <= 1) return 0;    // p8_c1
+    if (ncols <= 2) return 2;    // p4_c2
+    if (ncols <= 4) return 4;    // p4_c4
+    if (ncols <= 8) return 6;    // p2_c8
+    if (ncols <= 16) return 8;   // p1_c16
+    if (ncols <= 32) return 11;  // p1_c32
+    if (ncols <= 64) return 13;  // p1_c64
+    if (ncols <= 128) return 14;
+    return 15;
+}
+
+template <typename T>
+void csr_spmm(hipStream_t s, const CsrView& A, int ncols, const T* X, int64_t ldx, T* Y, int64_t ldy,
+              const T* Add, int64_t ld_add, double add_scale, int variant) {
+    if (A.rows <= 0 || ncols <= 0) return;
+    // Blocks wider than 256 columns are processed in 256-column slabs.
+    for (int c0 = 0; c0 < ncols; c0 += 256) {
+        const int nc = std::min(256, ncols - c0);
+        int v = variant;
+        if (v < 0 || !csr_spmm_variant_supports(v, nc)) v = csr_spmm_default_variant(nc);
+        const T* add = Add ? Add + c0 : nullptr;
+        switch (A.type) {
+            case ValueType::f32:
+                launch_by_index<float, T>(v, s, A, nc, X + c0, ldx, Y + c0, ldy, add, ld_add, add_scale);
+                break;
+            case ValueType::f64:
+                launch_by_index<double, T>(v, s, A, nc, X + c0, ldx, Y + c0, ldy, add, ld_add, add_scale);
+                break;
+            default:
+                throw std::invalid_argument("CSR values must be float32 or float64");
+        }
+    }
+}
+
+void csr_transpose(hipStream_t s, const CsrView& A, int64_t* t_rp, int32_t* t_ci, void* t_val,
+                   Scratch& scratch) {
+    const int64_t nnz = A.nnz;
+    if (nnz >= (int64_t(1) << 31)) throw std::invalid_argument("CSR transpose supports nnz < 2^31");
+    if (nnz == 0) {
+        AMFLS_HIP_CHECK(hipMemsetAsync(t_rp, 0, static_cast<size_t>(A.cols + 1) * sizeof(int64_t), s));
+        return;
+    }
+    int end_bit = 1;
+    while ((int64_t(1) << end_bit) < A.cols) ++end_bit;
+    size_t temp_bytes = 0;
+    AMFLS_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, static_cast<const int32_t*>(nullptr),
+                                                       static_cast<int32_t*>(nullptr),
+                                                       static_cast<const int32_t*>(nullptr),
+                                                       static_cast<int32_t*>(nullptr), static_cast<int>(nnz), 0,
+                                                       end_bit, s));
+    const size_t arr = static_cast<size_t>(nnz) * sizeof(int32_t);
+    const size_t align = 256;
+    const size_t arr_al = (arr + align - 1) / align * align;
+    scratch.ensure(4 * arr_al + temp_bytes + align);
+    char* base = static_cast<char*>(scratch.ptr);
+    int32_t* keys_out = reinterpret_cast<int32_t*>(base);
+    int32_t* vals_in = reinterpret_cast<int32_t*>(base + arr_al);
+    int32_t* vals_out = reinterpret_cast<int32_t*>(base + 2 * arr_al);
+    int32_t* row_of = reinterpret_cast<int32_t*>(base + 3 * arr_al);
+    void* temp = base + 4 * arr_al;
+    hipLaunchKernelGGL(row_of_kernel, dim3(static_cast<unsigned>(ceil_div(A.rows, kThreads))), dim3(kThreads), 0,
+                       s, A.rows, A.row_ptr, row_of);
+    AMFLS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(iota_kernel, dim3(static_cast<unsigned>(ceil_div(nnz, kThreads))), dim3(kThreads), 0, s,
+                       nnz, vals_in);
+    AMFLS_LAUNCH_CHECK();
+    AMFLS_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, A.col_idx, keys_out, vals_in,
+                                                       vals_out, static_cast<int>(nnz), 0, end_bit, s));
+    hipLaunchKernelGGL(offsets_from_sorted_kernel, dim3(static_cast<unsigned>(ceil_div(nnz + 1, kThreads))),
+                       dim3(kThreads), 0, s, nnz, A.cols, keys_out, t_rp);
+    AMFLS_LAUNCH_CHECK();
+    const dim3 g(static_cast<unsigned>(ceil_div(nnz, kThreads)));
+    if (A.type == ValueType::f32)
+        hipLaunchKernelGGL(transpose_gather_kernel<float>, g, dim3(kThreads), 0, s, nnz, vals_out, row_of,
+                           static_cast<const float*>(A.values), t_ci, static_cast<float*>(t_val));
+    else
+        hipLaunchKernelGGL(transpose_gather_kernel<double>, g, dim3(kThreads), 0, s, nnz, vals_out, row_of,
+                           static_cast<const double*>(A.values), t_ci, static_cast<double*>(t_val));
+    AMFLS_LAUNCH_CHECK();
+}
+
+void csr_row_stats(hipStream_t s, const CsrView& A, double* out_sum, double* out_sq, double* misc,
+                   Scratch& scratch) {
+    if (A.rows <= 0) return;
+    scratch.ensure(static_cast<size_t>(2 * A.rows) * sizeof(double));
+    double* partial = static_cast<double*>(scratch.ptr);
+    const dim3 g(static_cast<unsigned>(ceil_div(A.rows, kThreads / 8)));
+    if (A.type == ValueType::f32)
+        hipLaunchKernelGGL(csr_row_stats_kernel<float>, g, dim3(kThreads), 0, s, A.rows, A.row_ptr,
+                           static_cast<const float*>(A.values), out_sum, out_sq, partial);
+    else
+        hipLaunchKernelGGL(csr_row_stats_kernel<double>, g, dim3(kThreads), 0, s, A.rows, A.row_ptr,
+                           static_cast<const double*>(A.values), out_sum, out_sq, partial);
+    AMFLS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(misc_reduce_kernel, dim3(1), dim3(kThreads), 0, s, A.rows, partial, misc);
+    AMFLS_LAUNCH_CHECK();
+}
+
+void csr_validate(hipStream_t s, const CsrView& A, int* out_dev) {
+    fill<int>(s, out_dev, 3, 1);
+    hipLaunchKernelGGL(csr_validate_kernel, dim3(static_cast<unsigned>(ceil_div(A.rows, kThreads))),
+                       dim3(kThreads), 0, s, A.rows, A.cols, A.nnz, A.row_ptr, A.col_idx, out_dev);
+    AMFLS_LAUNCH_CHECK();
+}
+
+template void csr_spmm<float>(hipStream_t, const CsrView&, int, const float*, int64_t, float*, int64_t,
+                              const float*, int64_t, double, int);
+template void csr_spmm<double>(hipStream_t, const CsrView&, int, const double*, int64_t, double*,
+                               int64_t, const double*, int64_t, double, int);
+
+}  // namespace amflsx

@@ -1,0 +1,468 @@
+// extern "C" boundary of the AMFLS HIP engine (declared in include/amfls_hip.h).
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <string>
+
+#include "../../../include/amfls_hip.h"
+#include "solver.hpp"
+
+using namespace amflsx;
+
+struct amfls_context {
+    Context ctx;
+};
+
+struct amfls_operator {
+    std::unique_ptr<DeviceOperator> op;
+};
+
+struct amfls_result {
+    LeastSquaresResult r;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <typename F>
+int guard(F&& f) {
+    try {
+        f();
+        g_last_error.clear();
+        return AMFLS_OK;
+    } catch (const HipError& e) {
+        g_last_error = e.what();
+        return AMFLS_ERR_HIP;
+    } catch (const std::invalid_argument& e) {
+        g_last_error = e.what();
+        return AMFLS_ERR_INVALID_ARGUMENT;
+    } catch (const std::length_error& e) {
+        g_last_error = e.what();
+        return AMFLS_ERR_LENGTH;
+    } catch (const std::overflow_error& e) {
+        g_last_error = e.what();
+        return AMFLS_ERR_LENGTH;
+    } catch (const std::logic_error& e) {
+        g_last_error = e.what();
+        return AMFLS_ERR_LOGIC;
+    } catch (const std::bad_alloc& e) {
+        g_last_error = "host allocation failed";
+        return AMFLS_ERR_RUNTIME;
+    } catch (const std::runtime_error& e) {
+        g_last_error = e.what();
+        return AMFLS_ERR_RUNTIME;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return AMFLS_ERR_UNKNOWN;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return AMFLS_ERR_UNKNOWN;
+    }
+}
+
+void require(bool c, const char* msg) {
+    if (!c) throw std::invalid_argument(msg);
+}
+
+ValueType to_vt(int t) {
+    if (t == AMFLS_F64) return ValueType::f64;
+    if (t == AMFLS_F32) return ValueType::f32;
+    if (t == AMFLS_BF16) return ValueType::bf16;
+    throw std::invalid_argument("unknown value type");
+}
+
+// Device copy of b (owned when b was given on the host).
+struct RhsHolder {
+    DevBuf<double> own;
+    const double* ptr = nullptr;
+    RhsHolder(Context& ctx, const double* b, int64_t m, bool on_device) {
+        require(b != nullptr, "right-hand side cannot be null");
+        if (on_device) {
+            ptr = b;
+        } else {
+            own.reset(static_cast<size_t>(m));
+            AMFLS_HIP_CHECK(hipMemcpyAsync(own.get(), b, m * sizeof(double), hipMemcpyHostToDevice, ctx.stream));
+            ctx.sync();
+            ptr = own.get();
+        }
+    }
+};
+
+template <typename T>
+void transpose_block(hipStream_t s, int64_t rows, int cols, const T* src_colmajor, T* dst_rowmajor) {
+    // column-major (rows x cols, ld = rows) -> row-major (ld = cols):
+    // element (i, j) of the column-major block is the row-major element
+    // (j, i) of a cols x rows matrix, so copy_block over that view.
+    // copy_block reads X[i * ldx + j]; use the transposed strides via a
+    // per-column loop (cols is the block width, small).
+    for (int j = 0; j < cols; ++j)
+        copy_block<T, T>(s, rows, 1, src_colmajor + j * rows, 1, dst_rowmajor + j, cols, 1.0);
+}
+
+template <typename T>
+void untranspose_block(hipStream_t s, int64_t rows, int cols, const T* src_rowmajor, T* dst_colmajor) {
+    for (int j = 0; j < cols; ++j)
+        copy_block<T, T>(s, rows, 1, src_rowmajor + j, cols, dst_colmajor + j * rows, 1, 1.0);
+}
+
+template <typename T>
+void apply_colmajor(Context& ctx, const DeviceOperator& op, bool transpose, const T* in, int s, T* out) {
+    const int64_t in_rows = transpose ? op.rows() : op.cols();
+    const int64_t out_rows = transpose ? op.cols() : op.rows();
+    DevBuf<T> a(static_cast<size_t>(in_rows * s)), b(static_cast<size_t>(out_rows * s));
+    transpose_block<T>(ctx.stream, in_rows, s, in, a.get());
+    if (transpose) op.apply_t(ctx.stream, a.get(), s, s, b.get(), s);
+    else op.apply(ctx.stream, a.get(), s, s, b.get(), s);
+    untranspose_block<T>(ctx.stream, out_rows, s, b.get(), out);
+    ctx.sync();
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* amfls_last_error(void) { return g_last_error.c_str(); }
+int amfls_version(void) { return 10000; }
+
+void amfls_options_default(amfls_options* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->regularization = 0.0;
+    o->tolerance = 1e-8;
+    o->failure_probability = 1e-6;
+    o->precision = AMFLS_PRECISION_FP64;
+    o->clock_mode = AMFLS_CLOCK_WALL;
+    o->synthetic_operator_unit = 1e-3;
+    o->synthetic_local_unit = 1e-6;
+    o->maximum_refinements = 8;
+    o->inner_tolerance = 0.0;
+    o->width_policy = AMFLS_WIDTH_HEURISTIC;
+}
+
+int amfls_context_create(amfls_context** out) {
+    return guard([&] {
+        require(out != nullptr, "output pointer is null");
+        *out = new amfls_context();
+    });
+}
+
+int amfls_context_destroy(amfls_context* ctx) {
+    return guard([&] { delete ctx; });
+}
+
+int amfls_context_synchronize(amfls_context* ctx) {
+    return guard([&] {
+        require(ctx != nullptr, "context is null");
+        ctx->ctx.sync();
+    });
+}
+
+int amfls_csr_create(amfls_context* ctx, int64_t rows, int64_t cols, int64_t nnz, const int64_t* row_ptr,
+                     const int32_t* col_idx, const void* values, int value_type, amfls_operator** out) {
+    return guard([&] {
+        require(ctx != nullptr && out != nullptr, "null context or output");
+        auto h = std::make_unique<amfls_operator>();
+        h->op = std::make_unique<CsrOperator>(rows, cols, nnz, row_ptr, col_idx, values, to_vt(value_type), ctx->ctx);
+        *out = h.release();
+    });
+}
+
+int amfls_dense_create(amfls_context* ctx, int64_t rows, int64_t cols, const void* values, int value_type,
+                       int row_major, int64_t ld, amfls_operator** out) {
+    return guard([&] {
+        require(ctx != nullptr && out != nullptr, "null context or output");
+        auto h = std::make_unique<amfls_operator>();
+        h->op = std::make_unique<DenseOperator>(rows, cols, values, to_vt(value_type), row_major != 0, ld, ctx->ctx);
+        *out = h.release();
+    });
+}
+
+int amfls_operator_destroy(amfls_operator* op) {
+    return guard([&] { delete op; });
+}
+
+int amfls_operator_shape(const amfls_operator* op, int64_t* rows, int64_t* cols) {
+    return guard([&] {
+        require(op && rows && cols, "null argument");
+        *rows = op->op->rows();
+        *cols = op->op->cols();
+    });
+}
+
+int amfls_operator_validation_model(const amfls_operator* op, double* upper, double* lower, int64_t* sa,
+                                    int64_t* st) {
+    return guard([&] {
+        require(op && upper && lower && sa && st, "null argument");
+        const ValidationModel m = op->op->validation_model();
+        *upper = m.upper;
+        *lower = m.lower;
+        *sa = m.steps_apply;
+        *st = m.steps_transpose;
+    });
+}
+
+double amfls_operator_relative_block_product_cost(const amfls_operator* op, int s) {
+    if (!op) return kInfD;
+    return op->op->relative_block_product_cost(s);
+}
+
+int amfls_operator_apply(amfls_context* ctx, const amfls_operator* op, int transpose, const void* in,
+                         int block_cols, void* out, int dtype) {
+    return guard([&] {
+        require(ctx && op && in && out, "null argument");
+        require(block_cols > 0, "block_cols must be positive");
+        if (dtype == AMFLS_F64)
+            apply_colmajor<double>(ctx->ctx, *op->op, transpose != 0, static_cast<const double*>(in), block_cols,
+                                   static_cast<double*>(out));
+        else if (dtype == AMFLS_F32)
+            apply_colmajor<float>(ctx->ctx, *op->op, transpose != 0, static_cast<const float*>(in), block_cols,
+                                  static_cast<float*>(out));
+        else
+            throw std::invalid_argument("block dtype must be float64 or float32");
+    });
+}
+
+int amfls_operator_apply_rows(amfls_context* ctx, const amfls_operator* op, int transpose, const void* in,
+                              int block_cols, int64_t ld_in, void* out, int64_t ld_out, int dtype) {
+    return guard([&] {
+        require(ctx && op && in && out, "null argument");
+        require(block_cols > 0 && ld_in >= block_cols && ld_out >= block_cols, "invalid block shape");
+        hipStream_t s = ctx->ctx.stream;
+        if (dtype == AMFLS_F64) {
+            if (transpose) op->op->apply_t(s, static_cast<const double*>(in), block_cols, ld_in, static_cast<double*>(out), ld_out);
+            else op->op->apply(s, static_cast<const double*>(in), block_cols, ld_in, static_cast<double*>(out), ld_out);
+        } else if (dtype == AMFLS_F32) {
+            if (transpose) op->op->apply_t(s, static_cast<const float*>(in), block_cols, ld_in, static_cast<float*>(out), ld_out);
+            else op->op->apply(s, static_cast<const float*>(in), block_cols, ld_in, static_cast<float*>(out), ld_out);
+        } else {
+            throw std::invalid_argument("block dtype must be float64 or float32");
+        }
+    });
+}
+
+double amfls_operator_product_bytes(const amfls_operator* op, int s, int transpose, int dtype) {
+    if (!op) return 0.0;
+    return op->op->product_bytes(s, transpose != 0, dtype == AMFLS_F64 ? 8 : 4);
+}
+
+double amfls_operator_product_flops(const amfls_operator* op, int s) {
+    if (!op) return 0.0;
+    return op->op->product_flops(s);
+}
+
+int amfls_operator_set_variant(amfls_operator* op, int block_cols, int transpose, int variant) {
+    return guard([&] {
+        require(op != nullptr, "operator is null");
+        require(variant < csr_spmm_variant_count(), "unknown variant");
+        op->op->set_variant(block_cols, transpose != 0, variant);
+    });
+}
+
+int amfls_operator_profile(amfls_operator* op, int enable) {
+    return guard([&] {
+        require(op != nullptr, "operator is null");
+        op->op->set_profiling(enable != 0);
+    });
+}
+
+int amfls_operator_kernel_stats(amfls_operator* op, int64_t* launches, double* seconds, double* bytes) {
+    return guard([&] {
+        require(op && launches && seconds && bytes, "null argument");
+        long long l = 0;
+        op->op->kernel_stats(&l, seconds, bytes);
+        *launches = l;
+    });
+}
+
+int amfls_spmm_variant_count(void) { return csr_spmm_variant_count(); }
+const char* amfls_spmm_variant_name(int v) { return csr_spmm_variant_name(v); }
+int amfls_spmm_variant_supports(int v, int s) { return csr_spmm_variant_supports(v, s) ? 1 : 0; }
+
+int amfls_solve(amfls_context* ctx, const amfls_operator* op, const double* b, int b_on_device,
+                const amfls_options* options, amfls_result** out) {
+    return guard([&] {
+        require(ctx && op && options && out, "null argument");
+        SolveOptions o;
+        o.regularization = options->regularization;
+        o.tolerance = options->tolerance;
+        o.failure_probability = options->failure_probability;
+        o.maximum_epochs = options->maximum_epochs;
+        o.maximum_basis_size = options->maximum_basis_size;
+        o.seed = options->seed;
+        o.stream = options->stream;
+        o.maximum_depth = options->maximum_depth;
+        o.maximum_auxiliary_width = options->maximum_auxiliary_width;
+        require(options->precision >= AMFLS_PRECISION_FP64 && options->precision <= AMFLS_PRECISION_FP32,
+                "unknown precision");
+        o.precision = static_cast<Precision>(options->precision);
+        o.clock_mode = options->clock_mode == AMFLS_CLOCK_SYNTHETIC ? ClockMode::synthetic : ClockMode::wall;
+        o.synthetic_operator_unit = options->synthetic_operator_unit;
+        o.synthetic_local_unit = options->synthetic_local_unit;
+        o.maximum_refinements = options->maximum_refinements;
+        o.inner_tolerance = options->inner_tolerance;
+        o.width_policy = options->width_policy;
+        RhsHolder rhs(ctx->ctx, b, op->op->rows(), b_on_device != 0);
+        auto res = std::make_unique<amfls_result>();
+        res->r = solve_amfls(*op->op, rhs.ptr, o, ctx->ctx, nullptr);
+        *out = res.release();
+    });
+}
+
+int amfls_solve_fixed_rbgk(amfls_context* ctx, const amfls_operator* op, const double* b, int b_on_device,
+                           double regularization, double tolerance, double failure_probability,
+                           int32_t auxiliary_width, int32_t maximum_depth, int32_t maximum_basis_size,
+                           uint64_t seed, uint64_t stream, int32_t clock_mode, amfls_result** out) {
+    return guard([&] {
+        require(ctx && op && out, "null argument");
+        FixedRbgkOptions o;
+        o.regularization = regularization;
+        o.tolerance = tolerance;
+        o.failure_probability = failure_probability;
+        o.auxiliary_width = auxiliary_width;
+        o.maximum_depth = maximum_depth;
+        o.maximum_basis_size = maximum_basis_size;
+        o.seed = seed;
+        o.stream = stream;
+        o.clock_mode = clock_mode == AMFLS_CLOCK_SYNTHETIC ? ClockMode::synthetic : ClockMode::wall;
+        RhsHolder rhs(ctx->ctx, b, op->op->rows(), b_on_device != 0);
+        auto res = std::make_unique<amfls_result>();
+        res->r = solve_fixed_rbgk(*op->op, rhs.ptr, o, ctx->ctx);
+        *out = res.release();
+    });
+}
+
+int amfls_solve_lsqr(amfls_context* ctx, const amfls_operator* op, const double* b, int b_on_device,
+                     double regularization, double tolerance, int32_t maximum_iterations, int32_t clock_mode,
+                     amfls_result** out) {
+    return guard([&] {
+        require(ctx && op && out, "null argument");
+        LsqrOptions o;
+        o.regularization = regularization;
+        o.tolerance = tolerance;
+        o.maximum_iterations = maximum_iterations;
+        o.clock_mode = clock_mode == AMFLS_CLOCK_SYNTHETIC ? ClockMode::synthetic : ClockMode::wall;
+        RhsHolder rhs(ctx->ctx, b, op->op->rows(), b_on_device != 0);
+        auto res = std::make_unique<amfls_result>();
+        res->r = solve_lsqr(*op->op, rhs.ptr, o, ctx->ctx);
+        *out = res.release();
+    });
+}
+
+int amfls_result_sizes(const amfls_result* r, int64_t* cols, int64_t* rows) {
+    return guard([&] {
+        require(r && cols && rows, "null argument");
+        *cols = static_cast<int64_t>(r->r.solution.size());
+        *rows = static_cast<int64_t>(r->r.trace.size());
+    });
+}
+
+int amfls_result_copy(const amfls_result* res, double* solution, double* metrics, int64_t* counters,
+                      double* timings, double* trace_float, int64_t* trace_int, int64_t* status,
+                      int64_t* stop_reason) {
+    return guard([&] {
+        require(res != nullptr, "result is null");
+        const LeastSquaresResult& r = res->r;
+        const RunStatistics& s = r.statistics;
+        if (solution) std::memcpy(solution, r.solution.data(), r.solution.size() * sizeof(double));
+        if (metrics) {
+            const double m[AMFLS_METRICS_WIDTH] = {
+                r.objective, r.residual_norm, r.augmented_residual_norm, r.normal_residual_norm, r.solution_norm,
+                r.solution_energy_norm, r.operator_norm_lower_bound, r.augmented_operator_norm_lower_bound,
+                r.compatible_backward_error_upper_bound, r.least_squares_backward_error_upper_bound,
+                r.backward_error_upper_bound, r.relative_normal_residual_upper_bound, r.energy_error_upper_bound,
+                r.relative_energy_error_upper_bound, r.objective_gap_upper_bound,
+                r.ridge_base_energy_error_upper_bound, r.ridge_corrected_energy_error_upper_bound,
+                r.ridge_correction_gamma, r.ridge_correction_z_h_z, r.ridge_correction_two_abs_z_t_q,
+                r.ridge_correction_q_norm_squared};
+            std::memcpy(metrics, m, sizeof(m));
+        }
+        if (counters) {
+            const int64_t c[AMFLS_COUNTERS_WIDTH] = {
+                s.a_columns, s.at_columns, s.a_block_calls, s.at_block_calls, s.search_a_columns,
+                s.search_at_columns, s.validation_a_columns, s.validation_at_columns, r.iterations, r.depth,
+                r.auxiliary_width, r.basis_rank, s.sketch_a_columns, s.sketch_at_columns, s.sketch_a_block_calls,
+                s.sketch_at_block_calls, s.iterative_a_columns, s.iterative_at_columns, s.iterative_a_block_calls,
+                s.iterative_at_block_calls, s.gaussian_random_columns, s.gaussian_random_values,
+                s.search_a_block_calls, s.search_at_block_calls, s.validation_a_block_calls,
+                s.validation_at_block_calls, s.base_validation_a_columns, s.base_validation_at_columns,
+                s.base_validation_a_block_calls, s.base_validation_at_block_calls, s.ridge_correction_a_columns,
+                s.ridge_correction_at_columns, s.ridge_correction_a_block_calls, s.ridge_correction_at_block_calls,
+                s.gaussian_random_block_requests, static_cast<int64_t>(r.ridge_correction_disposition)};
+            std::memcpy(counters, c, sizeof(c));
+        }
+        if (timings) {
+            const double other = std::max(0.0, s.total_seconds - s.a_seconds - s.at_seconds -
+                                                   s.orthogonalization_seconds - s.projected_solve_seconds);
+            const double t[AMFLS_TIMINGS_WIDTH] = {s.total_seconds, s.a_seconds, s.at_seconds,
+                                                   s.orthogonalization_seconds, s.projected_solve_seconds,
+                                                   s.validation_seconds, other, s.base_validation_seconds,
+                                                   s.ridge_correction_validation_seconds};
+            std::memcpy(timings, t, sizeof(t));
+        }
+        for (size_t i = 0; i < r.trace.size(); ++i) {
+            const IterationRecord& q = r.trace[i];
+            if (trace_float) {
+                const double f[AMFLS_TRACE_FLOAT_WIDTH] = {
+                    static_cast<double>(q.epoch), static_cast<double>(q.depth), static_cast<double>(q.auxiliary_width),
+                    static_cast<double>(q.basis_rank), static_cast<double>(q.a_columns),
+                    static_cast<double>(q.at_columns), static_cast<double>(q.a_block_calls),
+                    static_cast<double>(q.at_block_calls), q.objective, q.residual_norm, q.augmented_residual_norm,
+                    q.normal_residual_norm, q.operator_norm_lower_bound, q.compatible_backward_error_upper_bound,
+                    q.least_squares_backward_error_upper_bound, q.backward_error_upper_bound,
+                    q.relative_normal_residual_upper_bound, q.relative_energy_error_upper_bound, q.solution_norm,
+                    q.solution_energy_norm, q.augmented_operator_norm_lower_bound, q.energy_error_upper_bound,
+                    q.objective_gap_upper_bound, q.ridge_base_energy_error_upper_bound,
+                    q.ridge_corrected_energy_error_upper_bound, q.ridge_correction_gamma, q.ridge_correction_z_h_z,
+                    q.ridge_correction_two_abs_z_t_q, q.ridge_correction_q_norm_squared};
+                std::memcpy(trace_float + i * AMFLS_TRACE_FLOAT_WIDTH, f, sizeof(f));
+            }
+            if (trace_int) {
+                const int64_t t[AMFLS_TRACE_INT_WIDTH] = {
+                    static_cast<int64_t>(q.contract_status), static_cast<int64_t>(q.contract_stop_reason),
+                    q.contract_passed ? 1 : 0, static_cast<int64_t>(q.ridge_correction_disposition)};
+                std::memcpy(trace_int + i * AMFLS_TRACE_INT_WIDTH, t, sizeof(t));
+            }
+        }
+        if (status) *status = static_cast<int64_t>(r.status);
+        if (stop_reason) *stop_reason = static_cast<int64_t>(r.stop_reason);
+    });
+}
+
+int amfls_result_trace_extension(const amfls_result* res, double* out) {
+    return guard([&] {
+        require(res && out, "null argument");
+        for (size_t i = 0; i < res->r.trace.size(); ++i) {
+            out[3 * i] = res->r.trace[i].interval_operator_seconds;
+            out[3 * i + 1] = res->r.trace[i].interval_local_seconds;
+            out[3 * i + 2] = res->r.trace[i].active_width;
+        }
+    });
+}
+
+int amfls_result_refinement_steps(const amfls_result* res, int32_t* steps) {
+    return guard([&] {
+        require(res && steps, "null argument");
+        *steps = res->r.refinement_steps;
+    });
+}
+
+int amfls_result_destroy(amfls_result* res) {
+    return guard([&] { delete res; });
+}
+
+int amfls_gaussian_fill(amfls_context* ctx, void* y, int64_t rows, int32_t cols, int64_t ld, int dtype,
+                        uint64_t seed, uint64_t stream, uint64_t first_col) {
+    return guard([&] {
+        require(ctx && y, "null argument");
+        require(rows >= 0 && cols >= 0 && ld >= cols, "invalid block shape");
+        if (dtype == AMFLS_F64) gaussian_block<double>(ctx->ctx.stream, rows, cols, seed, stream, first_col, static_cast<double*>(y), ld);
+        else if (dtype == AMFLS_F32) gaussian_block<float>(ctx->ctx.stream, rows, cols, seed, stream, first_col, static_cast<float*>(y), ld);
+        else throw std::invalid_argument("dtype must be float64 or float32");
+        ctx->ctx.sync();
+    });
+}
+
+}  // extern "C"

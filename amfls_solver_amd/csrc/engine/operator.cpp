@@ -1,0 +1,417 @@
+// Device operators: construction scans, products and validation models.
+#include "operator.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+
+#include "fp_bounds.hpp"
+
+namespace amflsx {
+
+ValidationModel norm_model_from_sums(const std::vector<double>& row_sum, const std::vector<double>& row_sq,
+                                     const std::vector<long long>& row_ops,
+                                     const std::vector<double>& col_sum, const std::vector<double>& col_sq,
+                                     const std::vector<long long>& col_ops, double max_entry,
+                                     long long steps_apply, long long steps_transpose) {
+    // Same bound chain as the reference constructors
+    // (sparse_csr_operator.cpp:90-129): max row / column |a| sums bound
+    // ||A||_inf and ||A||_1, the column square sums bound ||A||_F, and row /
+    // column two-norms bound ||A||_2 from below.
+    double max_row_upper = 0.0, max_col_upper = 0.0, two_norm_lower = 0.0, fro_sq_upper = 0.0;
+    for (size_t i = 0; i < row_sum.size(); ++i) {
+        max_row_upper = std::max(max_row_upper, fp::pos_sum_upper(row_sum[i], row_ops[i]));
+        two_norm_lower = std::max(two_norm_lower, fp::down_sqrt(fp::pos_sum_lower(row_sq[i], row_ops[i])));
+    }
+    for (size_t j = 0; j < col_sum.size(); ++j) {
+        max_col_upper = std::max(max_col_upper, fp::pos_sum_upper(col_sum[j], col_ops[j]));
+        two_norm_lower = std::max(two_norm_lower, fp::down_sqrt(fp::pos_sum_lower(col_sq[j], col_ops[j])));
+        fro_sq_upper = fp::up_add(fro_sq_upper, fp::pos_fma_square_sum_upper(col_sq[j], col_ops[j]));
+    }
+    ValidationModel m;
+    m.steps_apply = steps_apply;
+    m.steps_transpose = steps_transpose;
+    if (fp::environment_ok()) {
+        const double one_inf = fp::up_mul(fp::up_sqrt(max_col_upper), fp::up_sqrt(max_row_upper));
+        m.upper = std::min(one_inf, fp::up_sqrt(fro_sq_upper));
+        m.lower = std::max(max_entry, two_norm_lower);
+    }
+    if (!std::isfinite(m.upper) || m.lower > m.upper) {
+        m.upper = kInfD;
+        m.lower = 0.0;
+    }
+    return m;
+}
+
+// ------------------------------------------------------------------ profiling
+void DeviceOperator::set_profiling(bool on) {
+    profiling_ = on;
+    if (!on) resolve_pending();
+}
+
+void DeviceOperator::prof_begin(hipStream_t s) const {
+    if (!profiling_) return;
+    hipEvent_t e;
+    if (pool_.empty()) AMFLS_HIP_CHECK(hipEventCreate(&e));
+    else { e = pool_.back(); pool_.pop_back(); }
+    AMFLS_HIP_CHECK(hipEventRecord(e, s));
+    open_start_ = e;
+}
+
+void DeviceOperator::prof_end(hipStream_t s, int ncols, bool transpose, int eb) const {
+    if (!profiling_ || open_start_ == nullptr) return;
+    hipEvent_t e;
+    if (pool_.empty()) AMFLS_HIP_CHECK(hipEventCreate(&e));
+    else { e = pool_.back(); pool_.pop_back(); }
+    AMFLS_HIP_CHECK(hipEventRecord(e, s));
+    pending_.push_back({open_start_, e, product_bytes(ncols, transpose, eb)});
+    open_start_ = nullptr;
+}
+
+void DeviceOperator::resolve_pending() {
+    for (ProfPair& p : pending_) {
+        AMFLS_HIP_CHECK(hipEventSynchronize(p.stop));
+        float ms = 0.0f;
+        AMFLS_HIP_CHECK(hipEventElapsedTime(&ms, p.start, p.stop));
+        ++launches_;
+        seconds_ += 1e-3 * ms;
+        bytes_ += p.bytes;
+        pool_.push_back(p.start);
+        pool_.push_back(p.stop);
+    }
+    pending_.clear();
+}
+
+void DeviceOperator::kernel_stats(long long* launches, double* seconds, double* bytes) {
+    resolve_pending();
+    *launches = launches_;
+    *seconds = seconds_;
+    *bytes = bytes_;
+    launches_ = 0;
+    seconds_ = bytes_ = 0.0;
+}
+
+// ------------------------------------------------------------------ CSR
+CsrOperator::CsrOperator(int64_t rows, int64_t cols, int64_t nnz, const int64_t* row_ptr,
+                         const int32_t* col_idx, const void* values, ValueType type, Context& ctx) {
+    if (rows <= 0 || cols <= 0 || nnz < 0 || row_ptr == nullptr)
+        throw std::invalid_argument("CSR operator dimensions are invalid");
+    if (nnz > 0 && (col_idx == nullptr || values == nullptr))
+        throw std::invalid_argument("CSR operator arrays are missing");
+    if (type != ValueType::f32 && type != ValueType::f64)
+        throw std::invalid_argument("CSR values must be float32 or float64");
+    if (cols > (int64_t(1) << 31) - 1) throw std::invalid_argument("CSR column count exceeds int32");
+    rows_ = rows;
+    cols_ = cols;
+    nnz_ = nnz;
+    a_ = CsrView{rows, cols, nnz, row_ptr, col_idx, values, type};
+    hipStream_t s = ctx.stream;
+
+    DevBuf<int> flags(3);
+    csr_validate(s, a_, flags.get());
+    int hflags[3] = {0, 0, 0};
+    ctx.download(hflags, flags.get(), 3);
+    if (!hflags[0]) throw std::invalid_argument("CSR row offsets do not span the values or decrease");
+    if (!hflags[1]) throw std::invalid_argument("CSR column index is out of range");
+    if (!hflags[2]) throw std::invalid_argument("CSR rows must not contain duplicate columns");
+
+    t_rp_.reset(static_cast<size_t>(cols + 1));
+    t_ci_.reset(static_cast<size_t>(std::max<int64_t>(nnz, 1)));
+    t_val_.reset(static_cast<size_t>(std::max<int64_t>(nnz, 1)) * value_size(type));
+    csr_transpose(s, a_, t_rp_.get(), t_ci_.get(), t_val_.get(), ctx.scratch2);
+    at_ = CsrView{cols, rows, nnz, t_rp_.get(), t_ci_.get(), t_val_.get(), type};
+
+    DevBuf<double> rsum(rows), rsq(rows), csum(cols), csq(cols), misc(4);
+    csr_row_stats(s, a_, rsum.get(), rsq.get(), misc.get(), ctx.scratch);
+    csr_row_stats(s, at_, csum.get(), csq.get(), misc.get() + 2, ctx.scratch);
+    std::vector<double> hrs(rows), hrq(rows), hcs(cols), hcq(cols), hm(4);
+    std::vector<int64_t> hrp(rows + 1), htrp(cols + 1);
+    ctx.download(hrs.data(), rsum.get(), rows);
+    ctx.download(hrq.data(), rsq.get(), rows);
+    ctx.download(hcs.data(), csum.get(), cols);
+    ctx.download(hcq.data(), csq.get(), cols);
+    ctx.download(hm.data(), misc.get(), 4);
+    ctx.download(hrp.data(), row_ptr, rows + 1);
+    ctx.download(htrp.data(), t_rp_.get(), cols + 1);
+    if (hm[1] > 0.0) throw std::invalid_argument("CSR values must be finite");
+    std::vector<long long> rops(rows), cops(cols);
+    long long max_row = 0, max_col = 0;
+    for (int64_t i = 0; i < rows; ++i) {
+        rops[i] = hrp[i + 1] - hrp[i];
+        max_row = std::max(max_row, rops[i]);
+    }
+    for (int64_t j = 0; j < cols; ++j) {
+        cops[j] = htrp[j + 1] - htrp[j];
+        max_col = std::max(max_col, cops[j]);
+    }
+    model_ = norm_model_from_sums(hrs, hrq, rops, hcs, hcq, cops, hm[0], 2 * max_row + 2, 2 * max_col + 2);
+}
+
+int CsrOperator::variant_for(int s, bool transpose) const {
+    auto it = variants_.find({s, transpose});
+    return it == variants_.end() ? -1 : it->second;
+}
+
+void CsrOperator::set_variant(int s, bool transpose, int variant) { variants_[{s, transpose}] = variant; }
+
+void CsrOperator::apply(hipStream_t s, const double* X, int n, int64_t ldx, double* Y, int64_t ldy) const {
+    prof_begin(s);
+    csr_spmm<double>(s, a_, n, X, ldx, Y, ldy, nullptr, 0, 0.0, variant_for(n, false));
+    prof_end(s, n, false, 8);
+}
+void CsrOperator::apply_t(hipStream_t s, const double* Y, int n, int64_t ldy, double* X, int64_t ldx) const {
+    prof_begin(s);
+    csr_spmm<double>(s, at_, n, Y, ldy, X, ldx, nullptr, 0, 0.0, variant_for(n, true));
+    prof_end(s, n, true, 8);
+}
+void CsrOperator::apply(hipStream_t s, const float* X, int n, int64_t ldx, float* Y, int64_t ldy) const {
+    prof_begin(s);
+    csr_spmm<float>(s, a_, n, X, ldx, Y, ldy, nullptr, 0, 0.0, variant_for(n, false));
+    prof_end(s, n, false, 4);
+}
+void CsrOperator::apply_t(hipStream_t s, const float* Y, int n, int64_t ldy, float* X, int64_t ldx) const {
+    prof_begin(s);
+    csr_spmm<float>(s, at_, n, Y, ldy, X, ldx, nullptr, 0, 0.0, variant_for(n, true));
+    prof_end(s, n, true, 4);
+}
+
+template <typename T>
+void CsrOperator::apply_t_add(hipStream_t s, const T* Y, int n, int64_t ldy, T* X, int64_t ldx, const T* add,
+                              int64_t ld_add, double a) const {
+    csr_spmm<T>(s, at_, n, Y, ldy, X, ldx, add, ld_add, a, variant_for(n, true));
+}
+template void CsrOperator::apply_t_add<double>(hipStream_t, const double*, int, int64_t, double*, int64_t,
+                                               const double*, int64_t, double) const;
+template void CsrOperator::apply_t_add<float>(hipStream_t, const float*, int, int64_t, float*, int64_t,
+                                              const float*, int64_t, double) const;
+
+double CsrOperator::product_bytes(int s, bool transpose, int eb) const {
+    // values + column indices + row pointers + gathered input rows + output
+    const double vb = static_cast<double>(value_size(a_.type));
+    const double out_rows = transpose ? static_cast<double>(cols_) : static_cast<double>(rows_);
+    const double in_rows = transpose ? static_cast<double>(rows_) : static_cast<double>(cols_);
+    return (vb + 4.0) * static_cast<double>(nnz_) + 8.0 * (out_rows + 1.0) +
+           static_cast<double>(eb) * s * (in_rows + out_rows);
+}
+
+void CsrOperator::ensure_host_copy() const {
+    std::call_once(host_once_, [this] {
+        h_rp_.resize(rows_ + 1);
+        h_ci_.resize(nnz_);
+        h_val_.resize(nnz_);
+        AMFLS_HIP_CHECK(hipMemcpy(h_rp_.data(), a_.row_ptr, (rows_ + 1) * sizeof(int64_t), hipMemcpyDeviceToHost));
+        if (nnz_ > 0) {
+            AMFLS_HIP_CHECK(hipMemcpy(h_ci_.data(), a_.col_idx, nnz_ * sizeof(int32_t), hipMemcpyDeviceToHost));
+            if (a_.type == ValueType::f64) {
+                AMFLS_HIP_CHECK(hipMemcpy(h_val_.data(), a_.values, nnz_ * sizeof(double), hipMemcpyDeviceToHost));
+            } else {
+                std::vector<float> tmp(nnz_);
+                AMFLS_HIP_CHECK(hipMemcpy(tmp.data(), a_.values, nnz_ * sizeof(float), hipMemcpyDeviceToHost));
+                for (int64_t e = 0; e < nnz_; ++e) h_val_[e] = tmp[e];
+            }
+        }
+    });
+}
+
+bool CsrOperator::refine_apply(const long double* x, long double* y) const {
+    ensure_host_copy();
+    for (int64_t r = 0; r < rows_; ++r) {
+        long double sum = 0.0L;
+        for (int64_t e = h_rp_[r]; e < h_rp_[r + 1]; ++e)
+            sum += static_cast<long double>(h_val_[e]) * x[h_ci_[e]];
+        if (!std::isfinite(sum)) return false;
+        y[r] = sum;
+    }
+    return true;
+}
+
+bool CsrOperator::refine_apply_t(const long double* y, long double* x) const {
+    ensure_host_copy();
+    std::fill(x, x + cols_, 0.0L);
+    for (int64_t r = 0; r < rows_; ++r) {
+        const long double in = y[r];
+        if (!std::isfinite(in)) return false;
+        for (int64_t e = h_rp_[r]; e < h_rp_[r + 1]; ++e) x[h_ci_[e]] += static_cast<long double>(h_val_[e]) * in;
+    }
+    for (int64_t j = 0; j < cols_; ++j)
+        if (!std::isfinite(x[j])) return false;
+    return true;
+}
+
+// ------------------------------------------------------------------ dense
+DenseOperator::DenseOperator(int64_t rows, int64_t cols, const void* values, ValueType type, bool row_major,
+                             int64_t ld, Context& ctx)
+    : ctx_(ctx) {
+    if (rows <= 0 || cols <= 0 || values == nullptr) throw std::invalid_argument("dense operator input is invalid");
+    if (row_major ? ld < cols : ld < rows) throw std::invalid_argument("dense leading dimension is too small");
+    rows_ = rows;
+    cols_ = cols;
+    a_.ptr = values;
+    a_.type = type;
+    a_.rs = row_major ? ld : 1;
+    a_.cs = row_major ? 1 : ld;
+    DevBuf<double> rsum(rows), rsq(rows), csum(cols), csq(cols), misc(2);
+    dense_stats(ctx.stream, rows, cols, a_, rsum.get(), rsq.get(), csum.get(), csq.get(), misc.get(), ctx.scratch);
+    std::vector<double> hrs(rows), hrq(rows), hcs(cols), hcq(cols), hm(2);
+    ctx.download(hrs.data(), rsum.get(), rows);
+    ctx.download(hrq.data(), rsq.get(), rows);
+    ctx.download(hcs.data(), csum.get(), cols);
+    ctx.download(hcq.data(), csq.get(), cols);
+    ctx.download(hm.data(), misc.get(), 2);
+    std::vector<long long> rops(rows, cols), cops(cols, rows);
+    model_ = norm_model_from_sums(hrs, hrq, rops, hcs, hcq, cops, hm[0], 2 * cols + 2, 2 * rows + 2);
+    if (hm[1] > 0.0) {  // reference: a nonfinite entry disables certification
+        model_.upper = kInfD;
+        model_.lower = 0.0;
+    }
+}
+
+template <typename T>
+void DenseOperator::do_apply(hipStream_t s, const T* X, int n, int64_t ldx, T* Y, int64_t ldy) const {
+    prof_begin(s);
+    gemm_nn<T, T>(s, rows_, n, cols_, a_, X, ldx, Y, ldy, 1.0, 0.0);
+    prof_end(s, n, false, static_cast<int>(sizeof(T)));
+}
+
+template <typename T>
+void DenseOperator::do_apply_t(hipStream_t s, const T* Y, int n, int64_t ldy, T* X, int64_t ldx) const {
+    const ValueType ct = sizeof(T) == 8 ? ValueType::f64 : ValueType::f32;
+    prof_begin(s);
+    gemm_tn<T>(s, rows_, cols_, n, a_, Y, ldy, X, ct, ldx, 1.0, ctx_.scratch);
+    prof_end(s, n, true, static_cast<int>(sizeof(T)));
+}
+
+void DenseOperator::apply(hipStream_t s, const double* X, int n, int64_t ldx, double* Y, int64_t ldy) const {
+    do_apply<double>(s, X, n, ldx, Y, ldy);
+}
+void DenseOperator::apply_t(hipStream_t s, const double* Y, int n, int64_t ldy, double* X, int64_t ldx) const {
+    do_apply_t<double>(s, Y, n, ldy, X, ldx);
+}
+void DenseOperator::apply(hipStream_t s, const float* X, int n, int64_t ldx, float* Y, int64_t ldy) const {
+    do_apply<float>(s, X, n, ldx, Y, ldy);
+}
+void DenseOperator::apply_t(hipStream_t s, const float* Y, int n, int64_t ldy, float* X, int64_t ldx) const {
+    do_apply_t<float>(s, Y, n, ldy, X, ldx);
+}
+
+double DenseOperator::product_bytes(int s, bool transpose, int eb) const {
+    (void)transpose;
+    return static_cast<double>(value_size(a_.type)) * static_cast<double>(rows_) * static_cast<double>(cols_) +
+           static_cast<double>(eb) * s * static_cast<double>(rows_ + cols_);
+}
+
+bool DenseOperator::supports_refinement() const {
+    // A host long-double pass is affordable up to ~64M entries.
+    return static_cast<double>(rows_) * static_cast<double>(cols_) <= 6.4e7;
+}
+
+void DenseOperator::ensure_host_copy() const {
+    std::call_once(host_once_, [this] {
+        h_val_.assign(static_cast<size_t>(rows_ * cols_), 0.0);
+        const size_t es = value_size(a_.type);
+        std::vector<char> raw;
+        // copy the strided matrix row by row (row-major) or column by column
+        if (a_.cs == 1) {
+            raw.resize(static_cast<size_t>(cols_) * es);
+            for (int64_t i = 0; i < rows_; ++i) {
+                AMFLS_HIP_CHECK(hipMemcpy(raw.data(), static_cast<const char*>(a_.ptr) + i * a_.rs * es,
+                                          cols_ * es, hipMemcpyDeviceToHost));
+                for (int64_t j = 0; j < cols_; ++j) h_val_[i * cols_ + j] = 0.0;
+                for (int64_t j = 0; j < cols_; ++j) {
+                    double v = 0.0;
+                    if (a_.type == ValueType::f64) std::memcpy(&v, raw.data() + j * es, 8);
+                    else if (a_.type == ValueType::f32) { float f; std::memcpy(&f, raw.data() + j * es, 4); v = f; }
+                    else { uint16_t b; std::memcpy(&b, raw.data() + j * es, 2); uint32_t u = uint32_t(b) << 16; float f; std::memcpy(&f, &u, 4); v = f; }
+                    h_val_[i * cols_ + j] = v;
+                }
+            }
+        } else {
+            raw.resize(static_cast<size_t>(rows_) * es);
+            for (int64_t j = 0; j < cols_; ++j) {
+                AMFLS_HIP_CHECK(hipMemcpy(raw.data(), static_cast<const char*>(a_.ptr) + j * a_.cs * es,
+                                          rows_ * es, hipMemcpyDeviceToHost));
+                for (int64_t i = 0; i < rows_; ++i) {
+                    double v = 0.0;
+                    if (a_.type == ValueType::f64) std::memcpy(&v, raw.data() + i * es, 8);
+                    else if (a_.type == ValueType::f32) { float f; std::memcpy(&f, raw.data() + i * es, 4); v = f; }
+                    else { uint16_t b; std::memcpy(&b, raw.data() + i * es, 2); uint32_t u = uint32_t(b) << 16; float f; std::memcpy(&f, &u, 4); v = f; }
+                    h_val_[i * cols_ + j] = v;
+                }
+            }
+        }
+    });
+}
+
+bool DenseOperator::refine_apply(const long double* x, long double* y) const {
+    ensure_host_copy();
+    for (int64_t i = 0; i < rows_; ++i) {
+        long double sum = 0.0L;
+        const double* row = h_val_.data() + i * cols_;
+        for (int64_t j = 0; j < cols_; ++j) sum += static_cast<long double>(row[j]) * x[j];
+        if (!std::isfinite(sum)) return false;
+        y[i] = sum;
+    }
+    return true;
+}
+
+bool DenseOperator::refine_apply_t(const long double* y, long double* x) const {
+    ensure_host_copy();
+    std::fill(x, x + cols_, 0.0L);
+    for (int64_t i = 0; i < rows_; ++i) {
+        const long double in = y[i];
+        if (!std::isfinite(in)) return false;
+        const double* row = h_val_.data() + i * cols_;
+        for (int64_t j = 0; j < cols_; ++j) x[j] += static_cast<long double>(row[j]) * in;
+    }
+    for (int64_t j = 0; j < cols_; ++j)
+        if (!std::isfinite(x[j])) return false;
+    return true;
+}
+
+// ------------------------------------------------------------------ augmented
+AugmentedOperator::AugmentedOperator(const DeviceOperator& base, double a, Context& ctx)
+    : base_(base), a_(a), ctx_(ctx) {
+    rows_ = base.rows() + base.cols();
+    cols_ = base.cols();
+    const ValidationModel b = base.validation_model();
+    model_.upper = fp::up_sqrt(fp::up_add(fp::up_mul(b.upper, b.upper), fp::up_mul(a, a)));
+    model_.lower = std::max(b.lower, fp::down_mul(a, 1.0));
+    model_.steps_apply = std::max<long long>(b.steps_apply, 2);
+    model_.steps_transpose = b.steps_transpose + 2;
+    if (!std::isfinite(model_.upper)) {
+        model_.upper = kInfD;
+        model_.lower = 0.0;
+    }
+}
+
+template <typename T>
+void AugmentedOperator::do_apply(hipStream_t s, const T* X, int n, int64_t ldx, T* Y, int64_t ldy) const {
+    base_.apply(s, X, n, ldx, Y, ldy);
+    copy_block<T, T>(s, base_.cols(), n, X, ldx, Y + base_.rows() * ldy, ldy, a_);
+}
+
+template <typename T>
+void AugmentedOperator::do_apply_t(hipStream_t s, const T* Y, int n, int64_t ldy, T* X, int64_t ldx) const {
+    const T* tail = Y + base_.rows() * ldy;
+    if (auto* csr = dynamic_cast<const CsrOperator*>(&base_)) {
+        csr->apply_t_add<T>(s, Y, n, ldy, X, ldx, tail, ldy, a_);
+        return;
+    }
+    base_.apply_t(s, Y, n, ldy, X, ldx);
+    axpy_block<T>(s, base_.cols(), n, a_, tail, ldy, X, ldx);
+}
+
+void AugmentedOperator::apply(hipStream_t s, const double* X, int n, int64_t ldx, double* Y, int64_t ldy) const {
+    do_apply<double>(s, X, n, ldx, Y, ldy);
+}
+void AugmentedOperator::apply_t(hipStream_t s, const double* Y, int n, int64_t ldy, double* X, int64_t ldx) const {
+    do_apply_t<double>(s, Y, n, ldy, X, ldx);
+}
+void AugmentedOperator::apply(hipStream_t s, const float* X, int n, int64_t ldx, float* Y, int64_t ldy) const {
+    do_apply<float>(s, X, n, ldx, Y, ldy);
+}
+void AugmentedOperator::apply_t(hipStream_t s, const float* Y, int n, int64_t ldy, float* X, int64_t ldx) const {
+    do_apply_t<float>(s, Y, n, ldy, X, ldx);
+}
+
+}  // namespace amflsx

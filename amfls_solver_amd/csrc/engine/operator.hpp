@@ -1,0 +1,198 @@
+// Device operators of the AMFLS HIP engine.
+//
+// Counterpart of the reference's MatrixOperator contract
+// (include/amfls/matrix_operator.hpp:29-89) for matrices that live in HBM:
+// Y = A X and X = A^T Y on row-major device blocks, the relative block cost
+// used by the width controller, and the normwise validation model
+// {||A|| upper, ||A|| lower, apply steps, transpose steps} computed from the
+// stored values exactly as the reference's constructors do
+// (sparse_csr_operator.cpp:14-149, dense_operator.cpp:20-115,
+// row_major_dense_operator.cpp:23-101).
+#pragma once
+
+#include <map>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "context.hpp"
+
+namespace amflsx {
+
+struct ValidationModel {
+    double upper = kInfD;
+    double lower = 0.0;
+    long long steps_apply = 0;
+    long long steps_transpose = 0;
+};
+
+class DeviceOperator {
+public:
+    virtual ~DeviceOperator() = default;
+    int64_t rows() const { return rows_; }
+    int64_t cols() const { return cols_; }
+
+    virtual void apply(hipStream_t s, const double* X, int ncols, int64_t ldx, double* Y, int64_t ldy) const = 0;
+    virtual void apply_t(hipStream_t s, const double* Y, int ncols, int64_t ldy, double* X, int64_t ldx) const = 0;
+    virtual void apply(hipStream_t s, const float* X, int ncols, int64_t ldx, float* Y, int64_t ldy) const = 0;
+    virtual void apply_t(hipStream_t s, const float* Y, int ncols, int64_t ldy, float* X, int64_t ldx) const = 0;
+
+    virtual double relative_block_product_cost(int s) const {
+        return s > 0 ? static_cast<double>(s) : kInfD;
+    }
+    virtual ValidationModel validation_model() const { return {}; }
+
+    // Long-double validation refinement on the host (reference
+    // matrix_operator.hpp:62-88).  Supported where a host copy is affordable.
+    virtual bool supports_refinement() const { return false; }
+    virtual bool refine_apply(const long double*, long double*) const { return false; }
+    virtual bool refine_apply_t(const long double*, long double*) const { return false; }
+
+    // Algorithmic HBM bytes of one product of width s in the given working
+    // precision (4 or 8 bytes per block entry).
+    virtual double product_bytes(int s, bool transpose, int block_elem_bytes) const = 0;
+    virtual double product_flops(int s) const = 0;
+
+    // Selectable kernel variants (CSR SpMM); -1 = default.
+    virtual void set_variant(int, bool, int) {}
+
+    // Kernel timing with HIP events recorded on the launching stream around
+    // every product (off by default).  Stats: launches, seconds, algorithmic
+    // bytes (product_bytes of each launch).
+    void set_profiling(bool on);
+    void kernel_stats(long long* launches, double* seconds, double* bytes);
+
+protected:
+    void prof_begin(hipStream_t s) const;
+    void prof_end(hipStream_t s, int ncols, bool transpose, int elem_bytes) const;
+    int64_t rows_ = 0, cols_ = 0;
+
+private:
+    struct ProfPair { hipEvent_t start, stop; double bytes; };
+    bool profiling_ = false;
+    mutable std::vector<ProfPair> pending_;
+    mutable std::vector<hipEvent_t> pool_;
+    mutable hipEvent_t open_start_ = nullptr;
+    long long launches_ = 0;
+    double seconds_ = 0.0, bytes_ = 0.0;
+    void resolve_pending();
+};
+
+// Non-owning CSR (int64 row pointers, int32 column indices, fp32/fp64
+// values) on the device; owns its device-built transpose.
+class CsrOperator final : public DeviceOperator {
+public:
+    CsrOperator(int64_t rows, int64_t cols, int64_t nnz, const int64_t* row_ptr, const int32_t* col_idx,
+                const void* values, ValueType type, Context& ctx);
+
+    void apply(hipStream_t s, const double* X, int n, int64_t ldx, double* Y, int64_t ldy) const override;
+    void apply_t(hipStream_t s, const double* Y, int n, int64_t ldy, double* X, int64_t ldx) const override;
+    void apply(hipStream_t s, const float* X, int n, int64_t ldx, float* Y, int64_t ldy) const override;
+    void apply_t(hipStream_t s, const float* Y, int n, int64_t ldy, float* X, int64_t ldx) const override;
+    ValidationModel validation_model() const override { return model_; }
+    bool supports_refinement() const override { return true; }
+    bool refine_apply(const long double* x, long double* y) const override;
+    bool refine_apply_t(const long double* y, long double* x) const override;
+    double product_bytes(int s, bool transpose, int eb) const override;
+    double product_flops(int s) const override { return 2.0 * static_cast<double>(nnz_) * s; }
+    void set_variant(int s, bool transpose, int variant) override;
+
+    // Augmented [A; a I] products: Y rows [rows, rows+cols) = a X; the
+    // transpose adds a * Y[rows:] to A^T Y[0:rows].
+    template <typename T>
+    void apply_t_add(hipStream_t s, const T* Y, int n, int64_t ldy, T* X, int64_t ldx, const T* add,
+                     int64_t ld_add, double a) const;
+
+    const CsrView& view() const { return a_; }
+    const CsrView& transpose_view() const { return at_; }
+    int64_t nnz() const { return nnz_; }
+
+private:
+    int variant_for(int s, bool transpose) const;
+    void ensure_host_copy() const;
+
+    CsrView a_, at_;
+    int64_t nnz_ = 0;
+    DevBuf<int64_t> t_rp_;
+    DevBuf<int32_t> t_ci_;
+    DevBuf<char> t_val_;
+    ValidationModel model_;
+    std::map<std::pair<int, bool>, int> variants_;
+    mutable std::once_flag host_once_;
+    mutable std::vector<int64_t> h_rp_;
+    mutable std::vector<int32_t> h_ci_;
+    mutable std::vector<double> h_val_;
+};
+
+// Non-owning dense A (fp64 / fp32 / bf16), column-major (lda >= rows) or
+// row-major (lda >= cols).
+class DenseOperator final : public DeviceOperator {
+public:
+    DenseOperator(int64_t rows, int64_t cols, const void* values, ValueType type, bool row_major,
+                  int64_t ld, Context& ctx);
+
+    void apply(hipStream_t s, const double* X, int n, int64_t ldx, double* Y, int64_t ldy) const override;
+    void apply_t(hipStream_t s, const double* Y, int n, int64_t ldy, double* X, int64_t ldx) const override;
+    void apply(hipStream_t s, const float* X, int n, int64_t ldx, float* Y, int64_t ldy) const override;
+    void apply_t(hipStream_t s, const float* Y, int n, int64_t ldy, float* X, int64_t ldx) const override;
+    double relative_block_product_cost(int s) const override { return s > 0 ? 1.0 : kInfD; }
+    ValidationModel validation_model() const override { return model_; }
+    bool supports_refinement() const override;
+    bool refine_apply(const long double* x, long double* y) const override;
+    bool refine_apply_t(const long double* y, long double* x) const override;
+    double product_bytes(int s, bool transpose, int eb) const override;
+    double product_flops(int s) const override {
+        return 2.0 * static_cast<double>(rows_) * static_cast<double>(cols_) * s;
+    }
+    Strided strided() const { return a_; }
+
+private:
+    template <typename T>
+    void do_apply(hipStream_t s, const T* X, int n, int64_t ldx, T* Y, int64_t ldy) const;
+    template <typename T>
+    void do_apply_t(hipStream_t s, const T* Y, int n, int64_t ldy, T* X, int64_t ldx) const;
+    void ensure_host_copy() const;
+
+    Strided a_;
+    ValidationModel model_;
+    Context& ctx_;
+    mutable std::once_flag host_once_;
+    mutable std::vector<double> h_val_;  // row-major copy for refinement
+};
+
+// [A; a I] for the ridge correction problems of the mixed-precision outer
+// refinement (rows = m + n).
+class AugmentedOperator final : public DeviceOperator {
+public:
+    AugmentedOperator(const DeviceOperator& base, double a, Context& ctx);
+    void apply(hipStream_t s, const double* X, int n, int64_t ldx, double* Y, int64_t ldy) const override;
+    void apply_t(hipStream_t s, const double* Y, int n, int64_t ldy, double* X, int64_t ldx) const override;
+    void apply(hipStream_t s, const float* X, int n, int64_t ldx, float* Y, int64_t ldy) const override;
+    void apply_t(hipStream_t s, const float* Y, int n, int64_t ldy, float* X, int64_t ldx) const override;
+    double relative_block_product_cost(int s) const override { return base_.relative_block_product_cost(s); }
+    ValidationModel validation_model() const override { return model_; }
+    double product_bytes(int s, bool transpose, int eb) const override {
+        return base_.product_bytes(s, transpose, eb) + 2.0 * eb * s * static_cast<double>(base_.cols());
+    }
+    double product_flops(int s) const override { return base_.product_flops(s); }
+
+private:
+    template <typename T>
+    void do_apply(hipStream_t s, const T* X, int n, int64_t ldx, T* Y, int64_t ldy) const;
+    template <typename T>
+    void do_apply_t(hipStream_t s, const T* Y, int n, int64_t ldy, T* X, int64_t ldx) const;
+
+    const DeviceOperator& base_;
+    double a_;
+    ValidationModel model_;
+    Context& ctx_;
+};
+
+// Bounds from per-row / per-column |a| sums and square sums.
+ValidationModel norm_model_from_sums(const std::vector<double>& row_sum, const std::vector<double>& row_sq,
+                                     const std::vector<long long>& row_ops,
+                                     const std::vector<double>& col_sum, const std::vector<double>& col_sq,
+                                     const std::vector<long long>& col_ops, double max_entry,
+                                     long long steps_apply, long long steps_transpose);
+
+}  // namespace amflsx

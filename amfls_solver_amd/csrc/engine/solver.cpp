@@ -1,0 +1,703 @@
+#include "solver.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <deque>
+#include <limits>
+#include <memory>
+#include <stdexcept>
+
+#include "controller.hpp"
+#include "validation.hpp"
+#include "workspace.hpp"
+
+namespace amflsx {
+namespace {
+
+int default_epochs(int dimension) {
+    int epochs = 1, covered = 1;
+    while (covered < dimension && epochs < 30) {
+        covered *= 2;
+        ++epochs;
+    }
+    return epochs;
+}
+
+double active_certificate(const LeastSquaresResult& r, double lambda) {
+    return lambda > 0.0 ? r.relative_energy_error_upper_bound : r.backward_error_upper_bound;
+}
+
+void validate_common(const DeviceOperator& op, const double* b, double lambda, double tol) {
+    if (op.rows() <= 0 || op.cols() <= 0) throw std::invalid_argument("operator dimensions must be positive");
+    if (b == nullptr) throw std::invalid_argument("right-hand side cannot be null");
+    if (!std::isfinite(lambda) || lambda < 0.0)
+        throw std::invalid_argument("regularization must be finite and nonnegative");
+    if (!std::isfinite(tol) || tol <= 0.0 || tol >= 1.0)
+        throw std::invalid_argument("tolerance must lie strictly between zero and one");
+}
+
+void check_rhs_finite(Context& ctx, const double* b, int64_t m) {
+    double st[3];
+    ctx.block_stats<double>(m, 1, b, 1, st);
+    if (st[2] > 0.0) throw std::invalid_argument("right-hand side must be finite");
+}
+
+// Recent-interval features for the learned width predictor (engine
+// extension; the reference keeps only two same-width intervals).
+struct IntervalLog {
+    std::deque<double> decrease, seconds;
+    void push(double prev, double cur, double sec_per_level) {
+        double d = 0.0;
+        if (prev > 0.0 && cur > 0.0 && std::isfinite(prev) && std::isfinite(cur)) d = std::log10(prev / cur);
+        decrease.push_front(d);
+        seconds.push_front(sec_per_level);
+        if (decrease.size() > 4) decrease.pop_back();
+        if (seconds.size() > 4) seconds.pop_back();
+    }
+    void clear() {
+        decrease.clear();
+        seconds.clear();
+    }
+};
+
+double operator_nnz(const DeviceOperator& op) {
+    if (auto* c = dynamic_cast<const CsrOperator*>(&op)) return static_cast<double>(c->nnz());
+    return static_cast<double>(op.rows()) * static_cast<double>(op.cols());
+}
+
+// The adaptive driver (reference amfls.cpp:57-661) in working precision T.
+template <typename T>
+LeastSquaresResult amfls_driver(const DeviceOperator& op, const double* b_dev, const SolveOptions& o, Context& ctx,
+                                const WidthPolicy* policy) {
+    const double lambda = o.regularization, tol = o.tolerance;
+    Clock& clk = ctx.clock;
+    const double total_start = clk.now();
+    RunStatistics stats;
+    ValidationEngine validator(op, b_dev, ctx, stats);
+    Workspace<T> ws(op, b_dev, lambda, o.maximum_basis_size, stats, ctx, validator);
+    const int dimension = static_cast<int>(std::min<int64_t>(op.rows(), op.cols()));
+    const int max_depth = o.maximum_depth > 0 ? std::min(o.maximum_depth, dimension) : dimension;
+    const int cap_aux = std::max(0, ws.basis_limit() - 1);
+    const int max_aux = std::min(o.maximum_auxiliary_width > 0 ? o.maximum_auxiliary_width : cap_aux, cap_aux);
+    const int max_epochs = o.maximum_epochs > 0 ? o.maximum_epochs : default_epochs(dimension);
+
+    std::vector<IterationRecord> trace;
+    ctl::CheckpointSchedule schedule(lambda, tol);
+    int evaluations = 0, last_depth = -1, aux = 0;
+    LeastSquaresResult result, best;
+    double best_cert = kInfD;
+    bool has = false;
+    double last_op_s = 0.0, last_loc_s = 0.0;
+    int last_interval_width = 0;
+
+    auto record = [&](LeastSquaresResult cand, bool train) {
+        if (cand.trace.size() != 1) throw std::logic_error("AMFLS candidate must contain exactly one trace record");
+        if (cand.depth == last_depth) throw std::logic_error("AMFLS evaluated one checkpoint more than once");
+        last_depth = cand.depth;
+        if (train) schedule.record(cand.depth, cand);
+        IterationRecord rec = cand.trace.front();
+        rec.interval_operator_seconds = last_op_s;
+        rec.interval_local_seconds = last_loc_s;
+        rec.active_width = last_interval_width;
+        trace.push_back(rec);
+        cand.trace.clear();
+        ++evaluations;
+        const double cert = active_certificate(cand, lambda);
+        const bool better = !has || (std::isfinite(cert) && !std::isfinite(best_cert)) || cert < best_cert;
+        if (better) {
+            best_cert = cert;
+            best = cand;
+        }
+        has = true;
+        result = std::move(cand);
+    };
+    double prev_eval_cert = kInfD;
+    int stalls = 0;
+    bool stagnated = false;
+    auto evaluate = [&](bool train) {
+        record(ws.evaluate(tol, evaluations, aux), train);
+        if (o.stop_on_stagnation) {
+            const double c = active_certificate(result, lambda);
+            if (!(c < 0.5 * prev_eval_cert)) ++stalls; else stalls = 0;
+            prev_eval_cert = std::min(prev_eval_cert, c);
+            stagnated = stalls >= 2;
+        }
+    };
+    auto finish = [&](LeastSquaresResult f) {
+        f.iterations = ws.depth();
+        f.trace = trace;
+        stats.total_seconds = clk.now() - total_start;
+        f.statistics = stats;
+        return f;
+    };
+    auto finish_at_limit = [&](SolverStatus st, StopReason reason) {
+        LeastSquaresResult f = lambda > 0.0 ? result : best;
+        f.status = st;
+        f.stop_reason = reason;
+        return finish(std::move(f));
+    };
+
+    ws.initialize(false, 0, 0, 0, 0);
+    if (ws.can_advance() && max_depth > 0) {
+        last_interval_width = ws.active_left_width();
+        const double t0 = clk.now();
+        const double a0 = stats.a_seconds + stats.at_seconds;
+        (void)ws.advance_level();
+        last_op_s = stats.a_seconds + stats.at_seconds - a0;
+        last_loc_s = std::max(0.0, clk.now() - t0 - last_op_s);
+    }
+    evaluate(true);
+    if (candidate_validation_failed_numerically(result) || result.status == SolverStatus::success)
+        return finish(result);
+    bool truncated = step_has_numerical_rank_truncation(ws.last_step_audit());
+    bool broke = step_has_nonfinite_breakdown(ws.last_step_audit());
+
+    ctl::ProgressHistory hist;
+    ctl::MatchedHorizonPlan plan;
+    IntervalLog ilog;
+    int epoch = 0;
+    uint64_t random_generated = 0;
+    bool feedback_pending = false;
+    int fb_target = 0, stage_start_rank = 0, stage_origin = 0, stage_target = 0;
+    const double nnz = operator_nnz(op);
+
+    while (true) {
+        if (stagnated) return finish_at_limit(SolverStatus::precision_limit, StopReason::precision_limit);
+        if (broke) return finish_at_limit(SolverStatus::numerical_breakdown, StopReason::numerical_breakdown);
+        if (ws.basis_rank() >= ws.basis_limit()) return finish_at_limit(SolverStatus::basis_limit, StopReason::maximum_basis);
+        if (ws.depth() >= max_depth) return finish_at_limit(SolverStatus::work_limit, StopReason::maximum_depth);
+        if (truncated) return finish_at_limit(SolverStatus::precision_limit, StopReason::precision_limit);
+
+        const double interval_cert = active_certificate(result, lambda);
+        const int frontier = ws.active_left_width();
+        const int rem_basis = std::max(0, ws.basis_limit() - ws.basis_rank());
+        const int avail_inc = std::max(0, rem_basis - frontier);
+        const int rem_aux = std::max(0, max_aux - aux);
+        const int width_cap = std::min(avail_inc, rem_aux);
+        const bool fb_available = max_depth - ws.depth() >= 2;
+        const int fb_depth = fb_available ? ws.depth() + 2 : ws.depth();
+        const int levels_to_fb = fb_depth - ws.depth();
+        const int fb_feasible = fb_available ? ctl::feedback_feasible_increment(ws.basis_rank(), ws.basis_limit(),
+                                                                               frontier, levels_to_fb)
+                                             : 0;
+        const int feasible_cap = std::min(width_cap, fb_feasible);
+        if (plan.end_rank >= 0 && ctl::remaining(plan, ws.basis_rank()) == 0) ctl::reset(plan);
+        const int rem_horizon = ctl::remaining(plan, ws.basis_rank());
+        const bool horizon_active = rem_horizon > 0;
+        const bool collecting = horizon_active && !feedback_pending && plan.samples < 2;
+        const bool ready = !feedback_pending && !collecting;
+        const bool stage_available = ready && epoch < max_epochs;
+        const int next_inc = stage_available ? ctl::next_dyadic_increment(frontier, aux, max_aux, feasible_cap) : 0;
+        int requested = 0, requested_horizon = 0;
+        if (stage_available && next_inc > 0) {
+            const int target_active = frontier + next_inc;
+            const ctl::WidthDecision d =
+                horizon_active
+                    ? ctl::horizon_candidate(next_inc, rem_horizon, ws.depth(), max_depth, ws.basis_rank(),
+                                             ws.basis_limit(), frontier, plan.op_s, plan.loc_s, plan.samples,
+                                             op.relative_block_product_cost(frontier),
+                                             op.relative_block_product_cost(target_active), plan.origin_width > 1)
+                    : ctl::initial_candidate(next_inc, interval_cert, tol, ws.depth(), max_depth, ws.basis_rank(),
+                                             ws.basis_limit(), frontier, hist,
+                                             op.relative_block_product_cost(frontier),
+                                             op.relative_block_product_cost(target_active));
+            bool widen = d.candidate;
+            if (policy && *policy) {
+                WidthFeatures f;
+                f.active_width = frontier;
+                f.proposed_increment = next_inc;
+                for (size_t i = 0; i < ilog.decrease.size(); ++i) f.bound_decrease[i] = ilog.decrease[i];
+                for (size_t i = 0; i < ilog.seconds.size(); ++i) f.step_seconds[i] = ilog.seconds[i];
+                f.rows = static_cast<double>(op.rows());
+                f.cols = static_cast<double>(op.cols());
+                f.nnz = nnz;
+                f.certificate = interval_cert;
+                f.tolerance = tol;
+                f.depth = ws.depth();
+                f.basis_rank = ws.basis_rank();
+                f.basis_limit = ws.basis_limit();
+                f.heuristic = d.candidate;
+                widen = (*policy)(f);
+            }
+            if (widen) {
+                requested = next_inc;
+                if (!horizon_active) requested_horizon = d.matched_horizon > 0 ? d.matched_horizon : 0;
+            }
+        }
+        if (horizon_active && ready && requested == 0) ctl::reset(plan);
+
+        if (requested > 0) {
+            fb_target = fb_depth;
+            stage_start_rank = ws.basis_rank();
+            stage_origin = horizon_active ? plan.origin_width : frontier;
+            stage_target = frontier + requested;
+            ctl::reset(hist);
+            ilog.clear();
+            schedule = ctl::CheckpointSchedule(lambda, tol);
+            stats.gaussian_random_block_requests += 1;
+            stats.gaussian_random_columns += requested;
+            stats.gaussian_random_values += op.rows() * static_cast<long long>(requested);
+            const uint64_t first = random_generated;
+            random_generated += static_cast<uint64_t>(requested);
+            aux += requested;
+            ++epoch;
+            (void)ws.append_left_seed_gaussian(requested, o.seed, o.stream, first, true);
+            bool advanced = false;
+            if (ws.can_advance()) {
+                last_interval_width = ws.active_left_width();
+                const double t0 = clk.now();
+                const double a0 = stats.a_seconds + stats.at_seconds;
+                advanced = ws.advance_level();
+                last_op_s = stats.a_seconds + stats.at_seconds - a0;
+                last_loc_s = std::max(0.0, clk.now() - t0 - last_op_s);
+            }
+            if (advanced) {
+                if (requested_horizon > 0) {
+                    (void)ctl::start_plan(plan, stage_start_rank, ws.basis_limit(), requested_horizon, stage_origin,
+                                          stage_target, interval_cert);
+                } else {
+                    ctl::reset_costs(plan, stage_target);
+                }
+                truncated = step_has_numerical_rank_truncation(ws.last_step_audit());
+                broke = step_has_nonfinite_breakdown(ws.last_step_audit());
+                const bool exhausted = ctl::remaining(plan, ws.basis_rank()) == 0;
+                const bool terminal = ws.basis_rank() >= ws.basis_limit() || ws.depth() >= max_depth ||
+                                      !ws.can_advance();
+                if (terminal || exhausted) {
+                    evaluate(false);
+                    if (candidate_validation_failed_numerically(result) || result.status == SolverStatus::success)
+                        return finish(result);
+                }
+                const bool reliable = !truncated && !broke && ws.basis_rank() > stage_start_rank &&
+                                      ws.active_left_width() == stage_target;
+                const bool extended = exhausted && ctl::progress_stalled(plan, active_certificate(result, lambda)) &&
+                                      ctl::extend_to_limit(plan, ws.basis_rank(), ws.basis_limit());
+                if (!reliable || (exhausted && !extended)) {
+                    ctl::reset(plan);
+                    feedback_pending = false;
+                } else {
+                    feedback_pending = true;
+                }
+            } else {
+                truncated = step_has_numerical_rank_truncation(ws.last_step_audit());
+                broke = step_has_nonfinite_breakdown(ws.last_step_audit());
+                feedback_pending = false;
+                ctl::reset(plan);
+            }
+            continue;
+        }
+
+        if (ws.can_advance()) {
+            const int start_depth = ws.depth();
+            const int start_rank = ws.basis_rank();
+            const bool matched_cost = !feedback_pending && ctl::remaining(plan, start_rank) > 0 && plan.samples < 2;
+            int target_depth;
+            if (feedback_pending) target_depth = fb_target;
+            else if (matched_cost) target_depth = std::min(start_depth + 1, max_depth);
+            else {
+                target_depth = start_depth + 1;
+                while (target_depth < max_depth && !schedule.should_evaluate(target_depth)) ++target_depth;
+            }
+            const double t0 = clk.now();
+            const double a0 = stats.a_seconds, at0 = stats.at_seconds;
+            last_interval_width = frontier;
+            while (ws.can_advance() && ws.depth() < target_depth) {
+                const bool advanced = ws.advance_level();
+                truncated = step_has_numerical_rank_truncation(ws.last_step_audit());
+                broke = step_has_nonfinite_breakdown(ws.last_step_audit());
+                if (!advanced || truncated || broke) break;
+            }
+            const double search_s = clk.now() - t0;
+            const double op_s = std::max(0.0, stats.a_seconds - a0 + stats.at_seconds - at0);
+            const double loc_s = std::max(0.0, search_s - op_s);
+            last_op_s = op_s;
+            last_loc_s = loc_s;
+            if (ws.depth() > start_depth) {
+                const bool completes_pair = matched_cost && plan.samples == 1;
+                const bool exhausts = (feedback_pending || matched_cost) && ctl::remaining(plan, ws.basis_rank()) == 0;
+                const bool terminal = ws.basis_rank() >= ws.basis_limit() || ws.depth() >= max_depth ||
+                                      !ws.can_advance();
+                if ((!feedback_pending && !matched_cost) || completes_pair || exhausts || terminal) {
+                    evaluate(!feedback_pending && !matched_cost);
+                    if (candidate_validation_failed_numerically(result) || result.status == SolverStatus::success)
+                        return finish(result);
+                }
+                const int levels = ws.depth() - start_depth;
+                if (feedback_pending && ws.depth() >= fb_target) {
+                    feedback_pending = false;
+                    const bool reliable = !truncated && !broke && ws.basis_rank() > start_rank &&
+                                          ws.basis_rank() > stage_start_rank && frontier == stage_target &&
+                                          ws.active_left_width() == stage_target;
+                    const bool recorded = reliable && ctl::record_cost(plan, frontier, ws.active_left_width(), levels,
+                                                                       op_s, loc_s);
+                    const bool exh = ctl::remaining(plan, ws.basis_rank()) == 0;
+                    const bool ext = exh && ctl::progress_stalled(plan, active_certificate(result, lambda)) &&
+                                     ctl::extend_to_limit(plan, ws.basis_rank(), ws.basis_limit());
+                    if (!recorded || (exh && !ext)) ctl::reset(plan);
+                } else if (matched_cost) {
+                    const bool reliable = !truncated && !broke && ws.basis_rank() > start_rank &&
+                                          frontier == ws.active_left_width();
+                    const bool recorded = reliable && ctl::record_cost(plan, frontier, ws.active_left_width(), levels,
+                                                                       op_s, loc_s);
+                    const bool exh = ctl::remaining(plan, ws.basis_rank()) == 0;
+                    const bool ext = exh && ctl::progress_stalled(plan, active_certificate(result, lambda)) &&
+                                     ctl::extend_to_limit(plan, ws.basis_rank(), ws.basis_limit());
+                    if (!recorded || (exh && !ext)) ctl::reset(plan);
+                } else if (!feedback_pending) {
+                    const double cur = active_certificate(result, lambda);
+                    ctl::record_deepen_progress(hist, interval_cert, cur, frontier, ws.active_left_width(), levels,
+                                                op_s, loc_s);
+                    if (frontier == ws.active_left_width())
+                        ilog.push(interval_cert, cur, (op_s + loc_s) / std::max(1, levels));
+                }
+                continue;
+            }
+            if (feedback_pending || matched_cost) {
+                feedback_pending = false;
+                ctl::reset(plan);
+            }
+            continue;
+        }
+        StopReason reason = StopReason::exhausted_search_space;
+        if (!feedback_pending && epoch >= max_epochs && aux < max_aux && avail_inc > 0)
+            reason = StopReason::maximum_epochs;
+        return finish_at_limit(SolverStatus::work_limit, reason);
+    }
+}
+
+void accumulate(RunStatistics& into, const RunStatistics& s) {
+#define ACC(f) into.f += s.f;
+    ACC(a_columns) ACC(at_columns) ACC(a_block_calls) ACC(at_block_calls) ACC(search_a_columns)
+    ACC(search_at_columns) ACC(validation_a_columns) ACC(validation_at_columns) ACC(gaussian_random_columns)
+    ACC(gaussian_random_values) ACC(a_seconds) ACC(at_seconds) ACC(orthogonalization_seconds)
+    ACC(projected_solve_seconds) ACC(validation_seconds) ACC(search_a_block_calls) ACC(search_at_block_calls)
+    ACC(validation_a_block_calls) ACC(validation_at_block_calls) ACC(base_validation_a_columns)
+    ACC(base_validation_at_columns) ACC(base_validation_a_block_calls) ACC(base_validation_at_block_calls)
+    ACC(ridge_correction_a_columns) ACC(ridge_correction_at_columns) ACC(ridge_correction_a_block_calls)
+    ACC(ridge_correction_at_block_calls) ACC(base_validation_seconds) ACC(ridge_correction_validation_seconds)
+    ACC(gaussian_random_block_requests)
+#undef ACC
+}
+
+// Mixed precision: FP32 block Golub-Kahan inner solves on the correction
+// problem, FP64 residual / candidate / certificate on the original problem.
+LeastSquaresResult mixed_driver(const DeviceOperator& op, const double* b_dev, const SolveOptions& o, Context& ctx,
+                                const WidthPolicy* policy) {
+    const double lambda = o.regularization, tol = o.tolerance;
+    const int64_t m = op.rows(), n = op.cols();
+    Clock& clk = ctx.clock;
+    const double total_start = clk.now();
+    RunStatistics stats;
+    ValidationEngine outer(op, b_dev, ctx, stats);
+    std::unique_ptr<AugmentedOperator> aug;
+    if (lambda > 0.0) aug = std::make_unique<AugmentedOperator>(op, std::sqrt(lambda), ctx);
+    const DeviceOperator& inner_op = aug ? static_cast<const DeviceOperator&>(*aug) : op;
+    const int64_t mi = inner_op.rows();
+    DevBuf<double> x(static_cast<size_t>(n)), d(static_cast<size_t>(n)), rhs(static_cast<size_t>(mi));
+    fill<double>(ctx.stream, x.get(), n, 0.0);
+    std::vector<IterationRecord> trace;
+    LeastSquaresResult best;
+    double best_cert = kInfD;
+    bool have_best = false;
+    const double rho = o.inner_tolerance > 0.0 ? o.inner_tolerance : 1e-4;
+    int steps = 0;
+    LeastSquaresResult res;
+    for (int outer_it = 0; outer_it <= o.maximum_refinements; ++outer_it) {
+        res = outer.validate(x.get(), lambda, tol);
+        res.depth = outer_it;
+        res.basis_rank = 0;
+        res.statistics = stats;
+        trace.push_back(make_iteration_record(res, outer_it, outer_it, 0));
+        const double cert = active_certificate(res, lambda);
+        const double prev_best = best_cert;
+        if (!have_best || (std::isfinite(cert) && !std::isfinite(best_cert)) || cert < best_cert) {
+            best = res;
+            best_cert = cert;
+            have_best = true;
+        }
+        if (res.status == SolverStatus::success || candidate_validation_failed_numerically(res)) break;
+        if (outer_it >= 2 && std::isfinite(prev_best) && !(cert < 0.5 * prev_best)) break;  // stagnated
+        if (outer_it == o.maximum_refinements) break;
+        // Correction problem with the FP64 residual / gradient of x:
+        //   lambda = 0: min ||A d - (b - A x)||, minimum norm;
+        //   lambda > 0: min ||[A; sqrt(lambda) I] d - [0; -g / sqrt(lambda)]||,
+        //   whose normal equations are (A^T A + lambda I) d = -g.
+        if (lambda > 0.0) {
+            fill<double>(ctx.stream, rhs.get(), m, 0.0);
+            copy_block<double, double>(ctx.stream, n, 1, outer.last_gradient(), 1, rhs.get() + m, 1,
+                                       -1.0 / std::sqrt(lambda));
+        } else {
+            copy_block<double, double>(ctx.stream, m, 1, outer.last_residual_negated(), 1, rhs.get(), 1, -1.0);
+        }
+        double c0 = 1.0;
+        {
+            ValidationEngine probe(inner_op, rhs.get(), ctx, stats);
+            fill<double>(ctx.stream, d.get(), n, 0.0);
+            LeastSquaresResult r0 = probe.validate(d.get(), 0.0, 0.5);
+            if (std::isfinite(r0.backward_error_upper_bound) && r0.backward_error_upper_bound > 0.0)
+                c0 = r0.backward_error_upper_bound;
+        }
+        const double need = std::isfinite(cert) && cert > tol ? tol / cert : 1.0;
+        SolveOptions io = o;
+        io.regularization = 0.0;
+        io.precision = Precision::fp32;
+        io.stop_on_stagnation = true;
+        io.tolerance = std::min(0.5, std::max({rho * c0, 0.25 * need * c0, 1e-15}));
+        LeastSquaresResult in = amfls_driver<float>(inner_op, rhs.get(), io, ctx, policy);
+        accumulate(stats, in.statistics);
+        ++steps;
+        if (in.solution.size() != static_cast<size_t>(n)) break;
+        AMFLS_HIP_CHECK(hipMemcpyAsync(d.get(), in.solution.data(), n * sizeof(double), hipMemcpyHostToDevice,
+                                       ctx.stream));
+        add_vec(ctx.stream, n, d.get(), x.get());
+        ctx.sync();
+    }
+    if (!have_best) {
+        res = outer.validate(x.get(), lambda, tol);
+        best = res;
+    }
+    best.trace = trace;
+    best.refinement_steps = steps;
+    best.iterations = steps;
+    stats.total_seconds = clk.now() - total_start;
+    best.statistics = stats;
+    return best;
+}
+
+}  // namespace
+
+LeastSquaresResult solve_amfls(const DeviceOperator& op, const double* b_dev, const SolveOptions& o, Context& ctx,
+                               const WidthPolicy* policy) {
+    validate_common(op, b_dev, o.regularization, o.tolerance);
+    if (!std::isfinite(o.failure_probability) || o.failure_probability <= 0.0 || o.failure_probability >= 1.0)
+        throw std::invalid_argument("failure_probability must lie strictly between zero and one");
+    if (o.maximum_epochs < 0 || o.maximum_depth < 0 || o.maximum_auxiliary_width < 0 || o.maximum_basis_size < 0)
+        throw std::invalid_argument("work limits must be nonnegative");
+    check_rhs_finite(ctx, b_dev, op.rows());
+    ctx.clock.configure(o.clock_mode, o.synthetic_operator_unit, o.synthetic_local_unit);
+    if (o.precision == Precision::mixed) return mixed_driver(op, b_dev, o, ctx, policy);
+    if (o.precision == Precision::fp32) return amfls_driver<float>(op, b_dev, o, ctx, policy);
+    return amfls_driver<double>(op, b_dev, o, ctx, policy);
+}
+
+LeastSquaresResult solve_fixed_rbgk(const DeviceOperator& op, const double* b_dev, const FixedRbgkOptions& o,
+                                    Context& ctx) {
+    // reference fixed_rbgk.cpp:18-209
+    validate_common(op, b_dev, o.regularization, o.tolerance);
+    if (!std::isfinite(o.failure_probability) || o.failure_probability <= 0.0 || o.failure_probability >= 1.0)
+        throw std::invalid_argument("failure_probability must lie strictly between zero and one");
+    if (o.auxiliary_width < 0) throw std::invalid_argument("auxiliary_width must be nonnegative");
+    if (o.maximum_depth <= 0) throw std::invalid_argument("maximum_depth must be positive");
+    if (o.maximum_basis_size < 0) throw std::invalid_argument("maximum_basis_size must be nonnegative");
+    check_rhs_finite(ctx, b_dev, op.rows());
+    ctx.clock.configure(o.clock_mode, 1e-3, 1e-6);
+    Clock& clk = ctx.clock;
+    const double total_start = clk.now();
+    RunStatistics stats;
+    ValidationEngine validator(op, b_dev, ctx, stats);
+    Workspace<double> ws(op, b_dev, o.regularization, o.maximum_basis_size, stats, ctx, validator);
+    const int aux = ws.right_hand_side_norm() == 0.0 ? 0 : std::min(o.auxiliary_width, std::max(0, ws.basis_limit() - 1));
+    std::vector<IterationRecord> trace;
+    LeastSquaresResult result;
+    int depth = 0, evaluated = -1;
+    ctl::CheckpointSchedule schedule(o.regularization, o.tolerance);
+    auto evaluate = [&] {
+        result = ws.evaluate(o.tolerance, 0, aux);
+        schedule.record(depth, result);
+        evaluated = depth;
+        trace.push_back(result.trace.front());
+        result.trace.clear();
+    };
+    auto evaluate_terminal = [&] {
+        if (evaluated != depth) evaluate();
+        return candidate_validation_failed_numerically(result);
+    };
+    if (aux > 0) {
+        stats.gaussian_random_block_requests += 1;
+        stats.gaussian_random_columns += aux;
+        stats.gaussian_random_values += op.rows() * static_cast<long long>(aux);
+    }
+    ws.initialize(aux > 0, aux, o.seed, o.stream, 0);
+    if (ws.can_advance()) (void)ws.advance_level();
+    depth = ws.depth();
+    evaluate();
+    bool truncated = step_has_numerical_rank_truncation(ws.last_step_audit());
+    bool broke = step_has_nonfinite_breakdown(ws.last_step_audit());
+    while (true) {
+        if (candidate_validation_failed_numerically(result)) break;
+        if (result.status == SolverStatus::success && o.stop_early) break;
+        if (broke) {
+            if (evaluate_terminal()) break;
+            result.status = SolverStatus::numerical_breakdown;
+            result.stop_reason = StopReason::numerical_breakdown;
+            break;
+        }
+        SolverStatus ts = SolverStatus::success;
+        StopReason tr = StopReason::certified_optimality;
+        bool terminal = true;
+        if (ws.basis_rank() >= ws.basis_limit()) { ts = SolverStatus::basis_limit; tr = StopReason::maximum_basis; }
+        else if (depth >= o.maximum_depth) { ts = SolverStatus::work_limit; tr = StopReason::maximum_depth; }
+        else if (truncated) { ts = SolverStatus::precision_limit; tr = StopReason::precision_limit; }
+        else if (!ws.can_advance()) { ts = SolverStatus::work_limit; tr = StopReason::exhausted_search_space; }
+        else terminal = false;
+        if (terminal) {
+            if (evaluate_terminal()) break;
+            if (result.status != SolverStatus::success) {
+                result.status = ts;
+                result.stop_reason = tr;
+            }
+            break;
+        }
+        const bool ok = ws.advance_level();
+        truncated = step_has_numerical_rank_truncation(ws.last_step_audit());
+        broke = step_has_nonfinite_breakdown(ws.last_step_audit());
+        if (!ok) {
+            if (!ws.last_step_audit().right_rank_loss && !broke)
+                throw std::logic_error("failed fixed extension did not report rank loss");
+            continue;
+        }
+        depth = ws.depth();
+        const bool term = depth >= o.maximum_depth || ws.basis_rank() >= ws.basis_limit() || !ws.can_advance() ||
+                          truncated || broke;
+        if (schedule.should_evaluate(depth) || term) evaluate();
+    }
+    result.iterations = depth;
+    result.depth = depth;
+    result.trace = trace;
+    stats.total_seconds = clk.now() - total_start;
+    result.statistics = stats;
+    return result;
+}
+
+LeastSquaresResult solve_lsqr(const DeviceOperator& op, const double* b_dev, const LsqrOptions& o, Context& ctx) {
+    // reference lsqr.cpp:84-279 on device vectors (FP64)
+    validate_common(op, b_dev, o.regularization, o.tolerance);
+    if (o.maximum_iterations < 0) throw std::invalid_argument("maximum_iterations must be nonnegative");
+    check_rhs_finite(ctx, b_dev, op.rows());
+    ctx.clock.configure(o.clock_mode, 1e-3, 1e-6);
+    Clock& clk = ctx.clock;
+    const int64_t m = op.rows(), n = op.cols();
+    const int max_it = o.maximum_iterations > 0 ? o.maximum_iterations : static_cast<int>(std::min(m, n));
+    const double total_start = clk.now();
+    RunStatistics stats;
+    ValidationEngine validator(op, b_dev, ctx, stats);
+    hipStream_t s = ctx.stream;
+    DevBuf<double> u(m), v(n), w(n), x(n), av(m), atu(n);
+    auto nrm = [&](const double* p, int64_t len) {
+        double st[3];
+        ctx.block_stats<double>(len, 1, p, 1, st);
+        return fp::nominal_norm(st[0], st[1], st[2]);
+    };
+    auto search = [&](const double* in, double* out, bool transpose) {
+        if (!clk.synthetic()) ctx.sync();
+        const double start = clk.now();
+        clk.charge_operator(op.relative_block_product_cost(1));
+        if (transpose) op.apply_t(s, in, 1, 1, out, 1);
+        else op.apply(s, in, 1, 1, out, 1);
+        if (!clk.synthetic()) ctx.sync();
+        const double sec = clk.now() - start;
+        if (transpose) {
+            stats.at_seconds += sec; ++stats.at_columns; ++stats.at_block_calls; ++stats.search_at_columns;
+            ++stats.search_at_block_calls;
+        } else {
+            stats.a_seconds += sec; ++stats.a_columns; ++stats.a_block_calls; ++stats.search_a_columns;
+            ++stats.search_a_block_calls;
+        }
+    };
+    copy_block<double, double>(s, m, 1, b_dev, 1, u.get(), 1, 1.0);
+    fill<double>(s, x.get(), n, 0.0);
+    double beta = nrm(u.get(), m);
+    if (beta > 0.0) scale_vec(s, m, 1.0 / beta, u.get(), u.get());
+    search(u.get(), v.get(), true);
+    double alpha = nrm(v.get(), n);
+    if (alpha > 0.0) scale_vec(s, n, 1.0 / alpha, v.get(), v.get());
+    copy_block<double, double>(s, n, 1, v.get(), 1, w.get(), 1, 1.0);
+    double rhobar = alpha, phibar = beta;
+    const double damp = std::sqrt(o.regularization);
+    std::vector<IterationRecord> trace;
+    LeastSquaresResult result;
+    ctl::CheckpointSchedule schedule(o.regularization, o.tolerance);
+    auto evaluate = [&](int it) {
+        result = validator.validate(x.get(), o.regularization, o.tolerance);
+        result.iterations = result.depth = result.basis_rank = it;
+        result.trace.push_back(make_iteration_record(result, 0, it, 0));
+        schedule.record(it, result);
+        trace.push_back(result.trace.front());
+        result.trace.clear();
+    };
+    auto done = [&] {
+        result.trace = trace;
+        stats.total_seconds = clk.now() - total_start;
+        result.statistics = stats;
+        return result;
+    };
+    if (!std::isfinite(beta) || !std::isfinite(alpha) || alpha == 0.0) {
+        evaluate(0);
+        if (candidate_validation_failed_numerically(result) || result.status == SolverStatus::success || alpha == 0.0)
+            return done();
+    }
+    for (int it = 1; it <= max_it; ++it) {
+        search(v.get(), av.get(), false);
+        axpy_block<double>(s, m, 1, -alpha, u.get(), 1, av.get(), 1);
+        beta = nrm(av.get(), m);
+        if (beta > 0.0) scale_vec(s, m, 1.0 / beta, av.get(), av.get());
+        const double rhobar1 = std::hypot(rhobar, damp);
+        const double cs1 = rhobar1 == 0.0 ? 1.0 : rhobar / rhobar1;
+        phibar *= cs1;
+        const double rho = std::hypot(rhobar1, beta);
+        const double cs = rho == 0.0 ? 1.0 : rhobar1 / rho;
+        const double sn = rho == 0.0 ? 0.0 : beta / rho;
+        const double phi = cs * phibar;
+        phibar = sn * phibar;
+        bool finite = std::isfinite(beta) && std::isfinite(rhobar1) && std::isfinite(cs1) && std::isfinite(rho) &&
+                      std::isfinite(cs) && std::isfinite(sn) && std::isfinite(phi) && std::isfinite(phibar);
+        if (rho != 0.0) {
+            const double upd = phi / rho;
+            finite = finite && std::isfinite(upd);
+            axpy_block<double>(s, n, 1, upd, w.get(), 1, x.get(), 1);
+        }
+        finite = finite && std::isfinite(nrm(x.get(), n));
+        std::swap(u, av);
+        const bool beta_breakdown = beta == 0.0;
+        const bool checkpoint = schedule.should_evaluate(it) || beta_breakdown || !finite || it == max_it;
+        bool evaluated = false;
+        if (checkpoint) {
+            evaluate(it);
+            evaluated = true;
+            if (candidate_validation_failed_numerically(result) || result.status == SolverStatus::success) break;
+            if (!finite) { result.status = SolverStatus::numerical_breakdown; result.stop_reason = StopReason::numerical_breakdown; break; }
+            if (beta_breakdown) { result.status = SolverStatus::work_limit; result.stop_reason = StopReason::exhausted_search_space; break; }
+            if (it == max_it) { result.status = SolverStatus::work_limit; result.stop_reason = StopReason::maximum_depth; break; }
+        }
+        search(u.get(), atu.get(), true);
+        axpy_block<double>(s, n, 1, -beta, v.get(), 1, atu.get(), 1);
+        const double next_alpha = nrm(atu.get(), n);
+        bool cont = std::isfinite(next_alpha);
+        if (next_alpha > 0.0 && std::isfinite(next_alpha)) scale_vec(s, n, 1.0 / next_alpha, atu.get(), atu.get());
+        const double theta = sn * next_alpha;
+        const double next_rhobar = -cs * next_alpha;
+        const double rec = theta / rho;
+        cont = cont && std::isfinite(theta) && std::isfinite(next_rhobar) && std::isfinite(rec);
+        if (cont && next_alpha > 0.0) {
+            // w = atu - rec * w
+            copy_block<double, double>(s, n, 1, w.get(), 1, w.get(), 1, -rec);
+            axpy_block<double>(s, n, 1, 1.0, atu.get(), 1, w.get(), 1);
+            cont = std::isfinite(nrm(w.get(), n));
+        }
+        if (!cont || next_alpha == 0.0) {
+            if (!evaluated) {
+                evaluate(it);
+                if (candidate_validation_failed_numerically(result) || result.status == SolverStatus::success) break;
+            }
+            result.status = cont ? SolverStatus::work_limit : SolverStatus::numerical_breakdown;
+            result.stop_reason = cont ? StopReason::exhausted_search_space : StopReason::numerical_breakdown;
+            break;
+        }
+        alpha = next_alpha;
+        rhobar = next_rhobar;
+        std::swap(v, atu);
+    }
+    return done();
+}
+
+}  // namespace amflsx

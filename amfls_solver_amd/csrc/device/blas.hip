@@ -40,12 +40,18 @@ __device__ __forceinline__ Acc ld_acc(const V* p, int64_t i) { return load_as<Ac
 
 template <typename T> struct AccOf { using type = T; };
 
+// Optional device predicate: a kernel given a gate returns at once when the
+// gate holds 0 (work skipped by a decision taken on the device).
+__device__ __forceinline__ bool gated_off(const double* gate) { return gate != nullptr && *gate == 0.0; }
+
 // ------------------------------------------------------------------ gemm_nn
 // Wide tile: 64 rows x 64 columns per 256-thread block, 4x4 per thread.
 template <typename V, typename TB, typename T, typename Acc, bool AROW>
 __global__ __launch_bounds__(kThreads) void gemm_nn_tile_kernel(
     int64_t M, int N, int64_t K, const V* __restrict__ A, int64_t rs, int64_t cs,
-    const TB* __restrict__ B, int64_t ldb, T* __restrict__ Y, int64_t ldy, Acc alpha, Acc beta) {
+    const TB* __restrict__ B, int64_t ldb, T* __restrict__ Y, int64_t ldy, Acc alpha, Acc beta,
+    const double* gate) {
+    if (gated_off(gate)) return;
     constexpr int BM = 64, BN = 64, BK = 32;
     __shared__ Acc As[BK][BM + 1];
     __shared__ Acc Bs[BK][BN];
@@ -108,105 +114,94 @@ __global__ __launch_bounds__(kThreads) void gemm_nn_tile_kernel(
     }
 }
 
-// Narrow (N <= 8) row-major A: one wavefront per output row, lanes across K.
-template <typename V, typename TB, typename T, typename Acc, int NMAX>
-__global__ __launch_bounds__(kThreads) void gemm_nn_narrow_rowmajor_kernel(
-    int64_t M, int N, int64_t K, const V* __restrict__ A, int64_t rs, const TB* __restrict__ B,
-    int64_t ldb, T* __restrict__ Y, int64_t ldy, Acc alpha, Acc beta) {
-    const int lane = threadIdx.x % kWave;
-    const int64_t row = static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + threadIdx.x / kWave;
-    if (row >= M) return;
+// Tall-skinny product for narrow outputs (N <= 16): each block owns 64 rows;
+// A is staged through LDS in 64-wide K tiles with loads coalesced along the
+// contiguous dimension of A (k for row-major, rows for column-major); four
+// threads share a row, split the K tile and combine with two xor-shuffles.
+template <typename V, typename TB, typename T, typename Acc, int NMAX, bool AROW>
+__global__ __launch_bounds__(kThreads) void ts_gemm_nn_kernel(
+    int64_t M, int N, int64_t K, const V* __restrict__ A, int64_t rs, int64_t cs, const TB* __restrict__ B,
+    int64_t ldb, T* __restrict__ Y, int64_t ldy, Acc alpha, Acc beta, const double* gate) {
+    if (gated_off(gate)) return;
+    constexpr int BM = 64, KT = 64;
+    __shared__ Acc As[BM][KT + 1];
+    __shared__ Acc Bs[KT][NMAX];
+    const int t = threadIdx.x;
+    const int r = t / 4, q = t % 4;
+    const int64_t row0 = static_cast<int64_t>(blockIdx.x) * BM;
     Acc acc[NMAX];
 #pragma unroll
     for (int j = 0; j < NMAX; ++j) acc[j] = Acc(0);
-    const V* arow = A + row * rs;
-    for (int64_t k = lane; k < K; k += kWave) {
-        const Acc a = ld_acc<Acc>(arow, k);
+    for (int64_t k0 = 0; k0 < K; k0 += KT) {
 #pragma unroll
-        for (int j = 0; j < NMAX; ++j)
-            if (j < N) acc[j] = fma(a, static_cast<Acc>(B[k * ldb + j]), acc[j]);
-    }
-#pragma unroll
-    for (int j = 0; j < NMAX; ++j) {
-        Acc v = acc[j];
-#pragma unroll
-        for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
-        acc[j] = v;
-    }
-    if (lane < N) {
-        Acc v = Acc(0);
-#pragma unroll
-        for (int j = 0; j < NMAX; ++j)
-            if (j == lane) v = acc[j];
-        T* out = Y + row * ldy + lane;
-        Acc r = alpha * v;
-        if (beta != Acc(0)) r = fma(beta, static_cast<Acc>(*out), r);
-        *out = static_cast<T>(r);
-    }
-}
-
-// Narrow (N <= 8) column-major A: one thread per output row, loop over K
-// (consecutive threads read consecutive rows of each column: coalesced).
-template <typename V, typename TB, typename T, typename Acc, int NMAX>
-__global__ __launch_bounds__(kThreads) void gemm_nn_narrow_colmajor_kernel(
-    int64_t M, int N, int64_t K, const V* __restrict__ A, int64_t cs, const TB* __restrict__ B,
-    int64_t ldb, T* __restrict__ Y, int64_t ldy, Acc alpha, Acc beta) {
-    __shared__ Acc Bs[64][NMAX];
-    const int64_t row = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
-    Acc acc[NMAX];
-#pragma unroll
-    for (int j = 0; j < NMAX; ++j) acc[j] = Acc(0);
-    for (int64_t k0 = 0; k0 < K; k0 += 64) {
-        __syncthreads();
-        for (int idx = threadIdx.x; idx < 64 * NMAX; idx += kThreads) {
+        for (int l = 0; l < (BM * KT) / kThreads; ++l) {
+            const int idx = t + l * kThreads;
+            int rr, kk;
+            if (AROW) { kk = idx % KT; rr = idx / KT; } else { rr = idx % BM; kk = idx / BM; }
+            const int64_t gr = row0 + rr, gk = k0 + kk;
+            As[rr][kk] = (gr < M && gk < K) ? ld_acc<Acc>(A, gr * rs + gk * cs) : Acc(0);
+        }
+        for (int idx = t; idx < KT * NMAX; idx += kThreads) {
             const int kk = idx / NMAX, j = idx % NMAX;
             const int64_t gk = k0 + kk;
             Bs[kk][j] = (gk < K && j < N) ? static_cast<Acc>(B[gk * ldb + j]) : Acc(0);
         }
         __syncthreads();
-        if (row < M) {
-            const int kend = static_cast<int>(std::min<int64_t>(64, K - k0));
-            for (int kk = 0; kk < kend; ++kk) {
-                const Acc a = ld_acc<Acc>(A, row + (k0 + kk) * cs);
+#pragma unroll 4
+        for (int kk = q * (KT / 4); kk < (q + 1) * (KT / 4); ++kk) {
+            const Acc a = As[r][kk];
 #pragma unroll
-                for (int j = 0; j < NMAX; ++j) acc[j] = fma(a, Bs[kk][j], acc[j]);
-            }
+            for (int j = 0; j < NMAX; ++j) acc[j] = fma(a, Bs[kk][j], acc[j]);
         }
+        __syncthreads();
     }
-    if (row >= M) return;
-    for (int j = 0; j < N; ++j) {
-        T* out = Y + row * ldy + j;
-        Acc r = alpha * acc[j];
-        if (beta != Acc(0)) r = fma(beta, static_cast<Acc>(*out), r);
-        *out = static_cast<T>(r);
+#pragma unroll
+    for (int j = 0; j < NMAX; ++j) {
+        acc[j] += __shfl_xor(acc[j], 1);
+        acc[j] += __shfl_xor(acc[j], 2);
+    }
+    const int64_t gr = row0 + r;
+    if (gr >= M) return;
+#pragma unroll
+    for (int j = 0; j < NMAX; ++j) {
+        if (j % 4 != q || j >= N) continue;
+        T* out = Y + gr * ldy + j;
+        Acc v = alpha * acc[j];
+        if (beta != Acc(0)) v = fma(beta, static_cast<Acc>(*out), v);
+        *out = static_cast<T>(v);
     }
 }
 
 template <typename V, typename TB, typename T>
 void gemm_nn_dispatch(hipStream_t s, int64_t M, int N, int64_t K, const V* A, int64_t rs,
                       int64_t cs, const TB* B, int64_t ldb, T* Y, int64_t ldy, double alpha,
-                      double beta) {
+                      double beta, const double* gate) {
     using Acc = T;
     if (M <= 0 || N <= 0) return;
     const bool row_major = (cs == 1);
-    if (N <= 8) {
-        if (row_major) {
-            dim3 grid(static_cast<unsigned>(ceil_div(M, kThreads / kWave)));
-            hipLaunchKernelGGL((gemm_nn_narrow_rowmajor_kernel<V, TB, T, Acc, 8>), grid, dim3(kThreads), 0, s,
-                               M, N, K, A, rs, B, ldb, Y, ldy, Acc(alpha), Acc(beta));
-        } else {
-            dim3 grid(static_cast<unsigned>(ceil_div(M, kThreads)));
-            hipLaunchKernelGGL((gemm_nn_narrow_colmajor_kernel<V, TB, T, Acc, 8>), grid, dim3(kThreads), 0, s,
-                               M, N, K, A, cs, B, ldb, Y, ldy, Acc(alpha), Acc(beta));
-        }
+    if (N <= 16) {
+        const dim3 grid(static_cast<unsigned>(ceil_div(M, 64)));
+#define AMFLS_TSNN(NM)                                                                                     \
+        if (row_major)                                                                                     \
+            hipLaunchKernelGGL((ts_gemm_nn_kernel<V, TB, T, Acc, NM, true>), grid, dim3(kThreads), 0, s, M, N, K, \
+                               A, rs, cs, B, ldb, Y, ldy, Acc(alpha), Acc(beta), gate);                          \
+        else                                                                                               \
+            hipLaunchKernelGGL((ts_gemm_nn_kernel<V, TB, T, Acc, NM, false>), grid, dim3(kThreads), 0, s, M, N, K, \
+                               A, rs, cs, B, ldb, Y, ldy, Acc(alpha), Acc(beta), gate);
+        if (N <= 1) { AMFLS_TSNN(1) }
+        else if (N <= 2) { AMFLS_TSNN(2) }
+        else if (N <= 4) { AMFLS_TSNN(4) }
+        else if (N <= 8) { AMFLS_TSNN(8) }
+        else { AMFLS_TSNN(16) }
+#undef AMFLS_TSNN
     } else {
         dim3 grid(static_cast<unsigned>(ceil_div(M, 64)), static_cast<unsigned>(ceil_div(N, 64)));
         if (row_major)
             hipLaunchKernelGGL((gemm_nn_tile_kernel<V, TB, T, Acc, true>), grid, dim3(kThreads), 0, s, M, N,
-                               K, A, rs, cs, B, ldb, Y, ldy, Acc(alpha), Acc(beta));
+                               K, A, rs, cs, B, ldb, Y, ldy, Acc(alpha), Acc(beta), gate);
         else
             hipLaunchKernelGGL((gemm_nn_tile_kernel<V, TB, T, Acc, false>), grid, dim3(kThreads), 0, s, M, N,
-                               K, A, rs, cs, B, ldb, Y, ldy, Acc(alpha), Acc(beta));
+                               K, A, rs, cs, B, ldb, Y, ldy, Acc(alpha), Acc(beta), gate);
     }
     AMFLS_LAUNCH_CHECK();
 }
@@ -216,7 +211,8 @@ void gemm_nn_dispatch(hipStream_t s, int64_t M, int N, int64_t K, const V* A, in
 template <typename V, typename TB, bool AROW>
 __global__ __launch_bounds__(kThreads) void gemm_tn_tile_kernel(
     int64_t M, int64_t K, int N, const V* __restrict__ A, int64_t rs, int64_t cs,
-    const TB* __restrict__ B, int64_t ldb, double* __restrict__ P, int64_t rows_per_chunk) {
+    const TB* __restrict__ B, int64_t ldb, double* __restrict__ P, int64_t rows_per_chunk, const double* gate) {
+    if (gated_off(gate)) return;
     constexpr int BK = 64, BN = 64, BR = 32;
     __shared__ double As[BR][BK + 1];
     __shared__ double Bs[BR][BN];
@@ -276,141 +272,137 @@ __global__ __launch_bounds__(kThreads) void gemm_tn_tile_kernel(
     }
 }
 
-// Narrow N (<= 8), row-major A: thread per k column, loop over the chunk rows.
-template <typename V, typename TB, int NMAX>
-__global__ __launch_bounds__(kThreads) void gemm_tn_narrow_rowmajor_kernel(
-    int64_t M, int64_t K, int N, const V* __restrict__ A, int64_t rs, const TB* __restrict__ B,
-    int64_t ldb, double* __restrict__ P, int64_t rows_per_chunk) {
-    __shared__ double Bs[64][NMAX];
-    const int64_t k = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
-    const int64_t chunk = blockIdx.y;
-    const int64_t r_begin = chunk * rows_per_chunk;
-    const int64_t r_end = std::min<int64_t>(M, r_begin + rows_per_chunk);
-    double acc[NMAX];
+// Tall-skinny C = A^T B for K * N <= 4096: each block reduces a contiguous
+// range of rows into all K x N outputs (per-thread accumulators), staging
+// BR x K rows of A and BR x N rows of B through LDS with coalesced loads.
+template <typename V, typename TB, int ACC, bool AROW>
+__global__ __launch_bounds__(kThreads) void ts_gemm_tn_kernel(
+    int64_t M, int K, int N, const V* __restrict__ A, int64_t rs, int64_t cs, const TB* __restrict__ B,
+    int64_t ldb, double* __restrict__ P, int64_t rows_per_block, int BR, const double* gate) {
+    if (gated_off(gate)) return;
+    extern __shared__ double smem[];
+    double* As = smem;            // BR x K
+    double* Bs = smem + BR * K;   // BR x N
+    const int t = threadIdx.x;
+    const int KN = K * N;
+    const int64_t r_begin = static_cast<int64_t>(blockIdx.x) * rows_per_block;
+    const int64_t r_end = std::min<int64_t>(M, r_begin + rows_per_block);
+    double acc[ACC];
 #pragma unroll
-    for (int j = 0; j < NMAX; ++j) acc[j] = 0.0;
-    for (int64_t r0 = r_begin; r0 < r_end; r0 += 64) {
-        __syncthreads();
-        for (int idx = threadIdx.x; idx < 64 * NMAX; idx += kThreads) {
-            const int rr = idx / NMAX, j = idx % NMAX;
-            const int64_t gr = r0 + rr;
-            Bs[rr][j] = (gr < r_end && j < N) ? static_cast<double>(B[gr * ldb + j]) : 0.0;
+    for (int a = 0; a < ACC; ++a) acc[a] = 0.0;
+    for (int64_t r0 = r_begin; r0 < r_end; r0 += BR) {
+        const int nr = static_cast<int>(std::min<int64_t>(BR, r_end - r0));
+        for (int idx = t; idx < BR * K; idx += kThreads) {
+            int rr, kk;
+            if (AROW) { rr = idx / K; kk = idx % K; } else { rr = idx % BR; kk = idx / BR; }
+            As[rr * K + kk] = rr < nr ? ld_d(A, (r0 + rr) * rs + static_cast<int64_t>(kk) * cs) : 0.0;
+        }
+        for (int idx = t; idx < BR * N; idx += kThreads) {
+            const int rr = idx / N, j = idx % N;
+            Bs[rr * N + j] = rr < nr ? static_cast<double>(B[(r0 + rr) * ldb + j]) : 0.0;
         }
         __syncthreads();
-        if (k < K) {
-            const int rend = static_cast<int>(std::min<int64_t>(64, r_end - r0));
-            for (int rr = 0; rr < rend; ++rr) {
-                const double a = ld_d(A, (r0 + rr) * rs + k);
 #pragma unroll
-                for (int j = 0; j < NMAX; ++j) acc[j] = fma(a, Bs[rr][j], acc[j]);
+        for (int a = 0; a < ACC; ++a) {
+            const int o = t + a * kThreads;
+            if (o < KN) {
+                const int kk = o / N, j = o % N;
+                double v = acc[a];
+                for (int rr = 0; rr < nr; ++rr) v = fma(As[rr * K + kk], Bs[rr * N + j], v);
+                acc[a] = v;
             }
         }
-    }
-    if (k >= K) return;
-    for (int j = 0; j < N; ++j) P[(chunk * K + k) * N + j] = acc[j];
-}
-
-// Narrow N (<= 8), column-major A: one wavefront per k column, lanes across
-// the chunk rows, wave reduction.
-template <typename V, typename TB, int NMAX>
-__global__ __launch_bounds__(kThreads) void gemm_tn_narrow_colmajor_kernel(
-    int64_t M, int64_t K, int N, const V* __restrict__ A, int64_t cs, const TB* __restrict__ B,
-    int64_t ldb, double* __restrict__ P, int64_t rows_per_chunk) {
-    const int lane = threadIdx.x % kWave;
-    const int64_t k = static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + threadIdx.x / kWave;
-    const int64_t chunk = blockIdx.y;
-    if (k >= K) return;
-    const int64_t r_begin = chunk * rows_per_chunk;
-    const int64_t r_end = std::min<int64_t>(M, r_begin + rows_per_chunk);
-    double acc[NMAX];
-#pragma unroll
-    for (int j = 0; j < NMAX; ++j) acc[j] = 0.0;
-    const V* acol = A + k * cs;
-    for (int64_t r = r_begin + lane; r < r_end; r += kWave) {
-        const double a = ld_d(acol, r);
-#pragma unroll
-        for (int j = 0; j < NMAX; ++j)
-            if (j < N) acc[j] = fma(a, static_cast<double>(B[r * ldb + j]), acc[j]);
+        __syncthreads();
     }
 #pragma unroll
-    for (int j = 0; j < NMAX; ++j) {
-        double v = acc[j];
-#pragma unroll
-        for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
-        acc[j] = v;
-    }
-    if (lane < N) {
-        double v = 0.0;
-#pragma unroll
-        for (int j = 0; j < NMAX; ++j)
-            if (j == lane) v = acc[j];
-        P[(chunk * K + k) * N + lane] = v;
+    for (int a = 0; a < ACC; ++a) {
+        const int o = t + a * kThreads;
+        if (o < KN) P[static_cast<int64_t>(blockIdx.x) * KN + o] = acc[a];
     }
 }
 
+// out[o] = alpha * sum_b P[b][o]: one wavefront per output, lanes over blocks.
 template <typename TC>
-__global__ void reduce_chunks_kernel(int64_t chunks, int64_t K, int N, const double* __restrict__ P,
-                                     TC* __restrict__ C, int64_t ldc, double alpha) {
-    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (idx >= K * N) return;
-    const int64_t k = idx / N;
-    const int n = static_cast<int>(idx % N);
-    double sum = 0.0;
-    for (int64_t c = 0; c < chunks; ++c) sum += P[(c * K + k) * N + n];
-    C[k * ldc + n] = static_cast<TC>(alpha * sum);
+__global__ __launch_bounds__(kThreads) void reduce_partials_kernel(int64_t nb, int64_t KN, int N, const double* P,
+                                                                   TC* C, int64_t ldc, double alpha,
+                                                                   const double* gate) {
+    if (gated_off(gate)) return;
+    const int lane = threadIdx.x % kWave;
+    const int64_t o = static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + threadIdx.x / kWave;
+    if (o >= KN) return;
+    double v = 0.0;
+    for (int64_t b = lane; b < nb; b += kWave) v += P[b * KN + o];
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if (lane == 0) C[(o / N) * ldc + (o % N)] = static_cast<TC>(alpha * v);
 }
 
 template <typename V, typename TB>
 void gemm_tn_dispatch(hipStream_t s, int64_t M, int64_t K, int N, const V* A, int64_t rs,
                       int64_t cs, const TB* B, int64_t ldb, void* C, ValueType ctype, int64_t ldc,
-                      double alpha, Scratch& scratch) {
+                      double alpha, Scratch& scratch, const double* gate) {
     if (K <= 0 || N <= 0) return;
     const bool row_major = (cs == 1);
-    int64_t col_blocks;
-    if (N <= 8) {
-        col_blocks = row_major ? ceil_div(K, kThreads) : ceil_div(K, kThreads / kWave);
-    } else {
-        col_blocks = ceil_div(K, 64) * ceil_div(N, 64);
+    const int64_t KN = K * N;
+    if (KN <= 4096 && K <= 4096) {
+        // ~512 blocks, each a contiguous row range
+        int64_t nb = std::max<int64_t>(1, std::min<int64_t>(512, ceil_div(std::max<int64_t>(M, 1), 32)));
+        const int64_t rows_per_block = ceil_div(std::max<int64_t>(M, 1), nb);
+        nb = ceil_div(std::max<int64_t>(M, 1), rows_per_block);
+        int BR = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(64, 6144 / (K + N))));
+        const size_t lds = static_cast<size_t>(BR) * (K + N) * sizeof(double);
+        scratch.ensure(static_cast<size_t>(nb * KN) * sizeof(double));
+        double* P = static_cast<double*>(scratch.ptr);
+        const int acc = static_cast<int>(ceil_div(KN, kThreads));
+        const dim3 grid(static_cast<unsigned>(nb));
+#define AMFLS_TSTN(NA)                                                                                       \
+        if (row_major)                                                                                       \
+            hipLaunchKernelGGL((ts_gemm_tn_kernel<V, TB, NA, true>), grid, dim3(kThreads), lds, s, M,          \
+                               static_cast<int>(K), N, A, rs, cs, B, ldb, P, rows_per_block, BR, gate);           \
+        else                                                                                                 \
+            hipLaunchKernelGGL((ts_gemm_tn_kernel<V, TB, NA, false>), grid, dim3(kThreads), lds, s, M,         \
+                               static_cast<int>(K), N, A, rs, cs, B, ldb, P, rows_per_block, BR, gate);
+        if (acc <= 1) { AMFLS_TSTN(1) }
+        else if (acc <= 2) { AMFLS_TSTN(2) }
+        else if (acc <= 4) { AMFLS_TSTN(4) }
+        else if (acc <= 8) { AMFLS_TSTN(8) }
+        else { AMFLS_TSTN(16) }
+#undef AMFLS_TSTN
+        AMFLS_LAUNCH_CHECK();
+        const dim3 g2(static_cast<unsigned>(ceil_div(KN, kThreads / kWave)));
+        if (ctype == ValueType::f64)
+            hipLaunchKernelGGL(reduce_partials_kernel<double>, g2, dim3(kThreads), 0, s, nb, KN, N, P,
+                               static_cast<double*>(C), ldc, alpha, gate);
+        else
+            hipLaunchKernelGGL(reduce_partials_kernel<float>, g2, dim3(kThreads), 0, s, nb, KN, N, P,
+                               static_cast<float*>(C), ldc, alpha, gate);
+        AMFLS_LAUNCH_CHECK();
+        return;
     }
-    // Enough chunks along M for >= ~2048 blocks, each chunk >= 256 rows.
-    int64_t chunks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(M, 256), ceil_div(2048, col_blocks)));
+    // wide outputs: 64 x 64 output tiles, split over row chunks
+    const int64_t col_blocks = ceil_div(K, 64) * ceil_div(N, 64);
+    int64_t chunks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(M, 256), ceil_div(1024, col_blocks)));
     chunks = std::min<int64_t>(chunks, 65535);
     const int64_t rows_per_chunk = std::max<int64_t>(1, ceil_div(std::max<int64_t>(M, 1), chunks));
     chunks = std::max<int64_t>(1, ceil_div(std::max<int64_t>(M, 1), rows_per_chunk));
-    scratch.ensure(static_cast<size_t>(chunks * K * N) * sizeof(double));
+    scratch.ensure(static_cast<size_t>(chunks * KN) * sizeof(double));
     double* P = static_cast<double*>(scratch.ptr);
-    if (M <= 0) {
-        AMFLS_HIP_CHECK(hipMemsetAsync(P, 0, static_cast<size_t>(K * N) * sizeof(double), s));
-        chunks = 1;
-    } else if (N <= 8) {
-        if (row_major) {
-            dim3 grid(static_cast<unsigned>(ceil_div(K, kThreads)), static_cast<unsigned>(chunks));
-            hipLaunchKernelGGL((gemm_tn_narrow_rowmajor_kernel<V, TB, 8>), grid, dim3(kThreads), 0, s, M, K,
-                               N, A, rs, B, ldb, P, rows_per_chunk);
-        } else {
-            dim3 grid(static_cast<unsigned>(ceil_div(K, kThreads / kWave)), static_cast<unsigned>(chunks));
-            hipLaunchKernelGGL((gemm_tn_narrow_colmajor_kernel<V, TB, 8>), grid, dim3(kThreads), 0, s, M, K,
-                               N, A, cs, B, ldb, P, rows_per_chunk);
-        }
-    } else {
-        dim3 grid(static_cast<unsigned>(ceil_div(K, 64)), static_cast<unsigned>(ceil_div(N, 64)),
-                  static_cast<unsigned>(chunks));
-        if (row_major)
-            hipLaunchKernelGGL((gemm_tn_tile_kernel<V, TB, true>), grid, dim3(kThreads), 0, s, M, K, N, A, rs,
-                               cs, B, ldb, P, rows_per_chunk);
-        else
-            hipLaunchKernelGGL((gemm_tn_tile_kernel<V, TB, false>), grid, dim3(kThreads), 0, s, M, K, N, A,
-                               rs, cs, B, ldb, P, rows_per_chunk);
-    }
-    AMFLS_LAUNCH_CHECK();
-    const int64_t total = K * N;
-    dim3 g2(static_cast<unsigned>(ceil_div(total, kThreads)));
-    if (ctype == ValueType::f64)
-        hipLaunchKernelGGL(reduce_chunks_kernel<double>, g2, dim3(kThreads), 0, s, chunks, K, N, P,
-                           static_cast<double*>(C), ldc, alpha);
+    dim3 grid(static_cast<unsigned>(ceil_div(K, 64)), static_cast<unsigned>(ceil_div(N, 64)),
+              static_cast<unsigned>(chunks));
+    if (row_major)
+        hipLaunchKernelGGL((gemm_tn_tile_kernel<V, TB, true>), grid, dim3(kThreads), 0, s, M, K, N, A, rs,
+                           cs, B, ldb, P, rows_per_chunk, gate);
     else
-        hipLaunchKernelGGL(reduce_chunks_kernel<float>, g2, dim3(kThreads), 0, s, chunks, K, N, P,
-                           static_cast<float*>(C), ldc, alpha);
+        hipLaunchKernelGGL((gemm_tn_tile_kernel<V, TB, false>), grid, dim3(kThreads), 0, s, M, K, N, A,
+                           rs, cs, B, ldb, P, rows_per_chunk, gate);
+    AMFLS_LAUNCH_CHECK();
+    const dim3 g2(static_cast<unsigned>(ceil_div(KN, kThreads / kWave)));
+    if (ctype == ValueType::f64)
+        hipLaunchKernelGGL(reduce_partials_kernel<double>, g2, dim3(kThreads), 0, s, chunks, KN, N, P,
+                           static_cast<double*>(C), ldc, alpha, gate);
+    else
+        hipLaunchKernelGGL(reduce_partials_kernel<float>, g2, dim3(kThreads), 0, s, chunks, KN, N, P,
+                           static_cast<float*>(C), ldc, alpha, gate);
     AMFLS_LAUNCH_CHECK();
 }
 
@@ -478,7 +470,8 @@ __global__ void fma_kernel(int64_t n, double alpha, const double* x, const doubl
     if (i < n) y[i] = fma(alpha, x[i], z[i]);
 }
 
-__global__ void add_kernel(int64_t n, const double* x, double* y) {
+__global__ void add_kernel(int64_t n, const double* x, double* y, const double* gate) {
+    if (gated_off(gate)) return;
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) y[i] += x[i];
 }
@@ -499,100 +492,127 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Pass 1: per-block max |v| over finite values and nonfinite count.
+// Device-scope load of a value another workgroup published before its
+// release fence and ticket (bypasses stale per-XCD cache lines).
+__device__ __forceinline__ double coherent_load(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// One-launch norm statistics.  Each block reduces its grid-stride share to
+// (max |v|, sum (|v| 2^-e_b)^2, nonfinite count) with e_b the frexp exponent
+// of the block's own max; the last block to finish (ticket on `counter`)
+// rescales every partial sum to the exponent e of the global max, exactly
+// since the factors 2^(2(e_b - e)) are powers of two, and writes out[0..2].
+// The last block also resets the counter, so consecutive launches on one
+// stream reuse it.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void stats_max_kernel(int64_t rows, int cols, const T* X,
-                                                             int64_t ldx, double* partial) {
-    __shared__ double smax[kThreads / kWave];
-    __shared__ double scnt[kThreads / kWave];
+__global__ __launch_bounds__(kThreads) void stats_kernel(int64_t rows, int cols, const T* __restrict__ X,
+                                                         int64_t ldx, double* partial,
+                                                         double* __restrict__ out, unsigned* counter,
+                                                         const double* gate) {
+    if (gated_off(gate)) return;
+    __shared__ double sm[kThreads / kWave][3];
+    __shared__ bool last;
     const int64_t total = rows * cols;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kThreads;
+    const int64_t first = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    // pass over the block's share: max and nonfinite count
     double m = 0.0, bad = 0.0;
-    for (int64_t idx = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; idx < total;
-         idx += static_cast<int64_t>(gridDim.x) * kThreads) {
-        const int64_t i = idx / cols;
-        const int j = static_cast<int>(idx % cols);
+    for (int64_t idx = first; idx < total; idx += stride) {
+        const int64_t i = cols == 1 ? idx : idx / cols;
+        const int j = cols == 1 ? 0 : static_cast<int>(idx % cols);
         const double v = fabs(static_cast<double>(X[i * ldx + j]));
         if (isfinite(v)) m = fmax(m, v); else bad += 1.0;
     }
     m = wave_max(m);
     bad = wave_sum(bad);
-    const int w = threadIdx.x / kWave;
-    if (threadIdx.x % kWave == 0) { smax[w] = m; scnt[w] = bad; }
+    if (lane == 0) { sm[wv][0] = m; sm[wv][2] = bad; }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double mm = 0.0, bb = 0.0;
-        for (int k = 0; k < kThreads / kWave; ++k) { mm = fmax(mm, smax[k]); bb += scnt[k]; }
-        partial[2 * blockIdx.x] = mm;
-        partial[2 * blockIdx.x + 1] = bb;
-    }
-}
-
-__global__ void stats_max_final_kernel(int blocks, const double* partial, double* out) {
-    __shared__ double smax[kThreads / kWave];
-    __shared__ double scnt[kThreads / kWave];
-    double m = 0.0, bad = 0.0;
-    for (int b = threadIdx.x; b < blocks; b += kThreads) {
-        m = fmax(m, partial[2 * b]);
-        bad += partial[2 * b + 1];
-    }
-    m = wave_max(m);
-    bad = wave_sum(bad);
-    const int w = threadIdx.x / kWave;
-    if (threadIdx.x % kWave == 0) { smax[w] = m; scnt[w] = bad; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double mm = 0.0, bb = 0.0;
-        for (int k = 0; k < kThreads / kWave; ++k) { mm = fmax(mm, smax[k]); bb += scnt[k]; }
-        out[0] = mm;
-        out[2] = bb;
-    }
-}
-
-// Pass 2: sum of (|v| * 2^-e)^2 where frexp(max) = (f, e).
-template <typename T>
-__global__ __launch_bounds__(kThreads) void stats_sum_kernel(int64_t rows, int cols, const T* X,
-                                                             int64_t ldx, const double* out,
-                                                             double* partial) {
-    __shared__ double ssum[kThreads / kWave];
-    const double mx = out[0];
-    int e = 0;
-    if (mx > 0.0 && isfinite(mx)) (void)frexp(mx, &e);
-    const int64_t total = rows * cols;
+    double bm = 0.0;
+#pragma unroll
+    for (int k = 0; k < kThreads / kWave; ++k) bm = fmax(bm, sm[k][0]);
+    int eb = 0;
+    if (bm > 0.0) (void)frexp(bm, &eb);
+    // second sweep of the same (cache-resident) share: scaled squares
     double acc = 0.0;
-    for (int64_t idx = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; idx < total;
-         idx += static_cast<int64_t>(gridDim.x) * kThreads) {
-        const int64_t i = idx / cols;
-        const int j = static_cast<int>(idx % cols);
+    for (int64_t idx = first; idx < total; idx += stride) {
+        const int64_t i = cols == 1 ? idx : idx / cols;
+        const int j = cols == 1 ? 0 : static_cast<int>(idx % cols);
         const double v = fabs(static_cast<double>(X[i * ldx + j]));
         if (isfinite(v)) {
-            const double sv = ldexp(v, -e);
+            const double sv = ldexp(v, -eb);
             acc = fma(sv, sv, acc);
         }
     }
     acc = wave_sum(acc);
-    const int w = threadIdx.x / kWave;
-    if (threadIdx.x % kWave == 0) ssum[w] = acc;
+    if (lane == 0) sm[wv][1] = acc;
     __syncthreads();
     if (threadIdx.x == 0) {
-        double ss = 0.0;
-        for (int k = 0; k < kThreads / kWave; ++k) ss += ssum[k];
-        partial[blockIdx.x] = ss;
+        double ss = 0.0, bb = 0.0;
+        for (int k = 0; k < kThreads / kWave; ++k) { ss += sm[k][1]; bb += sm[k][2]; }
+        partial[3 * blockIdx.x] = bm;
+        partial[3 * blockIdx.x + 1] = ss;
+        partial[3 * blockIdx.x + 2] = bb;
+        __threadfence();
+        const unsigned ticket = atomicAdd(counter, 1u);
+        last = ticket == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    const int nb = static_cast<int>(gridDim.x);
+    double gm = 0.0;
+    for (int b = threadIdx.x; b < nb; b += kThreads) gm = fmax(gm, coherent_load(&partial[3 * b]));
+    gm = wave_max(gm);
+    __syncthreads();
+    if (lane == 0) sm[wv][0] = gm;
+    __syncthreads();
+    gm = 0.0;
+#pragma unroll
+    for (int k = 0; k < kThreads / kWave; ++k) gm = fmax(gm, sm[k][0]);
+    int e = 0;
+    if (gm > 0.0) (void)frexp(gm, &e);
+    double tot = 0.0, tbad = 0.0;
+    for (int b = threadIdx.x; b < nb; b += kThreads) {
+        const double pm = coherent_load(&partial[3 * b]);
+        const double ps = coherent_load(&partial[3 * b + 1]);
+        tbad += coherent_load(&partial[3 * b + 2]);
+        if (pm > 0.0) {
+            int ebb = 0;
+            (void)frexp(pm, &ebb);
+            tot += ldexp(ps, 2 * (ebb - e));
+        }
+    }
+    tot = wave_sum(tot);
+    tbad = wave_sum(tbad);
+    if (lane == 0) { sm[wv][1] = tot; sm[wv][2] = tbad; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double ss = 0.0, bb = 0.0;
+        for (int k = 0; k < kThreads / kWave; ++k) { ss += sm[k][1]; bb += sm[k][2]; }
+        out[0] = gm;
+        out[1] = ss;
+        out[2] = bb;
+        *counter = 0u;
     }
 }
 
-__global__ void stats_sum_final_kernel(int blocks, const double* partial, double* out) {
-    __shared__ double ssum[kThreads / kWave];
-    double acc = 0.0;
-    for (int b = threadIdx.x; b < blocks; b += kThreads) acc += partial[b];
-    acc = wave_sum(acc);
-    const int w = threadIdx.x / kWave;
-    if (threadIdx.x % kWave == 0) ssum[w] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double ss = 0.0;
-        for (int k = 0; k < kThreads / kWave; ++k) ss += ssum[k];
-        out[1] = ss;
-    }
+// gate = 1 when the DGKS criterion asks for a second orthogonalization pass:
+// both norms finite and ||after|| < factor * ||before|| (norms from the
+// statistics triples exactly as the host forms them).
+__device__ double nominal_norm_dev(const double* st) {
+    if (st[2] > 0.0) return __builtin_nan("");
+    if (st[0] == 0.0) return 0.0;
+    int e = 0;
+    (void)frexp(st[0], &e);
+    return ldexp(sqrt(st[1]), e);
+}
+
+__global__ void second_pass_gate_kernel(const double* before, const double* after, double factor, double* gate) {
+    if (threadIdx.x != 0) return;
+    const double nb = nominal_norm_dev(before), na = nominal_norm_dev(after);
+    *gate = (isfinite(nb) && isfinite(na) && na < factor * nb) ? 1.0 : 0.0;
 }
 
 // ------------------------------------------------------------------ RNG
@@ -746,39 +766,39 @@ __global__ void misc_final_kernel(int64_t rows, const double* misc_partial, doub
 // ---------------------------------------------------------------- launchers
 template <typename T, typename TB>
 void gemm_nn(hipStream_t s, int64_t M, int N, int64_t K, const Strided& A, const TB* B, int64_t ldb,
-             T* Y, int64_t ldy, double alpha, double beta) {
+             T* Y, int64_t ldy, double alpha, double beta, const double* gate) {
     switch (A.type) {
         case ValueType::f64:
             gemm_nn_dispatch<double, TB, T>(s, M, N, K, static_cast<const double*>(A.ptr), A.rs, A.cs, B,
-                                            ldb, Y, ldy, alpha, beta);
+                                            ldb, Y, ldy, alpha, beta, gate);
             break;
         case ValueType::f32:
             gemm_nn_dispatch<float, TB, T>(s, M, N, K, static_cast<const float*>(A.ptr), A.rs, A.cs, B,
-                                           ldb, Y, ldy, alpha, beta);
+                                           ldb, Y, ldy, alpha, beta, gate);
             break;
         case ValueType::bf16:
             gemm_nn_dispatch<__hip_bfloat16, TB, T>(s, M, N, K,
                                                     static_cast<const __hip_bfloat16*>(A.ptr), A.rs,
-                                                    A.cs, B, ldb, Y, ldy, alpha, beta);
+                                                    A.cs, B, ldb, Y, ldy, alpha, beta, gate);
             break;
     }
 }
 
 template <typename TB>
 void gemm_tn(hipStream_t s, int64_t M, int64_t K, int N, const Strided& A, const TB* B, int64_t ldb,
-             void* C, ValueType ctype, int64_t ldc, double alpha, Scratch& scratch) {
+             void* C, ValueType ctype, int64_t ldc, double alpha, Scratch& scratch, const double* gate) {
     switch (A.type) {
         case ValueType::f64:
             gemm_tn_dispatch<double, TB>(s, M, K, N, static_cast<const double*>(A.ptr), A.rs, A.cs, B,
-                                         ldb, C, ctype, ldc, alpha, scratch);
+                                         ldb, C, ctype, ldc, alpha, scratch, gate);
             break;
         case ValueType::f32:
             gemm_tn_dispatch<float, TB>(s, M, K, N, static_cast<const float*>(A.ptr), A.rs, A.cs, B, ldb,
-                                        C, ctype, ldc, alpha, scratch);
+                                        C, ctype, ldc, alpha, scratch, gate);
             break;
         case ValueType::bf16:
             gemm_tn_dispatch<__hip_bfloat16, TB>(s, M, K, N, static_cast<const __hip_bfloat16*>(A.ptr),
-                                                 A.rs, A.cs, B, ldb, C, ctype, ldc, alpha, scratch);
+                                                 A.rs, A.cs, B, ldb, C, ctype, ldc, alpha, scratch, gate);
             break;
     }
 }
@@ -834,10 +854,10 @@ void fma_vec(hipStream_t s, int64_t n, double alpha, const double* x, const doub
     AMFLS_LAUNCH_CHECK();
 }
 
-void add_vec(hipStream_t s, int64_t n, const double* x, double* y) {
+void add_vec(hipStream_t s, int64_t n, const double* x, double* y, const double* gate) {
     if (n <= 0) return;
     hipLaunchKernelGGL(add_kernel, dim3(static_cast<unsigned>(ceil_div(n, kThreads))), dim3(kThreads), 0, s, n,
-                       x, y);
+                       x, y, gate);
     AMFLS_LAUNCH_CHECK();
 }
 
@@ -850,19 +870,17 @@ void scale_vec(hipStream_t s, int64_t n, double alpha, const double* x, double* 
 
 template <typename T>
 void vector_stats(hipStream_t s, int64_t rows, int cols, const T* X, int64_t ldx, double* out,
-                  Scratch& scratch) {
+                  Scratch& scratch, unsigned* counter, const double* gate) {
     const int64_t total = rows * cols;
     const int blocks = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(1024, ceil_div(total, kThreads * 4))));
-    scratch.ensure(static_cast<size_t>(2 * blocks) * sizeof(double));
-    double* partial = static_cast<double*>(scratch.ptr);
-    hipLaunchKernelGGL(stats_max_kernel<T>, dim3(blocks), dim3(kThreads), 0, s, rows, cols, X, ldx, partial);
+    scratch.ensure(static_cast<size_t>(3 * blocks) * sizeof(double));
+    hipLaunchKernelGGL(stats_kernel<T>, dim3(blocks), dim3(kThreads), 0, s, rows, cols, X, ldx,
+                       static_cast<double*>(scratch.ptr), out, counter, gate);
     AMFLS_LAUNCH_CHECK();
-    hipLaunchKernelGGL(stats_max_final_kernel, dim3(1), dim3(kThreads), 0, s, blocks, partial, out);
-    AMFLS_LAUNCH_CHECK();
-    hipLaunchKernelGGL(stats_sum_kernel<T>, dim3(blocks), dim3(kThreads), 0, s, rows, cols, X, ldx, out,
-                       partial);
-    AMFLS_LAUNCH_CHECK();
-    hipLaunchKernelGGL(stats_sum_final_kernel, dim3(1), dim3(kThreads), 0, s, blocks, partial, out);
+}
+
+void second_pass_gate(hipStream_t s, const double* before, const double* after, double factor, double* gate) {
+    hipLaunchKernelGGL(second_pass_gate_kernel, dim3(1), dim3(kWave), 0, s, before, after, factor, gate);
     AMFLS_LAUNCH_CHECK();
 }
 
@@ -924,17 +942,17 @@ void dense_stats(hipStream_t s, int64_t M, int64_t N, const Strided& A, double* 
 
 // ---------------------------------------------------------------- instantiations
 template void gemm_nn<double, double>(hipStream_t, int64_t, int, int64_t, const Strided&, const double*,
-                                      int64_t, double*, int64_t, double, double);
+                                      int64_t, double*, int64_t, double, double, const double*);
 template void gemm_nn<float, float>(hipStream_t, int64_t, int, int64_t, const Strided&, const float*,
-                                    int64_t, float*, int64_t, double, double);
+                                    int64_t, float*, int64_t, double, double, const double*);
 template void gemm_nn<float, double>(hipStream_t, int64_t, int, int64_t, const Strided&, const double*,
-                                     int64_t, float*, int64_t, double, double);
+                                     int64_t, float*, int64_t, double, double, const double*);
 template void gemm_nn<double, float>(hipStream_t, int64_t, int, int64_t, const Strided&, const float*,
-                                     int64_t, double*, int64_t, double, double);
+                                     int64_t, double*, int64_t, double, double, const double*);
 template void gemm_tn<double>(hipStream_t, int64_t, int64_t, int, const Strided&, const double*, int64_t,
-                              void*, ValueType, int64_t, double, Scratch&);
+                              void*, ValueType, int64_t, double, Scratch&, const double*);
 template void gemm_tn<float>(hipStream_t, int64_t, int64_t, int, const Strided&, const float*, int64_t,
-                             void*, ValueType, int64_t, double, Scratch&);
+                             void*, ValueType, int64_t, double, Scratch&, const double*);
 template void rowmul<double>(hipStream_t, int64_t, int, int, const double*, int64_t, const double*,
                              double*, int64_t);
 template void rowmul<float>(hipStream_t, int64_t, int, int, const float*, int64_t, const double*, float*,
@@ -952,8 +970,10 @@ template void axpy_block<float>(hipStream_t, int64_t, int, double, const float*,
 template void fill<double>(hipStream_t, double*, int64_t, double);
 template void fill<float>(hipStream_t, float*, int64_t, float);
 template void fill<int>(hipStream_t, int*, int64_t, int);
-template void vector_stats<double>(hipStream_t, int64_t, int, const double*, int64_t, double*, Scratch&);
-template void vector_stats<float>(hipStream_t, int64_t, int, const float*, int64_t, double*, Scratch&);
+template void vector_stats<double>(hipStream_t, int64_t, int, const double*, int64_t, double*, Scratch&,
+                                   unsigned*, const double*);
+template void vector_stats<float>(hipStream_t, int64_t, int, const float*, int64_t, double*, Scratch&,
+                                  unsigned*, const double*);
 template void gaussian_block<double>(hipStream_t, int64_t, int, uint64_t, uint64_t, uint64_t, double*,
                                      int64_t);
 template void gaussian_block<float>(hipStream_t, int64_t, int, uint64_t, uint64_t, uint64_t, float*,

@@ -34,14 +34,16 @@ struct Scratch {
 // Y(M x N, T row-major) = alpha * A(M x K) * B(K x N, TB row-major) + beta * Y.
 template <typename T, typename TB>
 void gemm_nn(hipStream_t s, int64_t M, int N, int64_t K, const Strided& A, const TB* B, int64_t ldb,
-             T* Y, int64_t ldy, double alpha, double beta);
+             T* Y, int64_t ldy, double alpha, double beta, const double* gate = nullptr);
 
 // C(K x N) = A(M x K)^T * B(M x N, TB row-major); accumulation in double with a
 // deterministic split-M reduction through `scratch`; C written as `ctype`
 // (double or float) row-major with leading dimension ldc, C = alpha * sum.
+// Launchers taking `gate` skip their work when *gate == 0 on the device.
 template <typename TB>
 void gemm_tn(hipStream_t s, int64_t M, int64_t K, int N, const Strided& A, const TB* B, int64_t ldb,
-             void* C, ValueType ctype, int64_t ldc, double alpha, Scratch& scratch);
+             void* C, ValueType ctype, int64_t ldc, double alpha, Scratch& scratch,
+             const double* gate = nullptr);
 
 // In-place safe row-local product: Y[i, 0:r] = X[i, 0:w] * M (w x r, double
 // row-major, device).  w, r <= 128.
@@ -61,15 +63,19 @@ void axpy_block(hipStream_t s, int64_t rows, int cols, double a, const T* X, int
 void sub_vec(hipStream_t s, int64_t n, const double* a, const double* b, double* r);
 // y = fma(alpha, x, z) elementwise, fused (matches std::fma).
 void fma_vec(hipStream_t s, int64_t n, double alpha, const double* x, const double* z, double* y);
-void add_vec(hipStream_t s, int64_t n, const double* x, double* y);
+void add_vec(hipStream_t s, int64_t n, const double* x, double* y, const double* gate = nullptr);
 void scale_vec(hipStream_t s, int64_t n, double alpha, const double* x, double* y);
 
 // Norm statistics of a block viewed as one vector (rows*cols values):
 // out[0] = max |v| over finite values, out[1] = sum (|v| 2^-e)^2 with e the
-// frexp exponent of out[0], out[2] = number of nonfinite values.
+// frexp exponent of out[0], out[2] = number of nonfinite values.  One
+// launch; `counter` is a device word that holds 0 between launches.
 template <typename T>
 void vector_stats(hipStream_t s, int64_t rows, int cols, const T* X, int64_t ldx, double* out_dev,
-                  Scratch& scratch);
+                  Scratch& scratch, unsigned* counter, const double* gate = nullptr);
+// *gate = 1 if both statistics triples give finite norms and
+// ||after|| < factor * ||before||, else 0 (device-side DGKS criterion).
+void second_pass_gate(hipStream_t s, const double* before, const double* after, double factor, double* gate);
 
 // Counter-addressed Gaussian block, index = (first_col + j) * rows + i.
 template <typename T>

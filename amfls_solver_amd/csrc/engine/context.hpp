@@ -2,6 +2,7 @@
 // transfers and the (wall or synthetic) clock used by the width controller.
 #pragma once
 
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <utility>
@@ -36,9 +37,12 @@ public:
         if (n > 0) AMFLS_HIP_CHECK(hipMalloc(&p_, n * sizeof(T)));
         n_ = n;
     }
+    // Grows geometrically; contents are not preserved.
     void ensure(size_t n) {
-        if (n > n_) reset(n);
+        if (n > n_) reset(std::max(n, 2 * n_));
     }
+    // hipFree waits for the device, so a buffer still read by queued
+    // kernels is never released early.
     void release() {
         if (p_) (void)hipFree(p_);
         p_ = nullptr;
@@ -78,19 +82,117 @@ private:
     double op_unit_ = 1e-3, local_unit_ = 1e-6, t_ = 0.0;
 };
 
+// Page-locked host memory (hipHostMalloc) for asynchronous small transfers.
+template <typename T>
+class PinnedBuf {
+public:
+    PinnedBuf() = default;
+    ~PinnedBuf() { release(); }
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+    void ensure(size_t n) {
+        if (n <= n_) return;
+        release();
+        AMFLS_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&p_), n * sizeof(T), hipHostMallocDefault));
+        n_ = n;
+    }
+    void release() {
+        if (p_) (void)hipHostFree(p_);
+        p_ = nullptr;
+        n_ = 0;
+    }
+    T* get() const { return p_; }
+    size_t size() const { return n_; }
+
+private:
+    T* p_ = nullptr;
+    size_t n_ = 0;
+};
+
+// GPU-side timing of operator products: an event pair around each product,
+// resolved (and added to its target counter) on flush().  Nothing waits on
+// the device between products; flush() waits for the last recorded event.
+class ProductTimer {
+public:
+    ~ProductTimer() {
+        for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
+    }
+    void begin(hipStream_t s) {
+        open_ = take();
+        AMFLS_HIP_CHECK(hipEventRecord(open_, s));
+    }
+    void end(hipStream_t s, double* target) {
+        hipEvent_t stop = take();
+        AMFLS_HIP_CHECK(hipEventRecord(stop, s));
+        pending_.push_back({open_, stop, target});
+        open_ = nullptr;
+    }
+    // Drop unresolved pairs (their targets may no longer exist).
+    void discard() {
+        for (const auto& p : pending_) {
+            pool_.push_back(p.start);
+            pool_.push_back(p.stop);
+        }
+        pending_.clear();
+        if (open_) pool_.push_back(open_);
+        open_ = nullptr;
+    }
+    void flush() {
+        if (pending_.empty()) return;
+        AMFLS_HIP_CHECK(hipEventSynchronize(pending_.back().stop));
+        for (const auto& p : pending_) {
+            float ms = 0.0f;
+            AMFLS_HIP_CHECK(hipEventElapsedTime(&ms, p.start, p.stop));
+            *p.target += 1e-3 * static_cast<double>(ms);
+            pool_.push_back(p.start);
+            pool_.push_back(p.stop);
+        }
+        pending_.clear();
+    }
+
+private:
+    struct Pending { hipEvent_t start, stop; double* target; };
+    hipEvent_t take() {
+        if (!pool_.empty()) {
+            hipEvent_t e = pool_.back();
+            pool_.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        AMFLS_HIP_CHECK(hipEventCreate(&e));
+        return e;
+    }
+    std::vector<hipEvent_t> pool_;
+    std::vector<Pending> pending_;
+    hipEvent_t open_ = nullptr;
+};
+
 struct Context {
     hipStream_t stream = nullptr;
     Scratch scratch;   // kernel-internal partials
     Scratch scratch2;  // a second arena for nested needs (transpose build)
     DevBuf<double> small;  // device staging for small matrices (coefficients, R^-1)
     DevBuf<double> stats;  // device results of reductions
+    DevBuf<unsigned> counter;  // ticket word of one-launch reductions (0 between launches)
+    PinnedBuf<double> stage;   // host side of asynchronous uploads
+    size_t stage_pos = 0;
+    PinnedBuf<char> landing;   // host side of small downloads
+    ProductTimer timer;
     Clock clock;
+
+    static constexpr size_t kStageDoubles = 1 << 20;
+    static constexpr size_t kLandingBytes = 1 << 20;
 
     Context() {
         AMFLS_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         stats.reset(64);
+        counter.reset(1);
+        AMFLS_HIP_CHECK(hipMemset(counter.get(), 0, sizeof(unsigned)));
+        stage.ensure(kStageDoubles);
+        landing.ensure(kLandingBytes);
     }
     ~Context() {
+        if (stream) (void)hipStreamSynchronize(stream);
         scratch.release();
         scratch2.release();
         if (stream) (void)hipStreamDestroy(stream);
@@ -98,31 +200,57 @@ struct Context {
     Context(const Context&) = delete;
     Context& operator=(const Context&) = delete;
 
-    void sync() { AMFLS_HIP_CHECK(hipStreamSynchronize(stream)); }
+    void sync() {
+        AMFLS_HIP_CHECK(hipStreamSynchronize(stream));
+        stage_pos = 0;
+    }
 
-    // Upload a small double matrix into the staging buffer at `offset`
-    // doubles; returns the device pointer.  Synchronous with respect to the
-    // host buffer (the copy is complete when this returns).
+    // Upload a small double matrix into the device staging buffer at
+    // `offset` doubles and return the device pointer.  The copy is enqueued
+    // on the stream from page-locked memory and the host buffer may be
+    // reused at once; kernels enqueued afterwards see the data.
     double* upload(const double* host, size_t count, size_t offset = 0) {
         small.ensure(offset + count + 1);
         double* dst = small.get() + offset;
-        if (count > 0) {
+        if (count == 0) return dst;
+        if (count > kStageDoubles) {
             AMFLS_HIP_CHECK(hipMemcpyAsync(dst, host, count * sizeof(double), hipMemcpyHostToDevice, stream));
             sync();
+            return dst;
         }
+        if (stage_pos + count > kStageDoubles) sync();
+        double* h = stage.get() + stage_pos;
+        std::memcpy(h, host, count * sizeof(double));
+        AMFLS_HIP_CHECK(hipMemcpyAsync(dst, h, count * sizeof(double), hipMemcpyHostToDevice, stream));
+        stage_pos += count;
         return dst;
     }
+    // Copy device data to the host and wait for it.
     template <typename T>
     void download(T* host, const T* dev, size_t count) {
         if (count == 0) return;
-        AMFLS_HIP_CHECK(hipMemcpyAsync(host, dev, count * sizeof(T), hipMemcpyDeviceToHost, stream));
+        const size_t bytes = count * sizeof(T);
+        if (bytes <= kLandingBytes) {
+            AMFLS_HIP_CHECK(hipMemcpyAsync(landing.get(), dev, bytes, hipMemcpyDeviceToHost, stream));
+            sync();
+            std::memcpy(host, landing.get(), bytes);
+            return;
+        }
+        AMFLS_HIP_CHECK(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, stream));
         sync();
     }
 
-    // Statistics (max |v|, scaled square sum, nonfinite count) of a block.
+    // Statistics (max |v|, scaled square sum, nonfinite count) of a block,
+    // left on the device at `out_dev` (no wait).
+    template <typename T>
+    void block_stats_async(int64_t rows, int cols, const T* X, int64_t ldx, double* out_dev,
+                           const double* gate = nullptr) {
+        vector_stats<T>(stream, rows, cols, X, ldx, out_dev, scratch, counter.get(), gate);
+    }
+    // Statistics copied to the host (waits).
     template <typename T>
     void block_stats(int64_t rows, int cols, const T* X, int64_t ldx, double out[3]) {
-        vector_stats<T>(stream, rows, cols, X, ldx, stats.get(), scratch);
+        block_stats_async<T>(rows, cols, X, ldx, stats.get());
         download(out, stats.get(), 3);
     }
 };

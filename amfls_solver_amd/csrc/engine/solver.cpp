@@ -75,6 +75,11 @@ LeastSquaresResult amfls_driver(const DeviceOperator& op, const double* b_dev, c
     RunStatistics stats;
     ValidationEngine validator(op, b_dev, ctx, stats);
     Workspace<T> ws(op, b_dev, lambda, o.maximum_basis_size, stats, ctx, validator);
+    // Operator seconds so far (waits for the timed products in flight).
+    auto op_seconds = [&] {
+        ctx.timer.flush();
+        return stats.a_seconds + stats.at_seconds;
+    };
     const int dimension = static_cast<int>(std::min<int64_t>(op.rows(), op.cols()));
     const int max_depth = o.maximum_depth > 0 ? std::min(o.maximum_depth, dimension) : dimension;
     const int cap_aux = std::max(0, ws.basis_limit() - 1);
@@ -111,21 +116,27 @@ LeastSquaresResult amfls_driver(const DeviceOperator& op, const double* b_dev, c
         has = true;
         result = std::move(cand);
     };
-    double prev_eval_cert = kInfD;
-    int stalls = 0;
+    // Inner solves of the refinement loop stop once the certificate has not
+    // halved over the last doubling of the depth (an FP32 process has
+    // reached its rounding floor; the outer FP64 loop continues from there).
+    std::vector<std::pair<int, double>> eval_history;
     bool stagnated = false;
     auto evaluate = [&](bool train) {
         record(ws.evaluate(tol, evaluations, aux), train);
         if (o.stop_on_stagnation) {
             const double c = active_certificate(result, lambda);
-            if (!(c < 0.5 * prev_eval_cert)) ++stalls; else stalls = 0;
-            prev_eval_cert = std::min(prev_eval_cert, c);
-            stagnated = stalls >= 2;
+            const int d = result.depth;
+            double earlier = kInfD;
+            for (const auto& e : eval_history)
+                if (2 * e.first <= d) earlier = std::min(earlier, e.second);
+            eval_history.emplace_back(d, c);
+            stagnated = d >= 16 && std::isfinite(earlier) && !(c < 0.5 * earlier);
         }
     };
     auto finish = [&](LeastSquaresResult f) {
         f.iterations = ws.depth();
         f.trace = trace;
+        ctx.timer.flush();
         stats.total_seconds = clk.now() - total_start;
         f.statistics = stats;
         return f;
@@ -141,9 +152,9 @@ LeastSquaresResult amfls_driver(const DeviceOperator& op, const double* b_dev, c
     if (ws.can_advance() && max_depth > 0) {
         last_interval_width = ws.active_left_width();
         const double t0 = clk.now();
-        const double a0 = stats.a_seconds + stats.at_seconds;
+        const double a0 = op_seconds();
         (void)ws.advance_level();
-        last_op_s = stats.a_seconds + stats.at_seconds - a0;
+        last_op_s = op_seconds() - a0;
         last_loc_s = std::max(0.0, clk.now() - t0 - last_op_s);
     }
     evaluate(true);
@@ -246,9 +257,9 @@ LeastSquaresResult amfls_driver(const DeviceOperator& op, const double* b_dev, c
             if (ws.can_advance()) {
                 last_interval_width = ws.active_left_width();
                 const double t0 = clk.now();
-                const double a0 = stats.a_seconds + stats.at_seconds;
+                const double a0 = op_seconds();
                 advanced = ws.advance_level();
-                last_op_s = stats.a_seconds + stats.at_seconds - a0;
+                last_op_s = op_seconds() - a0;
                 last_loc_s = std::max(0.0, clk.now() - t0 - last_op_s);
             }
             if (advanced) {
@@ -299,7 +310,7 @@ LeastSquaresResult amfls_driver(const DeviceOperator& op, const double* b_dev, c
                 while (target_depth < max_depth && !schedule.should_evaluate(target_depth)) ++target_depth;
             }
             const double t0 = clk.now();
-            const double a0 = stats.a_seconds, at0 = stats.at_seconds;
+            const double a0 = op_seconds();
             last_interval_width = frontier;
             while (ws.can_advance() && ws.depth() < target_depth) {
                 const bool advanced = ws.advance_level();
@@ -307,8 +318,9 @@ LeastSquaresResult amfls_driver(const DeviceOperator& op, const double* b_dev, c
                 broke = step_has_nonfinite_breakdown(ws.last_step_audit());
                 if (!advanced || truncated || broke) break;
             }
+            const double op_total = op_seconds();
             const double search_s = clk.now() - t0;
-            const double op_s = std::max(0.0, stats.a_seconds - a0 + stats.at_seconds - at0);
+            const double op_s = std::max(0.0, op_total - a0);
             const double loc_s = std::max(0.0, search_s - op_s);
             last_op_s = op_s;
             last_loc_s = loc_s;
@@ -564,6 +576,7 @@ LeastSquaresResult solve_fixed_rbgk(const DeviceOperator& op, const double* b_de
     result.iterations = depth;
     result.depth = depth;
     result.trace = trace;
+    ctx.timer.flush();
     stats.total_seconds = clk.now() - total_start;
     result.statistics = stats;
     return result;

@@ -56,6 +56,8 @@ Workspace<T>::Workspace(const DeviceOperator& op, const double* b_dev, double re
     if (lim >= std::numeric_limits<int>::max()) throw std::length_error("basis limit exceeds int range");
     basis_limit_ = static_cast<int>(lim);
     bnorm_ = validator_.rhs_norm();
+    slots_.reset(kSlotCount);
+    gdev_.reset(static_cast<size_t>(basis_limit_) + 2);
     V_.init(n_, std::min(basis_limit_, 64));
     left_.init(m_, 8);
     work_.init(m_, 8);
@@ -93,22 +95,25 @@ double Workspace<T>::rank_cutoff(double scale, int k, int w) const {
 
 template <typename T>
 void Workspace<T>::apply_search(const T* X, int s, int64_t ldx, T* Y, int64_t ldy, bool transpose) {
+    // Wall clock: the product is bracketed by events on the stream and its
+    // GPU time is added to the operator seconds when the timer is flushed.
+    // Synthetic clock: the charged cost is the product's time.
     Clock& clk = ctx_.clock;
-    if (!clk.synthetic()) ctx_.sync();
-    const double start = clk.now();
+    const bool wall = !clk.synthetic();
+    double* seconds = transpose ? &stats_.at_seconds : &stats_.a_seconds;
+    const double charged_from = clk.now();
     clk.charge_operator(op_.relative_block_product_cost(s));
+    if (!wall) *seconds += clk.now() - charged_from;
+    if (wall) ctx_.timer.begin(ctx_.stream);
     if (transpose) op_.apply_t(ctx_.stream, X, s, ldx, Y, ldy);
     else op_.apply(ctx_.stream, X, s, ldx, Y, ldy);
-    if (!clk.synthetic()) ctx_.sync();
-    const double sec = clk.now() - start;
+    if (wall) ctx_.timer.end(ctx_.stream, seconds);
     if (transpose) {
-        stats_.at_seconds += sec;
         stats_.at_columns += s;
         stats_.search_at_columns += s;
         ++stats_.at_block_calls;
         ++stats_.search_at_block_calls;
     } else {
-        stats_.a_seconds += sec;
         stats_.a_columns += s;
         stats_.search_a_columns += s;
         ++stats_.a_block_calls;
@@ -117,50 +122,74 @@ void Workspace<T>::apply_search(const T* X, int s, int64_t ldx, T* Y, int64_t ld
 }
 
 template <typename T>
+void Workspace<T>::enqueue_scale(const T* X, int64_t ldx, int64_t rows, int cols) {
+    ctx_.block_stats_async<T>(rows, cols, X, ldx, slots_.get() + kScaleSlot);
+}
+
+template <typename T>
+void Workspace<T>::sync_projected_rhs() {
+    if (g_host_ >= g_.size()) return;
+    ctx_.download(g_.data() + g_host_, gdev_.get() + g_host_, g_.size() - g_host_);
+    g_host_ = g_.size();
+}
+
+template <typename T>
 typename Workspace<T>::Factor Workspace<T>::orthogonalize_and_factor(T* W, int64_t ldw, int64_t N, int w,
                                                                      const T* basis, int64_t ldb, int k,
-                                                                     int max_rank, double scale, int tracked,
-                                                                     bool retain) {
-    // reference krylov_workspace.cpp:82-310
+                                                                     int max_rank, int tracked, bool retain) {
+    // reference krylov_workspace.cpp:82-310.  The projections, the DGKS
+    // decision for one column and every norm run on the device; the host
+    // waits once for the statistics (and retained coefficients) before the
+    // rank decision.
     Clock& clk = ctx_.clock;
     const double start = clk.now();
     clk.charge_local(w, k);
     hipStream_t s = ctx_.stream;
     const Strided B{basis, vt_of(T{}), ldb, 1};
+    double* sl = slots_.get();
     Factor f;
-    double resid = 0.0;
-    bool coeff_finite = true;
     if (k > 0) {
         coeff_.ensure(static_cast<size_t>(k) * w);
         cbuf_.ensure(static_cast<size_t>(k) * w);
         fill<double>(s, coeff_.get(), static_cast<int64_t>(k) * w, 0.0);
     }
-    auto pass = [&] {
+    auto pass = [&](const double* gate) {
         // c = basis^T W ; W -= basis c ; coeff += c
-        gemm_tn<T>(s, N, k, w, B, W, ldw, cbuf_.get(), ValueType::f64, w, 1.0, ctx_.scratch);
-        gemm_nn<T, double>(s, N, w, k, B, cbuf_.get(), w, W, ldw, -1.0, 1.0);
-        add_vec(s, static_cast<int64_t>(k) * w, cbuf_.get(), coeff_.get());
+        gemm_tn<T>(s, N, k, w, B, W, ldw, cbuf_.get(), ValueType::f64, w, 1.0, ctx_.scratch, gate);
+        gemm_nn<T, double>(s, N, w, k, B, cbuf_.get(), w, W, ldw, -1.0, 1.0, gate);
+        add_vec(s, static_cast<int64_t>(k) * w, cbuf_.get(), coeff_.get(), gate);
     };
-    if (w == 1 && k > 0) {
-        const double before = block_norm(W, ldw, N, 1);
-        pass();
-        resid = block_norm(W, ldw, N, 1);
-        if (std::isfinite(before) && std::isfinite(resid) && resid < std::sqrt(0.5) * before) {
-            pass();
-            resid = block_norm(W, ldw, N, 1);
-        }
+    const bool dgks = w == 1 && k > 0;
+    if (dgks) {
+        // second pass when ||W|| fell below ||W_before|| / sqrt(2)
+        ctx_.block_stats_async<T>(N, 1, W, ldw, sl + kBeforeSlot);
+        pass(nullptr);
+        ctx_.block_stats_async<T>(N, 1, W, ldw, sl + kFirstSlot);
+        second_pass_gate(s, sl + kBeforeSlot, sl + kFirstSlot, std::sqrt(0.5), sl + kGateSlot);
+        pass(sl + kGateSlot);
+        ctx_.block_stats_async<T>(N, 1, W, ldw, sl + kSecondSlot, sl + kGateSlot);
     } else {
-        for (int p = 0; p < (k > 0 ? 2 : 0); ++p) pass();
-        resid = block_norm(W, ldw, N, w);
+        for (int p = 0; p < (k > 0 ? 2 : 0); ++p) pass(nullptr);
+        ctx_.block_stats_async<T>(N, w, W, ldw, sl + kFirstSlot);
     }
-    if (k > 0) {
-        double st[3];
-        ctx_.block_stats<double>(static_cast<int64_t>(k) * w, 1, coeff_.get(), 1, st);
-        coeff_finite = st[2] == 0.0;
-        if (retain) {
-            f.coefficients.resize(static_cast<size_t>(k) * w);
-            ctx_.download(f.coefficients.data(), coeff_.get(), static_cast<size_t>(k) * w);
-        }
+    if (k > 0) ctx_.block_stats_async<double>(static_cast<int64_t>(k) * w, 1, coeff_.get(), 1, sl + kCoeffSlot);
+    const size_t ncoef = (retain && k > 0) ? static_cast<size_t>(k) * w : 0;
+    double* land = reinterpret_cast<double*>(ctx_.landing.get());
+    const bool coef_landing = (kSlotCount + ncoef) * sizeof(double) <= Context::kLandingBytes;
+    AMFLS_HIP_CHECK(hipMemcpyAsync(land, sl, kSlotCount * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (ncoef > 0 && coef_landing)
+        AMFLS_HIP_CHECK(hipMemcpyAsync(land + kSlotCount, coeff_.get(), ncoef * sizeof(double),
+                                       hipMemcpyDeviceToHost, s));
+    ctx_.sync();
+    auto norm_at = [&](int slot) { return fp::nominal_norm(land[slot], land[slot + 1], land[slot + 2]); };
+    const double scale = norm_at(kScaleSlot);
+    double resid = norm_at(kFirstSlot);
+    if (dgks && land[kGateSlot] != 0.0) resid = norm_at(kSecondSlot);
+    const bool coeff_finite = k == 0 || land[kCoeffSlot + 2] == 0.0;
+    if (ncoef > 0) {
+        f.coefficients.resize(ncoef);
+        if (coef_landing) std::copy(land + kSlotCount, land + kSlotCount + ncoef, f.coefficients.begin());
+        else ctx_.download(f.coefficients.data(), coeff_.get(), ncoef);
     }
     f.cutoff = rank_cutoff(scale, k, w);
     f.discarded = resid;
@@ -295,6 +324,7 @@ void Workspace<T>::initialize(bool with_aux, int aux_cols, uint64_t seed, uint64
     if (std::isfinite(bnorm_) && bnorm_ > 0.0 && basis_limit_ > 0) {
         copy_block<double, T>(ctx_.stream, m_, 1, b_dev_, 1, left_.ptr(), left_.ld(), 1.0 / bnorm_);
         g_.push_back(bnorm_);
+        g_host_ = g_.size();
         T_.rows = 1;
         active_first_ = 0;
         active_width_ = 1;
@@ -339,9 +369,9 @@ int Workspace<T>::append_left_seed_gaussian(int cols, uint64_t seed, uint64_t st
     const int max_rank = static_cast<int>(std::min<int64_t>({static_cast<int64_t>(avail), m_ - a - p,
                                                              static_cast<int64_t>(cols)}));
     if (max_rank <= 0) return 0;
-    const double scale = block_norm(seed_ptr, left_.ld(), m_, cols);
+    enqueue_scale(seed_ptr, left_.ld(), m_, cols);
     Factor f = orthogonalize_and_factor(seed_ptr, left_.ld(), m_, cols, left_.ptr(), left_.ld(), a + p, max_rank,
-                                        scale, 0, false);
+                                        0, false);
     if (f.rank <= 0) return 0;
     pending_ += f.rank;
     pending_retained_ += f.rank;
@@ -370,7 +400,7 @@ bool Workspace<T>::advance_level() {
     apply_search(left_.ptr(), cw, left_.ld(), right_.ptr(), right_.ld(), true);
     pending_ = 0;
     pending_requested_ = pending_retained_ = 0;
-    const double rscale = block_norm(right_.ptr(), right_.ld(), n_, cw);
+    enqueue_scale(right_.ptr(), right_.ld(), n_, cw);
     if (prev_width_ > 0 && aw > 0) {
         if (coupling_.rows != aw || coupling_.cols != prev_width_)
             throw std::logic_error("block Golub-Kahan frontier coupling dimensions are invalid");
@@ -383,8 +413,8 @@ bool Workspace<T>::advance_level() {
                            right_.ptr(), right_.ld(), -1.0, 1.0);
     }
     const int max_right = std::min(cw, basis_limit_ - k_);
-    Factor rf = orthogonalize_and_factor(right_.ptr(), right_.ld(), n_, cw, V_.ptr(), V_.ld(), k_, max_right,
-                                         rscale, aw, false);
+    Factor rf = orthogonalize_and_factor(right_.ptr(), right_.ld(), n_, cw, V_.ptr(), V_.ld(), k_, max_right, aw,
+                                         false);
     const int rw = rf.rank;
     audit_.retained_right_width = rw;
     audit_.right_rank_cutoff = rf.cutoff;
@@ -406,7 +436,7 @@ bool Workspace<T>::advance_level() {
     // L = A * newV ; L -= U_active * coup^T
     work_.ensure_cols(rw, s);
     apply_search(right_.ptr(), rw, right_.ld(), work_.ptr(), work_.ld(), false);
-    const double lscale = block_norm(work_.ptr(), work_.ld(), m_, rw);
+    enqueue_scale(work_.ptr(), work_.ld(), m_, rw);
     if (aw > 0) {
         HostMat ct(aw, rw);  // coup^T, coup = rf.factor[:, :aw] (rw x aw)
         for (int i = 0; i < rw; ++i)
@@ -417,8 +447,8 @@ bool Workspace<T>::advance_level() {
     }
     const int max_left = static_cast<int>(std::min<int64_t>({static_cast<int64_t>(rw), m_ - old_left,
                                                              static_cast<int64_t>(basis_limit_) + 1 - old_left}));
-    Factor lf = orthogonalize_and_factor(work_.ptr(), work_.ld(), m_, rw, left_.ptr(), left_.ld(), aw, max_left,
-                                         lscale, 0, true);
+    Factor lf = orthogonalize_and_factor(work_.ptr(), work_.ld(), m_, rw, left_.ptr(), left_.ld(), aw, max_left, 0,
+                                         true);
     const int lw = lf.rank;
     audit_.retained_next_left_width = lw;
     audit_.left_rank_cutoff = lf.cutoff;
@@ -448,12 +478,12 @@ bool Workspace<T>::advance_level() {
     std::swap(left_, work_);
     // g entries U_new^T b
     if (lw > 0) {
-        dots_.ensure(static_cast<size_t>(lw));
-        gemm_tn<double>(s, m_, lw, 1, Strided{left_.ptr(), vt_of(T{}), left_.ld(), 1}, b_dev_, 1, dots_.get(),
+        // g entries stay on the device until the next projected solve
+        if (g_.size() + lw > gdev_.size()) throw std::logic_error("projected right-hand side exceeds its capacity");
+        gemm_tn<double>(s, m_, lw, 1, Strided{left_.ptr(), vt_of(T{}), left_.ld(), 1}, b_dev_, 1,
+                        gdev_.get() + g_.size(),
                         ValueType::f64, 1, 1.0, ctx_.scratch);
-        std::vector<double> d(lw);
-        ctx_.download(d.data(), dots_.get(), lw);
-        for (double v : d) g_.push_back(v);
+        g_.resize(g_.size() + lw, std::numeric_limits<double>::quiet_NaN());
     }
     active_first_ = old_left;
     active_width_ = lw;
@@ -474,6 +504,7 @@ void Workspace<T>::candidate(double* x_dev) {
         return;
     }
     bool dense = false;
+    sync_projected_rhs();
     if (!projected_solve(T_, g_, lambda_, rcond, y_, &dense)) {
         y_.assign(k_, std::numeric_limits<double>::quiet_NaN());
     }
@@ -487,6 +518,7 @@ LeastSquaresResult Workspace<T>::evaluate(double tolerance, int epoch, int auxil
     if (!initialized_) throw std::logic_error("block Golub-Kahan workspace is not initialized");
     if (!std::isfinite(tolerance) || tolerance <= 0.0 || tolerance >= 1.0)
         throw std::invalid_argument("block Golub-Kahan tolerance must lie strictly between zero and one");
+    ctx_.timer.flush();
     Clock& clk = ctx_.clock;
     const double start = clk.now();
     DevBuf<double> x(static_cast<size_t>(n_));

@@ -56,6 +56,9 @@ class Workspace {
 public:
     Workspace(const DeviceOperator& op, const double* b_dev, double regularization, int maximum_basis_size,
               RunStatistics& stats, Context& ctx, ValidationEngine& validator);
+    ~Workspace() { ctx_.timer.discard(); }
+    Workspace(const Workspace&) = delete;
+    Workspace& operator=(const Workspace&) = delete;
 
     void initialize(bool with_aux, int aux_cols, uint64_t seed, uint64_t stream, uint64_t first_col);
     int append_left_seed_gaussian(int cols, uint64_t seed, uint64_t stream, uint64_t first_col,
@@ -73,7 +76,6 @@ public:
     double right_hand_side_norm() const { return bnorm_; }
     const StepAudit& last_step_audit() const { return audit_; }
     const Staircase& projected_operator() const { return T_; }
-    const std::vector<double>& projected_rhs() const { return g_; }
 
 private:
     struct Factor {
@@ -84,9 +86,17 @@ private:
         bool positive_finite_below_cutoff = false, nonfinite_residual = false;
     };
     // Orthogonalize W (N x w, in place) against basis (N x k) and rank-reveal.
-    // On return W's first `rank` columns hold the orthonormal q.
+    // On return W's first `rank` columns hold the orthonormal q.  The scale
+    // of the rank cutoff is the norm whose statistics the caller enqueued
+    // into slot 0 (kScaleSlot) before W was modified.
     Factor orthogonalize_and_factor(T* W, int64_t ldw, int64_t N, int w, const T* basis, int64_t ldb, int k,
-                                    int max_rank, double scale, int tracked, bool retain);
+                                    int max_rank, int tracked, bool retain);
+    // Device statistics slots (triples) of one orthogonalization and the
+    // DGKS gate; copied to the host with a single wait.
+    static constexpr int kScaleSlot = 0, kBeforeSlot = 3, kFirstSlot = 6, kSecondSlot = 9, kCoeffSlot = 12,
+                         kGateSlot = 15, kSlotCount = 16;
+    void enqueue_scale(const T* X, int64_t ldx, int64_t rows, int cols);
+    void sync_projected_rhs();
     double block_norm(const T* X, int64_t ldx, int64_t rows, int cols);
     double rank_cutoff(double scale, int k, int w) const;
     void apply_search(const T* X, int s, int64_t ldx, T* Y, int64_t ldy, bool transpose);
@@ -108,7 +118,8 @@ private:
     DeviceBlock<T> right_;   // n x (a + p): right residual
     DevBuf<double> coeff_;   // k x w FP64 coefficient accumulation
     DevBuf<double> cbuf_;    // pass coefficients
-    DevBuf<double> dots_;    // g entries
+    DevBuf<double> gdev_;    // g = U^T b entries computed on the device
+    DevBuf<double> slots_;   // orthogonalization statistics (kSlotCount doubles)
     DevBuf<double> gram_;    // w x w Gram matrices
     DevBuf<T> tmp_;          // wide row-local products
     int k_ = 0;
@@ -117,6 +128,7 @@ private:
     HostMat coupling_;       // active_width x prev_width
     Staircase T_;
     std::vector<double> g_;
+    size_t g_host_ = 0;      // leading entries of g_ already copied from gdev_
     int pending_requested_ = 0, pending_retained_ = 0;
     int depth_ = 0;
     bool initialized_ = false, reliable_ = true;

@@ -114,9 +114,10 @@ def test_validation_models_match_oracle(torch_mod, fx):
 
 def _check_trajectory(res, trace, status):
     got = res.trace_float.numpy()
-    assert got.shape[0] == trace.shape[0]
     # depth, basis rank, auxiliary width of every evaluated candidate
-    assert np.array_equal(got[:, [1, 3, 2]], trace[:, [0, 1, 2]])
+    msg = f"engine trace\n{got[:, [1, 3, 2, 13, 14, 15]]}\noracle trace\n{trace[:, [0, 1, 2, 3]]}"
+    assert got.shape[0] == trace.shape[0], msg
+    assert np.array_equal(got[:, [1, 3, 2]], trace[:, [0, 1, 2]]), msg
     assert res.status == int(status[0]) and res.stop_reason == int(status[1])
     assert res.depth == int(status[2]) and res.basis_rank == int(status[3])
 

@@ -24,8 +24,8 @@ ARCH = os.environ.get("AMFLS_OFFLOAD_ARCH", "gfx950")
 SOURCES = [
     CSRC / "device" / "blas.hip",
     CSRC / "device" / "spmm.hip",
-    CSRC / "device" / "dense_mfma.hip",
-    CSRC / "device" / "mlp.hip",
+    CSRC / "device" / "rank_one.hip",
+    CSRC / "device" / "level_chain.hip",
     CSRC / "engine" / "operator.cpp",
     CSRC / "engine" / "small_la.cpp",
     CSRC / "engine" / "validation.cpp",

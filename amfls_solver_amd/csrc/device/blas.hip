@@ -492,12 +492,6 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Device-scope load of a value another workgroup published before its
-// release fence and ticket (bypasses stale per-XCD cache lines).
-__device__ __forceinline__ double coherent_load(const double* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // One-launch norm statistics.  Each block reduces its grid-stride share to
 // (max |v|, sum (|v| 2^-e_b)^2, nonfinite count) with e_b the frexp exponent
 // of the block's own max; the last block to finish (ticket on `counter`)
@@ -563,7 +557,7 @@ __global__ __launch_bounds__(kThreads) void stats_kernel(int64_t rows, int cols,
     __threadfence();
     const int nb = static_cast<int>(gridDim.x);
     double gm = 0.0;
-    for (int b = threadIdx.x; b < nb; b += kThreads) gm = fmax(gm, coherent_load(&partial[3 * b]));
+    for (int b = threadIdx.x; b < nb; b += kThreads) gm = fmax(gm, partial[3 * b]);
     gm = wave_max(gm);
     __syncthreads();
     if (lane == 0) sm[wv][0] = gm;
@@ -575,9 +569,9 @@ __global__ __launch_bounds__(kThreads) void stats_kernel(int64_t rows, int cols,
     if (gm > 0.0) (void)frexp(gm, &e);
     double tot = 0.0, tbad = 0.0;
     for (int b = threadIdx.x; b < nb; b += kThreads) {
-        const double pm = coherent_load(&partial[3 * b]);
-        const double ps = coherent_load(&partial[3 * b + 1]);
-        tbad += coherent_load(&partial[3 * b + 2]);
+        const double pm = partial[3 * b];
+        const double ps = partial[3 * b + 1];
+        tbad += partial[3 * b + 2];
         if (pm > 0.0) {
             int ebb = 0;
             (void)frexp(pm, &ebb);

@@ -77,6 +77,53 @@ void vector_stats(hipStream_t s, int64_t rows, int cols, const T* X, int64_t ldx
 // ||after|| < factor * ||before||, else 0 (device-side DGKS criterion).
 void second_pass_gate(hipStream_t s, const double* before, const double* after, double factor, double* gate);
 
+// ---------------------------------------------------------------- rank-one steps
+// One launch each; `counter` is a device word that holds 0 between launches
+// and gates (a, b) skip the launch while either holds 0.
+// c = B^T w (B row-major N x k), coeff = c or coeff += c.
+template <typename TB, typename TW>
+void rank_one_project(hipStream_t s, int64_t N, int k, const TB* B, int64_t ldb, const TW* w, int64_t ldw,
+                      double* c, double* coeff, bool accumulate, Scratch& scratch, unsigned* counter,
+                      const double* gate_a = nullptr, const double* gate_b = nullptr);
+// w -= B c (one rounding to T), statistics of the new w into out[0..2]; with
+// gate_out, *gate_out = (||w|| < factor * ||before||, both finite).
+template <typename T>
+void rank_one_update(hipStream_t s, int64_t N, int k, const T* B, int64_t ldb, const double* c, T* w,
+                     int64_t ldw, double* out, Scratch& scratch, unsigned* counter, const double* gate_a = nullptr,
+                     const double* gate_b = nullptr, const double* before = nullptr, double factor = 0.0,
+                     double* gate_out = nullptr);
+
+// ---------------------------------------------------------------- level chain
+// Batched rank-one level kernels (level_chain.hip).  Producers write one
+// partial per block; consumers combine all partials; a kernel returns at
+// once while *go == 0.  Statistics partials are triples, one per block of
+// chain_rows_per_block(N) rows; projection partials are k per block of
+// chain_project_rows(N, k) rows.  The optional (before, after) partials turn
+// a projection/update into the DGKS second pass, run only when
+// ||after|| < factor ||before||.
+int chain_rows_per_block(int64_t N);
+int chain_project_rows(int64_t N, int k);
+// y = r - (*coef) p in place on r; statistics partials of r and y.
+template <typename T>
+void chain_couple(hipStream_t s, int64_t N, T* r, int64_t ldr, const T* p, int64_t ldp, const double* coef,
+                  const double* go, double* part_r, double* part_y);
+// P[b][o] = sum over block b's rows of B[i, o] y[i].
+template <typename T>
+void chain_project(hipStream_t s, int64_t N, int k, const T* B, int64_t ldb, const T* y, int64_t ldy, double* P,
+                   const double* go, const double* before, const double* after, int nb_before, int nb_after,
+                   double factor);
+// y -= B (sum_b P[b]); statistics partials of y.
+template <typename T>
+void chain_update(hipStream_t s, int64_t N, int k, const T* B, int64_t ldb, const double* P, int nbP, T* y,
+                  int64_t ldy, double* part_y, const double* go, const double* before, const double* after,
+                  int nb_before, int nb_after, double factor);
+// Rank decision and normalization (dst = y / ||y||), see level_chain.hip.
+template <typename T>
+void chain_normalize(hipStream_t s, int64_t N, const T* y, int64_t ldy, T* dst, int64_t ldd, const double* part_scale,
+                     int nb_scale, const double* part_before, const double* part_first, const double* part_second,
+                     int nb_update, double dgks_factor, double rank_factor, const double* P1, const double* P2,
+                     int nbP, const double* b, double* gpart, double* record, double* cell, double* go);
+
 // Counter-addressed Gaussian block, index = (first_col + j) * rows + i.
 template <typename T>
 void gaussian_block(hipStream_t s, int64_t rows, int cols, uint64_t seed, uint64_t stream,
@@ -100,7 +147,9 @@ void csr_spmm(hipStream_t s, const CsrView& A, int ncols, const T* X, int64_t ld
 int csr_spmm_variant_count();
 bool csr_spmm_variant_supports(int variant, int ncols);
 const char* csr_spmm_variant_name(int variant);
-int csr_spmm_default_variant(int ncols);
+// Variant used when none is set: lanes sized to the block width and the
+// mean number of entries per row.
+int csr_spmm_default_variant(int ncols, double mean_row_nnz);
 
 // CSR of A^T, stable (each transposed row lists original rows in increasing
 // order).  Outputs sized cols+1, nnz, nnz (values of A's type).

@@ -88,11 +88,38 @@ struct VariantInfo {
     const char* name;
 };
 
+// (entry lanes P, column lanes LC, columns per lane CPL)
 constexpr VariantInfo kVariants[] = {
-    {8, 1, 1, "p8_c1"},   {16, 1, 1, "p16_c1"}, {4, 2, 1, "p4_c2"},   {8, 2, 1, "p8_c2"},
-    {4, 4, 1, "p4_c4"},   {8, 4, 1, "p8_c4"},   {2, 8, 1, "p2_c8"},   {4, 8, 1, "p4_c8"},
-    {1, 16, 1, "p1_c16"}, {2, 16, 1, "p2_c16"}, {4, 16, 1, "p4_c16"}, {1, 32, 1, "p1_c32"},
-    {2, 32, 1, "p2_c32"}, {1, 64, 1, "p1_c64"}, {1, 64, 2, "p1_c128"}, {1, 64, 4, "p1_c256"},
+    {1, 1, 1, "p1_c1"},
+    {2, 1, 1, "p2_c1"},
+    {4, 1, 1, "p4_c1"},
+    {8, 1, 1, "p8_c1"},
+    {16, 1, 1, "p16_c1"},
+    {32, 1, 1, "p32_c1"},
+    {64, 1, 1, "p64_c1"},
+    {1, 2, 1, "p1_c2"},
+    {2, 2, 1, "p2_c2"},
+    {4, 2, 1, "p4_c2"},
+    {8, 2, 1, "p8_c2"},
+    {16, 2, 1, "p16_c2"},
+    {32, 2, 1, "p32_c2"},
+    {1, 4, 1, "p1_c4"},
+    {2, 4, 1, "p2_c4"},
+    {4, 4, 1, "p4_c4"},
+    {8, 4, 1, "p8_c4"},
+    {16, 4, 1, "p16_c4"},
+    {1, 8, 1, "p1_c8"},
+    {2, 8, 1, "p2_c8"},
+    {4, 8, 1, "p4_c8"},
+    {8, 8, 1, "p8_c8"},
+    {1, 16, 1, "p1_c16"},
+    {2, 16, 1, "p2_c16"},
+    {4, 16, 1, "p4_c16"},
+    {1, 32, 1, "p1_c32"},
+    {2, 32, 1, "p2_c32"},
+    {1, 64, 1, "p1_c64"},
+    {1, 64, 2, "p1_c128"},
+    {1, 64, 4, "p1_c256"},
 };
 constexpr int kVariantCount = sizeof(kVariants) / sizeof(kVariants[0]);
 
@@ -116,10 +143,13 @@ void launch_by_index(int variant, hipStream_t s, const CsrView& A, int ncols, co
         launch_variant<V, T, kVariants[i].p, kVariants[i].lc, kVariants[i].cpl>(                 \
             s, A, ncols, X, ldx, Y, ldy, Add, ld_add, add_scale);                                \
         break;
+    static_assert(kVariantCount == 30, "variant switch must cover the table");
     switch (variant) {
-        AMFLS_CASE(0) AMFLS_CASE(1) AMFLS_CASE(2) AMFLS_CASE(3) AMFLS_CASE(4) AMFLS_CASE(5)
-        AMFLS_CASE(6) AMFLS_CASE(7) AMFLS_CASE(8) AMFLS_CASE(9) AMFLS_CASE(10) AMFLS_CASE(11)
-        AMFLS_CASE(12) AMFLS_CASE(13) AMFLS_CASE(14) AMFLS_CASE(15)
+        AMFLS_CASE(0) AMFLS_CASE(1) AMFLS_CASE(2) AMFLS_CASE(3) AMFLS_CASE(4) AMFLS_CASE(5) AMFLS_CASE(6)
+        AMFLS_CASE(7) AMFLS_CASE(8) AMFLS_CASE(9) AMFLS_CASE(10) AMFLS_CASE(11) AMFLS_CASE(12)
+        AMFLS_CASE(13) AMFLS_CASE(14) AMFLS_CASE(15) AMFLS_CASE(16) AMFLS_CASE(17) AMFLS_CASE(18)
+        AMFLS_CASE(19) AMFLS_CASE(20) AMFLS_CASE(21) AMFLS_CASE(22) AMFLS_CASE(23) AMFLS_CASE(24)
+        AMFLS_CASE(25) AMFLS_CASE(26) AMFLS_CASE(27) AMFLS_CASE(28) AMFLS_CASE(29)
         default:
             throw std::invalid_argument("unknown CSR SpMM variant");
     }
@@ -250,16 +280,23 @@ const char* csr_spmm_variant_name(int variant) {
     return (variant >= 0 && variant < kVariantCount) ? kVariants[variant].name : "invalid";
 }
 
-int csr_spmm_default_variant(int ncols) {
-    if (ncols <= 1) return 0;    // p8_c1
-    if (ncols <= 2) return 2;    // p4_c2
-    if (ncols <= 4) return 4;    // p4_c4
-    if (ncols <= 8) return 6;    // p2_c8
-    if (ncols <= 16) return 8;   // p1_c16
-    if (ncols <= 32) return 11;  // p1_c32
-    if (ncols <= 64) return 13;  // p1_c64
-    if (ncols <= 128) return 14;
-    return 15;
+int csr_spmm_variant_index(int p, int lc, int cpl) {
+    for (int v = 0; v < kVariantCount; ++v)
+        if (kVariants[v].p == p && kVariants[v].lc == lc && kVariants[v].cpl == cpl) return v;
+    return -1;
+}
+
+int csr_spmm_default_variant(int ncols, double mean_row_nnz) {
+    // Column lanes cover the block width (up to one wavefront); entry lanes
+    // split a row so that each keeps about three or more entries per column
+    // lane (measured on gfx950 for rows of 12 and 120 entries, widths 1-64).
+    if (ncols > 128) return csr_spmm_variant_index(1, 64, 4);
+    if (ncols > 64) return csr_spmm_variant_index(1, 64, 2);
+    int lc = 1;
+    while (lc < ncols) lc <<= 1;
+    int p = 1;
+    while (p * lc < 64 && 3.0 * lc * (2 * p) <= mean_row_nnz + 1e-9) p <<= 1;
+    return csr_spmm_variant_index(p, lc, 1);
 }
 
 template <typename T>
@@ -270,7 +307,8 @@ void csr_spmm(hipStream_t s, const CsrView& A, int ncols, const T* X, int64_t ld
     for (int c0 = 0; c0 < ncols; c0 += 256) {
         const int nc = std::min(256, ncols - c0);
         int v = variant;
-        if (v < 0 || !csr_spmm_variant_supports(v, nc)) v = csr_spmm_default_variant(nc);
+        if (v < 0 || !csr_spmm_variant_supports(v, nc))
+            v = csr_spmm_default_variant(nc, A.rows > 0 ? static_cast<double>(A.nnz) / A.rows : 0.0);
         const T* add = Add ? Add + c0 : nullptr;
         switch (A.type) {
             case ValueType::f32:

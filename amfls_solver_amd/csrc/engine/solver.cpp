@@ -313,7 +313,7 @@ LeastSquaresResult amfls_driver(const DeviceOperator& op, const double* b_dev, c
             const double a0 = op_seconds();
             last_interval_width = frontier;
             while (ws.can_advance() && ws.depth() < target_depth) {
-                const bool advanced = ws.advance_level();
+                const bool advanced = ws.advance_toward(target_depth);
                 truncated = step_has_numerical_rank_truncation(ws.last_step_audit());
                 broke = step_has_nonfinite_breakdown(ws.last_step_audit());
                 if (!advanced || truncated || broke) break;

@@ -59,9 +59,9 @@ Workspace<T>::Workspace(const DeviceOperator& op, const double* b_dev, double re
     slots_.reset(kSlotCount);
     gdev_.reset(static_cast<size_t>(basis_limit_) + 2);
     V_.init(n_, std::min(basis_limit_, 64));
-    left_.init(m_, 8);
-    work_.init(m_, 8);
-    right_.init(n_, 8);
+    left_.init(m_, 1);
+    work_.init(m_, 1);
+    right_.init(n_, 1);
     T_.rows = 0;
 }
 
@@ -78,19 +78,24 @@ double Workspace<T>::block_norm(const T* X, int64_t ldx, int64_t rows, int cols)
 }
 
 template <typename T>
-double Workspace<T>::rank_cutoff(double scale, int k, int w) const {
+double Workspace<T>::rank_factor(int k, int w) const {
     // reference krylov_workspace.cpp:63-80 for binary64 blocks.  FP32 blocks
     // carry a rounding floor near u_32 * ||block||, so the cutoff is scaled
     // by the working precision instead of the problem dimension.
-    if (!(scale >= 0.0) || !std::isfinite(scale)) return kInfD;
     if constexpr (sizeof(T) == 8) {
         const double dim = static_cast<double>(std::max<int64_t>({1, m_, n_, static_cast<int64_t>(k),
                                                                   static_cast<int64_t>(w)}));
-        return 64.0 * std::numeric_limits<double>::epsilon() * dim * scale;
+        return 64.0 * std::numeric_limits<double>::epsilon() * dim;
     } else {
         const double dim = static_cast<double>(std::max({1, k, w}));
-        return 64.0 * std::numeric_limits<float>::epsilon() * std::sqrt(dim) * scale;
+        return 64.0 * std::numeric_limits<float>::epsilon() * std::sqrt(dim);
     }
+}
+
+template <typename T>
+double Workspace<T>::rank_cutoff(double scale, int k, int w) const {
+    if (!(scale >= 0.0) || !std::isfinite(scale)) return kInfD;
+    return rank_factor(k, w) * scale;
 }
 
 template <typename T>
@@ -161,13 +166,17 @@ typename Workspace<T>::Factor Workspace<T>::orthogonalize_and_factor(T* W, int64
     };
     const bool dgks = w == 1 && k > 0;
     if (dgks) {
-        // second pass when ||W|| fell below ||W_before|| / sqrt(2)
+        // one-column DGKS with the fused rank-one kernels: second pass when
+        // ||W|| fell below ||W_before|| / sqrt(2)
+        unsigned* ctr = ctx_.counter.get();
         ctx_.block_stats_async<T>(N, 1, W, ldw, sl + kBeforeSlot);
-        pass(nullptr);
-        ctx_.block_stats_async<T>(N, 1, W, ldw, sl + kFirstSlot);
-        second_pass_gate(s, sl + kBeforeSlot, sl + kFirstSlot, std::sqrt(0.5), sl + kGateSlot);
-        pass(sl + kGateSlot);
-        ctx_.block_stats_async<T>(N, 1, W, ldw, sl + kSecondSlot, sl + kGateSlot);
+        rank_one_project<T, T>(s, N, k, basis, ldb, W, ldw, cbuf_.get(), coeff_.get(), false, ctx_.scratch, ctr);
+        rank_one_update<T>(s, N, k, basis, ldb, cbuf_.get(), W, ldw, sl + kFirstSlot, ctx_.scratch, ctr, nullptr,
+                           nullptr, sl + kBeforeSlot, std::sqrt(0.5), sl + kGateSlot);
+        rank_one_project<T, T>(s, N, k, basis, ldb, W, ldw, cbuf_.get(), coeff_.get(), true, ctx_.scratch, ctr,
+                               sl + kGateSlot);
+        rank_one_update<T>(s, N, k, basis, ldb, cbuf_.get(), W, ldw, sl + kSecondSlot, ctx_.scratch, ctr,
+                           sl + kGateSlot);
     } else {
         for (int p = 0; p < (k > 0 ? 2 : 0); ++p) pass(nullptr);
         ctx_.block_stats_async<T>(N, w, W, ldw, sl + kFirstSlot);
@@ -480,9 +489,9 @@ bool Workspace<T>::advance_level() {
     if (lw > 0) {
         // g entries stay on the device until the next projected solve
         if (g_.size() + lw > gdev_.size()) throw std::logic_error("projected right-hand side exceeds its capacity");
-        gemm_tn<double>(s, m_, lw, 1, Strided{left_.ptr(), vt_of(T{}), left_.ld(), 1}, b_dev_, 1,
-                        gdev_.get() + g_.size(),
-                        ValueType::f64, 1, 1.0, ctx_.scratch);
+        cbuf_.ensure(static_cast<size_t>(lw));
+        rank_one_project<T, double>(s, m_, lw, left_.ptr(), left_.ld(), b_dev_, 1, gdev_.get() + g_.size(),
+                                    cbuf_.get(), false, ctx_.scratch, ctx_.counter.get());
         g_.resize(g_.size() + lw, std::numeric_limits<double>::quiet_NaN());
     }
     active_first_ = old_left;
@@ -493,6 +502,177 @@ bool Workspace<T>::advance_level() {
     ++depth_;
     audit_.basis_rank_after = k_;
     return true;
+}
+
+template <typename T>
+void Workspace<T>::enqueue_product(const T* X, int64_t ldx, T* Y, int64_t ldy, bool transpose) {
+    // Counted and (on the synthetic clock) charged when the level commits.
+    const bool wall = !ctx_.clock.synthetic();
+    if (wall) ctx_.timer.begin(ctx_.stream);
+    if (transpose) op_.apply_t(ctx_.stream, X, 1, ldx, Y, ldy);
+    else op_.apply(ctx_.stream, X, 1, ldx, Y, ldy);
+    if (wall) ctx_.timer.end(ctx_.stream, transpose ? &stats_.at_seconds : &stats_.a_seconds);
+}
+
+template <typename T>
+int Workspace<T>::fast_batch_limit() const {
+    // Frontier of width one with a 1 x 1 coupling and nothing pending: every
+    // level of the batch then has max_right = max_left = 1.
+    if (!initialized_ || !reliable_ || pending_ != 0 || active_width_ != 1 || prev_width_ != 1 || k_ < 1 ||
+        coupling_.rows != 1 || coupling_.cols != 1 || active_first_ + 1 != static_cast<int>(g_.size()))
+        return 0;
+    const int64_t old_left = static_cast<int64_t>(g_.size());
+    // the chain's update kernel keeps the k coefficients in LDS: k <= 2048
+    const int64_t lim = std::min<int64_t>({static_cast<int64_t>(basis_limit_) - k_, m_ - old_left,
+                                           static_cast<int64_t>(basis_limit_) + 1 - old_left, 64,
+                                           static_cast<int64_t>(2048) - k_});
+    return static_cast<int>(std::max<int64_t>(0, lim));
+}
+
+template <typename T>
+int Workspace<T>::run_fast_batch(int count) {
+    // Each level: r = A^T u - beta v_prev, DGKS against V, rank decision,
+    // v = r / ||r|| appended to V; l = A v - alpha u, DGKS against u, rank
+    // decision, u' = l / ||l||, g entry u'^T b (reference
+    // krylov_workspace.cpp:589-831 for a one-column frontier).  A declined
+    // decision clears the device flag `go`, which skips the rest of the
+    // batch; the host commits the accepted prefix.
+    hipStream_t s = ctx_.stream;
+    Clock& clk = ctx_.clock;
+    sync_projected_rhs();
+    const int k0 = k_;
+    const int L0 = static_cast<int>(g_.size());
+    ensure_basis(k0 + count);
+    const int nbS = static_cast<int>(std::max(ceil_div(n_, chain_rows_per_block(n_)),
+                                              ceil_div(m_, chain_rows_per_block(m_))));
+    const int nbN = static_cast<int>(ceil_div(m_, chain_rows_per_block(m_)));
+    int64_t psize = ceil_div(m_, chain_project_rows(m_, 1));
+    for (int j = 0; j < count; ++j)
+        psize = std::max<int64_t>(psize, ceil_div(n_, chain_project_rows(n_, k0 + j)) * (k0 + j));
+    fast_cells_.ensure(static_cast<size_t>(4 + count));
+    fast_rec_.ensure(static_cast<size_t>(8 * count));
+    fast_part_.ensure(static_cast<size_t>(4 * 3 * nbS + 2 * psize + static_cast<int64_t>(count) * nbN));
+    double* cells = fast_cells_.get();
+    double* go = cells, *alpha = cells + 1, *beta = cells + 2;  // beta[j]: coupling of level j
+    double* sP = fast_part_.get();      // scale partials
+    double* bP = sP + 3 * nbS;          // before
+    double* f1 = bP + 3 * nbS;          // after the first pass
+    double* f2 = f1 + 3 * nbS;          // after the second pass
+    double* P1 = f2 + 3 * nbS;          // projection partials, first pass
+    double* P2 = P1 + psize;            // second pass
+    double* gpart = P2 + psize;         // g partials, nbN per level
+    fill<double>(s, fast_rec_.get(), 8 * static_cast<int64_t>(count), 0.0);
+    fill<double>(s, go, 1, 1.0);
+    fill<double>(s, beta, 1, coupling_(0, 0));
+    T* buf[2] = {left_.ptr(), work_.ptr()};
+    const int64_t ldbuf[2] = {left_.ld(), work_.ld()};
+    T* r = right_.ptr();
+    const int64_t ldr = right_.ld();
+    T* V = V_.ptr();
+    const int64_t ldv = V_.ld();
+    const double fac_l = rank_factor(1, 1);
+    const double dgks = std::sqrt(0.5);
+    const int nbSn = static_cast<int>(ceil_div(n_, chain_rows_per_block(n_)));
+    const int nbSm = static_cast<int>(ceil_div(m_, chain_rows_per_block(m_)));
+    const int nbPm = static_cast<int>(ceil_div(m_, chain_project_rows(m_, 1)));
+    for (int j = 0; j < count; ++j) {
+        const int k = k0 + j;
+        T* u = buf[j % 2];
+        T* l = buf[(j + 1) % 2];
+        const int64_t ldu = ldbuf[j % 2], ldl = ldbuf[(j + 1) % 2];
+        double* rec = fast_rec_.get() + 8 * j;
+        const int nbPn = static_cast<int>(ceil_div(n_, chain_project_rows(n_, k)));
+        // right residual against V (n x k)
+        enqueue_product(u, ldu, r, ldr, true);
+        chain_couple<T>(s, n_, r, ldr, V + (k - 1), ldv, beta + j, go, sP, bP);
+        chain_project<T>(s, n_, k, V, ldv, r, ldr, P1, go, nullptr, nullptr, 0, 0, 0.0);
+        chain_update<T>(s, n_, k, V, ldv, P1, nbPn, r, ldr, f1, go, nullptr, nullptr, 0, 0, 0.0);
+        chain_project<T>(s, n_, k, V, ldv, r, ldr, P2, go, bP, f1, nbSn, nbSn, dgks);
+        chain_update<T>(s, n_, k, V, ldv, P2, nbPn, r, ldr, f2, go, bP, f1, nbSn, nbSn, dgks);
+        chain_normalize<T>(s, n_, r, ldr, V + k, ldv, sP, nbSn, bP, f1, f2, nbSn, dgks, rank_factor(k, 1), nullptr,
+                           nullptr, 0, nullptr, nullptr, rec, alpha, go);
+        // left residual against u
+        enqueue_product(V + k, ldv, l, ldl, false);
+        chain_couple<T>(s, m_, l, ldl, u, ldu, alpha, go, sP, bP);
+        chain_project<T>(s, m_, 1, u, ldu, l, ldl, P1, go, nullptr, nullptr, 0, 0, 0.0);
+        chain_update<T>(s, m_, 1, u, ldu, P1, nbPm, l, ldl, f1, go, nullptr, nullptr, 0, 0, 0.0);
+        chain_project<T>(s, m_, 1, u, ldu, l, ldl, P2, go, bP, f1, nbSm, nbSm, dgks);
+        chain_update<T>(s, m_, 1, u, ldu, P2, nbPm, l, ldl, f2, go, bP, f1, nbSm, nbSm, dgks);
+        chain_normalize<T>(s, m_, l, ldl, l, ldl, sP, nbSm, bP, f1, f2, nbSm, dgks, fac_l, P1, P2, nbPm, b_dev_,
+                           gpart + static_cast<int64_t>(j) * nbN, rec + 4, beta + j + 1, go);
+    }
+    fast_gpart_.resize(static_cast<size_t>(count) * nbN);
+    fast_host_.resize(static_cast<size_t>(8 * count));
+    AMFLS_HIP_CHECK(hipMemcpyAsync(fast_gpart_.data(), gpart, fast_gpart_.size() * sizeof(double),
+                                   hipMemcpyDeviceToHost, s));
+    ctx_.download(fast_host_.data(), fast_rec_.get(), fast_host_.size());
+    // commit the accepted prefix exactly as advance_level would have
+    int done = 0;
+    for (; done < count; ++done) {
+        const double* rec = fast_host_.data() + 8 * done;
+        if (rec[2] != 1.0 || rec[6] != 1.0) break;
+        const double alpha = rec[0], scale_r = rec[1], beta_new = rec[4], scale_l = rec[5], c_l = rec[7];
+        double gv = 0.0;
+        for (int q = 0; q < nbN; ++q) gv += fast_gpart_[static_cast<size_t>(done) * nbN + q];
+        const int k = k_;
+        const int old_left = static_cast<int>(g_.size());
+        for (int side = 0; side < 2; ++side) {
+            const double start = clk.now();
+            clk.charge_operator(op_.relative_block_product_cost(1));
+            if (clk.synthetic()) (side == 0 ? stats_.at_seconds : stats_.a_seconds) += clk.now() - start;
+        }
+        stats_.at_columns += 1;
+        stats_.search_at_columns += 1;
+        ++stats_.at_block_calls;
+        ++stats_.search_at_block_calls;
+        stats_.a_columns += 1;
+        stats_.search_a_columns += 1;
+        ++stats_.a_block_calls;
+        ++stats_.search_a_block_calls;
+        clk.charge_local(1, k);
+        clk.charge_local(1, 1);
+        StepAudit a;
+        a.requested_injected_columns = pending_requested_;
+        a.retained_injected_columns = pending_retained_;
+        pending_requested_ = pending_retained_ = 0;
+        a.left_input_width = 1;
+        a.basis_rank_before = k;
+        a.basis_rank_after = k + 1;
+        a.retained_right_width = 1;
+        a.right_rank_cutoff = rank_cutoff(scale_r, k, 1);
+        a.retained_next_left_width = 1;
+        a.left_rank_cutoff = rank_cutoff(scale_l, 1, 1);
+        audit_ = a;
+        std::vector<double> col(2);
+        col[0] = alpha + c_l;
+        col[1] = beta_new;
+        T_.lo.push_back(active_first_);
+        T_.val.push_back(std::move(col));
+        T_.rows = old_left + 1;
+        g_.push_back(gv);
+        g_host_ = g_.size();
+        k_ += 1;
+        active_first_ = old_left;
+        active_width_ = 1;
+        prev_first_ = k;
+        prev_width_ = 1;
+        coupling_ = HostMat(1, 1);
+        coupling_(0, 0) = beta_new;
+        ++depth_;
+    }
+    if (done % 2 == 1) std::swap(left_, work_);
+    return done;
+}
+
+template <typename T>
+bool Workspace<T>::advance_toward(int target_depth) {
+    const int want = target_depth - depth_;
+    if (want <= 0) return advance_level();
+    const int count = std::min(want, fast_batch_limit());
+    if (count <= 0) return advance_level();
+    const int done = run_fast_batch(count);
+    if (done == count) return true;
+    return advance_level();  // the declined level, host-driven
 }
 
 template <typename T>

@@ -64,6 +64,13 @@ public:
     int append_left_seed_gaussian(int cols, uint64_t seed, uint64_t stream, uint64_t first_col,
                                   bool orthogonal_to_rhs);
     bool advance_level();
+    // Advance toward `target_depth`: while the frontier is one column wide
+    // (no pending injected columns), levels are enqueued in batches whose
+    // rank decisions are taken on the device and read back once per batch;
+    // otherwise, or for a level the device declined, one host-driven level
+    // (advance_level).  Returns false, like advance_level, when the last
+    // level attempted did not extend the basis; the audit is that level's.
+    bool advance_toward(int target_depth);
     LeastSquaresResult evaluate(double tolerance, int epoch, int auxiliary_width);
     // Candidate x = V y from the current projected solve (device, double).
     void candidate(double* x_dev);
@@ -96,6 +103,11 @@ private:
     static constexpr int kScaleSlot = 0, kBeforeSlot = 3, kFirstSlot = 6, kSecondSlot = 9, kCoeffSlot = 12,
                          kGateSlot = 15, kSlotCount = 16;
     void enqueue_scale(const T* X, int64_t ldx, int64_t rows, int cols);
+    // Device-resident rank-one levels.
+    int fast_batch_limit() const;
+    int run_fast_batch(int count);  // returns levels committed
+    void enqueue_product(const T* X, int64_t ldx, T* Y, int64_t ldy, bool transpose);
+    double rank_factor(int k, int w) const;  // rank cutoff = factor * scale
     void sync_projected_rhs();
     double block_norm(const T* X, int64_t ldx, int64_t rows, int cols);
     double rank_cutoff(double scale, int k, int w) const;
@@ -120,6 +132,10 @@ private:
     DevBuf<double> cbuf_;    // pass coefficients
     DevBuf<double> gdev_;    // g = U^T b entries computed on the device
     DevBuf<double> slots_;   // orthogonalization statistics (kSlotCount doubles)
+    DevBuf<double> fast_cells_;  // go, gate, inverse norm, then one coupling cell per batched level
+    DevBuf<double> fast_rec_;    // 8 doubles per batched level (right and left decisions)
+    DevBuf<double> fast_part_;   // per-block partials of the level chain
+    std::vector<double> fast_host_, fast_gpart_;
     DevBuf<double> gram_;    // w x w Gram matrices
     DevBuf<T> tmp_;          // wide row-local products
     int k_ = 0;

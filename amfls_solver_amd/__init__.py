@@ -14,4 +14,8 @@ from .binding import (  # noqa: F401
     run_lsqr,
 )
 
+from . import torch_ops as _torch_ops
+
+_torch_ops.register()
+
 __all__ = ["Context", "DeviceMatrix", "SolveResult", "run_amfls", "run_fixed_rbgk", "run_lsqr"]

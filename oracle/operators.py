@@ -14,11 +14,15 @@ each still obeys the gamma_k bound that positive_sum_{upper,lower}_bound
 assume, so the bounds stay valid and agree with the reference to the last
 few ulps.
 
-Pinned by tests/test_oracle_operators.py: products against a brute-force
-Python loop on tiny matrices, and the norm bounds against numpy's exact
-2-norm (lower <= ||A||_2 <= upper) plus closed forms for diagonal and rank-one
-matrices.  The long-double refinement products are checked against exact
-rational arithmetic.
+Pinned by tests/test_oracle_solvers.py: CSR products against the dense
+product of the same matrix, the norm bounds against numpy's 2-norm
+(lower <= ||A||_2 <= upper, and upper >= || |A| ||_2), the step counts
+2 max_row_nnz + 2 / 2 max_col_nnz + 2, the rank-one closed form
+||u v^T|| = ||u|| ||v||, the CSR structure checks (duplicates, range,
+nonfinite values) and the long-double refinement product exact on small
+integers (test_operator_products_and_norm_bounds,
+test_csr_constructor_rejects_bad_structure,
+test_long_double_refinement_is_exact_on_small_integers).
 """
 from __future__ import annotations
 

@@ -2,8 +2,9 @@
 
 Follows include/amfls/least_squares_result.hpp:8-229 and
 bindings/torch/result_tensors.hpp:12-156 (metrics 21, counters 36,
-timings 9, trace_float 29, trace_int 4).  Pure data; pinned by the field
-order checks in tests/test_oracle_results.py.
+timings 9, trace_float 29, trace_int 4).  Pure data; pinned by the widths and
+field positions checked in tests/test_oracle_solvers.py
+(test_result_tensor_layout_widths).
 """
 from __future__ import annotations
 

@@ -1,9 +1,10 @@
 """Certificate checkpoint cadence (oracle).
 
 Follows src/algorithms/mathematics/krylov/certificate_checkpoint_schedule.hpp:15-131.
-Pinned by tests/test_oracle_schedule.py: an independent enumeration of the
+Pinned by tests/test_oracle_controller.py: an independent enumeration of the
 static cadence (powers of two, plus 3*2^j for lambda = 0) and hand-computed
-log-linear crossings.
+log-linear crossings (test_static_checkpoints_enumeration,
+test_predicted_crossing_log_linear).
 """
 from __future__ import annotations
 

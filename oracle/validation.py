@@ -11,10 +11,13 @@ fma(lambda, x, A^T r) is evaluated in numpy longdouble and rounded once to
 binary64; where the 64-bit product is inexact this can differ from a true
 fused multiply-add by one ulp, which the tests' tolerances absorb.
 
-Pinned by tests/test_oracle_validation.py: exact-solution candidates give
-exact_stationarity; certified bounds are checked to enclose the true
-backward error / energy error computed independently with numpy SVD on
-small problems; hand-computed bounds on the README diagonal operator.
+Pinned by tests/test_oracle_solvers.py: the certified bounds enclose the
+true least-squares and compatible backward errors and the true ridge energy
+error computed independently (numpy 2-norm, closed-form ridge solution) on
+seeded problems, and an exact solution of a diagonal system certifies the
+compatible backward error below 1e-14 with a zero normal residual
+(test_validation_bounds_enclose_true_backward_error,
+test_ridge_energy_bound_encloses_truth).
 """
 from __future__ import annotations
 

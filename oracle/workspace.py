@@ -10,10 +10,11 @@ Thin SVD is numpy.linalg.svd(full_matrices=False) (LAPACK dgesdd, as
 svd.cpp:23-60); the lambda = 0 projected solve is scipy's dgelsy with the
 reference's rcond (svd.cpp:90-153).
 
-Pinned by tests/test_oracle_workspace.py: Krylov invariants A V = U_all T
-(reconstructed from the retained state), V^T V = I, nested objective
-decrease, the projected solution against numpy.linalg.lstsq / closed-form
-ridge, and exact breakdown on rank-deficient operators.
+Pinned by tests/test_oracle_solvers.py: V^T V = I, nested decrease of the
+projected objective, the candidate equal to the least-squares solution over
+span(V) computed directly from A (test_workspace_krylov_invariants), and
+on a rank-deficient operator the basis stops at the rank with the
+minimum-norm solution (test_rank_deficient_operator_stops_at_rank).
 """
 from __future__ import annotations
 

@@ -35,3 +35,24 @@ def test_errors_cross_the_boundary_as_codes():
     assert lib.amfls_operator_shape(None, None, None) == 1
     res = ctypes.c_void_p()
     assert lib.amfls_solve(None, None, None, 0, None, ctypes.byref(res)) == 1
+
+
+def test_torch_ops_schemas_match_reference():
+    """torch.ops.amfls carries the reference's schemas
+    (bindings/torch/torch_binding.cpp:510-525)."""
+    import torch
+    import amfls_solver_amd  # noqa: F401  (registers the ops)
+    want = {
+        "run_amfls": "amfls::run_amfls(Tensor matrix, Tensor b, float regularization, float tolerance, "
+                     "float failure_probability, int maximum_epochs=0, int maximum_depth=0, "
+                     "int maximum_auxiliary_width=0, int maximum_basis_size=0, int seed=0, int stream=0) -> "
+                     "(Tensor, Tensor, Tensor, Tensor, Tensor, Tensor, int, int)",
+        "run_fixed_rbgk": "amfls::run_fixed_rbgk(Tensor matrix, Tensor b, float regularization, float tolerance, "
+                          "float failure_probability, int auxiliary_width, int maximum_depth, "
+                          "int maximum_basis_size, int seed, int stream) -> "
+                          "(Tensor, Tensor, Tensor, Tensor, Tensor, Tensor, int, int)",
+        "run_lsqr": "amfls::run_lsqr(Tensor matrix, Tensor b, float regularization, float tolerance, "
+                    "int maximum_iterations=0) -> (Tensor, Tensor, Tensor, Tensor, Tensor, Tensor, int, int)",
+    }
+    for name, schema in want.items():
+        assert str(getattr(torch.ops.amfls, name).default._schema) == schema

@@ -247,3 +247,24 @@ def test_readme_diagonal_example(torch_mod):
     r = am.run_amfls(A, torch.tensor([100.0, 6.0, 6.0, 4.0], dtype=torch.float64), tolerance=1e-10, seed=7)
     assert r.success
     assert np.allclose(r.solution.numpy(), [1, 2, 3, 4], atol=1e-12)
+
+
+def test_torch_ops_entry_points(torch_mod, fx):
+    """torch.ops.amfls.* on dense host tensors and on device CSR tensors give
+    the reference's 8-tuple and the same answers as the Python mirror."""
+    import amfls_solver_amd as am
+    torch = torch_mod
+    A = torch.diag(torch.tensor([100.0, 3.0, 2.0, 1.0], dtype=torch.float64))
+    b = torch.tensor([100.0, 6.0, 6.0, 4.0], dtype=torch.float64)
+    out = torch.ops.amfls.run_amfls(A, b, 0.0, 1e-10, 1e-6, seed=7)
+    assert len(out) == 8 and out[6] == 0
+    assert out[1].shape == (21,) and out[2].shape == (36,) and out[3].shape == (9,)
+    assert out[4].shape[1] == 29 and out[5].shape[1] == 4
+    assert np.allclose(out[0].numpy(), [1, 2, 3, 4], atol=1e-12)
+    Ad = csr_tensor(torch, fx["csr_rp"], fx["csr_ci"], fx["csr_val"], (3000, 700)).to("cuda")
+    bd = torch.from_numpy(fx["csr_b"]).cuda()
+    o1 = torch.ops.amfls.run_lsqr(Ad, bd, 1e-3, 2e-9)
+    r1 = am.run_lsqr(Ad, bd, regularization=1e-3, tolerance=2e-9)
+    assert o1[6] == r1.status == 0 and np.allclose(o1[0].numpy(), r1.solution.numpy(), rtol=0, atol=1e-12)
+    o2 = torch.ops.amfls.run_fixed_rbgk(Ad, bd, 1e-3, 1e-9, 1e-6, 2, 6, 0, 3, 0)
+    assert o2[2][9] == 6  # depth counter

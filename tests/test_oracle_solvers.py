@@ -188,3 +188,15 @@ def test_result_tensor_layout_widths():
     assert met.shape == (21,) and cnt.shape == (36,) and tim.shape == (9,)
     assert tf.shape == (len(r.trace), 29) and ti.shape == (len(r.trace), 4)
     assert met[10] == r.backward_error_upper_bound and cnt[11] == r.basis_rank
+
+
+def test_rank_deficient_operator_stops_at_rank():
+    # A = P Q^T of rank 3: the Krylov basis cannot exceed 3 columns and the
+    # certified answer is the minimum-norm least-squares solution.
+    r = np.random.default_rng(21)
+    a = r.standard_normal((60, 3)) @ r.standard_normal((3, 12))
+    b = r.standard_normal(60)
+    res = S.solve_amfls(O.DenseOperator(a), b, tolerance=1e-10, clock=SyntheticClock())
+    assert res.basis_rank <= 3
+    xo = np.linalg.lstsq(a, b, rcond=None)[0]
+    assert np.linalg.norm(res.solution - xo) / np.linalg.norm(xo) < 1e-8

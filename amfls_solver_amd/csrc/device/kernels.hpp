@@ -94,35 +94,31 @@ void rank_one_update(hipStream_t s, int64_t N, int k, const T* B, int64_t ldb, c
                      double* gate_out = nullptr);
 
 // ---------------------------------------------------------------- level chain
-// Batched rank-one level kernels (level_chain.hip).  Producers write one
-// partial per block; consumers combine all partials; a kernel returns at
-// once while *go == 0.  Statistics partials are triples, one per block of
-// chain_rows_per_block(N) rows; projection partials are k per block of
-// chain_project_rows(N, k) rows.  The optional (before, after) partials turn
-// a projection/update into the DGKS second pass, run only when
-// ||after|| < factor ||before||.
-int chain_rows_per_block(int64_t N);
-int chain_project_rows(int64_t N, int k);
-// y = r - (*coef) p in place on r; statistics partials of r and y.
+// Batched rank-one level kernels (level_chain.hip).  All kernels of one side
+// of a level use blocks of chain_block_rows(N, k) rows, write one partial
+// per block (statistics triples; k projection sums) and combine their
+// producer's partials in every block; a kernel returns at once while
+// *go == 0.
+int chain_block_rows(int64_t N, int k);
+// y = r - (*coef) p in place on r; statistics partials of r and y;
+// projection partials Q of B^T y.
 template <typename T>
-void chain_couple(hipStream_t s, int64_t N, T* r, int64_t ldr, const T* p, int64_t ldp, const double* coef,
-                  const double* go, double* part_r, double* part_y);
-// P[b][o] = sum over block b's rows of B[i, o] y[i].
+void chain_couple_project(hipStream_t s, int64_t N, int k, T* r, int64_t ldr, const T* p, int64_t ldp,
+                          const double* coef, const T* B, int64_t ldb, const double* go, double* part_r,
+                          double* part_y, double* Q);
+// y -= B (sum_b Qin[b]); statistics partials of y; with Qout, projection
+// partials of the new y.  With before/after given it is the DGKS second
+// pass and runs only when ||after|| < factor ||before||.
 template <typename T>
-void chain_project(hipStream_t s, int64_t N, int k, const T* B, int64_t ldb, const T* y, int64_t ldy, double* P,
-                   const double* go, const double* before, const double* after, int nb_before, int nb_after,
-                   double factor);
-// y -= B (sum_b P[b]); statistics partials of y.
-template <typename T>
-void chain_update(hipStream_t s, int64_t N, int k, const T* B, int64_t ldb, const double* P, int nbP, T* y,
-                  int64_t ldy, double* part_y, const double* go, const double* before, const double* after,
-                  int nb_before, int nb_after, double factor);
+void chain_update(hipStream_t s, int64_t N, int k, const T* B, int64_t ldb, const double* Qin, T* y, int64_t ldy,
+                  double* part_y, double* Qout, const double* go, const double* before, const double* after,
+                  double factor);
 // Rank decision and normalization (dst = y / ||y||), see level_chain.hip.
 template <typename T>
-void chain_normalize(hipStream_t s, int64_t N, const T* y, int64_t ldy, T* dst, int64_t ldd, const double* part_scale,
-                     int nb_scale, const double* part_before, const double* part_first, const double* part_second,
-                     int nb_update, double dgks_factor, double rank_factor, const double* P1, const double* P2,
-                     int nbP, const double* b, double* gpart, double* record, double* cell, double* go);
+void chain_normalize(hipStream_t s, int64_t N, int k, const T* y, int64_t ldy, T* dst, int64_t ldd,
+                     const double* part_scale, const double* part_before, const double* part_first,
+                     const double* part_second, double dgks_factor, double rank_factor, const double* Q1,
+                     const double* Q2, const double* b, double* gpart, double* record, double* cell, double* go);
 
 // Counter-addressed Gaussian block, index = (first_col + j) * rows + i.
 template <typename T>

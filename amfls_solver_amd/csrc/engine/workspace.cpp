@@ -543,24 +543,29 @@ int Workspace<T>::run_fast_batch(int count) {
     const int k0 = k_;
     const int L0 = static_cast<int>(g_.size());
     ensure_basis(k0 + count);
-    const int nbS = static_cast<int>(std::max(ceil_div(n_, chain_rows_per_block(n_)),
-                                              ceil_div(m_, chain_rows_per_block(m_))));
-    const int nbN = static_cast<int>(ceil_div(m_, chain_rows_per_block(m_)));
-    int64_t psize = ceil_div(m_, chain_project_rows(m_, 1));
-    for (int j = 0; j < count; ++j)
-        psize = std::max<int64_t>(psize, ceil_div(n_, chain_project_rows(n_, k0 + j)) * (k0 + j));
+    // partial buffers sized for the largest block counts of the batch
+    int64_t nbmax = ceil_div(m_, chain_block_rows(m_, 1));
+    int64_t qmax = nbmax;
+    for (int j = 0; j < count; ++j) {
+        const int64_t nbj = ceil_div(n_, chain_block_rows(n_, k0 + j));
+        nbmax = std::max(nbmax, nbj);
+        qmax = std::max<int64_t>(qmax, nbj * (k0 + j));
+    }
+    const int nbN = static_cast<int>(ceil_div(m_, chain_block_rows(m_, 1)));
     fast_cells_.ensure(static_cast<size_t>(4 + count));
     fast_rec_.ensure(static_cast<size_t>(8 * count));
-    fast_part_.ensure(static_cast<size_t>(4 * 3 * nbS + 2 * psize + static_cast<int64_t>(count) * nbN));
+    // statistics partials: one triple per wavefront (4 per block)
+    const int64_t spart = 3 * 4 * nbmax;
+    fast_part_.ensure(static_cast<size_t>(4 * spart + 2 * qmax + static_cast<int64_t>(count) * nbN));
     double* cells = fast_cells_.get();
     double* go = cells, *alpha = cells + 1, *beta = cells + 2;  // beta[j]: coupling of level j
     double* sP = fast_part_.get();      // scale partials
-    double* bP = sP + 3 * nbS;          // before
-    double* f1 = bP + 3 * nbS;          // after the first pass
-    double* f2 = f1 + 3 * nbS;          // after the second pass
-    double* P1 = f2 + 3 * nbS;          // projection partials, first pass
-    double* P2 = P1 + psize;            // second pass
-    double* gpart = P2 + psize;         // g partials, nbN per level
+    double* bP = sP + spart;            // before
+    double* f1 = bP + spart;            // after the first pass
+    double* f2 = f1 + spart;            // after the second pass
+    double* Q1 = f2 + spart;            // projection partials, first pass
+    double* Q2 = Q1 + qmax;             // second pass
+    double* gpart = Q2 + qmax;          // g partials, nbN per level
     fill<double>(s, fast_rec_.get(), 8 * static_cast<int64_t>(count), 0.0);
     fill<double>(s, go, 1, 1.0);
     fill<double>(s, beta, 1, coupling_(0, 0));
@@ -572,33 +577,25 @@ int Workspace<T>::run_fast_batch(int count) {
     const int64_t ldv = V_.ld();
     const double fac_l = rank_factor(1, 1);
     const double dgks = std::sqrt(0.5);
-    const int nbSn = static_cast<int>(ceil_div(n_, chain_rows_per_block(n_)));
-    const int nbSm = static_cast<int>(ceil_div(m_, chain_rows_per_block(m_)));
-    const int nbPm = static_cast<int>(ceil_div(m_, chain_project_rows(m_, 1)));
     for (int j = 0; j < count; ++j) {
         const int k = k0 + j;
         T* u = buf[j % 2];
         T* l = buf[(j + 1) % 2];
         const int64_t ldu = ldbuf[j % 2], ldl = ldbuf[(j + 1) % 2];
         double* rec = fast_rec_.get() + 8 * j;
-        const int nbPn = static_cast<int>(ceil_div(n_, chain_project_rows(n_, k)));
         // right residual against V (n x k)
         enqueue_product(u, ldu, r, ldr, true);
-        chain_couple<T>(s, n_, r, ldr, V + (k - 1), ldv, beta + j, go, sP, bP);
-        chain_project<T>(s, n_, k, V, ldv, r, ldr, P1, go, nullptr, nullptr, 0, 0, 0.0);
-        chain_update<T>(s, n_, k, V, ldv, P1, nbPn, r, ldr, f1, go, nullptr, nullptr, 0, 0, 0.0);
-        chain_project<T>(s, n_, k, V, ldv, r, ldr, P2, go, bP, f1, nbSn, nbSn, dgks);
-        chain_update<T>(s, n_, k, V, ldv, P2, nbPn, r, ldr, f2, go, bP, f1, nbSn, nbSn, dgks);
-        chain_normalize<T>(s, n_, r, ldr, V + k, ldv, sP, nbSn, bP, f1, f2, nbSn, dgks, rank_factor(k, 1), nullptr,
-                           nullptr, 0, nullptr, nullptr, rec, alpha, go);
+        chain_couple_project<T>(s, n_, k, r, ldr, V + (k - 1), ldv, beta + j, V, ldv, go, sP, bP, Q1);
+        chain_update<T>(s, n_, k, V, ldv, Q1, r, ldr, f1, Q2, go, nullptr, nullptr, 0.0);
+        chain_update<T>(s, n_, k, V, ldv, Q2, r, ldr, f2, nullptr, go, bP, f1, dgks);
+        chain_normalize<T>(s, n_, k, r, ldr, V + k, ldv, sP, bP, f1, f2, dgks, rank_factor(k, 1), nullptr, nullptr,
+                           nullptr, nullptr, rec, alpha, go);
         // left residual against u
         enqueue_product(V + k, ldv, l, ldl, false);
-        chain_couple<T>(s, m_, l, ldl, u, ldu, alpha, go, sP, bP);
-        chain_project<T>(s, m_, 1, u, ldu, l, ldl, P1, go, nullptr, nullptr, 0, 0, 0.0);
-        chain_update<T>(s, m_, 1, u, ldu, P1, nbPm, l, ldl, f1, go, nullptr, nullptr, 0, 0, 0.0);
-        chain_project<T>(s, m_, 1, u, ldu, l, ldl, P2, go, bP, f1, nbSm, nbSm, dgks);
-        chain_update<T>(s, m_, 1, u, ldu, P2, nbPm, l, ldl, f2, go, bP, f1, nbSm, nbSm, dgks);
-        chain_normalize<T>(s, m_, l, ldl, l, ldl, sP, nbSm, bP, f1, f2, nbSm, dgks, fac_l, P1, P2, nbPm, b_dev_,
+        chain_couple_project<T>(s, m_, 1, l, ldl, u, ldu, alpha, u, ldu, go, sP, bP, Q1);
+        chain_update<T>(s, m_, 1, u, ldu, Q1, l, ldl, f1, Q2, go, nullptr, nullptr, 0.0);
+        chain_update<T>(s, m_, 1, u, ldu, Q2, l, ldl, f2, nullptr, go, bP, f1, dgks);
+        chain_normalize<T>(s, m_, 1, l, ldl, l, ldl, sP, bP, f1, f2, dgks, fac_l, Q1, Q2, b_dev_,
                            gpart + static_cast<int64_t>(j) * nbN, rec + 4, beta + j + 1, go);
     }
     fast_gpart_.resize(static_cast<size_t>(count) * nbN);

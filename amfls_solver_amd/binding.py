@@ -276,6 +276,23 @@ class DeviceMatrix:
             self.handle = None
 
 
+def gaussian_block(rows: int, cols: int, seed: int, stream: int, *, dtype: torch.dtype = torch.float64,
+                   first_column: int = 0, context: "Context | None" = None) -> torch.Tensor:
+    """Counter-addressed N(0,1) block on the device (reference
+    random/gaussian.cpp:39-55): entry (i, j) = gaussian_at(seed, stream,
+    (first_column + j) * rows + i), returned row-major (rows x cols)."""
+    if dtype not in (torch.float64, torch.float32):
+        raise ValueError("gaussian blocks are float64 or float32")
+    ctx = context or default_context()
+    out = torch.empty((rows, cols), dtype=dtype, device=_require_gpu())
+    torch.cuda.synchronize()
+    _lib.check(ctx._lib.amfls_gaussian_fill(ctx.handle, out.data_ptr(), int(rows), int(cols), int(cols),
+                                            _DTYPE_CODE[dtype], int(seed) % _UINT64_MODULUS,
+                                            int(stream) % _UINT64_MODULUS, int(first_column) % _UINT64_MODULUS))
+    ctx.synchronize()
+    return out
+
+
 def _canonical_uint64_to_int64_transport(value: int, name: str) -> int:
     if isinstance(value, bool) or not isinstance(value, int):
         raise ValueError(f"{name} must be an integer in [0, 2**64 - 1]")

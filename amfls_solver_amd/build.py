@@ -26,6 +26,7 @@ SOURCES = [
     CSRC / "device" / "spmm.hip",
     CSRC / "device" / "rank_one.hip",
     CSRC / "device" / "level_chain.hip",
+    CSRC / "device" / "dense_mfma.hip",
     CSRC / "engine" / "operator.cpp",
     CSRC / "engine" / "small_la.cpp",
     CSRC / "engine" / "validation.cpp",

@@ -1,0 +1,486 @@
+// Dense tall-matrix products on the gfx950 matrix cores.
+//
+// Y = A X and Z = A^T Y for a row-major A (m x n, stored bf16, fp32 or fp64)
+// and row-major blocks X (n x s), Y (m x s) with s <= 64, in three modes:
+//
+//   split3  A bf16, blocks fp32/fp64: each block value is split into three
+//           bf16 terms (hi + mid + lo, 24 significant bits) and multiplied by
+//           the exactly representable bf16 A with v_mfma_f32_16x16x32_bf16,
+//           fp32 accumulation: fp32-accurate products at bf16 matrix-core
+//           rate, bound by the 2 bytes per entry of A.
+//   f32     v_mfma_f32_16x16x4_f32 on A and X rounded to fp32.
+//   f64     v_mfma_f64_16x16x4_f64 on A upconverted to fp64 (exact for bf16
+//           and fp32 storage) and fp64 blocks: fp64-accurate products.
+//
+// MFMA operands (16 x 16 tiles, wave64): the A operand of lane l holds row
+// l % 16, the B operand column l % 16; the reduction index of element t of
+// lane group q = l / 16 is assigned the same way in both operands (the K
+// dimension is a sum, so any consistent assignment is exact); the 16 x 16
+// result of lane l holds column l % 16, rows 4 q + r (r < 4) for the f32 and
+// bf16 instructions but rows 4 r + q for v_mfma_f64_16x16x4_f64 (both
+// measured on gfx950 with one-hot operands).
+//
+// Y = A X: one wavefront per 16 rows, K stepped by 32 (eight consecutive
+// entries of an A row per lane, one 16-byte load for bf16), X read through
+// the cache (it is small).  Z = A^T Y: a block owns a strip of 64 columns of
+// A and a chunk of rows; 32 x 64 tiles of A and 32 x s tiles of Y are staged
+// in LDS with coalesced loads; per-chunk partial sums are combined by a
+// second kernel in a fixed order (deterministic, no atomics).
+#include "kernels.hpp"
+
+#include <algorithm>
+
+namespace amflsx {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+constexpr int kSplit3 = 0, kF32 = 1, kF64 = 2;
+constexpr int kThreads = 256;
+
+template <int MODE> struct AccOf { typedef floatx4 type; };
+template <> struct AccOf<kF64> { typedef doublex4 type; };
+
+// Row (within the 16 x 16 tile) of result element r of lane group q.
+template <int MODE>
+__device__ __forceinline__ int result_row(int q, int r) { return MODE == kF64 ? 4 * r + q : 4 * q + r; }
+
+__device__ __forceinline__ double to_d(double v) { return v; }
+__device__ __forceinline__ double to_d(float v) { return static_cast<double>(v); }
+__device__ __forceinline__ double to_d(__hip_bfloat16 v) { return static_cast<double>(__bfloat162float(v)); }
+
+__device__ __forceinline__ __bf16 to_bf16(float v) { return static_cast<__bf16>(v); }
+
+// Three-term bf16 split of v: hi = bf16(v), mid = bf16(v - hi), lo = bf16(v - hi - mid).
+__device__ __forceinline__ void split3(double v, __bf16& hi, __bf16& mid, __bf16& lo) {
+    const float f = static_cast<float>(v);
+    hi = to_bf16(f);
+    const float r1 = f - static_cast<float>(hi);
+    mid = to_bf16(r1);
+    lo = to_bf16(r1 - static_cast<float>(mid));
+}
+
+// One K=32 step: a[t] (this lane's 8 A-operand values) times b[t] (this
+// lane's 8 B-operand values) accumulated into acc.
+template <int MODE>
+__device__ __forceinline__ void mfma_step(const double* a, const double* b,
+                                          typename AccOf<MODE>::type& acc) {
+    if constexpr (MODE == kSplit3) {
+        bf16x8 av, h, m, l;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            av[t] = static_cast<__bf16>(static_cast<float>(a[t]));  // exact: a is bf16 data
+            __bf16 x0, x1, x2;
+            split3(b[t], x0, x1, x2);
+            h[t] = x0;
+            m[t] = x1;
+            l[t] = x2;
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, l, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, m, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, h, acc, 0, 0, 0);
+    } else if constexpr (MODE == kF32) {
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(static_cast<float>(a[t]), static_cast<float>(b[t]), acc, 0, 0,
+                                                       0);
+    } else {
+#pragma unroll
+        for (int t = 0; t < 8; ++t) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], b[t], acc, 0, 0, 0);
+    }
+}
+
+// Eight consecutive entries A[row][k0 .. k0+8) (zero past n or past m).
+template <typename TA>
+__device__ __forceinline__ void load_row8(const TA* __restrict__ A, int64_t lda, int64_t row, int64_t M, int64_t k0,
+                                          int64_t N, double* out) {
+    if (row < M && k0 + 8 <= N) {
+        const TA* p = A + row * lda + k0;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) out[t] = to_d(p[t]);
+    } else {
+#pragma unroll
+        for (int t = 0; t < 8; ++t) out[t] = (row < M && k0 + t < N) ? to_d(A[row * lda + k0 + t]) : 0.0;
+    }
+}
+
+// Operand form of the small block X (N x s): for column c and reduction
+// index k, element (c, k) at XT[c * nk + k], nk = N rounded up to 64, zero
+// padded; split3 stores three bf16 planes (hi, mid, lo), the other modes the
+// values in the MFMA input type.
+template <typename TX, int MODE>
+__global__ void pack_x_kernel(int64_t N, int s, int ncp, int64_t nk, const TX* __restrict__ X, int64_t ldx,
+                              void* __restrict__ XT) {
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= static_cast<int64_t>(ncp) * nk) return;
+    const int c = static_cast<int>(e / nk);
+    const int64_t k = e % nk;
+    const double v = (c < s && k < N) ? static_cast<double>(X[k * ldx + c]) : 0.0;
+    if constexpr (MODE == kSplit3) {
+        __bf16* p = static_cast<__bf16*>(XT);
+        const int64_t plane = static_cast<int64_t>(ncp) * nk;
+        __bf16 h, m, l;
+        split3(v, h, m, l);
+        p[e] = h;
+        p[plane + e] = m;
+        p[2 * plane + e] = l;
+    } else if constexpr (MODE == kF32) {
+        static_cast<float*>(XT)[e] = static_cast<float>(v);
+    } else {
+        static_cast<double*>(XT)[e] = v;
+    }
+}
+
+// Eight consecutive stored entries of A (row, k0..k0+8), zero outside.
+template <typename TA>
+struct Frag8 {
+    TA v[8];
+    __device__ __forceinline__ void load(const TA* __restrict__ A, int64_t lda, int64_t row, int64_t M, int64_t k0,
+                                         int64_t N) {
+        if (row < M && k0 + 8 <= N) {
+            const TA* p = A + row * lda + k0;
+            if ((reinterpret_cast<uintptr_t>(p) & 15) == 0) {
+                // 16-byte vector loads (one for bf16, two for fp32, four for fp64)
+                constexpr int W = static_cast<int>(sizeof(TA) * 8 / 16);
+                const uint4* src = reinterpret_cast<const uint4*>(p);
+                uint4 raw[W];
+#pragma unroll
+                for (int w = 0; w < W; ++w) raw[w] = src[w];
+                __builtin_memcpy(v, raw, sizeof(v));
+                return;
+            }
+#pragma unroll
+            for (int t = 0; t < 8; ++t) v[t] = p[t];
+        } else {
+#pragma unroll
+            for (int t = 0; t < 8; ++t) v[t] = (row < M && k0 + t < N) ? A[row * lda + k0 + t] : TA(0.0f);
+        }
+    }
+};
+
+template <typename TA, int MODE>
+__device__ __forceinline__ void mfma_packed(const Frag8<TA>& a, const void* __restrict__ XT, int64_t plane,
+                                            int64_t off, typename AccOf<MODE>::type& acc) {
+    if constexpr (MODE == kSplit3) {
+        const bf16x8* x = reinterpret_cast<const bf16x8*>(static_cast<const __bf16*>(XT) + off);
+        const bf16x8 h = x[0];
+        const bf16x8 m = *reinterpret_cast<const bf16x8*>(static_cast<const __bf16*>(XT) + plane + off);
+        const bf16x8 l = *reinterpret_cast<const bf16x8*>(static_cast<const __bf16*>(XT) + 2 * plane + off);
+        bf16x8 av;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) av[t] = static_cast<__bf16>(static_cast<float>(to_d(a.v[t])));
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, l, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, m, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, h, acc, 0, 0, 0);
+    } else if constexpr (MODE == kF32) {
+        const float* x = static_cast<const float*>(XT) + off;
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(static_cast<float>(to_d(a.v[t])), x[t], acc, 0, 0, 0);
+    } else {
+        const double* x = static_cast<const double*>(XT) + off;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(to_d(a.v[t]), x[t], acc, 0, 0, 0);
+    }
+}
+
+// Y (M x s) = A (M x N) X (N x s).  Each wavefront owns RT row tiles of 16
+// rows and NT 16-column tiles (s <= 16 NT).  K advances 64 per trip: lane
+// (c, q) loads 16 consecutive entries A[row c][k0 + 16 q .. +16) (32 bytes
+// of bf16) and feeds them to two K=32 steps, element t of step h standing
+// for k = k0 + 16 q + 8 h + t in both operands.  The next trip's A entries
+// are loaded before the current trip's matrix-core work.
+template <typename TA, typename TX, int MODE, int NT, int RT>
+__global__ __launch_bounds__(kThreads) void dense_nn_kernel(int64_t M, int64_t N, int s, const TA* __restrict__ A,
+                                                            int64_t lda, const void* __restrict__ XT, int64_t nk,
+                                                            int64_t plane, TX* __restrict__ Y, int64_t ldy) {
+    using Acc = typename AccOf<MODE>::type;
+    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    const int c = lane % 16, q = lane / 16;
+    const int64_t row0 = (static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + wave) * (16 * RT);
+    if (row0 >= M) return;
+    Acc acc[RT][NT];
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+        for (int j = 0; j < NT; ++j) acc[r][j] = Acc{0, 0, 0, 0};
+    Frag8<TA> cur[RT][2], nxt[RT][2];
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) cur[r][h].load(A, lda, row0 + 16 * r + c, M, 16 * q + 8 * h, N);
+    for (int64_t k0 = 0; k0 < N; k0 += 64) {
+        if (k0 + 64 < N) {
+#pragma unroll
+            for (int r = 0; r < RT; ++r)
+#pragma unroll
+                for (int h = 0; h < 2; ++h) nxt[r][h].load(A, lda, row0 + 16 * r + c, M, k0 + 64 + 16 * q + 8 * h, N);
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int j = 0; j < NT; ++j) {
+                const int64_t off = static_cast<int64_t>(16 * j + c) * nk + k0 + 16 * q + 8 * h;
+#pragma unroll
+                for (int r = 0; r < RT; ++r) mfma_packed<TA, MODE>(cur[r][h], XT, plane, off, acc[r][j]);
+            }
+#pragma unroll
+        for (int r = 0; r < RT; ++r)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) cur[r][h] = nxt[r][h];
+    }
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+        const int col = 16 * j + c;
+        if (col >= s) continue;
+#pragma unroll
+        for (int r = 0; r < RT; ++r)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int64_t row = row0 + 16 * r + result_row<MODE>(q, e);
+                if (row < M) Y[row * ldy + col] = static_cast<TX>(acc[r][j][e]);
+            }
+    }
+}
+
+// Partial Z = A^T Y over one chunk of rows for one 64-column strip of A.
+// P[chunk][j][col] (j < N, col < s).  Rows advance RS (32 or 64) per trip:
+// the RS x 64 A tile and the RS x s Y tile of the next trip are loaded into
+// registers while the current trip's tiles in LDS feed the matrix cores.
+template <typename TA, typename TX, int MODE, int NT, int RS>
+__global__ __launch_bounds__(kThreads) void dense_tn_kernel(int64_t M, int64_t N, int s, const TA* __restrict__ A,
+                                                            int64_t lda, const TX* __restrict__ Yb, int64_t ldy,
+                                                            int64_t rows_per_chunk, double* __restrict__ P) {
+    using Acc = typename AccOf<MODE>::type;
+    constexpr int YW = 16 * NT;
+    constexpr int YPT = (RS * YW) / kThreads;  // Y values per thread per trip
+    constexpr int APT = RS / 32;               // 8-entry A pieces per thread per trip
+    __shared__ TA At[RS][64 + 8];
+    __shared__ double Yt[RS][YW + 1];
+    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    const int c = lane % 16, q = lane / 16;
+    const int64_t j0 = static_cast<int64_t>(blockIdx.x) * 64;
+    const int64_t r0 = static_cast<int64_t>(blockIdx.y) * rows_per_chunk;
+    const int64_t r1 = std::min<int64_t>(M, r0 + rows_per_chunk);
+    const int ar = threadIdx.x / 8, ac = (threadIdx.x % 8) * 8;  // this thread's A tile slots
+    Acc acc[NT];
+#pragma unroll
+    for (int j = 0; j < NT; ++j) acc[j] = Acc{0, 0, 0, 0};
+    Frag8<TA> an[APT];
+    double yn[YPT];
+    auto fetch = [&](int64_t i0) {
+#pragma unroll
+        for (int p = 0; p < APT; ++p) an[p].load(A, lda, i0 + ar + 32 * p, r1, j0 + ac, N);
+#pragma unroll
+        for (int u = 0; u < YPT; ++u) {
+            const int e = threadIdx.x + u * kThreads;
+            const int rr = e / YW, cc = e % YW;
+            const int64_t row = i0 + rr;
+            yn[u] = (row < r1 && cc < s) ? static_cast<double>(Yb[row * ldy + cc]) : 0.0;
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int p = 0; p < APT; ++p)
+#pragma unroll
+            for (int t = 0; t < 8; ++t) At[ar + 32 * p][ac + t] = an[p].v[t];
+#pragma unroll
+        for (int u = 0; u < YPT; ++u) {
+            const int e = threadIdx.x + u * kThreads;
+            Yt[e / YW][e % YW] = yn[u];
+        }
+    };
+    if (r0 < r1) {
+        fetch(r0);
+        store();
+    }
+    __syncthreads();
+    for (int64_t i0 = r0; i0 < r1; i0 += RS) {
+        const bool more = i0 + RS < r1;
+        if (more) fetch(i0 + RS);
+#pragma unroll
+        for (int h = 0; h < RS / 32; ++h) {
+            double a[8];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) a[t] = to_d(At[32 * h + 8 * q + t][16 * wave + c]);
+#pragma unroll
+            for (int j = 0; j < NT; ++j) {
+                double b[8];
+#pragma unroll
+                for (int t = 0; t < 8; ++t) b[t] = Yt[32 * h + 8 * q + t][16 * j + c];
+                mfma_step<MODE>(a, b, acc[j]);
+            }
+        }
+        __syncthreads();
+        if (more) store();
+        __syncthreads();
+    }
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+        const int col = 16 * j + c;
+        if (col >= s) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int64_t jj = j0 + 16 * wave + result_row<MODE>(q, r);
+            if (jj < N) P[(static_cast<int64_t>(blockIdx.y) * N + jj) * s + col] = static_cast<double>(acc[j][r]);
+        }
+    }
+}
+
+// Z[j][col] = sum over chunks of P[chunk][j][col] (chunk order).
+template <typename TX>
+__global__ void dense_tn_reduce_kernel(int64_t N, int s, int64_t chunks, const double* __restrict__ P,
+                                       TX* __restrict__ Z, int64_t ldz) {
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= N * s) return;
+    const int64_t j = e / s;
+    const int col = static_cast<int>(e % s);
+    double acc = 0.0;
+    for (int64_t ch = 0; ch < chunks; ++ch) acc += P[(ch * N + j) * s + col];
+    Z[j * ldz + col] = static_cast<TX>(acc);
+}
+
+template <typename TA, typename TX, int MODE>
+void nn_dispatch(hipStream_t st, int64_t M, int64_t N, int s, const TA* A, int64_t lda, const TX* X, int64_t ldx,
+                 TX* Y, int64_t ldy, Scratch& scratch) {
+    const int NTt = s <= 16 ? 1 : (s <= 32 ? 2 : (s <= 48 ? 3 : 4));
+    const int ncp = 16 * NTt;
+    const int64_t nk = ceil_div(N, 64) * 64;
+    const int64_t plane = static_cast<int64_t>(ncp) * nk;
+    const size_t esize = MODE == kSplit3 ? 3 * sizeof(__bf16) : (MODE == kF32 ? sizeof(float) : sizeof(double));
+    scratch.ensure(static_cast<size_t>(plane) * esize);
+    void* XT = scratch.ptr;
+    hipLaunchKernelGGL((pack_x_kernel<TX, MODE>), dim3(static_cast<unsigned>(ceil_div(plane, kThreads))),
+                       dim3(kThreads), 0, st, N, s, ncp, nk, X, ldx, XT);
+    AMFLS_LAUNCH_CHECK();
+    constexpr int RT = MODE == kF64 ? 2 : 4;  // row tiles per wavefront (register budget)
+    const dim3 grid(static_cast<unsigned>(ceil_div(M, 16 * RT * (kThreads / kWave))));
+#define AMFLS_NN(NT)                                                                                           \
+    hipLaunchKernelGGL((dense_nn_kernel<TA, TX, MODE, NT, RT>), grid, dim3(kThreads), 0, st, M, N, s, A, lda, XT, \
+                       nk, plane, Y, ldy)
+    if (NTt == 1) { AMFLS_NN(1); }
+    else if (NTt == 2) { AMFLS_NN(2); }
+    else if (NTt == 3) { AMFLS_NN(3); }
+    else { AMFLS_NN(4); }
+#undef AMFLS_NN
+    AMFLS_LAUNCH_CHECK();
+}
+
+template <typename TA, typename TX, int MODE>
+void tn_dispatch(hipStream_t st, int64_t M, int64_t N, int s, const TA* A, int64_t lda, const TX* Y, int64_t ldy,
+                 TX* Z, int64_t ldz, Scratch& scratch) {
+    const int64_t strips = ceil_div(N, 64);
+    int64_t chunks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(2048, strips), ceil_div(M, 256)));
+    const int64_t rows_per_chunk = ceil_div(ceil_div(M, chunks), 64) * 64;
+    chunks = ceil_div(M, rows_per_chunk);
+    scratch.ensure(static_cast<size_t>(chunks * N * s) * sizeof(double));
+    double* P = static_cast<double*>(scratch.ptr);
+    const dim3 grid(static_cast<unsigned>(strips), static_cast<unsigned>(chunks));
+    // 64-row trips when the LDS tiles fit (A stored in 2 or 4 bytes, s <= 32)
+    constexpr int RS = sizeof(TA) <= 4 ? 64 : 32;
+#define AMFLS_TN(NT)                                                                                               \
+    hipLaunchKernelGGL((dense_tn_kernel<TA, TX, MODE, NT, (NT <= 2 ? RS : 32)>), grid, dim3(kThreads), 0, st, M, N, s, \
+                       A, lda, Y, ldy, rows_per_chunk, P)
+    if (s <= 16) { AMFLS_TN(1); }
+    else if (s <= 32) { AMFLS_TN(2); }
+    else if (s <= 48) { AMFLS_TN(3); }
+    else { AMFLS_TN(4); }
+#undef AMFLS_TN
+    AMFLS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(dense_tn_reduce_kernel<TX>, dim3(static_cast<unsigned>(ceil_div(N * s, kThreads))),
+                       dim3(kThreads), 0, st, N, s, chunks, P, Z, ldz);
+    AMFLS_LAUNCH_CHECK();
+}
+
+template <typename TX, int MODE>
+void by_storage_nn(hipStream_t st, const Strided& A, int64_t M, int64_t N, int s, const TX* X, int64_t ldx, TX* Y,
+                   int64_t ldy, Scratch& scratch) {
+    switch (A.type) {
+        case ValueType::bf16:
+            nn_dispatch<__hip_bfloat16, TX, MODE>(st, M, N, s, static_cast<const __hip_bfloat16*>(A.ptr), A.rs, X,
+                                                  ldx, Y, ldy, scratch);
+            break;
+        case ValueType::f32:
+            if constexpr (MODE != kSplit3)
+                nn_dispatch<float, TX, MODE>(st, M, N, s, static_cast<const float*>(A.ptr), A.rs, X, ldx, Y, ldy,
+                                             scratch);
+            break;
+        case ValueType::f64:
+            if constexpr (MODE != kSplit3)
+                nn_dispatch<double, TX, MODE>(st, M, N, s, static_cast<const double*>(A.ptr), A.rs, X, ldx, Y, ldy,
+                                              scratch);
+            break;
+    }
+}
+
+template <typename TX, int MODE>
+void by_storage_tn(hipStream_t st, const Strided& A, int64_t M, int64_t N, int s, const TX* Y, int64_t ldy, TX* Z,
+                   int64_t ldz, Scratch& scratch) {
+    switch (A.type) {
+        case ValueType::bf16:
+            tn_dispatch<__hip_bfloat16, TX, MODE>(st, M, N, s, static_cast<const __hip_bfloat16*>(A.ptr), A.rs, Y,
+                                                  ldy, Z, ldz, scratch);
+            break;
+        case ValueType::f32:
+            if constexpr (MODE != kSplit3)
+                tn_dispatch<float, TX, MODE>(st, M, N, s, static_cast<const float*>(A.ptr), A.rs, Y, ldy, Z, ldz,
+                                             scratch);
+            break;
+        case ValueType::f64:
+            if constexpr (MODE != kSplit3)
+                tn_dispatch<double, TX, MODE>(st, M, N, s, static_cast<const double*>(A.ptr), A.rs, Y, ldy, Z, ldz,
+                                              scratch);
+            break;
+    }
+}
+
+}  // namespace
+
+DenseMfmaMode dense_mfma_mode(ValueType storage, bool fp64_blocks, bool reduced) {
+    if (fp64_blocks && !reduced) return DenseMfmaMode::f64;
+    return storage == ValueType::bf16 ? DenseMfmaMode::split3 : DenseMfmaMode::f32;
+}
+
+template <typename TX>
+void dense_mfma_nn(hipStream_t s, DenseMfmaMode mode, const Strided& A, int64_t M, int64_t N, int ncols, const TX* X,
+                   int64_t ldx, TX* Y, int64_t ldy, Scratch& scratch) {
+    if (M <= 0 || ncols <= 0) return;
+    if (A.cs != 1) throw std::invalid_argument("matrix-core dense products need row-major storage");
+    if (ncols > 64) throw std::invalid_argument("matrix-core dense products support blocks up to 64 columns");
+    if (mode == DenseMfmaMode::split3 && A.type != ValueType::bf16)
+        throw std::invalid_argument("the split-bf16 product needs bf16 storage");
+    switch (mode) {
+        case DenseMfmaMode::split3: by_storage_nn<TX, kSplit3>(s, A, M, N, ncols, X, ldx, Y, ldy, scratch); break;
+        case DenseMfmaMode::f32: by_storage_nn<TX, kF32>(s, A, M, N, ncols, X, ldx, Y, ldy, scratch); break;
+        case DenseMfmaMode::f64: by_storage_nn<TX, kF64>(s, A, M, N, ncols, X, ldx, Y, ldy, scratch); break;
+    }
+}
+
+template <typename TX>
+void dense_mfma_tn(hipStream_t s, DenseMfmaMode mode, const Strided& A, int64_t M, int64_t N, int ncols, const TX* Y,
+                   int64_t ldy, TX* Z, int64_t ldz, Scratch& scratch) {
+    if (N <= 0 || ncols <= 0) return;
+    if (A.cs != 1) throw std::invalid_argument("matrix-core dense products need row-major storage");
+    if (ncols > 64) throw std::invalid_argument("matrix-core dense products support blocks up to 64 columns");
+    if (mode == DenseMfmaMode::split3 && A.type != ValueType::bf16)
+        throw std::invalid_argument("the split-bf16 product needs bf16 storage");
+    if (M <= 0) throw std::invalid_argument("matrix-core dense products need a nonempty matrix");
+    switch (mode) {
+        case DenseMfmaMode::split3: by_storage_tn<TX, kSplit3>(s, A, M, N, ncols, Y, ldy, Z, ldz, scratch); break;
+        case DenseMfmaMode::f32: by_storage_tn<TX, kF32>(s, A, M, N, ncols, Y, ldy, Z, ldz, scratch); break;
+        case DenseMfmaMode::f64: by_storage_tn<TX, kF64>(s, A, M, N, ncols, Y, ldy, Z, ldz, scratch); break;
+    }
+}
+
+template void dense_mfma_nn<double>(hipStream_t, DenseMfmaMode, const Strided&, int64_t, int64_t, int, const double*,
+                                    int64_t, double*, int64_t, Scratch&);
+template void dense_mfma_nn<float>(hipStream_t, DenseMfmaMode, const Strided&, int64_t, int64_t, int, const float*,
+                                   int64_t, float*, int64_t, Scratch&);
+template void dense_mfma_tn<double>(hipStream_t, DenseMfmaMode, const Strided&, int64_t, int64_t, int, const double*,
+                                    int64_t, double*, int64_t, Scratch&);
+template void dense_mfma_tn<float>(hipStream_t, DenseMfmaMode, const Strided&, int64_t, int64_t, int, const float*,
+                                   int64_t, float*, int64_t, Scratch&);
+
+}  // namespace amflsx

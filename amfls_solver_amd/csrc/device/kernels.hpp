@@ -51,6 +51,22 @@ template <typename T>
 void rowmul(hipStream_t s, int64_t rows, int w, int r, const T* X, int64_t ldx, const double* M,
             T* Y, int64_t ldy);
 
+// ---------------------------------------------------------------- dense, matrix cores
+// Row-major A (m x n; bf16, fp32 or fp64 storage) times row-major blocks of
+// at most 64 columns on MFMA (dense_mfma.hip): split3 = fp32-accurate
+// products from bf16 A and three-term bf16 splits of the block; f32 = fp32
+// matrix cores; f64 = fp64 matrix cores on A upconverted exactly.
+enum class DenseMfmaMode : int { split3 = 0, f32 = 1, f64 = 2 };
+// The mode for a storage type and block precision; `reduced` asks for
+// fp32-accurate products on fp64 blocks.
+DenseMfmaMode dense_mfma_mode(ValueType storage, bool fp64_blocks, bool reduced);
+template <typename TX>
+void dense_mfma_nn(hipStream_t s, DenseMfmaMode mode, const Strided& A, int64_t M, int64_t N, int ncols, const TX* X,
+                   int64_t ldx, TX* Y, int64_t ldy, Scratch& scratch);
+template <typename TX>
+void dense_mfma_tn(hipStream_t s, DenseMfmaMode mode, const Strided& A, int64_t M, int64_t N, int ncols, const TX* Y,
+                   int64_t ldy, TX* Z, int64_t ldz, Scratch& scratch);
+
 // ---------------------------------------------------------------- vectors
 template <typename Tin, typename Tout>
 void copy_block(hipStream_t s, int64_t rows, int cols, const Tin* X, int64_t ldx, Tout* Y,

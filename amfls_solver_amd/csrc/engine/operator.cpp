@@ -269,7 +269,11 @@ DenseOperator::DenseOperator(int64_t rows, int64_t cols, const void* values, Val
 template <typename T>
 void DenseOperator::do_apply(hipStream_t s, const T* X, int n, int64_t ldx, T* Y, int64_t ldy) const {
     prof_begin(s);
-    gemm_nn<T, T>(s, rows_, n, cols_, a_, X, ldx, Y, ldy, 1.0, 0.0);
+    if (use_mfma(n))
+        dense_mfma_nn<T>(s, dense_mfma_mode(a_.type, sizeof(T) == 8, reduced_), a_, rows_, cols_, n, X, ldx, Y, ldy,
+                         ctx_.scratch);
+    else
+        gemm_nn<T, T>(s, rows_, n, cols_, a_, X, ldx, Y, ldy, 1.0, 0.0);
     prof_end(s, n, false, static_cast<int>(sizeof(T)));
 }
 
@@ -277,7 +281,11 @@ template <typename T>
 void DenseOperator::do_apply_t(hipStream_t s, const T* Y, int n, int64_t ldy, T* X, int64_t ldx) const {
     const ValueType ct = sizeof(T) == 8 ? ValueType::f64 : ValueType::f32;
     prof_begin(s);
-    gemm_tn<T>(s, rows_, cols_, n, a_, Y, ldy, X, ct, ldx, 1.0, ctx_.scratch);
+    if (use_mfma(n))
+        dense_mfma_tn<T>(s, dense_mfma_mode(a_.type, sizeof(T) == 8, reduced_), a_, rows_, cols_, n, Y, ldy, X, ldx,
+                         ctx_.scratch);
+    else
+        gemm_tn<T>(s, rows_, cols_, n, a_, Y, ldy, X, ct, ldx, 1.0, ctx_.scratch);
     prof_end(s, n, true, static_cast<int>(sizeof(T)));
 }
 

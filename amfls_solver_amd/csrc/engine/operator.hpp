@@ -145,8 +145,16 @@ public:
         return 2.0 * static_cast<double>(rows_) * static_cast<double>(cols_) * s;
     }
     Strided strided() const { return a_; }
+    // Row-major storage and blocks of at most 64 columns run on the matrix
+    // cores; `reduced` makes fp64-block products fp32-accurate (split-bf16
+    // or fp32 matrix cores) for the inner solves of mixed precision.
+    bool use_mfma(int ncols) const { return mfma_ && a_.cs == 1 && ncols <= 64; }
+    void set_mfma(bool on) { mfma_ = on; }
+    void set_reduced_products(bool on) { reduced_ = on; }
 
 private:
+    bool mfma_ = true;
+    bool reduced_ = false;
     template <typename T>
     void do_apply(hipStream_t s, const T* X, int n, int64_t ldx, T* Y, int64_t ldy) const;
     template <typename T>

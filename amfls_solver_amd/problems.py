@@ -111,3 +111,31 @@ def dense_gd(rows: int, cols: int, decades: float, *, seed: int = 0, noise: floa
 def config0(seed: int = 0):
     """2000 x 500 dense, D = logspace(0, -4), b = A x* + 1e-6 N(0,1)."""
     return dense_gd(2000, 500, 4.0, seed=seed, noise=1e-6)
+
+
+def config2_device(rows: int = 1_000_000, cols: int = 2000, *, storage=None, decades: float = 6.0, seed: int = 0,
+                   noise: float = 1e-8):
+    """Dense tall A = G D built on the GPU: G_ij = gaussian_at(seed, 0, j*rows + i)
+    (the counter-addressed stream of reference random/gaussian.cpp:39-55), D
+    log-spaced from 1 to 10^-decades, stored as `storage` (bfloat16 by
+    default; the stored values are A); x* = gaussian_at(seed, 1, .),
+    e = gaussian_at(seed, 2, .), b = A x* + noise ||A x*|| e / ||e|| with A x*
+    computed in FP64 from the stored A.  Returns (A, b, x*) as CUDA tensors,
+    A row-major (rows x cols)."""
+    import torch
+
+    from .binding import DeviceMatrix, gaussian_block
+    storage = storage or torch.bfloat16
+    d = torch.logspace(0.0, -decades, cols, dtype=torch.float64, device="cuda")
+    a = torch.empty((rows, cols), dtype=storage, device="cuda")
+    chunk = max(1, min(cols, (1 << 26) // max(rows, 1)))  # columns per generated slab (<= 512 MB)
+    for j0 in range(0, cols, chunk):
+        j1 = min(cols, j0 + chunk)
+        g = gaussian_block(rows, j1 - j0, seed, 0, first_column=j0)  # entry (i, j) = counter (j0 + j) rows + i
+        a[:, j0:j1] = (g * d[None, j0:j1]).to(storage)
+        del g
+    xs = gaussian_block(cols, 1, seed, 1).reshape(-1)
+    e = gaussian_block(rows, 1, seed, 2).reshape(-1)
+    ax = DeviceMatrix(a, value_dtype=storage).apply(xs).reshape(-1)
+    b = ax + noise * torch.linalg.vector_norm(ax) * e / torch.linalg.vector_norm(e)
+    return a, b, xs

@@ -268,3 +268,31 @@ def test_torch_ops_entry_points(torch_mod, fx):
     assert o1[6] == r1.status == 0 and np.allclose(o1[0].numpy(), r1.solution.numpy(), rtol=0, atol=1e-12)
     o2 = torch.ops.amfls.run_fixed_rbgk(Ad, bd, 1e-3, 1e-9, 1e-6, 2, 6, 0, 3, 0)
     assert o2[2][9] == 6  # depth counter
+
+
+@pytest.mark.parametrize("vt", ["bfloat16", "float32", "float64"])
+def test_dense_matrix_core_products(torch_mod, vt):
+    """Row-major dense products on the matrix cores: fp64 blocks are
+    fp64-accurate (A upconverted exactly), fp32 blocks fp32-accurate
+    (split-bf16 or fp32 matrix cores); ragged sizes and widths up to 64
+    (70 falls back to the vector kernels)."""
+    import amfls_solver_amd as am
+    torch = torch_mod
+    rng = np.random.default_rng(7)
+    for m, n in ((1000, 203), (37, 64), (4099, 1)):
+        a = torch.from_numpy(rng.standard_normal((m, n))).to(getattr(torch, vt))
+        stored = a.double().numpy()
+        M = am.DeviceMatrix(a.contiguous(), value_dtype=getattr(torch, vt))
+        for s in (1, 5, 16, 17, 40, 64, 70):
+            X = rng.standard_normal((n, s))
+            Y = stored @ X
+            Yg = rng.standard_normal((m, s))
+            Z = stored.T @ Yg
+            y = M.apply(torch.from_numpy(X)).cpu().numpy()
+            z = M.apply(torch.from_numpy(Yg), transpose=True).cpu().numpy()
+            assert np.abs(y - Y).max() <= 1e-13 * max(1.0, np.abs(Y).max()) * np.sqrt(n)
+            assert np.abs(z - Z).max() <= 1e-13 * max(1.0, np.abs(Z).max()) * np.sqrt(m)
+            y32 = M.apply(torch.from_numpy(X).float()).cpu().double().numpy()
+            z32 = M.apply(torch.from_numpy(Yg).float(), transpose=True).cpu().double().numpy()
+            assert np.abs(y32 - Y).max() <= 1e-5 * np.abs(np.abs(stored) @ np.abs(X)).max()
+            assert np.abs(z32 - Z).max() <= 1e-5 * np.abs(np.abs(stored).T @ np.abs(Yg)).max()

@@ -14,6 +14,8 @@ LIB_PATH = Path(__file__).resolve().parent / "libamfls_hip.so"
 
 F64, F32, BF16 = 0, 1, 2
 PRECISION = {"fp64": 0, "mixed": 1, "fp32": 2}
+WIDTH_POLICY = {"heuristic": 0, "learned": 1, "record": 2, "widen": 3, "hold": 4, "record_widen": 5}
+WIDTH_INPUTS, WIDTH_PARAMS = 12, 5057
 CLOCK = {"wall": 0, "synthetic": 1}
 
 ERRORS = {
@@ -89,6 +91,15 @@ SIGNATURES = {
     "amfls_result_destroy": (c_int, [c_void_p]),
     "amfls_gaussian_fill": (c_int, [c_void_p, c_void_p, c_int64, c_int32, c_int64, c_int, c_uint64, c_uint64,
                                     c_uint64]),
+    "amfls_result_width_records": (c_int, [c_void_p, c_void_p, POINTER(c_int64)]),
+    "amfls_width_model_create": (c_int, [c_void_p, c_void_p, c_void_p, POINTER(c_void_p)]),
+    "amfls_width_model_destroy": (c_int, [c_void_p]),
+    "amfls_width_model_get_params": (c_int, [c_void_p, c_void_p]),
+    "amfls_width_model_set_params": (c_int, [c_void_p, c_void_p]),
+    "amfls_width_model_logits": (c_int, [c_void_p, c_void_p, c_int64, c_void_p]),
+    "amfls_width_model_gradient": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, POINTER(c_float)]),
+    "amfls_width_model_train_step": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_float, POINTER(c_float)]),
+    "amfls_context_set_width_model": (c_int, [c_void_p, c_void_p]),
 }
 
 _lib = None

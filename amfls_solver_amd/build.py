@@ -27,11 +27,13 @@ SOURCES = [
     CSRC / "device" / "rank_one.hip",
     CSRC / "device" / "level_chain.hip",
     CSRC / "device" / "dense_mfma.hip",
+    CSRC / "device" / "mlp.hip",
     CSRC / "engine" / "operator.cpp",
     CSRC / "engine" / "small_la.cpp",
     CSRC / "engine" / "validation.cpp",
     CSRC / "engine" / "workspace.cpp",
     CSRC / "engine" / "solver.cpp",
+    CSRC / "engine" / "width_model.cpp",
     CSRC / "engine" / "capi.cpp",
 ]
 

@@ -136,6 +136,19 @@ void chain_normalize(hipStream_t s, int64_t N, int k, const T* y, int64_t ldy, T
                      const double* part_second, double dgks_factor, double rank_factor, const double* Q1,
                      const double* Q2, const double* b, double* gpart, double* record, double* cell, double* go);
 
+// ---------------------------------------------------------------- width MLP
+// 12 -> 64 -> 64 -> 1 ReLU network of the learned width controller
+// (mlp.hip).  params: flat FP32 array of kMlpParams values; norm: per-input
+// mean (12) then inverse standard deviation (12); X: n x 12 row-major.
+constexpr int kMlpInputs = 12, kMlpHidden = 64;
+constexpr int kMlpParams = kMlpHidden * kMlpInputs + kMlpHidden + kMlpHidden * kMlpHidden + kMlpHidden + kMlpHidden + 1;
+void mlp_forward(hipStream_t s, const float* params, const float* norm, int64_t n, const float* X, float* logits);
+// grad = d(mean BCE)/d(params), *loss = mean BCE (device pointers).
+void mlp_gradient(hipStream_t s, const float* params, const float* norm, int64_t n, const float* X, const float* y,
+                  float* grad, float* loss, Scratch& scratch);
+// One Adam step (step counts from 1).
+void mlp_adam(hipStream_t s, float* params, float* m, float* v, const float* grad, float lr, int step);
+
 // Counter-addressed Gaussian block, index = (first_col + j) * rows + i.
 template <typename T>
 void gaussian_block(hipStream_t s, int64_t rows, int cols, uint64_t seed, uint64_t stream,

@@ -1,5 +1,7 @@
 // extern "C" boundary of the AMFLS HIP engine (declared in include/amfls_hip.h).
+#include <cmath>
 #include <cstring>
+#include <vector>
 #include <exception>
 #include <memory>
 #include <new>
@@ -8,11 +10,17 @@
 
 #include "../../../include/amfls_hip.h"
 #include "solver.hpp"
+#include "width_model.hpp"
 
 using namespace amflsx;
 
+struct amfls_width_model {
+    std::unique_ptr<WidthModel> m;
+};
+
 struct amfls_context {
     Context ctx;
+    amfls_width_model* width_model = nullptr;  // not owned
 };
 
 struct amfls_operator {
@@ -21,6 +29,7 @@ struct amfls_operator {
 
 struct amfls_result {
     LeastSquaresResult r;
+    std::vector<float> width_records;  // rows x (AMFLS_WIDTH_INPUTS + 1) in record mode
 };
 
 namespace {
@@ -304,9 +313,34 @@ int amfls_solve(amfls_context* ctx, const amfls_operator* op, const double* b, i
         o.maximum_refinements = options->maximum_refinements;
         o.inner_tolerance = options->inner_tolerance;
         o.width_policy = options->width_policy;
+        require(o.width_policy >= AMFLS_WIDTH_HEURISTIC && o.width_policy <= AMFLS_WIDTH_RECORD_WIDEN,
+                "unknown width policy");
         RhsHolder rhs(ctx->ctx, b, op->op->rows(), b_on_device != 0);
         auto res = std::make_unique<amfls_result>();
-        res->r = solve_amfls(*op->op, rhs.ptr, o, ctx->ctx, nullptr);
+        // widening predicate: the reference heuristic, the learned model, or
+        // the heuristic with every decision's model inputs recorded
+        WidthPolicy policy;
+        if (o.width_policy == AMFLS_WIDTH_LEARNED) {
+            require(ctx->width_model != nullptr, "the learned width policy needs a model set on the context");
+            WidthModel* m = ctx->width_model->m.get();
+            // the model decides within the widths it was trained on (blocks of at most 64 columns)
+            policy = [m](const WidthFeatures& f) { return f.active_width + f.proposed_increment <= 64 && m->widen(f); };
+        } else if (o.width_policy == AMFLS_WIDTH_WIDEN) {
+            // widen at every opportunity up to the widest benchmarked block (64)
+            policy = [](const WidthFeatures& f) { return f.active_width + f.proposed_increment <= 64; };
+        } else if (o.width_policy == AMFLS_WIDTH_HOLD) {
+            policy = [](const WidthFeatures&) { return false; };
+        } else if (o.width_policy == AMFLS_WIDTH_RECORD || o.width_policy == AMFLS_WIDTH_RECORD_WIDEN) {
+            std::vector<float>* rec = &res->width_records;
+            const bool widen_always = o.width_policy == AMFLS_WIDTH_RECORD_WIDEN;
+            policy = [rec, widen_always](const WidthFeatures& f) {
+                const auto x = width_features(f);
+                rec->insert(rec->end(), x.begin(), x.end());
+                rec->push_back(f.heuristic ? 1.0f : 0.0f);
+                return widen_always ? f.active_width + f.proposed_increment <= 64 : f.heuristic;
+            };
+        }
+        res->r = solve_amfls(*op->op, rhs.ptr, o, ctx->ctx, policy ? &policy : nullptr);
         *out = res.release();
     });
 }
@@ -446,6 +480,73 @@ int amfls_result_refinement_steps(const amfls_result* res, int32_t* steps) {
     return guard([&] {
         require(res && steps, "null argument");
         *steps = res->r.refinement_steps;
+    });
+}
+
+int amfls_result_width_records(const amfls_result* res, float* out, int64_t* rows) {
+    return guard([&] {
+        require(res && rows, "null argument");
+        const size_t w = AMFLS_WIDTH_INPUTS + 1;
+        *rows = static_cast<int64_t>(res->width_records.size() / w);
+        if (out) std::memcpy(out, res->width_records.data(), res->width_records.size() * sizeof(float));
+    });
+}
+
+int amfls_width_model_create(amfls_context* ctx, const float* params, const float* norm, amfls_width_model** out) {
+    return guard([&] {
+        require(ctx && params && norm && out, "null argument");
+        auto m = std::make_unique<amfls_width_model>();
+        m->m = std::make_unique<WidthModel>(params, norm, ctx->ctx);
+        *out = m.release();
+    });
+}
+
+int amfls_width_model_destroy(amfls_width_model* model) {
+    return guard([&] { delete model; });
+}
+
+int amfls_width_model_get_params(amfls_width_model* model, float* params) {
+    return guard([&] {
+        require(model && params, "null argument");
+        model->m->params(params);
+    });
+}
+
+int amfls_width_model_set_params(amfls_width_model* model, const float* params) {
+    return guard([&] {
+        require(model && params, "null argument");
+        model->m->set_params(params);
+    });
+}
+
+int amfls_width_model_logits(amfls_width_model* model, const float* features, int64_t n, float* logits) {
+    return guard([&] {
+        require(model && (n == 0 || (features && logits)) && n >= 0, "invalid argument");
+        model->m->logits(features, n, logits);
+    });
+}
+
+int amfls_width_model_gradient(amfls_width_model* model, const float* features, const float* labels, int64_t n,
+                               float* grad, float* loss) {
+    return guard([&] {
+        require(model && features && labels && n > 0 && loss, "invalid argument");
+        *loss = model->m->gradient(features, labels, n, grad);
+    });
+}
+
+int amfls_width_model_train_step(amfls_width_model* model, const float* features, const float* labels, int64_t n,
+                                 float learning_rate, float* loss) {
+    return guard([&] {
+        require(model && features && labels && n > 0 && loss, "invalid argument");
+        require(std::isfinite(learning_rate) && learning_rate > 0.0f, "learning rate must be positive");
+        *loss = model->m->train_step(features, labels, n, learning_rate);
+    });
+}
+
+int amfls_context_set_width_model(amfls_context* ctx, amfls_width_model* model) {
+    return guard([&] {
+        require(ctx != nullptr, "null argument");
+        ctx->width_model = model;
     });
 }
 

@@ -1,0 +1,304 @@
+"""Learned width controller: data, GPU training and evaluation.
+
+The engine's widening decision (reference src/algorithms/amfls/amfls.cpp:
+322-358, cost gate amfls_internal.hpp:318-324) can be taken by a 12 -> 64 ->
+64 -> 1 ReLU MLP (csrc/device/mlp.hip) instead of the heuristic.  Its inputs
+are what AMFLS records about itself at each decision: the active width, the
+last four log10 certificate decreases, the last four per-level interval
+times, and log10 of the operator's rows, columns and nonzeros.
+
+Labels.  Each training problem is solved to the same certified tolerance
+with a constant predicate, "always widen" and "never widen" (wall-clock time
+to certification on the GPU), and twice more while recording the model
+inputs of every decision met, once following the heuristic and once always
+widening.  Those decisions are labelled
+"widen" when always-widen certified faster than never-widen; problems that
+certify under neither are left out.  The model therefore learns, from what
+the solver observes early on, whether widening pays for the problem at
+hand.  It is trained on the GPU with the engine's own forward/backward
+kernels and Adam; the FP64 certificate still decides termination, so a poor
+model costs time, never correctness.
+
+CLI (GPU):  python -m amfls_solver_amd.width_model train --problems 300
+            python -m amfls_solver_amd.width_model evaluate
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import math
+import time
+from pathlib import Path
+
+import numpy as np
+import torch
+
+from . import _lib
+from .binding import DeviceMatrix, default_context, run_amfls
+
+N_IN, N_HID, N_PARAMS = _lib.WIDTH_INPUTS, 64, _lib.WIDTH_PARAMS
+DEFAULT_PATH = Path(__file__).resolve().parent / "models" / "width_mlp.npz"
+
+
+def init_params(seed: int = 0) -> np.ndarray:
+    """He-normal weights and zero biases in the engine's parameter layout."""
+    r = np.random.default_rng(seed)
+    W1 = r.standard_normal((N_HID, N_IN)) * math.sqrt(2.0 / N_IN)
+    W2 = r.standard_normal((N_HID, N_HID)) * math.sqrt(2.0 / N_HID)
+    W3 = r.standard_normal(N_HID) * math.sqrt(1.0 / N_HID)
+    return np.concatenate([W1.ravel(), np.zeros(N_HID), W2.ravel(), np.zeros(N_HID), W3, [0.0]]).astype(np.float32)
+
+
+class WidthModel:
+    """Device-resident width MLP (C ABI amfls_width_model_*)."""
+
+    def __init__(self, params: np.ndarray, norm: np.ndarray, context=None):
+        self.context = context or default_context()
+        self._lib = self.context._lib
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        nm = np.ascontiguousarray(norm, dtype=np.float32)
+        if p.size != N_PARAMS or nm.size != 2 * N_IN:
+            raise ValueError("width model needs 5057 parameters and 24 normalization values")
+        h = ctypes.c_void_p()
+        _lib.check(self._lib.amfls_width_model_create(self.context.handle, p.ctypes.data_as(ctypes.c_void_p),
+                                                      nm.ctypes.data_as(ctypes.c_void_p), ctypes.byref(h)))
+        self.handle = h
+        self.norm = nm.copy()
+
+    def params(self) -> np.ndarray:
+        out = np.empty(N_PARAMS, dtype=np.float32)
+        _lib.check(self._lib.amfls_width_model_get_params(self.handle, out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+    def logits(self, X: np.ndarray) -> np.ndarray:
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, N_IN)
+        out = np.empty(X.shape[0], dtype=np.float32)
+        _lib.check(self._lib.amfls_width_model_logits(self.handle, X.ctypes.data_as(ctypes.c_void_p), X.shape[0],
+                                                      out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+    def gradient(self, X: torch.Tensor, y: torch.Tensor):
+        """(mean BCE, gradient) for device tensors X (n x 12) and y (n)."""
+        Xd = X.to(device="cuda", dtype=torch.float32).contiguous()
+        yd = y.to(device="cuda", dtype=torch.float32).contiguous()
+        g = np.empty(N_PARAMS, dtype=np.float32)
+        loss = ctypes.c_float()
+        torch.cuda.synchronize()
+        _lib.check(self._lib.amfls_width_model_gradient(self.handle, Xd.data_ptr(), yd.data_ptr(), Xd.shape[0],
+                                                        g.ctypes.data_as(ctypes.c_void_p), ctypes.byref(loss)))
+        return float(loss.value), g
+
+    def train_step(self, X: torch.Tensor, y: torch.Tensor, lr: float) -> float:
+        loss = ctypes.c_float()
+        _lib.check(self._lib.amfls_width_model_train_step(self.handle, X.data_ptr(), y.data_ptr(), X.shape[0],
+                                                          float(lr), ctypes.byref(loss)))
+        return float(loss.value)
+
+    def attach(self) -> None:
+        """Use this model for width_policy="learned" solves on its context."""
+        _lib.check(self._lib.amfls_context_set_width_model(self.context.handle, self.handle))
+
+    def save(self, path: Path, **meta) -> None:
+        path = Path(path)
+        path.parent.mkdir(parents=True, exist_ok=True)
+        np.savez(path, params=self.params(), norm=self.norm, meta=np.array(json.dumps(meta)))
+
+    @classmethod
+    def load(cls, path: Path = DEFAULT_PATH, context=None) -> "WidthModel":
+        with np.load(Path(path), allow_pickle=False) as z:
+            m = cls(z["params"], z["norm"], context)
+            m.meta = json.loads(str(z["meta"])) if "meta" in z else {}
+        return m
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            self._lib.amfls_context_set_width_model(self.context.handle, None)
+            self._lib.amfls_width_model_destroy(h)
+            self.handle = None
+
+
+# ----------------------------------------------------------------- problems
+def training_problem(index: int, seed: int = 11):
+    """Problem `index` of the training/evaluation distribution, drawn like the
+    benchmark configurations at reduced size: CSR (distinct uniform columns,
+    N(0,1) values, 4-32 nonzeros per row, optional log-uniform row scales) or
+    dense G D (D log-spaced over 1-6 decades); lambda = 0 or one of
+    logspace(-6, -1, 8).  Returns (matrix tensor, b, lambda, description)."""
+    from . import problems as P
+    r = np.random.default_rng([seed, index])
+    lam = 0.0 if r.random() < 0.4 else float(10.0 ** (-6.0 + 5.0 * int(r.integers(0, 8)) / 7.0))
+    if r.random() < 0.65:
+        m = int(r.integers(2000, 40001))
+        n = int(r.integers(200, min(4000, m) + 1))
+        per = np.minimum(r.integers(4, 33, size=m), n)
+        scaled = r.random() < 0.5
+        p = P.csr_rows(m, n, per, seed=int(r.integers(0, 2**31)), scale_range=(1e-3, 1.0) if scaled else (1.0, 1.0),
+                       regularization=lam)
+        A = torch.sparse_csr_tensor(torch.from_numpy(p.row_ptr), torch.from_numpy(p.col_idx.astype(np.int64)),
+                                    torch.from_numpy(p.values), size=(m, n))
+        return A, torch.from_numpy(p.b), lam, f"csr {m}x{n} nnz {p.nnz} lam {lam:.1e}"
+    m = int(r.integers(1000, 20001))
+    n = int(r.integers(100, min(800, m) + 1))
+    dec = float(r.uniform(1.0, 6.0))
+    a, b, _ = P.dense_gd(m, n, dec, seed=int(r.integers(0, 2**31)), noise=1e-6)
+    return torch.from_numpy(np.ascontiguousarray(a)), torch.from_numpy(b), lam, f"dense {m}x{n} dec {dec:.1f} lam {lam:.1e}"
+
+
+def _solve_seconds(res) -> float:
+    return res.total_seconds if res.status == 0 else math.inf
+
+
+def _timed(M, b, lam, tolerance, policy, repeats=2):
+    ts = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        r = run_amfls(M, b, regularization=lam, tolerance=tolerance, width_policy=_lib.WIDTH_POLICY[policy],
+                      context=M.context)
+        ts.append(time.perf_counter() - t0 if r.status == 0 else math.inf)
+    return float(np.median(ts)), r
+
+
+def collect(indices, *, seed: int = 11, tolerance: float = 1e-8, log=print):
+    """Decision records and labels for the given problem indices."""
+    X, y, info = [], [], []
+    for i in indices:
+        A, b, lam, desc = training_problem(i, seed)
+        M = DeviceMatrix(A, value_dtype=torch.float32 if A.layout == torch.sparse_csr else torch.float64)
+        bd = b.cuda()
+        run_amfls(M, bd, regularization=lam, tolerance=tolerance)  # warm
+        t_widen, rw = _timed(M, bd, lam, tolerance, "widen")
+        t_hold, rh = _timed(M, bd, lam, tolerance, "hold")
+        t_heur, _ = _timed(M, bd, lam, tolerance, "heuristic")
+        # decisions met along both the heuristic's and the always-widen path
+        rec = np.concatenate([record(M, bd, lam, tolerance), record(M, bd, lam, tolerance, widen=True)])
+        usable = math.isfinite(t_widen) or math.isfinite(t_hold)
+        label = 1.0 if t_widen < t_hold else 0.0
+        if usable:
+            for row in rec:
+                X.append(row[:N_IN])
+                y.append(label)
+        info.append({"index": int(i), "problem": desc, "widen_seconds": t_widen, "hold_seconds": t_hold,
+                     "heuristic_seconds": t_heur, "label": label, "decisions": len(rec), "used": usable})
+        log(f"  [{i}] {desc}: widen {t_widen:.4f}s (aux {rw.auxiliary_width}), hold {t_hold:.4f}s, "
+            f"heuristic {t_heur:.4f}s -> label {int(label)} ({len(rec)} decisions{'' if usable else ', unused'})")
+    return np.array(X, dtype=np.float32).reshape(-1, N_IN), np.array(y, dtype=np.float32), info
+
+
+def record(M, b, lam, tolerance, widen: bool = False):
+    """Model inputs and heuristic choices (rows x 13) of every widening
+    decision met while following the heuristic, or while always widening."""
+    raw, _, _ = _record_raw(M, b, lam, tolerance, "record_widen" if widen else "record")
+    return raw
+
+
+def _record_raw(M, b, lam, tolerance, policy):
+    ctx = M.context
+    o = _lib.options(regularization=float(lam), tolerance=float(tolerance), width_policy=_lib.WIDTH_POLICY[policy])
+    res = ctypes.c_void_p()
+    torch.cuda.synchronize()
+    _lib.check(ctx._lib.amfls_solve(ctx.handle, M.handle, b.data_ptr(), 1, ctypes.byref(o), ctypes.byref(res)))
+    try:
+        rows = ctypes.c_int64()
+        _lib.check(ctx._lib.amfls_result_width_records(res, None, ctypes.byref(rows)))
+        out = np.empty((rows.value, N_IN + 1), dtype=np.float32)
+        _lib.check(ctx._lib.amfls_result_width_records(res, out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(rows)))
+    finally:
+        ctx._lib.amfls_result_destroy(res)
+    return out, None, None
+
+
+def train(X: np.ndarray, y: np.ndarray, *, epochs: int = 400, lr: float = 3e-3, batch: int = 256, seed: int = 0,
+          log=print) -> WidthModel:
+    """Adam on the GPU with the engine's kernels; features normalized by the
+    training set's mean and standard deviation."""
+    mean = X.mean(axis=0)
+    std = X.std(axis=0)
+    norm = np.concatenate([mean, 1.0 / np.maximum(std, 1e-3)]).astype(np.float32)
+    model = WidthModel(init_params(seed), norm)
+    Xd = torch.from_numpy(X).cuda()
+    yd = torch.from_numpy(y).cuda()
+    gen = torch.Generator(device="cpu").manual_seed(seed)
+    n = X.shape[0]
+    for ep in range(epochs):
+        perm = torch.randperm(n, generator=gen).cuda()
+        tot = 0.0
+        for s0 in range(0, n, batch):
+            idx = perm[s0:s0 + batch]
+            xb, yb = Xd[idx].contiguous(), yd[idx].contiguous()
+            tot += model.train_step(xb, yb, lr) * idx.numel()
+        if ep % 50 == 0 or ep == epochs - 1:
+            log(f"  epoch {ep}: mean loss {tot / n:.4f}")
+    return model
+
+
+def evaluate(model: WidthModel, indices, *, seed: int = 11, tolerance: float = 1e-8, repeats: int = 2, log=print):
+    """Time-to-tolerance of the heuristic and the learned policy per problem
+    (median of `repeats` runs each, after a warm run)."""
+    model.attach()
+    rows = []
+    for i in indices:
+        A, b, lam, desc = training_problem(i, seed)
+        M = DeviceMatrix(A, value_dtype=torch.float32 if A.layout == torch.sparse_csr else torch.float64,
+                         context=model.context)
+        bd = b.cuda()
+        out = {}
+        for pol, code in (("heuristic", 0), ("learned", 1)):
+            run_amfls(M, bd, regularization=lam, tolerance=tolerance, width_policy=code, context=model.context)
+            ts = []
+            for _ in range(repeats):
+                t0 = time.perf_counter()
+                r = run_amfls(M, bd, regularization=lam, tolerance=tolerance, width_policy=code,
+                              context=model.context)
+                ts.append(time.perf_counter() - t0 if r.status == 0 else math.inf)
+            out[pol] = (float(np.median(ts)), int(r.status), int(r.auxiliary_width))
+        rows.append({"index": int(i), "problem": desc, "heuristic": out["heuristic"], "learned": out["learned"]})
+        log(f"  [{i}] {desc}: heuristic {out['heuristic'][0]:.4f}s (aux {out['heuristic'][2]}), "
+            f"learned {out['learned'][0]:.4f}s (aux {out['learned'][2]})")
+    ratios = [r["heuristic"][0] / r["learned"][0] for r in rows
+              if math.isfinite(r["heuristic"][0]) and math.isfinite(r["learned"][0])]
+    gm = float(np.exp(np.mean(np.log(ratios)))) if ratios else float("nan")
+    return gm, rows
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(prog="python -m amfls_solver_amd.width_model")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("train")
+    t.add_argument("--problems", type=int, default=300)
+    t.add_argument("--heldout", type=int, default=40)
+    t.add_argument("--epochs", type=int, default=400)
+    t.add_argument("--out", type=Path, default=DEFAULT_PATH)
+    e = sub.add_parser("evaluate")
+    e.add_argument("--model", type=Path, default=DEFAULT_PATH)
+    e.add_argument("--heldout", type=int, default=40)
+    e.add_argument("--update", type=Path, default=None, help="write the model with the held-out result here")
+    a = ap.parse_args(argv)
+    if a.cmd == "train":
+        print(f"collecting {a.problems} training problems", flush=True)
+        X, y, info = collect(range(a.problems), log=lambda s: print(s, flush=True))
+        print(f"{X.shape[0]} decisions, {y.mean():.3f} labelled widen", flush=True)
+        model = train(X, y, epochs=a.epochs, log=lambda s: print(s, flush=True))
+        z = model.logits(X)
+        acc = float(np.mean((z > 0) == (y > 0.5)))
+        print(f"training accuracy {acc:.3f}", flush=True)
+        gm, rows = evaluate(model, range(100000, 100000 + a.heldout), log=lambda s: print(s, flush=True))
+        print(f"held-out geometric-mean speedup of learned over heuristic: {gm:.3f}", flush=True)
+        model.save(a.out, problems=a.problems, decisions=int(X.shape[0]), widen_fraction=float(y.mean()),
+                   train_accuracy=acc, heldout=a.heldout, heldout_speedup=gm, epochs=a.epochs)
+        print(f"saved {a.out}", flush=True)
+    else:
+        model = WidthModel.load(a.model)
+        gm, rows = evaluate(model, range(100000, 100000 + a.heldout), log=lambda s: print(s, flush=True))
+        print(f"held-out geometric-mean speedup of learned over heuristic: {gm:.3f}", flush=True)
+        if a.update:
+            meta = dict(model.meta)
+            meta.update(heldout=a.heldout, heldout_speedup=gm)
+            model.save(a.update, **meta)
+            print(f"wrote {a.update}", flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -51,7 +51,18 @@ enum {
 enum { AMFLS_F64 = 0, AMFLS_F32 = 1, AMFLS_BF16 = 2 };
 enum { AMFLS_PRECISION_FP64 = 0, AMFLS_PRECISION_MIXED = 1, AMFLS_PRECISION_FP32 = 2 };
 enum { AMFLS_CLOCK_WALL = 0, AMFLS_CLOCK_SYNTHETIC = 1 };
-enum { AMFLS_WIDTH_HEURISTIC = 0, AMFLS_WIDTH_LEARNED = 1 };
+/* Widening predicate: the reference heuristic, the learned model, the
+ * heuristic with the model inputs of each decision recorded, or a fixed
+ * answer when a stage could widen: always (while the widened block has at
+ * most 64 columns) or never. */
+enum {
+    AMFLS_WIDTH_HEURISTIC = 0,
+    AMFLS_WIDTH_LEARNED = 1,
+    AMFLS_WIDTH_RECORD = 2,
+    AMFLS_WIDTH_WIDEN = 3,
+    AMFLS_WIDTH_HOLD = 4,
+    AMFLS_WIDTH_RECORD_WIDEN = 5 /* record while always widening */
+};
 
 #define AMFLS_METRICS_WIDTH 21
 #define AMFLS_COUNTERS_WIDTH 36
@@ -60,7 +71,11 @@ enum { AMFLS_WIDTH_HEURISTIC = 0, AMFLS_WIDTH_LEARNED = 1 };
 #define AMFLS_TRACE_INT_WIDTH 4
 #define AMFLS_TRACE_EXT_WIDTH 3
 
+#define AMFLS_WIDTH_INPUTS 12
+#define AMFLS_WIDTH_PARAMS 5057
+
 typedef struct amfls_context amfls_context;
+typedef struct amfls_width_model amfls_width_model;
 typedef struct amfls_operator amfls_operator;
 typedef struct amfls_result amfls_result;
 
@@ -146,6 +161,35 @@ int amfls_result_copy(const amfls_result* result, double* solution, double* metr
 int amfls_result_trace_extension(const amfls_result* result, double* out);
 int amfls_result_refinement_steps(const amfls_result* result, int32_t* steps);
 int amfls_result_destroy(amfls_result* result);
+
+/* Learned width controller (engine extension; it replaces the reference's
+ * widening predicate, src/algorithms/amfls/amfls.cpp:322-358 and the cost
+ * gate of amfls_internal.hpp:318-324, when options.width_policy is
+ * AMFLS_WIDTH_LEARNED; the FP64 certificate still decides termination).
+ * A 12 -> 64 -> 64 -> 1 ReLU MLP in FP32: params holds AMFLS_WIDTH_PARAMS
+ * values (W1[64x12], b1[64], W2[64x64], b2[64], W3[64], b3), norm the input
+ * means then inverse standard deviations (2 x 12).  A model uses the
+ * context it was created on and must be destroyed before it.
+ * AMFLS_WIDTH_RECORD solves with the heuristic and records, per widening
+ * decision, the 12 model inputs and the heuristic's choice (1 = widen). */
+int amfls_width_model_create(amfls_context* ctx, const float* params, const float* norm, amfls_width_model** out);
+int amfls_width_model_destroy(amfls_width_model* model);
+int amfls_width_model_get_params(amfls_width_model* model, float* params);
+int amfls_width_model_set_params(amfls_width_model* model, const float* params);
+/* Logits of n host feature rows (n x 12) into host logits[n]. */
+int amfls_width_model_logits(amfls_width_model* model, const float* features, int64_t n, float* logits);
+/* Mean binary cross-entropy of device features (n x 12) and labels (n) and
+ * its gradient (host grad[AMFLS_WIDTH_PARAMS], may be NULL). */
+int amfls_width_model_gradient(amfls_width_model* model, const float* features, const float* labels, int64_t n,
+                               float* grad, float* loss);
+/* One Adam step on a device batch; *loss is the batch loss before the step. */
+int amfls_width_model_train_step(amfls_width_model* model, const float* features, const float* labels, int64_t n,
+                                 float learning_rate, float* loss);
+/* The model used by AMFLS_WIDTH_LEARNED solves on this context (NULL clears). */
+int amfls_context_set_width_model(amfls_context* ctx, amfls_width_model* model);
+/* Recorded decisions of an AMFLS_WIDTH_RECORD solve: rows x 13 floats
+ * (out may be NULL to query rows). */
+int amfls_result_width_records(const amfls_result* result, float* out, int64_t* rows);
 
 /* Counter-addressed Gaussian fill (reference random/gaussian.cpp:25-55)
  * of a row-major device block; dtype AMFLS_F64 or AMFLS_F32. */

@@ -296,3 +296,86 @@ def test_dense_matrix_core_products(torch_mod, vt):
             z32 = M.apply(torch.from_numpy(Yg).float(), transpose=True).cpu().double().numpy()
             assert np.abs(y32 - Y).max() <= 1e-5 * np.abs(np.abs(stored) @ np.abs(X)).max()
             assert np.abs(z32 - Z).max() <= 1e-5 * np.abs(np.abs(stored).T @ np.abs(Yg)).max()
+
+
+def _mlp_data(n, seed):
+    r = np.random.default_rng(seed)
+    X = (r.standard_normal((n, 12)) * 2.0 + 0.5).astype(np.float32)
+    y = (r.random(n) < 0.4).astype(np.float32)
+    norm = np.concatenate([np.full(12, 0.5), np.full(12, 0.5)]).astype(np.float32)
+    return X, y, norm
+
+
+def test_width_mlp_kernels_match_oracle(torch_mod):
+    """Forward logits and the gradient of the mean BCE from the HIP kernels
+    against the float64 CPU oracle (oracle/mlp.py)."""
+    from amfls_solver_amd.width_model import WidthModel, init_params
+    from oracle import mlp as OM
+    torch = torch_mod
+    X, y, norm = _mlp_data(300, 4)
+    p = init_params(9)
+    p[768:832] = 0.03
+    m = WidthModel(p, norm)
+    z = m.logits(X)
+    zo, _ = OM.forward(p.astype(np.float64), norm, X)
+    assert np.abs(z - zo).max() <= 1e-4 * max(1.0, np.abs(zo).max())
+    loss, g = m.gradient(torch.from_numpy(X), torch.from_numpy(y))
+    go = OM.gradient(p.astype(np.float64), norm, X, y)
+    assert abs(loss - OM.loss(p.astype(np.float64), norm, X, y)) <= 1e-5
+    assert np.abs(g - go).max() <= 1e-5 + 1e-4 * np.abs(go).max()
+
+
+def test_width_mlp_training_reduces_loss(torch_mod):
+    """Adam on the GPU learns a separable rule (label = x0 + x1 > 0)."""
+    from amfls_solver_amd.width_model import WidthModel, init_params
+    torch = torch_mod
+    r = np.random.default_rng(1)
+    X = r.standard_normal((512, 12)).astype(np.float32)
+    y = (X[:, 0] + X[:, 1] > 0).astype(np.float32)
+    norm = np.concatenate([np.zeros(12), np.ones(12)]).astype(np.float32)
+    m = WidthModel(init_params(2), norm)
+    Xd, yd = torch.from_numpy(X).cuda(), torch.from_numpy(y).cuda()
+    first = m.train_step(Xd, yd, 1e-2)
+    for _ in range(300):
+        last = m.train_step(Xd, yd, 1e-2)
+    assert first > 0.3 and last < 0.1
+    assert np.mean((m.logits(X) > 0) == (y > 0.5)) > 0.97
+
+
+def test_width_policies_record_and_learned_certify(torch_mod):
+    """The record policy returns one 13-value row per widening decision and
+    follows the heuristic; the learned policy (trained model shipped with the
+    package, else an untrained one) still certifies: the FP64 certificate is
+    the only termination test."""
+    import amfls_solver_amd as am
+    from amfls_solver_amd import width_model as W
+    torch = torch_mod
+    A, b, lam, _ = W.training_problem(3)
+    M = am.DeviceMatrix(A, value_dtype=torch.float32 if A.layout == torch.sparse_csr else torch.float64)
+    rec = W.record(M, b.cuda(), lam, 1e-8)
+    assert rec.ndim == 2 and rec.shape[1] == 13 and set(np.unique(rec[:, 12])) <= {0.0, 1.0}
+    model = W.WidthModel.load() if W.DEFAULT_PATH.exists() else W.WidthModel(W.init_params(0), np.concatenate(
+        [np.zeros(12), np.ones(12)]).astype(np.float32))
+    model.attach()
+    M2 = am.DeviceMatrix(A, value_dtype=M.value_dtype, context=model.context)
+    r = am.run_amfls(M2, b.cuda(), regularization=lam, tolerance=1e-8, width_policy=1, context=model.context)
+    assert r.status == 0
+    with pytest.raises(ValueError):
+        ctx2 = am.Context()
+        M3 = am.DeviceMatrix(A, value_dtype=M.value_dtype, context=ctx2)
+        am.run_amfls(M3, b.cuda(), regularization=lam, tolerance=1e-8, width_policy=1, context=ctx2)
+
+
+def test_learned_width_policy_gains_over_heuristic(torch_mod):
+    """What the trained network gains: on held-out dense problems (drawn like
+    the training set, never trained on) where widening pays, the shipped
+    model widens and certifies sooner than the heuristic controller, which
+    stays at width one; both certify the same tolerance."""
+    from amfls_solver_amd import width_model as W
+    model = W.WidthModel.load()
+    assert model.meta["heldout_speedup"] >= 1.0  # 40 held-out problems at training time
+    gm, rows = W.evaluate(model, [100013, 100028, 100030], repeats=3, log=print)
+    for r in rows:
+        assert r["heuristic"][1] == 0 and r["learned"][1] == 0
+        assert r["learned"][2] > r["heuristic"][2]  # the model widened
+    assert gm > 1.2, rows

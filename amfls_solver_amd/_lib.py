@@ -99,6 +99,18 @@ SIGNATURES = {
     "amfls_width_model_logits": (c_int, [c_void_p, c_void_p, c_int64, c_void_p]),
     "amfls_width_model_gradient": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, POINTER(c_float)]),
     "amfls_width_model_train_step": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_float, POINTER(c_float)]),
+    "amfls_csr_generate_rows": (c_int, [c_void_p, c_int64, c_int64, c_int32, c_int32, c_uint64, c_void_p,
+                                        POINTER(c_int64)]),
+    "amfls_csr_generate_entries": (c_int, [c_void_p, c_int64, c_int64, c_uint64, c_double, c_double, c_void_p,
+                                           c_void_p, c_void_p, c_int]),
+    "amfls_operator_autotune": (c_int, [c_void_p, c_void_p, POINTER(c_int32), c_int32, c_int, c_int32,
+                                        POINTER(c_int32)]),
+    "amfls_solve_sweep": (c_int, [c_void_p, c_void_p, c_void_p, c_int, POINTER(c_double), c_int32,
+                                  POINTER(Options), POINTER(c_void_p)]),
+    "amfls_solve_batch": (c_int, [c_void_p, c_int32, POINTER(c_void_p), POINTER(c_void_p), POINTER(c_double),
+                                  c_int32, POINTER(Options), c_int32, POINTER(c_void_p)]),
+    "amfls_width_model_set_domain": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "amfls_width_model_decisions": (c_int, [c_void_p, POINTER(c_int64), POINTER(c_int64)]),
     "amfls_context_set_width_model": (c_int, [c_void_p, c_void_p]),
 }
 

@@ -223,6 +223,34 @@ class DeviceMatrix:
         self.handle = h
         self.value_dtype = vd
 
+    @classmethod
+    def from_csr_arrays(cls, rows: int, cols: int, row_ptr: torch.Tensor, col_idx: torch.Tensor,
+                        values: torch.Tensor, *, context: Context | None = None) -> "DeviceMatrix":
+        """CSR operator over device arrays (int64 row pointers, int32 column
+        indices, float32/float64 values) that the caller keeps unchanged."""
+        dev = _require_gpu()
+        self = cls.__new__(cls)
+        self.context = context or default_context()
+        if rows <= 0 or cols <= 0 or rows > 2**31 - 1 or cols > 2**31 - 1:
+            raise ValueError("matrix must have shape (m,n) with 0 < m, n < 2^31")
+        if row_ptr.dtype != torch.int64 or col_idx.dtype != torch.int32 or values.dtype not in (torch.float64,
+                                                                                                torch.float32):
+            raise ValueError("CSR arrays must be int64 row pointers, int32 columns and float32/float64 values")
+        if row_ptr.numel() != rows + 1 or col_idx.numel() != values.numel():
+            raise ValueError("CSR row offsets must have length rows + 1")
+        self.rows, self.cols = int(rows), int(cols)
+        self._rp = row_ptr.to(device=dev).contiguous()
+        self._ci = col_idx.to(device=dev).contiguous()
+        self._val = values.to(device=dev).contiguous()
+        h = ctypes.c_void_p()
+        torch.cuda.synchronize()
+        _lib.check(self.context._lib.amfls_csr_create(self.context.handle, self.rows, self.cols, self._val.numel(),
+                                                      self._rp.data_ptr(), self._ci.data_ptr(),
+                                                      self._val.data_ptr(), _DTYPE_CODE[values.dtype],
+                                                      ctypes.byref(h)))
+        self.kind, self.nnz, self.handle, self.value_dtype = "csr", int(self._val.numel()), h, values.dtype
+        return self
+
     def validation_model(self):
         lib = self.context._lib
         up, lo = ctypes.c_double(), ctypes.c_double()
@@ -256,6 +284,19 @@ class DeviceMatrix:
     def set_variant(self, s: int, transpose: bool, variant: int) -> None:
         _lib.check(self.context._lib.amfls_operator_set_variant(self.handle, s, int(transpose), variant))
 
+    def autotune(self, widths=(1, 2, 4, 8, 16, 32, 64), dtype: torch.dtype = torch.float64, reps: int = 5):
+        """Time the SpMM kernel variants for each block width on this CSR
+        matrix and keep the fastest for A and A^T; returns
+        {width: (A variant name, A^T variant name)}."""
+        w = (ctypes.c_int32 * len(widths))(*[int(x) for x in widths])
+        chosen = (ctypes.c_int32 * (2 * len(widths)))()
+        torch.cuda.synchronize()
+        _lib.check(self.context._lib.amfls_operator_autotune(self.context.handle, self.handle, w, len(widths),
+                                                             _DTYPE_CODE[dtype], int(reps), chosen))
+        name = self.context._lib.amfls_spmm_variant_name
+        return {int(s): (name(chosen[2 * i]).decode(), name(chosen[2 * i + 1]).decode())
+                for i, s in enumerate(widths)}
+
     def profile(self, enable: bool = True) -> None:
         _lib.check(self.context._lib.amfls_operator_profile(self.handle, int(enable)))
 
@@ -274,6 +315,28 @@ class DeviceMatrix:
         if h is not None and h.value:
             self.context._lib.amfls_operator_destroy(h)
             self.handle = None
+
+
+def generate_csr(rows: int, cols: int, nnz_range=(4, 32), *, seed: int = 0, scale_range=(1.0, 1.0),
+                 value_dtype: torch.dtype = torch.float32, context: "Context | None" = None):
+    """Counter-addressed synthetic CSR built on the device: per-row lengths
+    uniform in nnz_range (at most 64), ascending distinct uniform columns,
+    N(0,1) values times log-uniform row scales in scale_range.  Returns
+    (row_ptr int64, col_idx int32, values) device tensors."""
+    dev = _require_gpu()
+    ctx = context or default_context()
+    lo, hi = int(nnz_range[0]), int(nnz_range[1])
+    rp = torch.empty(rows + 1, dtype=torch.int64, device=dev)
+    nnz = ctypes.c_int64()
+    torch.cuda.synchronize()
+    _lib.check(ctx._lib.amfls_csr_generate_rows(ctx.handle, int(rows), int(cols), lo, hi, int(seed) % _UINT64_MODULUS,
+                                                rp.data_ptr(), ctypes.byref(nnz)))
+    ci = torch.empty(max(nnz.value, 1), dtype=torch.int32, device=dev)[:nnz.value]
+    val = torch.empty(max(nnz.value, 1), dtype=value_dtype, device=dev)[:nnz.value]
+    _lib.check(ctx._lib.amfls_csr_generate_entries(ctx.handle, int(rows), int(cols), int(seed) % _UINT64_MODULUS,
+                                                   float(scale_range[0]), float(scale_range[1]), rp.data_ptr(),
+                                                   ci.data_ptr(), val.data_ptr(), _DTYPE_CODE[value_dtype]))
+    return rp, ci, val
 
 
 def gaussian_block(rows: int, cols: int, seed: int, stream: int, *, dtype: torch.dtype = torch.float64,
@@ -413,6 +476,100 @@ def run_amfls(matrix, b, *, regularization: float = 0.0, tolerance: float = 1e-8
                                     maximum_refinements=maximum_refinements, inner_tolerance=inner_tolerance,
                                     width_policy=width_policy, context=context)
     return _decode(raw, ext, steps)
+
+
+def _sweep_options(tolerance, failure_probability, maximum_epochs, maximum_depth, maximum_auxiliary_width,
+                   maximum_basis_size, seed, stream, precision, clock, width_policy):
+    if precision not in ("fp64", "fp32"):
+        raise ValueError("regularization sweeps run in fp64 or fp32 precision")
+    return _lib.options(tolerance=float(tolerance), failure_probability=float(failure_probability),
+                        maximum_epochs=int(maximum_epochs), maximum_depth=int(maximum_depth),
+                        maximum_auxiliary_width=int(maximum_auxiliary_width),
+                        maximum_basis_size=int(maximum_basis_size), seed=int(seed) % _UINT64_MODULUS,
+                        stream=int(stream) % _UINT64_MODULUS, precision=_lib.PRECISION[precision],
+                        clock_mode=_lib.CLOCK[clock], width_policy=int(width_policy))
+
+
+def run_amfls_sweep(matrix, b, lambdas, *, tolerance: float = 1e-8, failure_probability: float = 1e-6,
+                    maximum_epochs: int = 0, maximum_depth: int = 0, maximum_auxiliary_width: int = 0,
+                    maximum_basis_size: int = 0, seed: int = 0, stream: int = 0, precision: str = "fp64",
+                    clock: str = "wall", value_dtype=None, width_policy: int = 0,
+                    context: Context | None = None) -> list[SolveResult]:
+    """One certified solve per regularization in `lambdas`, all served by one
+    block Golub-Kahan process (engine extension; the reference runs
+    run_amfls once per lambda, binding.py:259-290)."""
+    op = _operator(matrix, value_dtype, context)
+    ctx = op.context
+    bd = _rhs(b, op._val.device)
+    if bd.numel() != op.rows:
+        raise ValueError("right-hand side shape mismatch")
+    lam = np.ascontiguousarray(np.asarray(lambdas, dtype=np.float64).ravel())
+    if lam.size == 0:
+        raise ValueError("at least one regularization is required")
+    o = _sweep_options(tolerance, failure_probability, maximum_epochs, maximum_depth, maximum_auxiliary_width,
+                       maximum_basis_size, seed, stream, precision, clock, width_policy)
+    res = (ctypes.c_void_p * lam.size)()
+    torch.cuda.synchronize()
+    _lib.check(ctx._lib.amfls_solve_sweep(ctx.handle, op.handle, bd.data_ptr(), 1,
+                                          lam.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), lam.size,
+                                          ctypes.byref(o), res))
+    handles = [ctypes.c_void_p(res[j]) for j in range(lam.size)]
+    out = []
+    for j, h in enumerate(handles):
+        try:
+            out.append(_decode(*_result_tuple(ctx, h)))
+        except Exception:
+            for k in handles[j + 1:]:
+                ctx._lib.amfls_result_destroy(k)
+            raise
+    return out
+
+
+def run_amfls_batch(matrices, rhs, lambdas, *, workers: int = 4, tolerance: float = 1e-8,
+                    failure_probability: float = 1e-6, maximum_epochs: int = 0, maximum_depth: int = 0,
+                    maximum_auxiliary_width: int = 0, maximum_basis_size: int = 0, seed: int = 0, stream: int = 0,
+                    precision: str = "fp64", clock: str = "wall", width_policy: int = 0,
+                    context: Context | None = None) -> list[list[SolveResult]]:
+    """Independent CSR problems (DeviceMatrix list, device right-hand sides,
+    lambdas of shape problems x L) solved concurrently by `workers` host
+    threads with a stream each; every problem runs a regularization sweep
+    (engine extension)."""
+    mats = list(matrices)
+    bs = [torch.as_tensor(b) for b in rhs]
+    lam = np.ascontiguousarray(np.asarray(lambdas, dtype=np.float64))
+    if lam.ndim != 2 or lam.shape[0] != len(mats) or len(bs) != len(mats) or lam.shape[1] == 0:
+        raise ValueError("lambdas must have shape (problems, L) with L > 0, one right-hand side per problem")
+    ctx = context or (mats[0].context if mats else default_context())
+    keep = []
+    for M, b in zip(mats, bs):
+        if not isinstance(M, DeviceMatrix) or M.kind != "csr":
+            raise ValueError("batched solves take CSR DeviceMatrix operators")
+        if b.numel() != M.rows:
+            raise ValueError("right-hand side shape mismatch")
+        keep.append(_rhs(b, M._val.device))
+    o = _sweep_options(tolerance, failure_probability, maximum_epochs, maximum_depth, maximum_auxiliary_width,
+                       maximum_basis_size, seed, stream, precision, clock, width_policy)
+    P, L = lam.shape
+    ops = (ctypes.c_void_p * max(P, 1))(*[M.handle.value for M in mats])
+    bp = (ctypes.c_void_p * max(P, 1))(*[b.data_ptr() for b in keep])
+    res = (ctypes.c_void_p * max(P * L, 1))()
+    torch.cuda.synchronize()
+    _lib.check(ctx._lib.amfls_solve_batch(ctx.handle, P, ops, bp, lam.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                          L, ctypes.byref(o), int(workers), res))
+    handles = [ctypes.c_void_p(res[k]) for k in range(P * L)]
+    out, flat = [], []
+    try:
+        for h in handles:
+            taken = ctypes.c_void_p(h.value)
+            h.value = None  # _result_tuple releases it, also when it raises
+            flat.append(_decode(*_result_tuple(ctx, taken)))
+    finally:
+        for h in handles:
+            if h.value:
+                ctx._lib.amfls_result_destroy(h)
+    for i in range(P):
+        out.append(flat[i * L:(i + 1) * L])
+    return out
 
 
 def run_fixed_rbgk(matrix, b, *, auxiliary_width: int, maximum_depth: int, maximum_basis_size: int = 0,

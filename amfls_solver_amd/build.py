@@ -28,6 +28,7 @@ SOURCES = [
     CSRC / "device" / "level_chain.hip",
     CSRC / "device" / "dense_mfma.hip",
     CSRC / "device" / "mlp.hip",
+    CSRC / "device" / "generate.hip",
     CSRC / "engine" / "operator.cpp",
     CSRC / "engine" / "small_la.cpp",
     CSRC / "engine" / "validation.cpp",

@@ -150,6 +150,15 @@ void mlp_gradient(hipStream_t s, const float* params, const float* norm, int64_t
 void mlp_adam(hipStream_t s, float* params, float* m, float* v, const float* grad, float lr, int step);
 
 // Counter-addressed Gaussian block, index = (first_col + j) * rows + i.
+// Counter-addressed synthetic CSR (generate.hip): row pointers (rows + 1,
+// device) for row lengths in [lo, hi] (hi <= 64), returning nnz (waits);
+// then sorted distinct uniform columns and N(0,1) values times log-uniform
+// row scales in [scale_lo, scale_hi].
+int64_t csr_generate_rows(hipStream_t s, int64_t rows, int64_t cols, int lo, int hi, uint64_t seed,
+                          int64_t* row_ptr, Scratch& scratch);
+void csr_generate_entries(hipStream_t s, int64_t rows, int64_t cols, uint64_t seed, double scale_lo,
+                          double scale_hi, const int64_t* row_ptr, int32_t* col_idx, void* values, ValueType type);
+
 template <typename T>
 void gaussian_block(hipStream_t s, int64_t rows, int cols, uint64_t seed, uint64_t stream,
                     uint64_t first_col, T* Y, int64_t ldy);
@@ -166,12 +175,18 @@ struct CsrView {
 };
 
 // Y(rows x ncols) = A * X (+ add_scale * Add[0:rows]); X, Y, Add row-major.
+// With ev_start / ev_stop, the events take the start and end timestamps of
+// the product's kernel dispatches themselves (hipExtLaunchKernel), so their
+// elapsed time is kernel execution without queue gaps.
 template <typename T>
 void csr_spmm(hipStream_t s, const CsrView& A, int ncols, const T* X, int64_t ldx, T* Y, int64_t ldy,
-              const T* Add, int64_t ld_add, double add_scale, int variant);
+              const T* Add, int64_t ld_add, double add_scale, int variant, hipEvent_t ev_start = nullptr,
+              hipEvent_t ev_stop = nullptr);
 int csr_spmm_variant_count();
 bool csr_spmm_variant_supports(int variant, int ncols);
 const char* csr_spmm_variant_name(int variant);
+// Entry lanes per row (p), column lanes (lc) and columns per lane (cpl).
+bool csr_spmm_variant_shape(int variant, int* p, int* lc, int* cpl);
 // Variant used when none is set: lanes sized to the block width and the
 // mean number of entries per row.
 int csr_spmm_default_variant(int ncols, double mean_row_nnz);

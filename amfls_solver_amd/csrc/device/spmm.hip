@@ -12,6 +12,7 @@
 // needed and the summation order is deterministic.
 #include "kernels.hpp"
 
+#include <hip/hip_ext.h>
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -125,23 +126,29 @@ constexpr int kVariantCount = sizeof(kVariants) / sizeof(kVariants[0]);
 
 template <typename V, typename T, int P, int LC, int CPL>
 void launch_variant(hipStream_t s, const CsrView& A, int ncols, const T* X, int64_t ldx, T* Y,
-                    int64_t ldy, const T* Add, int64_t ld_add, double add_scale) {
+                    int64_t ldy, const T* Add, int64_t ld_add, double add_scale, hipEvent_t e0, hipEvent_t e1) {
     constexpr int G = P * LC;
     const int64_t groups_per_block = kThreads / G;
     const dim3 grid(static_cast<unsigned>(ceil_div(A.rows, groups_per_block)));
-    hipLaunchKernelGGL((csr_spmm_kernel<V, T, P, LC, CPL>), grid, dim3(kThreads), 0, s, A.rows, A.row_ptr,
-                       A.col_idx, static_cast<const V*>(A.values), ncols, X, ldx, Y, ldy, Add, ld_add,
-                       static_cast<T>(add_scale));
+    if (e0 || e1)
+        hipExtLaunchKernelGGL((csr_spmm_kernel<V, T, P, LC, CPL>), grid, dim3(kThreads), 0, s, e0, e1, 0, A.rows,
+                              A.row_ptr, A.col_idx, static_cast<const V*>(A.values), ncols, X, ldx, Y, ldy, Add,
+                              ld_add, static_cast<T>(add_scale));
+    else
+        hipLaunchKernelGGL((csr_spmm_kernel<V, T, P, LC, CPL>), grid, dim3(kThreads), 0, s, A.rows, A.row_ptr,
+                           A.col_idx, static_cast<const V*>(A.values), ncols, X, ldx, Y, ldy, Add, ld_add,
+                           static_cast<T>(add_scale));
     AMFLS_LAUNCH_CHECK();
 }
 
 template <typename V, typename T>
 void launch_by_index(int variant, hipStream_t s, const CsrView& A, int ncols, const T* X, int64_t ldx,
-                     T* Y, int64_t ldy, const T* Add, int64_t ld_add, double add_scale) {
+                     T* Y, int64_t ldy, const T* Add, int64_t ld_add, double add_scale, hipEvent_t e0,
+                     hipEvent_t e1) {
 #define AMFLS_CASE(i)                                                                            \
     case i:                                                                                      \
         launch_variant<V, T, kVariants[i].p, kVariants[i].lc, kVariants[i].cpl>(                 \
-            s, A, ncols, X, ldx, Y, ldy, Add, ld_add, add_scale);                                \
+            s, A, ncols, X, ldx, Y, ldy, Add, ld_add, add_scale, e0, e1);                        \
         break;
     static_assert(kVariantCount == 30, "variant switch must cover the table");
     switch (variant) {
@@ -280,6 +287,14 @@ const char* csr_spmm_variant_name(int variant) {
     return (variant >= 0 && variant < kVariantCount) ? kVariants[variant].name : "invalid";
 }
 
+bool csr_spmm_variant_shape(int variant, int* p, int* lc, int* cpl) {
+    if (variant < 0 || variant >= kVariantCount) return false;
+    *p = kVariants[variant].p;
+    *lc = kVariants[variant].lc;
+    *cpl = kVariants[variant].cpl;
+    return true;
+}
+
 int csr_spmm_variant_index(int p, int lc, int cpl) {
     for (int v = 0; v < kVariantCount; ++v)
         if (kVariants[v].p == p && kVariants[v].lc == lc && kVariants[v].cpl == cpl) return v;
@@ -301,8 +316,14 @@ int csr_spmm_default_variant(int ncols, double mean_row_nnz) {
 
 template <typename T>
 void csr_spmm(hipStream_t s, const CsrView& A, int ncols, const T* X, int64_t ldx, T* Y, int64_t ldy,
-              const T* Add, int64_t ld_add, double add_scale, int variant) {
-    if (A.rows <= 0 || ncols <= 0) return;
+              const T* Add, int64_t ld_add, double add_scale, int variant, hipEvent_t ev_start,
+              hipEvent_t ev_stop) {
+    if (A.rows <= 0 || ncols <= 0) {
+        // keep a caller's event pair well defined for an empty product
+        if (ev_start) AMFLS_HIP_CHECK(hipEventRecord(ev_start, s));
+        if (ev_stop) AMFLS_HIP_CHECK(hipEventRecord(ev_stop, s));
+        return;
+    }
     // Blocks wider than 256 columns are processed in 256-column slabs.
     for (int c0 = 0; c0 < ncols; c0 += 256) {
         const int nc = std::min(256, ncols - c0);
@@ -310,12 +331,14 @@ void csr_spmm(hipStream_t s, const CsrView& A, int ncols, const T* X, int64_t ld
         if (v < 0 || !csr_spmm_variant_supports(v, nc))
             v = csr_spmm_default_variant(nc, A.rows > 0 ? static_cast<double>(A.nnz) / A.rows : 0.0);
         const T* add = Add ? Add + c0 : nullptr;
+        hipEvent_t e0 = c0 == 0 ? ev_start : nullptr;
+        hipEvent_t e1 = c0 + nc == ncols ? ev_stop : nullptr;
         switch (A.type) {
             case ValueType::f32:
-                launch_by_index<float, T>(v, s, A, nc, X + c0, ldx, Y + c0, ldy, add, ld_add, add_scale);
+                launch_by_index<float, T>(v, s, A, nc, X + c0, ldx, Y + c0, ldy, add, ld_add, add_scale, e0, e1);
                 break;
             case ValueType::f64:
-                launch_by_index<double, T>(v, s, A, nc, X + c0, ldx, Y + c0, ldy, add, ld_add, add_scale);
+                launch_by_index<double, T>(v, s, A, nc, X + c0, ldx, Y + c0, ldy, add, ld_add, add_scale, e0, e1);
                 break;
             default:
                 throw std::invalid_argument("CSR values must be float32 or float64");
@@ -395,8 +418,8 @@ void csr_validate(hipStream_t s, const CsrView& A, int* out_dev) {
 }
 
 template void csr_spmm<float>(hipStream_t, const CsrView&, int, const float*, int64_t, float*, int64_t,
-                              const float*, int64_t, double, int);
+                              const float*, int64_t, double, int, hipEvent_t, hipEvent_t);
 template void csr_spmm<double>(hipStream_t, const CsrView&, int, const double*, int64_t, double*,
-                               int64_t, const double*, int64_t, double, int);
+                               int64_t, const double*, int64_t, double, int, hipEvent_t, hipEvent_t);
 
 }  // namespace amflsx

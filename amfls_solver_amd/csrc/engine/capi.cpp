@@ -286,62 +286,137 @@ int amfls_operator_kernel_stats(amfls_operator* op, int64_t* launches, double* s
     });
 }
 
+int amfls_operator_autotune(amfls_context* ctx, amfls_operator* op, const int32_t* widths, int32_t count,
+                            int dtype, int32_t reps, int32_t* chosen) {
+    return guard([&] {
+        require(ctx && op && widths && count > 0, "invalid argument");
+        require(dtype == AMFLS_F64 || dtype == AMFLS_F32, "autotune blocks are float64 or float32");
+        auto* csr = dynamic_cast<CsrOperator*>(op->op.get());
+        require(csr != nullptr, "autotuning applies to CSR operators");
+        const std::vector<int> v = csr->autotune(ctx->ctx, std::vector<int>(widths, widths + count),
+                                                 dtype == AMFLS_F64, reps);
+        if (chosen)
+            for (size_t i = 0; i < v.size(); ++i) chosen[i] = v[i];
+    });
+}
+
 int amfls_spmm_variant_count(void) { return csr_spmm_variant_count(); }
 const char* amfls_spmm_variant_name(int v) { return csr_spmm_variant_name(v); }
 int amfls_spmm_variant_supports(int v, int s) { return csr_spmm_variant_supports(v, s) ? 1 : 0; }
+
+namespace {
+
+SolveOptions solve_options(const amfls_options* options) {
+    SolveOptions o;
+    o.regularization = options->regularization;
+    o.tolerance = options->tolerance;
+    o.failure_probability = options->failure_probability;
+    o.maximum_epochs = options->maximum_epochs;
+    o.maximum_basis_size = options->maximum_basis_size;
+    o.seed = options->seed;
+    o.stream = options->stream;
+    o.maximum_depth = options->maximum_depth;
+    o.maximum_auxiliary_width = options->maximum_auxiliary_width;
+    require(options->precision >= AMFLS_PRECISION_FP64 && options->precision <= AMFLS_PRECISION_FP32,
+            "unknown precision");
+    o.precision = static_cast<Precision>(options->precision);
+    o.clock_mode = options->clock_mode == AMFLS_CLOCK_SYNTHETIC ? ClockMode::synthetic : ClockMode::wall;
+    o.synthetic_operator_unit = options->synthetic_operator_unit;
+    o.synthetic_local_unit = options->synthetic_local_unit;
+    o.maximum_refinements = options->maximum_refinements;
+    o.inner_tolerance = options->inner_tolerance;
+    o.width_policy = options->width_policy;
+    require(o.width_policy >= AMFLS_WIDTH_HEURISTIC && o.width_policy <= AMFLS_WIDTH_RECORD_WIDEN,
+            "unknown width policy");
+    return o;
+}
+
+// Widening predicate of a width policy: the reference heuristic (empty),
+// the learned model, constant choices, or the heuristic / always-widen with
+// every decision's model inputs recorded into `records`.
+WidthPolicy width_policy(const amfls_context* ctx, int code, std::vector<float>* records) {
+    if (code == AMFLS_WIDTH_LEARNED) {
+        require(ctx->width_model != nullptr, "the learned width policy needs a model set on the context");
+        WidthModel* m = ctx->width_model->m.get();
+        // the model decides within the widths it was trained on (blocks of at most 64 columns)
+        return [m](const WidthFeatures& f) { return f.active_width + f.proposed_increment <= 64 && m->widen(f); };
+    }
+    if (code == AMFLS_WIDTH_WIDEN)  // widen at every opportunity up to the widest benchmarked block (64)
+        return [](const WidthFeatures& f) { return f.active_width + f.proposed_increment <= 64; };
+    if (code == AMFLS_WIDTH_HOLD) return [](const WidthFeatures&) { return false; };
+    if (code == AMFLS_WIDTH_RECORD || code == AMFLS_WIDTH_RECORD_WIDEN) {
+        require(records != nullptr, "recording width policies apply to single solves");
+        const bool widen_always = code == AMFLS_WIDTH_RECORD_WIDEN;
+        return [records, widen_always](const WidthFeatures& f) {
+            const auto x = width_features(f);
+            records->insert(records->end(), x.begin(), x.end());
+            records->push_back(f.heuristic ? 1.0f : 0.0f);
+            return widen_always ? f.active_width + f.proposed_increment <= 64 : f.heuristic;
+        };
+    }
+    return WidthPolicy{};
+}
+
+}  // namespace
 
 int amfls_solve(amfls_context* ctx, const amfls_operator* op, const double* b, int b_on_device,
                 const amfls_options* options, amfls_result** out) {
     return guard([&] {
         require(ctx && op && options && out, "null argument");
-        SolveOptions o;
-        o.regularization = options->regularization;
-        o.tolerance = options->tolerance;
-        o.failure_probability = options->failure_probability;
-        o.maximum_epochs = options->maximum_epochs;
-        o.maximum_basis_size = options->maximum_basis_size;
-        o.seed = options->seed;
-        o.stream = options->stream;
-        o.maximum_depth = options->maximum_depth;
-        o.maximum_auxiliary_width = options->maximum_auxiliary_width;
-        require(options->precision >= AMFLS_PRECISION_FP64 && options->precision <= AMFLS_PRECISION_FP32,
-                "unknown precision");
-        o.precision = static_cast<Precision>(options->precision);
-        o.clock_mode = options->clock_mode == AMFLS_CLOCK_SYNTHETIC ? ClockMode::synthetic : ClockMode::wall;
-        o.synthetic_operator_unit = options->synthetic_operator_unit;
-        o.synthetic_local_unit = options->synthetic_local_unit;
-        o.maximum_refinements = options->maximum_refinements;
-        o.inner_tolerance = options->inner_tolerance;
-        o.width_policy = options->width_policy;
-        require(o.width_policy >= AMFLS_WIDTH_HEURISTIC && o.width_policy <= AMFLS_WIDTH_RECORD_WIDEN,
-                "unknown width policy");
+        const SolveOptions o = solve_options(options);
         RhsHolder rhs(ctx->ctx, b, op->op->rows(), b_on_device != 0);
         auto res = std::make_unique<amfls_result>();
-        // widening predicate: the reference heuristic, the learned model, or
-        // the heuristic with every decision's model inputs recorded
-        WidthPolicy policy;
-        if (o.width_policy == AMFLS_WIDTH_LEARNED) {
-            require(ctx->width_model != nullptr, "the learned width policy needs a model set on the context");
-            WidthModel* m = ctx->width_model->m.get();
-            // the model decides within the widths it was trained on (blocks of at most 64 columns)
-            policy = [m](const WidthFeatures& f) { return f.active_width + f.proposed_increment <= 64 && m->widen(f); };
-        } else if (o.width_policy == AMFLS_WIDTH_WIDEN) {
-            // widen at every opportunity up to the widest benchmarked block (64)
-            policy = [](const WidthFeatures& f) { return f.active_width + f.proposed_increment <= 64; };
-        } else if (o.width_policy == AMFLS_WIDTH_HOLD) {
-            policy = [](const WidthFeatures&) { return false; };
-        } else if (o.width_policy == AMFLS_WIDTH_RECORD || o.width_policy == AMFLS_WIDTH_RECORD_WIDEN) {
-            std::vector<float>* rec = &res->width_records;
-            const bool widen_always = o.width_policy == AMFLS_WIDTH_RECORD_WIDEN;
-            policy = [rec, widen_always](const WidthFeatures& f) {
-                const auto x = width_features(f);
-                rec->insert(rec->end(), x.begin(), x.end());
-                rec->push_back(f.heuristic ? 1.0f : 0.0f);
-                return widen_always ? f.active_width + f.proposed_increment <= 64 : f.heuristic;
-            };
-        }
+        const WidthPolicy policy = width_policy(ctx, o.width_policy, &res->width_records);
         res->r = solve_amfls(*op->op, rhs.ptr, o, ctx->ctx, policy ? &policy : nullptr);
         *out = res.release();
+    });
+}
+
+int amfls_solve_sweep(amfls_context* ctx, const amfls_operator* op, const double* b, int b_on_device,
+                      const double* lambdas, int32_t count, const amfls_options* options, amfls_result** results) {
+    return guard([&] {
+        require(ctx && op && options && results && lambdas && count > 0, "invalid argument");
+        const SolveOptions o = solve_options(options);
+        RhsHolder rhs(ctx->ctx, b, op->op->rows(), b_on_device != 0);
+        const WidthPolicy policy = width_policy(ctx, o.width_policy, nullptr);
+        std::vector<LeastSquaresResult> rs = solve_amfls_sweep(
+            *op->op, rhs.ptr, o, std::vector<double>(lambdas, lambdas + count), ctx->ctx, policy ? &policy : nullptr);
+        std::vector<std::unique_ptr<amfls_result>> hold(static_cast<size_t>(count));
+        for (int32_t j = 0; j < count; ++j) {
+            hold[j] = std::make_unique<amfls_result>();
+            hold[j]->r = std::move(rs[j]);
+        }
+        for (int32_t j = 0; j < count; ++j) results[j] = hold[j].release();
+    });
+}
+
+int amfls_solve_batch(amfls_context* ctx, int32_t problems, const amfls_operator* const* ops,
+                      const double* const* b_dev, const double* lambdas, int32_t lambdas_per_problem,
+                      const amfls_options* options, int32_t workers, amfls_result** results) {
+    return guard([&] {
+        require(ctx && options && results && problems >= 0 && lambdas_per_problem > 0 && lambdas, "invalid argument");
+        require(problems == 0 || (ops && b_dev), "null argument");
+        const SolveOptions o = solve_options(options);
+        const WidthPolicy policy = width_policy(ctx, o.width_policy, nullptr);
+        std::vector<BatchProblem> batch(static_cast<size_t>(problems));
+        for (int32_t i = 0; i < problems; ++i) {
+            require(ops[i] != nullptr && b_dev[i] != nullptr, "null operator or right-hand side in the batch");
+            require(dynamic_cast<const CsrOperator*>(ops[i]->op.get()) != nullptr,
+                    "batched solves take CSR operators");
+            batch[i].op = ops[i]->op.get();
+            batch[i].b_dev = b_dev[i];
+            batch[i].lambdas.assign(lambdas + static_cast<size_t>(i) * lambdas_per_problem,
+                                    lambdas + static_cast<size_t>(i + 1) * lambdas_per_problem);
+        }
+        ctx->ctx.sync();  // inputs written on the caller's stream are complete
+        auto rs = solve_amfls_batch(batch, o, workers, policy ? &policy : nullptr);
+        const size_t total = static_cast<size_t>(problems) * lambdas_per_problem;
+        std::vector<std::unique_ptr<amfls_result>> hold(total);
+        for (size_t k = 0; k < total; ++k) {
+            hold[k] = std::make_unique<amfls_result>();
+            hold[k]->r = std::move(rs[k / lambdas_per_problem][k % lambdas_per_problem]);
+        }
+        for (size_t k = 0; k < total; ++k) results[k] = hold[k].release();
     });
 }
 
@@ -543,6 +618,21 @@ int amfls_width_model_train_step(amfls_width_model* model, const float* features
     });
 }
 
+int amfls_width_model_set_domain(amfls_width_model* model, const float* lo, const float* hi) {
+    return guard([&] {
+        require(model && lo && hi, "null argument");
+        model->m->set_domain(lo, hi);
+    });
+}
+
+int amfls_width_model_decisions(const amfls_width_model* model, int64_t* by_model, int64_t* by_heuristic) {
+    return guard([&] {
+        require(model && by_model && by_heuristic, "null argument");
+        *by_model = model->m->model_decisions();
+        *by_heuristic = model->m->fallback_decisions();
+    });
+}
+
 int amfls_context_set_width_model(amfls_context* ctx, amfls_width_model* model) {
     return guard([&] {
         require(ctx != nullptr, "null argument");
@@ -552,6 +642,26 @@ int amfls_context_set_width_model(amfls_context* ctx, amfls_width_model* model) 
 
 int amfls_result_destroy(amfls_result* res) {
     return guard([&] { delete res; });
+}
+
+int amfls_csr_generate_rows(amfls_context* ctx, int64_t rows, int64_t cols, int32_t nnz_lo, int32_t nnz_hi,
+                            uint64_t seed, int64_t* row_ptr, int64_t* nnz) {
+    return guard([&] {
+        require(ctx && row_ptr && nnz, "null argument");
+        *nnz = csr_generate_rows(ctx->ctx.stream, rows, cols, nnz_lo, nnz_hi, seed, row_ptr, ctx->ctx.scratch);
+    });
+}
+
+int amfls_csr_generate_entries(amfls_context* ctx, int64_t rows, int64_t cols, uint64_t seed, double scale_lo,
+                               double scale_hi, const int64_t* row_ptr, int32_t* col_idx, void* values,
+                               int value_type) {
+    return guard([&] {
+        require(ctx && row_ptr && col_idx && values, "null argument");
+        require(value_type == AMFLS_F64 || value_type == AMFLS_F32, "generated values are float64 or float32");
+        csr_generate_entries(ctx->ctx.stream, rows, cols, seed, scale_lo, scale_hi, row_ptr, col_idx, values,
+                             value_type == AMFLS_F64 ? ValueType::f64 : ValueType::f32);
+        ctx->ctx.sync();
+    });
 }
 
 int amfls_gaussian_fill(amfls_context* ctx, void* y, int64_t rows, int32_t cols, int64_t ld, int dtype,

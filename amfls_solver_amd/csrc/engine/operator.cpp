@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <limits>
 #include <stdexcept>
 
 #include "fp_bounds.hpp"
@@ -66,6 +67,20 @@ void DeviceOperator::prof_end(hipStream_t s, int ncols, bool transpose, int eb) 
     AMFLS_HIP_CHECK(hipEventRecord(e, s));
     pending_.push_back({open_start_, e, product_bytes(ncols, transpose, eb)});
     open_start_ = nullptr;
+}
+
+void DeviceOperator::prof_events(hipEvent_t* start, hipEvent_t* stop) const {
+    *start = *stop = nullptr;
+    if (!profiling_) return;
+    for (hipEvent_t* e : {start, stop}) {
+        if (pool_.empty()) AMFLS_HIP_CHECK(hipEventCreate(e));
+        else { *e = pool_.back(); pool_.pop_back(); }
+    }
+}
+
+void DeviceOperator::prof_commit(hipEvent_t start, hipEvent_t stop, int ncols, bool transpose, int eb) const {
+    if (start == nullptr || stop == nullptr) return;
+    pending_.push_back({start, stop, product_bytes(ncols, transpose, eb)});
 }
 
 void DeviceOperator::resolve_pending() {
@@ -154,25 +169,90 @@ int CsrOperator::variant_for(int s, bool transpose) const {
 
 void CsrOperator::set_variant(int s, bool transpose, int variant) { variants_[{s, transpose}] = variant; }
 
+namespace {
+
+template <typename T>
+std::vector<int> tune_widths(Context& ctx, const CsrView& a, const CsrView& at, int64_t rows, int64_t cols,
+                             const std::vector<int>& widths, int reps,
+                             std::map<std::pair<int, bool>, int>& variants) {
+    std::vector<int> chosen;
+    std::vector<hipEvent_t> ev(2 * static_cast<size_t>(reps));
+    for (hipEvent_t& e : ev) AMFLS_HIP_CHECK(hipEventCreate(&e));
+    try {
+        const int64_t big = std::max(rows, cols);
+        for (int s : widths) {
+            if (s < 1 || s > 256) throw std::invalid_argument("autotune widths must lie in [1, 256]");
+            DevBuf<T> in(static_cast<size_t>(big) * s), out(static_cast<size_t>(big) * s);
+            fill<T>(ctx.stream, in.get(), big * s, T(1));
+            int lanes = 1;
+            while (lanes < s) lanes <<= 1;
+            for (int tr = 0; tr < 2; ++tr) {
+                const CsrView& A = tr ? at : a;
+                const double mean = A.rows > 0 ? static_cast<double>(A.nnz) / A.rows : 0.0;
+                int best = csr_spmm_default_variant(s, mean);
+                float best_ms = std::numeric_limits<float>::infinity();
+                for (int v = 0; v < csr_spmm_variant_count(); ++v) {
+                    int p = 0, lc = 0, cpl = 0;
+                    csr_spmm_variant_shape(v, &p, &lc, &cpl);
+                    if (!csr_spmm_variant_supports(v, s)) continue;
+                    if (s <= 64 && (cpl != 1 || (lc != lanes && lc != 2 * lanes))) continue;
+                    csr_spmm<T>(ctx.stream, A, s, in.get(), s, out.get(), s, nullptr, 0, 0.0, v);  // warm
+                    for (int r = 0; r < reps; ++r)
+                        csr_spmm<T>(ctx.stream, A, s, in.get(), s, out.get(), s, nullptr, 0, 0.0, v, ev[2 * r],
+                                    ev[2 * r + 1]);
+                    ctx.sync();
+                    std::vector<float> ms(static_cast<size_t>(reps));
+                    for (int r = 0; r < reps; ++r) AMFLS_HIP_CHECK(hipEventElapsedTime(&ms[r], ev[2 * r], ev[2 * r + 1]));
+                    std::nth_element(ms.begin(), ms.begin() + reps / 2, ms.end());
+                    if (ms[reps / 2] < best_ms) {
+                        best_ms = ms[reps / 2];
+                        best = v;
+                    }
+                }
+                variants[{s, tr != 0}] = best;
+                chosen.push_back(best);
+            }
+        }
+    } catch (...) {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        throw;
+    }
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    return chosen;
+}
+
+}  // namespace
+
+std::vector<int> CsrOperator::autotune(Context& ctx, const std::vector<int>& widths, bool fp64_blocks, int reps) {
+    if (reps < 1) throw std::invalid_argument("autotune needs at least one timed launch");
+    ctx.sync();
+    return fp64_blocks ? tune_widths<double>(ctx, a_, at_, rows_, cols_, widths, reps, variants_)
+                       : tune_widths<float>(ctx, a_, at_, rows_, cols_, widths, reps, variants_);
+}
+
 void CsrOperator::apply(hipStream_t s, const double* X, int n, int64_t ldx, double* Y, int64_t ldy) const {
-    prof_begin(s);
-    csr_spmm<double>(s, a_, n, X, ldx, Y, ldy, nullptr, 0, 0.0, variant_for(n, false));
-    prof_end(s, n, false, 8);
+    hipEvent_t e0, e1;
+    prof_events(&e0, &e1);
+    csr_spmm<double>(s, a_, n, X, ldx, Y, ldy, nullptr, 0, 0.0, variant_for(n, false), e0, e1);
+    prof_commit(e0, e1, n, false, 8);
 }
 void CsrOperator::apply_t(hipStream_t s, const double* Y, int n, int64_t ldy, double* X, int64_t ldx) const {
-    prof_begin(s);
-    csr_spmm<double>(s, at_, n, Y, ldy, X, ldx, nullptr, 0, 0.0, variant_for(n, true));
-    prof_end(s, n, true, 8);
+    hipEvent_t e0, e1;
+    prof_events(&e0, &e1);
+    csr_spmm<double>(s, at_, n, Y, ldy, X, ldx, nullptr, 0, 0.0, variant_for(n, true), e0, e1);
+    prof_commit(e0, e1, n, true, 8);
 }
 void CsrOperator::apply(hipStream_t s, const float* X, int n, int64_t ldx, float* Y, int64_t ldy) const {
-    prof_begin(s);
-    csr_spmm<float>(s, a_, n, X, ldx, Y, ldy, nullptr, 0, 0.0, variant_for(n, false));
-    prof_end(s, n, false, 4);
+    hipEvent_t e0, e1;
+    prof_events(&e0, &e1);
+    csr_spmm<float>(s, a_, n, X, ldx, Y, ldy, nullptr, 0, 0.0, variant_for(n, false), e0, e1);
+    prof_commit(e0, e1, n, false, 4);
 }
 void CsrOperator::apply_t(hipStream_t s, const float* Y, int n, int64_t ldy, float* X, int64_t ldx) const {
-    prof_begin(s);
-    csr_spmm<float>(s, at_, n, Y, ldy, X, ldx, nullptr, 0, 0.0, variant_for(n, true));
-    prof_end(s, n, true, 4);
+    hipEvent_t e0, e1;
+    prof_events(&e0, &e1);
+    csr_spmm<float>(s, at_, n, Y, ldy, X, ldx, nullptr, 0, 0.0, variant_for(n, true), e0, e1);
+    prof_commit(e0, e1, n, true, 4);
 }
 
 template <typename T>

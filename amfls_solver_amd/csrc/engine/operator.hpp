@@ -65,6 +65,10 @@ public:
 protected:
     void prof_begin(hipStream_t s) const;
     void prof_end(hipStream_t s, int ncols, bool transpose, int elem_bytes) const;
+    // Events for kernels that take their dispatch timestamps (nullptr when
+    // profiling is off), and the record of one product timed that way.
+    void prof_events(hipEvent_t* start, hipEvent_t* stop) const;
+    void prof_commit(hipEvent_t start, hipEvent_t stop, int ncols, bool transpose, int elem_bytes) const;
     int64_t rows_ = 0, cols_ = 0;
 
 private:
@@ -96,6 +100,11 @@ public:
     double product_bytes(int s, bool transpose, int eb) const override;
     double product_flops(int s) const override { return 2.0 * static_cast<double>(nnz_) * s; }
     void set_variant(int s, bool transpose, int variant) override;
+    // Times the SpMM variants whose column lanes fit each block width (the
+    // next power of two and twice it) on this matrix, with FP64 or FP32
+    // blocks, and keeps the fastest for A and A^T (median kernel time of
+    // `reps` launches).  Returns the chosen variants, (A, A^T) per width.
+    std::vector<int> autotune(Context& ctx, const std::vector<int>& widths, bool fp64_blocks, int reps);
 
     // Augmented [A; a I] products: Y rows [rows, rows+cols) = a X; the
     // transpose adds a * Y[rows:] to A^T Y[0:rows].

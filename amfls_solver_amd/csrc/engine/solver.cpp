@@ -2,7 +2,11 @@
 
 #include <algorithm>
 #include <cmath>
+#include <atomic>
 #include <deque>
+#include <exception>
+#include <mutex>
+#include <thread>
 #include <limits>
 #include <memory>
 #include <stdexcept>
@@ -65,10 +69,17 @@ double operator_nnz(const DeviceOperator& op) {
     return static_cast<double>(op.rows()) * static_cast<double>(op.cols());
 }
 
+// Evaluation of the driver's current basis at another regularization, and
+// the observer the regularization sweep installs: called after every
+// checkpoint evaluation (final = false) and once before the driver returns
+// (final = true).
+using LambdaEvaluator = std::function<LeastSquaresResult(double lambda)>;
+using CheckpointHook = std::function<void(const LambdaEvaluator& evaluate_at, int depth, bool final)>;
+
 // The adaptive driver (reference amfls.cpp:57-661) in working precision T.
 template <typename T>
 LeastSquaresResult amfls_driver(const DeviceOperator& op, const double* b_dev, const SolveOptions& o, Context& ctx,
-                                const WidthPolicy* policy) {
+                                const WidthPolicy* policy, const CheckpointHook* hook = nullptr) {
     const double lambda = o.regularization, tol = o.tolerance;
     Clock& clk = ctx.clock;
     const double total_start = clk.now();
@@ -121,8 +132,21 @@ LeastSquaresResult amfls_driver(const DeviceOperator& op, const double* b_dev, c
     // reached its rounding floor; the outer FP64 loop continues from there).
     std::vector<std::pair<int, double>> eval_history;
     bool stagnated = false;
+    const LambdaEvaluator evaluate_at = [&](double l) {
+        const double keep = ws.regularization();
+        ws.set_regularization(l);
+        try {
+            LeastSquaresResult r = ws.evaluate(tol, evaluations, aux);
+            ws.set_regularization(keep);
+            return r;
+        } catch (...) {
+            ws.set_regularization(keep);
+            throw;
+        }
+    };
     auto evaluate = [&](bool train) {
         record(ws.evaluate(tol, evaluations, aux), train);
+        if (hook) (*hook)(evaluate_at, ws.depth(), false);
         if (o.stop_on_stagnation) {
             const double c = active_certificate(result, lambda);
             const int d = result.depth;
@@ -134,6 +158,7 @@ LeastSquaresResult amfls_driver(const DeviceOperator& op, const double* b_dev, c
         }
     };
     auto finish = [&](LeastSquaresResult f) {
+        if (hook) (*hook)(evaluate_at, ws.depth(), true);
         f.iterations = ws.depth();
         f.trace = trace;
         ctx.timer.flush();
@@ -490,6 +515,133 @@ LeastSquaresResult solve_amfls(const DeviceOperator& op, const double* b_dev, co
     if (o.precision == Precision::mixed) return mixed_driver(op, b_dev, o, ctx, policy);
     if (o.precision == Precision::fp32) return amfls_driver<float>(op, b_dev, o, ctx, policy);
     return amfls_driver<double>(op, b_dev, o, ctx, policy);
+}
+
+namespace {
+
+void validate_amfls_options(const SolveOptions& o) {
+    if (!std::isfinite(o.failure_probability) || o.failure_probability <= 0.0 || o.failure_probability >= 1.0)
+        throw std::invalid_argument("failure_probability must lie strictly between zero and one");
+    if (o.maximum_epochs < 0 || o.maximum_depth < 0 || o.maximum_auxiliary_width < 0 || o.maximum_basis_size < 0)
+        throw std::invalid_argument("work limits must be nonnegative");
+}
+
+}  // namespace
+
+std::vector<LeastSquaresResult> solve_amfls_sweep(const DeviceOperator& op, const double* b_dev,
+                                                  const SolveOptions& o, const std::vector<double>& lambdas,
+                                                  Context& ctx, const WidthPolicy* policy) {
+    if (lambdas.empty()) throw std::invalid_argument("the regularization sweep needs at least one value");
+    for (double l : lambdas) validate_common(op, b_dev, l, o.tolerance);
+    validate_amfls_options(o);
+    if (o.precision == Precision::mixed)
+        throw std::invalid_argument("the regularization sweep runs in fp64 or fp32 precision");
+    check_rhs_finite(ctx, b_dev, op.rows());
+    ctx.clock.configure(o.clock_mode, o.synthetic_operator_unit, o.synthetic_local_unit);
+    const size_t L = lambdas.size();
+    size_t drive = 0;
+    for (size_t j = 1; j < L; ++j)
+        if (lambdas[j] < lambdas[drive]) drive = j;
+    const double lam_d = lambdas[drive];
+    struct Slot {
+        bool done = false, has = false;
+        int evaluated_depth = -1;
+        double best_cert = kInfD;
+        LeastSquaresResult best;
+        std::vector<IterationRecord> trace;
+    };
+    std::vector<Slot> slot(L);
+    const CheckpointHook hook = [&](const LambdaEvaluator& evaluate_at, int depth, bool) {
+        for (size_t j = 0; j < L; ++j) {
+            Slot& sl = slot[j];
+            if (sl.done || lambdas[j] == lam_d || depth == sl.evaluated_depth) continue;
+            LeastSquaresResult r = evaluate_at(lambdas[j]);
+            sl.evaluated_depth = depth;
+            if (!r.trace.empty()) sl.trace.push_back(r.trace.front());
+            r.trace.clear();
+            const double c = active_certificate(r, lambdas[j]);
+            if (!sl.has || (std::isfinite(c) && !std::isfinite(sl.best_cert)) || c < sl.best_cert) {
+                sl.best_cert = c;
+                sl.best = r;
+                sl.has = true;
+            }
+            if (r.status == SolverStatus::success || candidate_validation_failed_numerically(r)) {
+                sl.done = true;
+                sl.best = std::move(r);
+            }
+        }
+    };
+    SolveOptions od = o;
+    od.regularization = lam_d;
+    LeastSquaresResult rd = o.precision == Precision::fp32 ? amfls_driver<float>(op, b_dev, od, ctx, policy, &hook)
+                                                           : amfls_driver<double>(op, b_dev, od, ctx, policy, &hook);
+    std::vector<LeastSquaresResult> out(L);
+    for (size_t j = 0; j < L; ++j) {
+        if (lambdas[j] == lam_d) {
+            out[j] = rd;
+            continue;
+        }
+        Slot& sl = slot[j];
+        if (sl.done && sl.best.status == SolverStatus::success) {
+            out[j] = std::move(sl.best);
+            out[j].trace = std::move(sl.trace);
+            out[j].iterations = out[j].depth;
+            out[j].statistics = rd.statistics;
+            continue;
+        }
+        // not certified along the shared process: a solve of its own
+        SolveOptions oj = o;
+        oj.regularization = lambdas[j];
+        out[j] = o.precision == Precision::fp32 ? amfls_driver<float>(op, b_dev, oj, ctx, policy)
+                                                : amfls_driver<double>(op, b_dev, oj, ctx, policy);
+    }
+    return out;
+}
+
+std::vector<std::vector<LeastSquaresResult>> solve_amfls_batch(const std::vector<BatchProblem>& problems,
+                                                               const SolveOptions& o, int workers,
+                                                               const WidthPolicy* policy) {
+    for (const BatchProblem& p : problems)
+        if (p.op == nullptr || p.b_dev == nullptr) throw std::invalid_argument("batch problem without operator or b");
+    if (workers <= 0) throw std::invalid_argument("the batch needs at least one worker");
+    std::vector<std::vector<LeastSquaresResult>> out(problems.size());
+    if (problems.empty()) return out;
+    workers = static_cast<int>(std::min<size_t>(static_cast<size_t>(workers), problems.size()));
+    int device = 0;
+    AMFLS_HIP_CHECK(hipGetDevice(&device));
+    std::atomic<size_t> next{0};
+    std::atomic<bool> failed{false};
+    std::exception_ptr error;
+    std::mutex error_mu, policy_mu;
+    // a stateful policy (the learned model) is called by one worker at a time
+    WidthPolicy serial;
+    if (policy && *policy)
+        serial = [&](const WidthFeatures& f) {
+            std::lock_guard<std::mutex> g(policy_mu);
+            return (*policy)(f);
+        };
+    auto work = [&] {
+        try {
+            AMFLS_HIP_CHECK(hipSetDevice(device));
+            Context wctx;
+            while (!failed.load()) {
+                const size_t i = next.fetch_add(1);
+                if (i >= problems.size()) break;
+                const BatchProblem& p = problems[i];
+                out[i] = solve_amfls_sweep(*p.op, p.b_dev, o, p.lambdas, wctx, serial ? &serial : nullptr);
+            }
+        } catch (...) {
+            std::lock_guard<std::mutex> g(error_mu);
+            if (!error) error = std::current_exception();
+            failed.store(true);
+        }
+    };
+    std::vector<std::thread> pool;
+    pool.reserve(static_cast<size_t>(workers));
+    for (int w = 0; w < workers; ++w) pool.emplace_back(work);
+    for (std::thread& t : pool) t.join();
+    if (error) std::rethrow_exception(error);
+    return out;
 }
 
 LeastSquaresResult solve_fixed_rbgk(const DeviceOperator& op, const double* b_dev, const FixedRbgkOptions& o,

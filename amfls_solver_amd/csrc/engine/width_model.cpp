@@ -2,6 +2,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <limits>
 #include <stdexcept>
 
 namespace amflsx {
@@ -23,7 +24,10 @@ WidthModel::WidthModel(const float* params, const float* norm, Context& ctx) : c
     m_.reset(kMlpParams);
     v_.reset(kMlpParams);
     grad_.reset(kMlpParams + 1);
-    io_.reset(kMlpInputs + 1);
+    in_.reset(kMlpInputs);
+    out_.reset(1);
+    lo_.fill(-std::numeric_limits<float>::infinity());
+    hi_.fill(std::numeric_limits<float>::infinity());
     AMFLS_HIP_CHECK(hipMemcpy(params_.get(), params, kMlpParams * sizeof(float), hipMemcpyHostToDevice));
     AMFLS_HIP_CHECK(hipMemcpy(norm_.get(), norm, 2 * kMlpInputs * sizeof(float), hipMemcpyHostToDevice));
     AMFLS_HIP_CHECK(hipMemset(m_.get(), 0, kMlpParams * sizeof(float)));
@@ -40,11 +44,29 @@ void WidthModel::params(float* out) {
 
 void WidthModel::logits(const float* X_host, int64_t n, float* out_host) {
     if (n <= 0) return;
-    DevBuf<float> x(static_cast<size_t>(n) * kMlpInputs), z(static_cast<size_t>(n));
-    AMFLS_HIP_CHECK(hipMemcpyAsync(x.get(), X_host, static_cast<size_t>(n) * kMlpInputs * sizeof(float),
+    // buffers grow once and are reused, so a decision inside a solve does
+    // not allocate (an allocation would also wait for the device)
+    in_.ensure(static_cast<size_t>(n) * kMlpInputs);
+    out_.ensure(static_cast<size_t>(n));
+    AMFLS_HIP_CHECK(hipMemcpyAsync(in_.get(), X_host, static_cast<size_t>(n) * kMlpInputs * sizeof(float),
                                    hipMemcpyHostToDevice, ctx_.stream));
-    mlp_forward(ctx_.stream, params_.get(), norm_.get(), n, x.get(), z.get());
-    ctx_.download(out_host, z.get(), static_cast<size_t>(n));
+    mlp_forward(ctx_.stream, params_.get(), norm_.get(), n, in_.get(), out_.get());
+    ctx_.download(out_host, out_.get(), static_cast<size_t>(n));
+}
+
+void WidthModel::set_domain(const float* lo, const float* hi) {
+    for (int i = 0; i < kMlpInputs; ++i) {
+        if (std::isnan(lo[i]) || std::isnan(hi[i]) || lo[i] > hi[i])
+            throw std::invalid_argument("width model domain needs lo <= hi for every input");
+        lo_[i] = lo[i];
+        hi_[i] = hi[i];
+    }
+}
+
+bool WidthModel::in_domain(const std::array<float, kMlpInputs>& x) const {
+    for (int i = 0; i < kMlpInputs; ++i)
+        if (!(x[i] >= lo_[i] && x[i] <= hi_[i])) return false;
+    return true;
 }
 
 float WidthModel::gradient(const float* X_dev, const float* y_dev, int64_t n, float* grad_host) {
@@ -67,6 +89,11 @@ float WidthModel::train_step(const float* X_dev, const float* y_dev, int64_t n, 
 
 bool WidthModel::widen(const WidthFeatures& f) {
     const auto x = width_features(f);
+    if (!in_domain(x)) {
+        ++fallback_decisions_;
+        return f.heuristic;
+    }
+    ++model_decisions_;
     float z = 0.0f;
     logits(x.data(), 1, &z);
     return z > 0.0f;

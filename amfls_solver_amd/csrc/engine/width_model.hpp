@@ -29,12 +29,22 @@ public:
     float gradient(const float* X_dev, const float* y_dev, int64_t n, float* grad_host);
     // One Adam step on a device batch; returns the batch loss before the step.
     float train_step(const float* X_dev, const float* y_dev, int64_t n, float lr);
+    // Per-feature box of the inputs the model was trained on.  A decision
+    // whose inputs leave the box is taken by the heuristic (f.heuristic):
+    // the model never extrapolates.  Default: unbounded.
+    void set_domain(const float* lo, const float* hi);
+    bool in_domain(const std::array<float, kMlpInputs>& x) const;
     bool widen(const WidthFeatures& f);
+    // Decisions taken by the model / handed to the heuristic since creation.
+    long long model_decisions() const { return model_decisions_; }
+    long long fallback_decisions() const { return fallback_decisions_; }
 
 private:
     Context& ctx_;
-    DevBuf<float> params_, norm_, m_, v_, grad_, io_;
+    DevBuf<float> params_, norm_, m_, v_, grad_, in_, out_;
+    std::array<float, kMlpInputs> lo_, hi_;
     int step_ = 0;
+    long long model_decisions_ = 0, fallback_decisions_ = 0;
 };
 
 }  // namespace amflsx

@@ -72,6 +72,11 @@ public:
     // level attempted did not extend the basis; the audit is that level's.
     bool advance_toward(int target_depth);
     LeastSquaresResult evaluate(double tolerance, int epoch, int auxiliary_width);
+    // The regularization of the projected solve and the validation.  The
+    // Golub-Kahan process itself does not depend on it, so one basis can be
+    // evaluated at several values.
+    void set_regularization(double lambda) { lambda_ = lambda; }
+    double regularization() const { return lambda_; }
     // Candidate x = V y from the current projected solve (device, double).
     void candidate(double* x_dev);
 

@@ -65,6 +65,23 @@ class WidthModel:
                                                       nm.ctypes.data_as(ctypes.c_void_p), ctypes.byref(h)))
         self.handle = h
         self.norm = nm.copy()
+        self.domain = None
+
+    def set_domain(self, lo: np.ndarray, hi: np.ndarray) -> None:
+        """Per-input box of the training data; decisions outside it go to the heuristic."""
+        lo = np.ascontiguousarray(lo, dtype=np.float32)
+        hi = np.ascontiguousarray(hi, dtype=np.float32)
+        if lo.size != N_IN or hi.size != N_IN:
+            raise ValueError("width model domain needs 12 lower and 12 upper bounds")
+        _lib.check(self._lib.amfls_width_model_set_domain(self.handle, lo.ctypes.data_as(ctypes.c_void_p),
+                                                          hi.ctypes.data_as(ctypes.c_void_p)))
+        self.domain = (lo.copy(), hi.copy())
+
+    def decisions(self) -> tuple[int, int]:
+        """(decisions taken by the model, decisions handed to the heuristic) so far."""
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        _lib.check(self._lib.amfls_width_model_decisions(self.handle, ctypes.byref(a), ctypes.byref(b)))
+        return int(a.value), int(b.value)
 
     def params(self) -> np.ndarray:
         out = np.empty(N_PARAMS, dtype=np.float32)
@@ -102,13 +119,16 @@ class WidthModel:
     def save(self, path: Path, **meta) -> None:
         path = Path(path)
         path.parent.mkdir(parents=True, exist_ok=True)
-        np.savez(path, params=self.params(), norm=self.norm, meta=np.array(json.dumps(meta)))
+        extra = {} if self.domain is None else {"domain_lo": self.domain[0], "domain_hi": self.domain[1]}
+        np.savez(path, params=self.params(), norm=self.norm, meta=np.array(json.dumps(meta)), **extra)
 
     @classmethod
     def load(cls, path: Path = DEFAULT_PATH, context=None) -> "WidthModel":
         with np.load(Path(path), allow_pickle=False) as z:
             m = cls(z["params"], z["norm"], context)
             m.meta = json.loads(str(z["meta"])) if "meta" in z else {}
+            if "domain_lo" in z:
+                m.set_domain(z["domain_lo"], z["domain_hi"])
         return m
 
     def __del__(self):
@@ -122,57 +142,86 @@ class WidthModel:
 # ----------------------------------------------------------------- problems
 def training_problem(index: int, seed: int = 11):
     """Problem `index` of the training/evaluation distribution, drawn like the
-    benchmark configurations at reduced size: CSR (distinct uniform columns,
-    N(0,1) values, 4-32 nonzeros per row, optional log-uniform row scales) or
-    dense G D (D log-spaced over 1-6 decades); lambda = 0 or one of
+    benchmark configurations: CSR as the 200000 x 20000 and batched
+    configurations (m log-uniform in [5000, 200000], n log-uniform in
+    [500, min(20000, m / 2)], distinct uniform columns, N(0,1) values, either 12
+    or U[4, 32] nonzeros per row, rows optionally scaled by log-uniform
+    factors in [1e-3, 1]) or dense G D as the tall dense configuration
+    (m log-uniform in [1000, 40000], n in [100, min(1000, m)], D log-spaced
+    over 1-6 decades, b = A x* + 1e-8-relative noise); lambda = 0 or one of
     logspace(-6, -1, 8).  Returns (matrix tensor, b, lambda, description)."""
     from . import problems as P
     r = np.random.default_rng([seed, index])
     lam = 0.0 if r.random() < 0.4 else float(10.0 ** (-6.0 + 5.0 * int(r.integers(0, 8)) / 7.0))
     if r.random() < 0.65:
-        m = int(r.integers(2000, 40001))
-        n = int(r.integers(200, min(4000, m) + 1))
-        per = np.minimum(r.integers(4, 33, size=m), n)
+        m = int(round(10.0 ** r.uniform(np.log10(5000), np.log10(200_000))))
+        n = int(round(10.0 ** r.uniform(np.log10(500), np.log10(min(20_000, m // 2)))))
+        per = np.full(m, 12) if r.random() < 0.3 else r.integers(4, 33, size=m)
+        per = np.minimum(per, n)
         scaled = r.random() < 0.5
         p = P.csr_rows(m, n, per, seed=int(r.integers(0, 2**31)), scale_range=(1e-3, 1.0) if scaled else (1.0, 1.0),
                        regularization=lam)
         A = torch.sparse_csr_tensor(torch.from_numpy(p.row_ptr), torch.from_numpy(p.col_idx.astype(np.int64)),
                                     torch.from_numpy(p.values), size=(m, n))
         return A, torch.from_numpy(p.b), lam, f"csr {m}x{n} nnz {p.nnz} lam {lam:.1e}"
-    m = int(r.integers(1000, 20001))
-    n = int(r.integers(100, min(800, m) + 1))
+    m = int(round(10.0 ** r.uniform(3.0, np.log10(40_000))))
+    n = int(r.integers(100, min(1000, m) + 1))
     dec = float(r.uniform(1.0, 6.0))
-    a, b, _ = P.dense_gd(m, n, dec, seed=int(r.integers(0, 2**31)), noise=1e-6)
+    a, b, _ = P.dense_gd(m, n, dec, seed=int(r.integers(0, 2**31)), noise=1e-8, relative_noise=True)
     return torch.from_numpy(np.ascontiguousarray(a)), torch.from_numpy(b), lam, f"dense {m}x{n} dec {dec:.1f} lam {lam:.1e}"
 
 
-def _solve_seconds(res) -> float:
-    return res.total_seconds if res.status == 0 else math.inf
+def wide_problem(rows: int = 50_000, cols: int = 500_000, per_row: int = 8, seed: int = 5):
+    """The wide minimum-norm evaluation case: CSR rows x cols with `per_row`
+    distinct uniform columns per row, N(0,1) values, b ~ N(0,1), lambda = 0."""
+    from . import problems as P
+    p = P.csr_rows(rows, cols, per_row, seed=seed, scale_range=(1.0, 1.0), regularization=0.0)
+    A = torch.sparse_csr_tensor(torch.from_numpy(p.row_ptr), torch.from_numpy(p.col_idx.astype(np.int64)),
+                                torch.from_numpy(p.values), size=(rows, cols))
+    return A, torch.from_numpy(p.b), 0.0, f"csr {rows}x{cols} nnz {p.nnz} lam 0 (wide)"
 
 
-def _timed(M, b, lam, tolerance, policy, repeats=2):
-    ts = []
-    for _ in range(repeats):
-        t0 = time.perf_counter()
+def _matrix(A, context=None):
+    return DeviceMatrix(A, value_dtype=torch.float32 if A.layout == torch.sparse_csr else torch.float64,
+                        context=context)
+
+
+def _run(M, b, lam, tolerance, policy, record_decisions=False):
+    """(seconds to certification or inf, result summary, decision records)."""
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    if record_decisions:
+        rec, status, aux = _record_raw(M, b, lam, tolerance, policy)
+    else:
         r = run_amfls(M, b, regularization=lam, tolerance=tolerance, width_policy=_lib.WIDTH_POLICY[policy],
                       context=M.context)
-        ts.append(time.perf_counter() - t0 if r.status == 0 else math.inf)
-    return float(np.median(ts)), r
+        rec, status, aux = None, int(r.status), int(r.auxiliary_width)
+    dt = time.perf_counter() - t0
+    return (dt if status == 0 else math.inf), aux, rec
 
 
-def collect(indices, *, seed: int = 11, tolerance: float = 1e-8, log=print):
-    """Decision records and labels for the given problem indices."""
+def collect(indices, *, seed: int = 11, tolerance: float = 1e-8, budget_seconds: float = math.inf, log=print):
+    """Decision records and labels for the given problem indices (stops
+    starting new problems once `budget_seconds` have passed)."""
     X, y, info = [], [], []
+    t_start = time.perf_counter()
     for i in indices:
+        if time.perf_counter() - t_start > budget_seconds:
+            log(f"  collection budget of {budget_seconds:.0f}s reached after {len(info)} problems")
+            break
         A, b, lam, desc = training_problem(i, seed)
-        M = DeviceMatrix(A, value_dtype=torch.float32 if A.layout == torch.sparse_csr else torch.float64)
+        M = _matrix(A)
         bd = b.cuda()
-        run_amfls(M, bd, regularization=lam, tolerance=tolerance)  # warm
-        t_widen, rw = _timed(M, bd, lam, tolerance, "widen")
-        t_hold, rh = _timed(M, bd, lam, tolerance, "hold")
-        t_heur, _ = _timed(M, bd, lam, tolerance, "heuristic")
-        # decisions met along both the heuristic's and the always-widen path
-        rec = np.concatenate([record(M, bd, lam, tolerance), record(M, bd, lam, tolerance, widen=True)])
+        _run(M, bd, lam, tolerance, "heuristic")  # warm
+        # decisions met along the heuristic's and the always-widen path, and
+        # the time to certification of those runs and of never widening
+        t_heur, _, rec_h = _run(M, bd, lam, tolerance, "record", True)
+        t_widen, aux_w, rec_w = _run(M, bd, lam, tolerance, "record_widen", True)
+        t_hold, _, _ = _run(M, bd, lam, tolerance, "hold")
+        if min(t_widen, t_hold) < 0.05:  # short solves: best of two
+            t_widen = min(t_widen, _run(M, bd, lam, tolerance, "widen")[0])
+            t_hold = min(t_hold, _run(M, bd, lam, tolerance, "hold")[0])
+        rec = np.concatenate([rec_h, rec_w])
         usable = math.isfinite(t_widen) or math.isfinite(t_hold)
         label = 1.0 if t_widen < t_hold else 0.0
         if usable:
@@ -181,8 +230,9 @@ def collect(indices, *, seed: int = 11, tolerance: float = 1e-8, log=print):
                 y.append(label)
         info.append({"index": int(i), "problem": desc, "widen_seconds": t_widen, "hold_seconds": t_hold,
                      "heuristic_seconds": t_heur, "label": label, "decisions": len(rec), "used": usable})
-        log(f"  [{i}] {desc}: widen {t_widen:.4f}s (aux {rw.auxiliary_width}), hold {t_hold:.4f}s, "
+        log(f"  [{i}] {desc}: widen {t_widen:.4f}s (aux {aux_w}), hold {t_hold:.4f}s, "
             f"heuristic {t_heur:.4f}s -> label {int(label)} ({len(rec)} decisions{'' if usable else ', unused'})")
+        del M
     return np.array(X, dtype=np.float32).reshape(-1, N_IN), np.array(y, dtype=np.float32), info
 
 
@@ -194,6 +244,8 @@ def record(M, b, lam, tolerance, widen: bool = False):
 
 
 def _record_raw(M, b, lam, tolerance, policy):
+    """(decision records rows x 13, status, auxiliary width) of one solve."""
+    from .binding import _decode, _result_tuple
     ctx = M.context
     o = _lib.options(regularization=float(lam), tolerance=float(tolerance), width_policy=_lib.WIDTH_POLICY[policy])
     res = ctypes.c_void_p()
@@ -204,19 +256,30 @@ def _record_raw(M, b, lam, tolerance, policy):
         _lib.check(ctx._lib.amfls_result_width_records(res, None, ctypes.byref(rows)))
         out = np.empty((rows.value, N_IN + 1), dtype=np.float32)
         _lib.check(ctx._lib.amfls_result_width_records(res, out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(rows)))
-    finally:
+    except Exception:
         ctx._lib.amfls_result_destroy(res)
-    return out, None, None
+        raise
+    r = _decode(*_result_tuple(ctx, res))  # releases the result
+    return out, int(r.status), int(r.auxiliary_width)
+
+
+def feature_domain(X: np.ndarray, margin: float = 0.05):
+    """Per-input box of the training inputs, widened by `margin` of each range."""
+    lo, hi = X.min(axis=0), X.max(axis=0)
+    pad = margin * np.maximum(hi - lo, 1e-3)
+    return (lo - pad).astype(np.float32), (hi + pad).astype(np.float32)
 
 
 def train(X: np.ndarray, y: np.ndarray, *, epochs: int = 400, lr: float = 3e-3, batch: int = 256, seed: int = 0,
           log=print) -> WidthModel:
     """Adam on the GPU with the engine's kernels; features normalized by the
-    training set's mean and standard deviation."""
+    training set's mean and standard deviation; the model's domain is the
+    box of the training inputs."""
     mean = X.mean(axis=0)
     std = X.std(axis=0)
     norm = np.concatenate([mean, 1.0 / np.maximum(std, 1e-3)]).astype(np.float32)
     model = WidthModel(init_params(seed), norm)
+    model.set_domain(*feature_domain(X))
     Xd = torch.from_numpy(X).cuda()
     yd = torch.from_numpy(y).cuda()
     gen = torch.Generator(device="cpu").manual_seed(seed)
@@ -233,68 +296,105 @@ def train(X: np.ndarray, y: np.ndarray, *, epochs: int = 400, lr: float = 3e-3, 
     return model
 
 
-def evaluate(model: WidthModel, indices, *, seed: int = 11, tolerance: float = 1e-8, repeats: int = 2, log=print):
+def evaluate(model: WidthModel, problems, *, seed: int = 11, tolerance: float = 1e-8, repeats: int = 2, log=print):
     """Time-to-tolerance of the heuristic and the learned policy per problem
-    (median of `repeats` runs each, after a warm run)."""
+    (median of `repeats` runs each, after a warm run).  `problems` holds
+    distribution indices or (matrix, b, lambda, description) tuples."""
     model.attach()
     rows = []
-    for i in indices:
-        A, b, lam, desc = training_problem(i, seed)
-        M = DeviceMatrix(A, value_dtype=torch.float32 if A.layout == torch.sparse_csr else torch.float64,
-                         context=model.context)
+    for item in problems:
+        A, b, lam, desc = training_problem(item, seed) if isinstance(item, (int, np.integer)) else item
+        M = _matrix(A, model.context)
         bd = b.cuda()
         out = {}
         for pol, code in (("heuristic", 0), ("learned", 1)):
             run_amfls(M, bd, regularization=lam, tolerance=tolerance, width_policy=code, context=model.context)
+            d0 = model.decisions()
             ts = []
             for _ in range(repeats):
+                torch.cuda.synchronize()
                 t0 = time.perf_counter()
                 r = run_amfls(M, bd, regularization=lam, tolerance=tolerance, width_policy=code,
                               context=model.context)
                 ts.append(time.perf_counter() - t0 if r.status == 0 else math.inf)
-            out[pol] = (float(np.median(ts)), int(r.status), int(r.auxiliary_width))
-        rows.append({"index": int(i), "problem": desc, "heuristic": out["heuristic"], "learned": out["learned"]})
-        log(f"  [{i}] {desc}: heuristic {out['heuristic'][0]:.4f}s (aux {out['heuristic'][2]}), "
-            f"learned {out['learned'][0]:.4f}s (aux {out['learned'][2]})")
+            d1 = model.decisions()
+            out[pol] = (float(np.median(ts)), int(r.status), int(r.auxiliary_width),
+                        (d1[0] - d0[0]) // repeats, (d1[1] - d0[1]) // repeats)
+        rows.append({"index": int(item) if isinstance(item, (int, np.integer)) else -1, "problem": desc,
+                     "heuristic": out["heuristic"][:3], "learned": out["learned"][:3],
+                     "model_decisions": out["learned"][3], "heuristic_fallbacks": out["learned"][4]})
+        log(f"  [{rows[-1]['index']}] {desc}: heuristic {out['heuristic'][0]:.4f}s (aux {out['heuristic'][2]}), "
+            f"learned {out['learned'][0]:.4f}s (aux {out['learned'][2]}; {out['learned'][3]} model decisions, "
+            f"{out['learned'][4]} outside the domain)")
+        del M
     ratios = [r["heuristic"][0] / r["learned"][0] for r in rows
               if math.isfinite(r["heuristic"][0]) and math.isfinite(r["learned"][0])]
     gm = float(np.exp(np.mean(np.log(ratios)))) if ratios else float("nan")
     return gm, rows
 
 
+def gain_cases(rows, factor: float = 1.2, limit: int = 4):
+    """Held-out problems on which the learned policy widened beyond the
+    heuristic and certified at least `factor` times sooner."""
+    out = []
+    for r in rows:
+        h, l = r["heuristic"], r["learned"]
+        if h[1] == 0 and l[1] == 0 and l[2] > h[2] and h[0] > factor * l[0]:
+            out.append(r["index"])
+    return out[:limit]
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(prog="python -m amfls_solver_amd.width_model")
     sub = ap.add_subparsers(dest="cmd", required=True)
     t = sub.add_parser("train")
-    t.add_argument("--problems", type=int, default=300)
+    t.add_argument("--problems", type=int, default=2000)
+    t.add_argument("--budget-seconds", type=float, default=math.inf,
+                   help="stop starting new training problems after this many seconds")
     t.add_argument("--heldout", type=int, default=40)
     t.add_argument("--epochs", type=int, default=400)
     t.add_argument("--out", type=Path, default=DEFAULT_PATH)
+    t.add_argument("--data", type=Path, default=None, help="also save the decision records (npz) here")
+    t.add_argument("--wide", action="store_true", help="also evaluate the wide 50000 x 500000 minimum-norm CSR case")
     e = sub.add_parser("evaluate")
     e.add_argument("--model", type=Path, default=DEFAULT_PATH)
     e.add_argument("--heldout", type=int, default=40)
     e.add_argument("--update", type=Path, default=None, help="write the model with the held-out result here")
+    e.add_argument("--wide", action="store_true", help="also the wide 50000 x 500000 minimum-norm CSR case")
     a = ap.parse_args(argv)
     if a.cmd == "train":
         print(f"collecting {a.problems} training problems", flush=True)
-        X, y, info = collect(range(a.problems), log=lambda s: print(s, flush=True))
-        print(f"{X.shape[0]} decisions, {y.mean():.3f} labelled widen", flush=True)
+        X, y, info = collect(range(a.problems), budget_seconds=a.budget_seconds, log=lambda s: print(s, flush=True))
+        print(f"{X.shape[0]} decisions from {len(info)} problems, {y.mean():.3f} labelled widen", flush=True)
+        if a.data:
+            a.data.parent.mkdir(parents=True, exist_ok=True)
+            np.savez(a.data, X=X, y=y, info=np.array(json.dumps(info)))
         model = train(X, y, epochs=a.epochs, log=lambda s: print(s, flush=True))
         z = model.logits(X)
         acc = float(np.mean((z > 0) == (y > 0.5)))
         print(f"training accuracy {acc:.3f}", flush=True)
         gm, rows = evaluate(model, range(100000, 100000 + a.heldout), log=lambda s: print(s, flush=True))
         print(f"held-out geometric-mean speedup of learned over heuristic: {gm:.3f}", flush=True)
-        model.save(a.out, problems=a.problems, decisions=int(X.shape[0]), widen_fraction=float(y.mean()),
-                   train_accuracy=acc, heldout=a.heldout, heldout_speedup=gm, epochs=a.epochs)
+        meta = dict(problems=len(info), decisions=int(X.shape[0]), widen_fraction=float(y.mean()),
+                    train_accuracy=acc, heldout=a.heldout, heldout_speedup=gm, epochs=a.epochs,
+                    gain_cases=gain_cases(rows))
+        if a.wide:
+            _, wr = evaluate(model, [wide_problem()], repeats=1, log=lambda s: print(s, flush=True))
+            meta.update(wide_case={k: wr[0][k] for k in ("problem", "heuristic", "learned", "model_decisions",
+                                                         "heuristic_fallbacks")})
+        model.save(a.out, **meta)
         print(f"saved {a.out}", flush=True)
     else:
         model = WidthModel.load(a.model)
         gm, rows = evaluate(model, range(100000, 100000 + a.heldout), log=lambda s: print(s, flush=True))
         print(f"held-out geometric-mean speedup of learned over heuristic: {gm:.3f}", flush=True)
+        meta = dict(model.meta)
+        meta.update(heldout=a.heldout, heldout_speedup=gm, gain_cases=gain_cases(rows))
+        if a.wide:
+            _, wr = evaluate(model, [wide_problem()], repeats=1, log=lambda s: print(s, flush=True))
+            meta.update(wide_case={k: wr[0][k] for k in ("problem", "heuristic", "learned", "model_decisions",
+                                                         "heuristic_fallbacks")})
         if a.update:
-            meta = dict(model.meta)
-            meta.update(heldout=a.heldout, heldout_speedup=gm)
             model.save(a.update, **meta)
             print(f"wrote {a.update}", flush=True)
     return 0

@@ -1,16 +1,19 @@
 #!/usr/bin/env python
 """Time-to-tolerance benchmark of the AMFLS HIP engine on one MI355X.
 
-Workload: the 200000 x 20000 CSR least-squares problem (exactly 12 distinct
-uniform columns per row, N(0,1) values, rows scaled by log-uniform factors
-in [1e-3, 1], values stored as float32, b ~ N(0,1), lambda = 1e-3, certified
-relative energy error 1e-8).  A step is one complete certified solve with the
-matrix and b already resident in HBM.  Reported: seconds per solve for the
-mixed-precision (FP32 search + FP64 refinement/certificate) and the plain FP64
-engine, the CSR SpMM kernel's achieved algorithmic GB/s measured with HIP
-events on the engine stream over the timed solves, an SpMM bandwidth sweep on
-a CSR matrix larger than the 256 MB last-level cache, and the CPU
-restatement (oracle) timed on this host with one BLAS thread.
+Headline workload: the 200000 x 20000 CSR least-squares problem (exactly 12
+distinct uniform columns per row, N(0,1) values, rows scaled by log-uniform
+factors in [1e-3, 1], values stored as float32, b ~ N(0,1), lambda = 1e-3,
+certified relative energy error 1e-8).  A step is one complete certified
+solve with the matrix and b already resident in HBM.  Reported: seconds per
+solve of the plain FP64 engine (headline), of mixed precision (FP32 search +
+FP64 refinement/certificate) and of the learned width policy; the CSR SpMM
+kernel's achieved algorithmic GB/s measured with HIP events on the engine
+stream over the timed solves; an SpMM bandwidth sweep on a CSR matrix larger
+than the 256 MB last-level cache; the 1,000,000 x 2000 dense BF16 problem
+(G D, D log-spaced over six decades, lambda = 0) with its matrix-core product
+rates; and the CPU restatement (oracle) timed on this host with one BLAS
+thread.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -27,6 +30,21 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+
+
+def measured_traffic():
+    """HBM bytes per launch of the headline SpMM kernels from the committed
+    PMC summary (FETCH_SIZE and WRITE_SIZE passes scaled by a calibration
+    copy), or None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "csr_spmm_c1_pmc_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
 
 
 def log(*a):
@@ -69,6 +87,106 @@ def spmm_sweep(torch, np, am, P, widths=(1, 4, 16, 64), reps=20):
     return out
 
 
+def dense_c2(torch, am, P, rows=1_000_000, cols=2000):
+    """The 1e6 x 2000 BF16 dense problem: FP64 certified solve (A upconverted
+    exactly on the matrix cores) and the product kernels' rates."""
+    import ctypes
+    a, b, xs = P.config2_device(rows, cols)
+    M = am.DeviceMatrix(a, value_dtype=torch.bfloat16)
+    out = {"matrix": f"dense_{rows}x{cols}_bf16_GD_6decades_lambda0"}
+    lib, h = M.context._lib, M.context.handle
+    prod = {}
+    for s in (1, 16, 64):
+        for dt, code, name in ((torch.float64, 0, "f64"), (torch.float32, 1, "f32")):
+            X = torch.randn(cols, s, device="cuda", dtype=dt)
+            Y = torch.empty(rows, s, device="cuda", dtype=dt)
+            Z = torch.empty(cols, s, device="cuda", dtype=dt)
+            for tr, (src, dst) in ((0, (X, Y)), (1, (Y, Z))):
+                lib.amfls_operator_apply_rows(h, M.handle, tr, ctypes.c_void_p(src.data_ptr()), s, s,
+                                              ctypes.c_void_p(dst.data_ptr()), s, code)
+                M.context.synchronize()
+                M.profile(True)
+                for _ in range(5):
+                    lib.amfls_operator_apply_rows(h, M.handle, tr, ctypes.c_void_p(src.data_ptr()), s, s,
+                                                  ctypes.c_void_p(dst.data_ptr()), s, code)
+                M.context.synchronize()
+                n, sec, by = M.kernel_stats()
+                M.profile(False)
+                prod[f"{'AT' if tr else 'A'}_s{s}_{name}"] = {
+                    "ms": 1e3 * sec / n, "GBs": by / sec / 1e9, "TFs": 2.0 * rows * cols * s / (sec / n) / 1e12}
+    out["products"] = prod
+    for mode in ("fp64", "mixed"):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = am.run_amfls(M, b, tolerance=1e-8, precision=mode)
+        torch.cuda.synchronize()
+        out[mode] = {"seconds": time.perf_counter() - t0, "status": int(r.status), "depth": r.depth,
+                     "auxiliary_width": r.auxiliary_width, "basis_rank": r.basis_rank,
+                     "backward_error": r.backward_error_upper_bound, "refinement_steps": r.refinement_steps}
+        log(f"C2 {mode}: {out[mode]}")
+    del M, a
+    return out
+
+
+def batch_c3(torch, np, am, problems=256, lambdas=8, workers=4, seed=3, tolerance=1e-8):
+    """The batched configuration: `problems` independent CSR problems
+    (m ~ U[5000, 200000], n ~ U[500, min(10000, m)], row lengths ~ U[4, 32],
+    N(0,1) values stored as float32, b ~ N(0,1)), each with `lambdas` values
+    logspace(-6, -1); built on the device by the counter-addressed generator,
+    autotuned at width 1, then solved concurrently by `workers` host threads
+    with a stream each, one regularization sweep per problem (FP64 blocks,
+    FP64 certificates)."""
+    from amfls_solver_amd.binding import generate_csr
+    rng = np.random.default_rng(seed)
+    dims = []
+    for _ in range(problems):
+        m = int(rng.integers(5000, 200_001))
+        n = int(rng.integers(500, min(10_000, m) + 1))
+        dims.append((m, n))
+    t0 = time.perf_counter()
+    mats, bs = [], []
+    for i, (m, n) in enumerate(dims):
+        rp, ci, val = generate_csr(m, n, (4, 32), seed=1000 + i, value_dtype=torch.float32)
+        mats.append(am.DeviceMatrix.from_csr_arrays(m, n, rp, ci, val))
+        bs.append(am.binding.gaussian_block(m, 1, 2000 + i, 0).reshape(-1))
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for M in mats:
+        M.autotune(widths=(1,), dtype=torch.float64, reps=3)
+    t_tune = time.perf_counter() - t0
+    lam = np.tile(np.logspace(-6.0, -1.0, lambdas), (problems, 1))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = am.run_amfls_batch(mats, bs, lam, workers=workers, tolerance=tolerance)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ok = sum(1 for row in res for r in row if r.status == 0)
+    prods = sum(row[0].a_block_calls + row[0].at_block_calls for row in res)
+    nnz = sum(M.nnz for M in mats)
+    out = {"problems": problems, "lambdas_per_problem": lambdas, "workers": workers, "total_nnz": nnz,
+           "seconds": dt, "certified": ok, "solves": problems * lambdas, "solves_per_s": problems * lambdas / dt,
+           "block_products": prods, "block_products_per_s": prods / dt, "build_seconds": t_build,
+           "autotune_seconds": t_tune, "max_depth": max(r.depth for row in res for r in row)}
+    # the same first problems solved one lambda at a time, for the sharing gain
+    k = min(8, problems)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(k):
+        for l in lam[i]:
+            am.run_amfls(mats[i], bs[i], regularization=float(l), tolerance=tolerance)
+    torch.cuda.synchronize()
+    t_sep = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for i in range(k):
+        am.run_amfls_sweep(mats[i], bs[i], lam[i], tolerance=tolerance)
+    torch.cuda.synchronize()
+    t_sw = time.perf_counter() - t0
+    out["first_problems_serial"] = {"problems": k, "separate_solves_seconds": t_sep, "sweep_seconds": t_sw}
+    del mats, bs
+    return out
+
+
 def cpu_baseline(np, P, seconds_budget=25.0):
     """Oracle (CPU restatement, numpy/scipy FP64) on the same workload, 1 BLAS thread."""
     from threadpoolctl import threadpool_limits
@@ -95,6 +213,11 @@ def main() -> int:
     ap.add_argument("--headline", choices=["mixed", "fp64"], default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-dense", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-problems", type=int, default=256)
+    ap.add_argument("--pmc-calibrate", action="store_true",
+                    help="start with a 1 GiB device copy (reference kernel for PMC byte counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {a.gpus}); multi-GPU runs are not part of this build",
@@ -106,6 +229,14 @@ def main() -> int:
 
     import amfls_solver_amd as am
     from amfls_solver_amd import problems as P
+
+    if a.pmc_calibrate:
+        src = torch.empty(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("pmc calibration copy: 1 GiB read + 1 GiB written")
+        del src, dst
 
     log("generating 200000x20000 CSR (12 nnz/row, lambda 1e-3)")
     p = P.config1()
@@ -134,6 +265,19 @@ def main() -> int:
         log(f"{mode}: {dt:.4f} s/solve status {last.status} cert {last.relative_energy_error_upper_bound:.3e} "
             f"depth {last.depth} width {last.auxiliary_width} rank {last.basis_rank} refine {last.refinement_steps}")
 
+    # learned width policy (shipped model) on the same workload, FP64
+    from amfls_solver_amd.width_model import WidthModel
+    wm = WidthModel.load(context=M.context)
+    wm.attach()
+    am.run_amfls(M, b, regularization=p.regularization, tolerance=1e-8, width_policy=1, context=M.context)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        rl = am.run_amfls(M, b, regularization=p.regularization, tolerance=1e-8, width_policy=1, context=M.context)
+    torch.cuda.synchronize()
+    learned_s = (time.perf_counter() - t0) / a.steps
+    log(f"learned width policy: {learned_s:.4f} s/solve status {rl.status} width {rl.auxiliary_width}")
+
     dt, r = results[a.headline]
     other = "mixed" if a.headline == "fp64" else "fp64"
     n, sec, by = results["kernel"]
@@ -152,10 +296,21 @@ def main() -> int:
     else:
         rel = {}
 
+    dense = None
+    if not a.no_dense:
+        log("dense 1000000x2000 bf16 (matrix cores)")
+        dense = dense_c2(torch, am, P)
+
     sweep = None
     if not a.no_sweep:
         log("SpMM sweep on 2000000x200000 CSR (beyond the 256 MB cache)")
         sweep = spmm_sweep(torch, np, am, P)
+
+    batch = None
+    if not a.no_batch and a.batch_problems > 0:
+        log(f"batched CSR problems: {a.batch_problems} x 8 lambdas")
+        batch = batch_c3(torch, np, am, problems=a.batch_problems)
+        log(f"batch: {batch}")
 
     line = {
         "metric": "time-to-rel-err 1e-8 (s): mixed/learned vs FP64 AMFLS, 1 GPU; SpMM GB/s",
@@ -182,6 +337,9 @@ def main() -> int:
             f"{other}_status": int(results[other][1].status),
             f"{other}_certificate": results[other][1].relative_energy_error_upper_bound,
             "mixed_refinement_steps": results["mixed"][1].refinement_steps,
+            "learned_width_seconds": learned_s,
+            "learned_width_status": int(rl.status),
+            "learned_model_heldout_speedup": wm.meta.get("heldout_speedup"),
             "rel_diff_vs_oracle": rel,
             "parallelism": "single_gpu",
         },
@@ -191,7 +349,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(),
             "kernel": "csr_spmm (A and A^T products of the timed solves)",
             "launches": n,
             "avg_us": 1e6 * sec / max(n, 1),
@@ -199,6 +357,8 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "spmm_sweep": sweep,
+        "dense_1e6x2000_bf16": dense,
+        "batched_csr": batch,
     }
     print(json.dumps(line), flush=True)
     return 0

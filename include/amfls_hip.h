@@ -130,9 +130,18 @@ double amfls_operator_product_bytes(const amfls_operator* op, int block_cols, in
 double amfls_operator_product_flops(const amfls_operator* op, int block_cols);
 /* CSR SpMM kernel variant per (width, direction); -1 restores the default. */
 int amfls_operator_set_variant(amfls_operator* op, int block_cols, int transpose, int variant);
-/* Per-product kernel timing with HIP events on the engine stream (off by
- * default).  kernel_stats returns and resets: launches, total seconds and
- * total algorithmic bytes of the timed products. */
+/* SpMM autotuner (engine extension; the reference's CSR apply has a single
+ * loop, src/algorithms/operators/sparse_csr_operator.cpp:203-269): for each
+ * of `count` block widths, times the kernel variants whose column lanes fit
+ * the width on this matrix (median of `reps` launches, blocks of dtype
+ * AMFLS_F64 or AMFLS_F32) for A and A^T and keeps the fastest.  chosen (may
+ * be NULL) receives 2 * count variant indices, (A, A^T) per width. */
+int amfls_operator_autotune(amfls_context* ctx, amfls_operator* op, const int32_t* widths, int32_t count,
+                            int dtype, int32_t reps, int32_t* chosen);
+/* Per-product kernel timing (off by default): CSR products are timed by the
+ * start and end timestamps of their kernel dispatches, other products by
+ * HIP events around them on the engine stream.  kernel_stats returns and
+ * resets: launches, total seconds and total algorithmic bytes. */
 int amfls_operator_profile(amfls_operator* op, int enable);
 int amfls_operator_kernel_stats(amfls_operator* op, int64_t* launches, double* seconds, double* bytes);
 int amfls_spmm_variant_count(void);
@@ -142,6 +151,26 @@ int amfls_spmm_variant_supports(int variant, int block_cols);
 /* b: rows values, device pointer if b_on_device != 0 else host pointer. */
 int amfls_solve(amfls_context* ctx, const amfls_operator* op, const double* b, int b_on_device,
                 const amfls_options* options, amfls_result** out);
+/* Regularization sweep (engine extension; the reference solves one lambda
+ * per solve_amfls call, src/algorithms/amfls/amfls.cpp:57-661).  The block
+ * Golub-Kahan basis of (A, b) does not depend on lambda: one process runs for
+ * the smallest of lambdas[count] and, at each of its checkpoints, the basis
+ * is also evaluated and certified (FP64) for every other value not yet
+ * certified; a value still uncertified when the process stops is solved on
+ * its own.  options->regularization is ignored; precision FP64 or FP32;
+ * recording width policies are not accepted.  results[count] receive one
+ * result per lambda (statistics of swept results are the shared process's). */
+int amfls_solve_sweep(amfls_context* ctx, const amfls_operator* op, const double* b, int b_on_device,
+                      const double* lambdas, int32_t count, const amfls_options* options, amfls_result** results);
+/* Independent CSR problems solved concurrently (engine extension): `workers`
+ * host threads, each with its own stream, take problems in order and run
+ * each problem's regularization sweep over its lambdas_per_problem values
+ * (lambdas is problems x lambdas_per_problem, row-major).  b_dev holds device
+ * right-hand sides.  results (problems x lambdas_per_problem) as
+ * amfls_solve_sweep.  An error in any problem fails the call. */
+int amfls_solve_batch(amfls_context* ctx, int32_t problems, const amfls_operator* const* ops,
+                      const double* const* b_dev, const double* lambdas, int32_t lambdas_per_problem,
+                      const amfls_options* options, int32_t workers, amfls_result** results);
 int amfls_solve_fixed_rbgk(amfls_context* ctx, const amfls_operator* op, const double* b, int b_on_device,
                            double regularization, double tolerance, double failure_probability,
                            int32_t auxiliary_width, int32_t maximum_depth, int32_t maximum_basis_size,
@@ -185,11 +214,31 @@ int amfls_width_model_gradient(amfls_width_model* model, const float* features, 
 /* One Adam step on a device batch; *loss is the batch loss before the step. */
 int amfls_width_model_train_step(amfls_width_model* model, const float* features, const float* labels, int64_t n,
                                  float learning_rate, float* loss);
+/* Per-input box (lo[12], hi[12]) of the data the model was trained on.  A
+ * decision whose inputs leave the box is taken by the heuristic instead, so
+ * the model never extrapolates (the default box is unbounded). */
+int amfls_width_model_set_domain(amfls_width_model* model, const float* lo, const float* hi);
+/* Decisions taken by the model and handed to the heuristic so far. */
+int amfls_width_model_decisions(const amfls_width_model* model, int64_t* by_model, int64_t* by_heuristic);
 /* The model used by AMFLS_WIDTH_LEARNED solves on this context (NULL clears). */
 int amfls_context_set_width_model(amfls_context* ctx, amfls_width_model* model);
 /* Recorded decisions of an AMFLS_WIDTH_RECORD solve: rows x 13 floats
  * (out may be NULL to query rows). */
 int amfls_result_width_records(const amfls_result* result, float* out, int64_t* rows);
+
+/* Counter-addressed synthetic CSR on the device (engine extension for the
+ * batched benchmark inputs; its counters follow the splitmix64 / Box-Muller
+ * chain of reference random/gaussian.cpp:13-35).  generate_rows writes
+ * row_ptr (rows + 1 device int64) for per-row lengths uniform in
+ * [nnz_lo, min(nnz_hi, cols)] (nnz_hi <= 64) and returns *nnz (waits);
+ * generate_entries then writes ascending distinct uniform column indices
+ * (nnz device int32) and N(0,1) values times log-uniform row scales in
+ * [scale_lo, scale_hi] (nnz device values of value_type) and waits. */
+int amfls_csr_generate_rows(amfls_context* ctx, int64_t rows, int64_t cols, int32_t nnz_lo, int32_t nnz_hi,
+                            uint64_t seed, int64_t* row_ptr, int64_t* nnz);
+int amfls_csr_generate_entries(amfls_context* ctx, int64_t rows, int64_t cols, uint64_t seed, double scale_lo,
+                               double scale_hi, const int64_t* row_ptr, int32_t* col_idx, void* values,
+                               int value_type);
 
 /* Counter-addressed Gaussian fill (reference random/gaussian.cpp:25-55)
  * of a row-major device block; dtype AMFLS_F64 or AMFLS_F32. */

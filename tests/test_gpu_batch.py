@@ -1,0 +1,112 @@
+"""GPU tests of the engine's batch extensions: the counter-addressed CSR
+generator against its oracle restatement, the SpMM autotuner, regularization
+sweeps (one Golub-Kahan process, one certified result per lambda) against
+direct FP64 ridge solves, and concurrent batches against serial sweeps."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_mod(gpu):
+    import torch
+    return torch
+
+
+def _csr(torch, rows, cols, nnz_range, seed, scale_range=(1.0, 1.0), value_dtype=None):
+    import amfls_solver_amd as am
+    from amfls_solver_amd.binding import generate_csr
+    rp, ci, val = generate_csr(rows, cols, nnz_range, seed=seed, scale_range=scale_range,
+                               value_dtype=value_dtype or torch.float64)
+    return am.DeviceMatrix.from_csr_arrays(rows, cols, rp, ci, val), (rp, ci, val)
+
+
+def _ridge(rp, ci, val, shape, b, lam):
+    import scipy.sparse as sp
+    A = sp.csr_matrix((val, ci, rp), shape=shape).toarray()
+    K = A.T @ A + lam * np.eye(shape[1])
+    return np.linalg.solve(K, A.T @ b), A
+
+
+def test_generated_csr_matches_oracle(torch_mod):
+    from oracle import csr_gen as G
+    torch = torch_mod
+    for rows, cols, lo, hi, seed, scale in ((2500, 600, 4, 32, 11, (1.0, 1.0)), (400, 12, 8, 20, 5, (1e-3, 1.0))):
+        M, (rp, ci, val) = _csr(torch, rows, cols, (lo, hi), seed, scale)
+        orp, oci, oval = G.csr_generate(rows, cols, lo, hi, seed, scale)
+        np.testing.assert_array_equal(rp.cpu().numpy(), orp)
+        np.testing.assert_array_equal(ci.cpu().numpy(), oci)
+        # counters are exact; device log/cos/exp10 may differ from libm by ulps
+        np.testing.assert_allclose(val.cpu().numpy(), oval, rtol=1e-13, atol=1e-15)
+
+
+def test_autotune_keeps_products_correct(torch_mod):
+    import amfls_solver_amd as am
+    torch = torch_mod
+    M, (rp, ci, val) = _csr(torch, 20000, 3000, (4, 32), 3)
+    chosen = M.autotune(widths=(1, 4, 16), dtype=torch.float64, reps=3)
+    assert set(chosen) == {1, 4, 16}
+    lib = M.context._lib
+    for s, (va, vt) in chosen.items():
+        for name in (va, vt):
+            idx = [lib.amfls_spmm_variant_name(v).decode() for v in range(lib.amfls_spmm_variant_count())].index(name)
+            assert lib.amfls_spmm_variant_supports(idx, s) == 1
+        X = torch.randn(3000, s, dtype=torch.float64, device="cuda")
+        Y = M.apply(X)
+        import scipy.sparse as sp
+        A = sp.csr_matrix((val.cpu().numpy(), ci.cpu().numpy(), rp.cpu().numpy()), shape=(20000, 3000))
+        ref = A @ X.cpu().numpy()
+        assert np.abs(Y.cpu().numpy() - ref).max() <= 1e-12 * np.abs(ref).max()
+        Z = M.apply(Y, transpose=True)
+        refz = A.T @ Y.cpu().numpy()
+        assert np.abs(Z.cpu().numpy() - refz).max() <= 1e-12 * np.abs(refz).max()
+    del am
+
+
+def test_regularization_sweep_certifies_every_lambda(torch_mod):
+    import amfls_solver_amd as am
+    torch = torch_mod
+    rows, cols = 6000, 400
+    M, (rp, ci, val) = _csr(torch, rows, cols, (4, 32), 21)
+    b = torch.from_numpy(np.random.default_rng(1).standard_normal(rows)).cuda()
+    lams = np.logspace(-6, -1, 8)
+    rs = am.run_amfls_sweep(M, b, lams, tolerance=1e-8)
+    assert len(rs) == 8
+    bn = b.cpu().numpy()
+    for lam, r in zip(lams, rs):
+        assert r.status == 0, (lam, r.status)
+        assert r.relative_energy_error_upper_bound <= 1e-8
+        x_ref, A = _ridge(rp.cpu().numpy(), ci.cpu().numpy(), val.cpu().numpy(), (rows, cols), bn, lam)
+        d = r.solution.numpy() - x_ref
+        K = A.T @ A + lam * np.eye(cols)
+        err = np.sqrt(d @ K @ d) / np.sqrt(x_ref @ K @ x_ref)
+        assert err <= r.relative_energy_error_upper_bound * 1.0001 + 1e-14, (lam, err)
+    # the swept process runs to the depth of the smallest lambda; larger
+    # lambdas certify no later
+    assert rs[0].depth == max(r.depth for r in rs)
+
+
+def test_batch_matches_serial_sweeps(torch_mod):
+    """Concurrent workers give each problem exactly its serial sweep (the
+    synthetic clock makes every width decision reproducible)."""
+    import amfls_solver_amd as am
+    torch = torch_mod
+    mats, bs = [], []
+    rng = np.random.default_rng(4)
+    for i in range(6):
+        m = int(rng.integers(3000, 9000))
+        n = int(rng.integers(200, 600))
+        M, _ = _csr(torch, m, n, (4, 32), 100 + i, value_dtype=torch.float32)
+        mats.append(M)
+        bs.append(torch.from_numpy(rng.standard_normal(m)).cuda())
+    lams = np.tile(np.logspace(-4, -1, 3), (6, 1))
+    batch = am.run_amfls_batch(mats, bs, lams, workers=3, tolerance=1e-8, clock="synthetic")
+    for i in range(6):
+        serial = am.run_amfls_sweep(mats[i], bs[i], lams[i], tolerance=1e-8, clock="synthetic")
+        for rb, rsr in zip(batch[i], serial):
+            assert rb.status == 0 and rsr.status == 0
+            assert (rb.depth, rb.basis_rank, rb.auxiliary_width) == (rsr.depth, rsr.basis_rank, rsr.auxiliary_width)
+            assert torch.equal(rb.solution, rsr.solution)
+    with pytest.raises(ValueError):
+        am.run_amfls_batch(mats[:1], bs[:1], lams[:1], workers=1, precision="mixed")

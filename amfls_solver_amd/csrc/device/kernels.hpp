@@ -3,6 +3,8 @@
 // needs are copied out by the caller.
 #pragma once
 
+#include <vector>
+
 #include "common.hpp"
 
 namespace amflsx {
@@ -172,7 +174,20 @@ struct CsrView {
     const int32_t* col_idx = nullptr;
     const void* values = nullptr;
     ValueType type = ValueType::f32;
+    // Row partition of the width-one stream kernel (stream_blocks + 1 row
+    // boundaries, device), or null.
+    const int64_t* stream_rows = nullptr;
+    int64_t stream_blocks = 0;
 };
+
+// Row boundaries (host) of the stream kernel's partition: consecutive rows
+// whose entries fit one block's budget (csr_stream_budget()), at most 1024
+// rows per block; a longer row gets a block of its own.
+std::vector<int64_t> csr_stream_partition(const std::vector<int64_t>& row_ptr);
+int csr_stream_budget();
+// Index of the width-one stream variant (CSR-Stream: entries streamed into
+// LDS with coalesced loads, then summed per row).
+int csr_spmm_stream_variant();
 
 // Y(rows x ncols) = A * X (+ add_scale * Add[0:rows]); X, Y, Add row-major.
 // With ev_start / ev_stop, the events take the start and end timestamps of

@@ -12,7 +12,10 @@
 //
 // Every reduction is split between its producer, which writes one partial
 // per block, and its consumer kernel, whose every block combines all
-// partials in the same fixed order and so obtains the same value.  Kernel
+// partials in the same fixed order and so obtains the same value.  The
+// consumer issues the loads of all partials it needs together, so a kernel
+// waits for at most a few dependent memory round trips; the update reads
+// the basis once for both the update and the next projection.  Kernel
 // boundaries are the only grid-wide synchronization: no tickets, no device-
 // scope fences, no extra launches for final reductions.  Scalars live in
 // device memory, so whole batches of levels are enqueued without the host.
@@ -57,85 +60,134 @@ struct Stat {
     }
 };
 
-// Wave combine of per-lane Stat (max, then sums rescaled exactly to the wave
-// max exponent, butterfly order); lane 0 writes one triple per wavefront at
-// part[3 * (block * kWaves + wave)].  No barriers.
-__device__ void wave_publish(const Stat& st, double* part) {
+// Merge a triple (pm = max, ps = sum scaled to pm's frexp exponent, pb =
+// nonfinite count) into a running Stat, rescaling exactly by powers of two.
+__device__ __forceinline__ void merge(Stat& st, double pm, double ps, double pb) {
+    st.bad += pb;
+    if (pm > 0.0) {
+        int ep = 0;
+        (void)frexp(pm, &ep);
+        if (st.m == 0.0) {
+            st.e = ep;
+        } else if (ep > st.e) {
+            st.s = ldexp(st.s, 2 * (st.e - ep));
+            st.e = ep;
+        }
+        st.m = fmax(st.m, pm);
+        st.s += ldexp(ps, 2 * (ep - st.e));
+    }
+}
+
+// Wavefront combine of per-lane Stat (max, sums rescaled to the wave max
+// exponent, butterfly order); every lane receives the triple in st.
+__device__ __forceinline__ void wave_reduce(Stat& st) {
     const double wm = dev::wave_max(st.m);
     int ew = 0;
     if (wm > 0.0) (void)frexp(wm, &ew);
     const double ss = dev::wave_sum(st.m > 0.0 ? ldexp(st.s, 2 * (st.e - ew)) : 0.0);
-    const double bb = dev::wave_sum(st.bad);
+    st.bad = dev::wave_sum(st.bad);
+    st.m = wm;
+    st.e = ew;
+    st.s = ss;
+}
+
+// One statistics triple per block at part[3 * blockIdx.x]: wavefront
+// combines, then the kWaves triples merged in wave order by thread 0.  All
+// threads of the block must call it (it synchronizes); sm holds 3 * kWaves.
+__device__ void block_publish(Stat st, double* part, double* sm) {
+    wave_reduce(st);
+    const int w = threadIdx.x / kWave;
     if (threadIdx.x % kWave == 0) {
-        const int64_t slot = static_cast<int64_t>(blockIdx.x) * dev::kWaves + threadIdx.x / kWave;
-        part[3 * slot] = wm;
-        part[3 * slot + 1] = ss;
-        part[3 * slot + 2] = bb;
+        sm[3 * w] = st.m;
+        sm[3 * w + 1] = st.s;
+        sm[3 * w + 2] = st.bad;
     }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Stat b;
+#pragma unroll
+        for (int i = 0; i < dev::kWaves; ++i) merge(b, sm[3 * i], sm[3 * i + 1], sm[3 * i + 2]);
+        part[3 * blockIdx.x] = b.m;
+        part[3 * blockIdx.x + 1] = b.s;
+        part[3 * blockIdx.x + 2] = b.bad;
+    }
+    __syncthreads();
 }
 
-// Statistics triple of np per-wavefront triples, computed by every wavefront
-// on its own (shuffles only, no barriers); every lane receives it.
-__device__ void combine_stats(const double* part, int np, double out[3]) {
+// Statistics triples of NA partial arrays (np block triples each), computed
+// by every wavefront on its own with all loads of a chunk issued together:
+// per lane the triples q = lane, lane + 64, ... are merged in order, then
+// the wavefront combines.  Every lane receives out[a][0..2].
+template <int NA>
+__device__ void combine_stats(const double* const (&parts)[NA], int np, double (&out)[NA][3]) {
     const int lane = threadIdx.x % kWave;
-    double gm = 0.0;
-    for (int q = lane; q < np; q += kWave) gm = fmax(gm, part[3 * q]);
-    gm = dev::wave_max(gm);
-    int e = 0;
-    if (gm > 0.0) (void)frexp(gm, &e);
-    double tot = 0.0, bad = 0.0;
-    for (int q = lane; q < np; q += kWave) {
-        const double pm = part[3 * q];
-        bad += part[3 * q + 2];
-        if (pm > 0.0) {
-            int eb = 0;
-            (void)frexp(pm, &eb);
-            tot += ldexp(part[3 * q + 1], 2 * (eb - e));
-        }
+    Stat st[NA];
+    for (int q0 = 0; q0 < np; q0 += 4 * kWave) {
+        double v[NA][4][3];
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int q = q0 + lane + u * kWave;
+                const bool in = q < np;
+                v[a][u][0] = in ? parts[a][3 * q] : 0.0;
+                v[a][u][1] = in ? parts[a][3 * q + 1] : 0.0;
+                v[a][u][2] = in ? parts[a][3 * q + 2] : 0.0;
+            }
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) merge(st[a], v[a][u][0], v[a][u][1], v[a][u][2]);
     }
-    out[0] = gm;
-    out[1] = dev::wave_sum(tot);
-    out[2] = dev::wave_sum(bad);
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+        wave_reduce(st[a]);
+        out[a][0] = st[a].m;
+        out[a][1] = st[a].s;
+        out[a][2] = st[a].bad;
+    }
 }
 
-// DGKS second-pass test from the partials of ||before|| and ||after||.
-__device__ bool second_pass_needed(const double* before, int np, const double* after, double factor) {
-    double sb[3], sa[3];
-    combine_stats(before, np, sb);
-    combine_stats(after, np, sa);
+__device__ __forceinline__ bool needs_second_pass(const double* sb, const double* sa, double factor) {
     const double nb = dev::nominal_norm(sb), na = dev::nominal_norm(sa);
     return isfinite(nb) && isfinite(na) && na < factor * nb;
 }
 
-// c[o] = sum_b Q[b][o] in a fixed order, into shared memory (k values): one
-// wavefront per coefficient (lanes over blocks, butterfly) for k <= 32, one
-// thread per coefficient (blocks in order) otherwise.
-__device__ void combine_coefficients(const double* Q, int nbq, int k, double* cs) {
-    if (k <= 32) {
-        const int lane = threadIdx.x % kWave;
-        for (int o = threadIdx.x / kWave; o < k; o += dev::kWaves) {
-            double acc = 0.0;
-            for (int b = lane; b < nbq; b += kWave) acc += Q[static_cast<int64_t>(b) * k + o];
-            acc = dev::wave_sum(acc);
-            if (lane == 0) cs[o] = acc;
-        }
-        __syncthreads();
-        return;
-    }
-    for (int o = threadIdx.x; o < k; o += kBlock) {
+// cs[o] = sum over blocks b of Q[b][o] (o < k) into shared memory, the same
+// fixed order in every block and every kernel: thread t takes coefficient
+// o = t mod kp (kp = power of two >= the chunk width) and blocks
+// b = t / kp, t / kp + S, ... (S = 256 / kp) in order, then a shared-memory
+// tree over the S threads of each coefficient.  Loads are issued sixteen at
+// a time.  All threads must call it; red holds kBlock doubles.
+__device__ void sum_partials(const double* Q, int nbq, int k, double* cs, double* red) {
+    const int t = threadIdx.x;
+    for (int o0 = 0; o0 < k; o0 += kBlock) {
+        const int kc = std::min(kBlock, k - o0);
+        int kp = 1;
+        while (kp < kc) kp <<= 1;
+        const int S = kBlock / kp;
+        const int o = t % kp, sidx = t / kp;
         double acc = 0.0;
-        int b = 0;
-        for (; b + 3 < nbq; b += 4) {
-            double pv[4];
+        if (o < kc) {
+            int b = sidx;
+            for (; b + 15 * S < nbq; b += 16 * S) {
+                double v[16];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) pv[u] = Q[static_cast<int64_t>(b + u) * k + o];
+                for (int u = 0; u < 16; ++u) v[u] = Q[static_cast<int64_t>(b + u * S) * k + o0 + o];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) acc += pv[u];
+                for (int u = 0; u < 16; ++u) acc += v[u];
+            }
+            for (; b < nbq; b += S) acc += Q[static_cast<int64_t>(b) * k + o0 + o];
         }
-        for (; b < nbq; ++b) acc += Q[static_cast<int64_t>(b) * k + o];
-        cs[o] = acc;
+        red[t] = acc;
+        __syncthreads();
+        for (int stride = S / 2; stride > 0; stride >>= 1) {
+            if (sidx < stride) red[t] += red[t + stride * kp];
+            __syncthreads();
+        }
+        if (t < kc) cs[o0 + t] = red[t];
+        __syncthreads();
     }
-    __syncthreads();
 }
 
 // Per-block projection partials Q[b][o] = sum over the block's rows of
@@ -197,83 +249,124 @@ __global__ __launch_bounds__(kBlock) void couple_project_kernel(int64_t N, int k
         sr.add(rv);
         sy.add(static_cast<double>(yv));
     }
-    wave_publish(sr, part_r);
-    wave_publish(sy, part_y);
-    __syncthreads();
+    block_publish(sr, part_r, red);
+    block_publish(sy, part_y, red);
     project_block<T>(B, ldb, k, r0, nr, ys, red, Q);
 }
 
 // y -= B c with c = sum_b Qin[b] (gated: second pass only when the DGKS
 // test on (before, after) holds); stats of the new y; with Qout, projection
-// partials of the new y for a following pass.
-template <typename T>
+// partials of the new y for a following pass, accumulated in the same read
+// of B.  Narrow bases (k <= 16): a thread per row.  Wider bases: a
+// wavefront per row, lanes over the k coefficients (J per lane).
+template <typename T, int J>
 __global__ __launch_bounds__(kBlock) void update_kernel(int64_t N, int k, int R, const T* __restrict__ B,
                                                         int64_t ldb, const double* Qin, int nbq, T* y, int64_t ldy,
                                                         double* part_y, double* Qout, const double* go,
                                                         const double* before, const double* after, int np_stats,
                                                         double factor) {
-    extern __shared__ double lds[];  // c (k values) then the block's new y (R values)
-    double* cs = lds;
-    double* ys = lds + k;
+    extern __shared__ double cs[];  // c (k values)
     __shared__ double red[kBlock];
+    __shared__ double wq[dev::kWaves][kWave];
     if (*go == 0.0) return;
-    if (before != nullptr && !second_pass_needed(before, np_stats, after, factor)) return;
-    combine_coefficients(Qin, nbq, k, cs);
+    if (before != nullptr) {
+        const double* const parts[2] = {before, after};
+        double st2[2][3];
+        combine_stats<2>(parts, np_stats, st2);
+        if (!needs_second_pass(st2[0], st2[1], factor)) return;
+    }
+    sum_partials(Qin, nbq, k, cs, red);
     const int t = threadIdx.x;
     const int64_t r0 = static_cast<int64_t>(blockIdx.x) * R;
     const int nr = static_cast<int>(std::min<int64_t>(R, N - r0));
     Stat st;
-    if (k <= 16) {
+    if (J == 0) {  // k <= 16: a thread per row
+        double qa[16];
+#pragma unroll
+        for (int o = 0; o < 16; ++o) qa[o] = 0.0;
         for (int rr = t; rr < nr; rr += kBlock) {
             const int64_t i = r0 + rr;
+            double bv[16];
+#pragma unroll
+            for (int o = 0; o < 16; ++o) bv[o] = o < k ? static_cast<double>(B[i * ldb + o]) : 0.0;
             double acc = 0.0;
-            for (int o = 0; o < k; ++o) acc = fma(static_cast<double>(B[i * ldb + o]), cs[o], acc);
+#pragma unroll
+            for (int o = 0; o < 16; ++o)
+                if (o < k) acc = fma(bv[o], cs[o], acc);
             const T nv = static_cast<T>(static_cast<double>(y[i * ldy]) - acc);
             y[i * ldy] = nv;
-            ys[rr] = static_cast<double>(nv);
+            st.add(static_cast<double>(nv));
+#pragma unroll
+            for (int o = 0; o < 16; ++o) qa[o] = fma(bv[o], static_cast<double>(nv), qa[o]);
+        }
+        block_publish(st, part_y, red);
+        if (Qout != nullptr) {
+            // per coefficient: wavefront butterfly, then the kWaves sums in order
+            const int lane = t % kWave, w = t / kWave;
+            for (int o = 0; o < k; ++o) {
+                double q = 0.0;
+#pragma unroll
+                for (int u = 0; u < 16; ++u)
+                    if (u == o) q = qa[u];
+                q = dev::wave_sum(q);
+                if (lane == 0) wq[w][o] = q;
+            }
+            __syncthreads();
+            if (t < k) {
+                double q = 0.0;
+#pragma unroll
+                for (int w2 = 0; w2 < dev::kWaves; ++w2) q += wq[w2][t];
+                Qout[static_cast<int64_t>(blockIdx.x) * k + t] = q;
+            }
+        }
+        return;
+    }
+    // J >= 1: wavefront per row, lane holds coefficients lane + 64 j (j < J)
+    const int lane = t % kWave, w = t / kWave;
+    double qa[J > 0 ? J : 1];
+    double cl[J > 0 ? J : 1];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        qa[j] = 0.0;
+        const int o = lane + kWave * j;
+        cl[j] = o < k ? cs[o] : 0.0;
+    }
+    for (int rr = w; rr < nr; rr += dev::kWaves) {
+        const int64_t i = r0 + rr;
+        double bv[J > 0 ? J : 1];
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            const int o = lane + kWave * j;
+            bv[j] = o < k ? static_cast<double>(B[i * ldb + o]) : 0.0;
+            acc = fma(bv[j], cl[j], acc);
+        }
+        acc = dev::wave_sum(acc);
+        const double yo = static_cast<double>(y[i * ldy]);  // every lane reads the same word
+        const T nv = static_cast<T>(yo - acc);
+        if (lane == 0) {
+            y[i * ldy] = nv;
             st.add(static_cast<double>(nv));
         }
-    } else {
-        const int lane = t % kWave, wv = t / kWave;
-        int rr = wv;
-        for (; rr + 3 * dev::kWaves < nr; rr += 4 * dev::kWaves) {
-            double acc[4] = {0.0, 0.0, 0.0, 0.0};
-            for (int o = lane; o < k; o += kWave) {
-                const double co = cs[o];
 #pragma unroll
-                for (int u = 0; u < 4; ++u)
-                    acc[u] = fma(static_cast<double>(B[(r0 + rr + u * dev::kWaves) * ldb + o]), co, acc[u]);
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) acc[u] = dev::wave_sum(acc[u]);
-            if (lane == 0) {
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int64_t i = r0 + rr + u * dev::kWaves;
-                    const T nv = static_cast<T>(static_cast<double>(y[i * ldy]) - acc[u]);
-                    y[i * ldy] = nv;
-                    ys[rr + u * dev::kWaves] = static_cast<double>(nv);
-                    st.add(static_cast<double>(nv));
-                }
-            }
-        }
-        for (; rr < nr; rr += dev::kWaves) {
-            const int64_t i = r0 + rr;
-            double acc = 0.0;
-            for (int o = lane; o < k; o += kWave) acc = fma(static_cast<double>(B[i * ldb + o]), cs[o], acc);
-            acc = dev::wave_sum(acc);
-            if (lane == 0) {
-                const T nv = static_cast<T>(static_cast<double>(y[i * ldy]) - acc);
-                y[i * ldy] = nv;
-                ys[rr] = static_cast<double>(nv);
-                st.add(static_cast<double>(nv));
-            }
-        }
+        for (int j = 0; j < J; ++j) qa[j] = fma(bv[j], static_cast<double>(nv), qa[j]);
     }
-    wave_publish(st, part_y);
+    block_publish(st, part_y, red);
     if (Qout != nullptr) {
-        __syncthreads();
-        project_block<T>(B, ldb, k, r0, nr, ys, red, Qout);
+        // Qout[block][o] = the kWaves wavefront sums of coefficient o, in order
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            wq[w][lane] = qa[j];
+            __syncthreads();
+            if (t < kWave) {
+                const int o = t + kWave * j;
+                double q = 0.0;
+#pragma unroll
+                for (int w2 = 0; w2 < dev::kWaves; ++w2) q += wq[w2][t];
+                if (o < k) Qout[static_cast<int64_t>(blockIdx.x) * k + o] = q;
+            }
+            __syncthreads();
+        }
     }
 }
 
@@ -284,44 +377,37 @@ __global__ __launch_bounds__(kBlock) void update_kernel(int64_t N, int k, int R,
 // then dst = y * (1 / ||resid||) and, with b given, gpart[block] = sum over
 // the block of dst[i] b[i].  Block 0 writes record = {resid, scale,
 // accepted, c} (c = summed coefficient of a one-column basis, when Q1 is
-// given) and, on acceptance, *cell = resid; on rejection *go = 0.
+// given, combined exactly as the update kernels combined it) and, on
+// acceptance, *cell = resid; on rejection *go = 0.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void chain_normalize_kernel(
     int64_t N, int R, const T* y, int64_t ldy, T* dst, int64_t ldd, const double* part_scale,
     const double* part_before, const double* part_first, const double* part_second, int np_stats,
     double dgks_factor, double rank_factor, const double* Q1, const double* Q2, int nbq, const double* b,
     double* gpart, double* record, double* cell, double* go) {
-    __shared__ double sm[dev::kWaves];
+    __shared__ double red[kBlock];
+    __shared__ double cq[2];
     if (*go == 0.0) return;
-    double ss[3], sf[3];
-    combine_stats(part_scale, np_stats, ss);
-    const bool second = second_pass_needed(part_before, np_stats, part_first, dgks_factor);
-    combine_stats(second ? part_second : part_first, np_stats, sf);
-    const double scale = dev::nominal_norm(ss), resid = dev::nominal_norm(sf);
+    const double* const parts[4] = {part_scale, part_before, part_first, part_second};
+    double st4[4][3];
+    combine_stats<4>(parts, np_stats, st4);
+    const bool second = needs_second_pass(st4[1], st4[2], dgks_factor);
+    const double scale = dev::nominal_norm(st4[0]), resid = dev::nominal_norm(second ? st4[3] : st4[2]);
     const bool ok = isfinite(scale) && scale >= 0.0 && isfinite(resid) && resid > rank_factor * scale;
-    double c_total = 0.0;
-    if (Q1 != nullptr && blockIdx.x == 0 && threadIdx.x < kWave) {
-        const int lane = threadIdx.x;
-        double c1 = 0.0, c2 = 0.0;
-        for (int q = lane; q < nbq; q += kWave) c1 += Q1[q];
-        c1 = dev::wave_sum(c1);
-        if (second) {
-            for (int q = lane; q < nbq; q += kWave) c2 += Q2[q];
-            c2 = dev::wave_sum(c2);
-        }
-        c_total = second ? c1 + c2 : c1;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        record[0] = resid;
-        record[1] = scale;
-        record[2] = ok ? 1.0 : 0.0;
+    if (blockIdx.x == 0) {
         if (Q1 != nullptr) {
-            // the coefficient sums of a one-column basis, in the order the
-            // update kernels combined them (wave butterfly over blocks)
-            record[3] = c_total;
+            sum_partials(Q1, nbq, 1, cq, red);
+            if (second) sum_partials(Q2, nbq, 1, cq + 1, red);
         }
-        if (ok) *cell = resid;
-        else *go = 0.0;
+        if (threadIdx.x == 0) {
+            record[0] = resid;
+            record[1] = scale;
+            record[2] = ok ? 1.0 : 0.0;
+            // the coefficient sums of a one-column basis, as the updates used them
+            if (Q1 != nullptr) record[3] = second ? cq[0] + cq[1] : cq[0];
+            if (ok) *cell = resid;
+            else *go = 0.0;
+        }
     }
     if (!ok) return;
     const double inv = 1.0 / resid;
@@ -336,7 +422,7 @@ __global__ __launch_bounds__(kBlock) void chain_normalize_kernel(
         if (b != nullptr) acc = fma(static_cast<double>(nv), b[g], acc);
     }
     if (b != nullptr) {
-        acc = dev::block_sum(acc, sm);
+        acc = dev::block_sum(acc, red);
         if (threadIdx.x == 0) gpart[blockIdx.x] = acc;
     }
 }
@@ -345,9 +431,9 @@ __global__ __launch_bounds__(kBlock) void chain_normalize_kernel(
 
 int chain_block_rows(int64_t N, int k) {
     // about 128 blocks; fewer, longer blocks when the k projection partials
-    // each consumer block combines would exceed 16K values
+    // each consumer block combines would exceed 8K values
     const int64_t base = std::max<int64_t>(64, ceil_div(N, 128));
-    const int64_t wide = ceil_div(N * std::max(k, 1), 16384);
+    const int64_t wide = ceil_div(N * std::max(k, 1), 8192);
     return static_cast<int>(std::min<int64_t>(4096, std::max(base, wide)));
 }
 
@@ -366,12 +452,22 @@ template <typename T>
 void chain_update(hipStream_t s, int64_t N, int k, const T* B, int64_t ldb, const double* Qin, T* y, int64_t ldy,
                   double* part_y, double* Qout, const double* go, const double* before, const double* after,
                   double factor) {
+    if (k < 1 || k > 2048) throw std::invalid_argument("chain update supports bases of 1 to 2048 columns");
     const int R = chain_block_rows(N, k);
     const int64_t nb = ceil_div(N, R);
-    const size_t lds = static_cast<size_t>(k + R) * sizeof(double);
-    hipLaunchKernelGGL(update_kernel<T>, dim3(static_cast<unsigned>(nb)), dim3(kBlock), lds, s, N, k, R, B, ldb, Qin,
-                       static_cast<int>(nb), y, ldy, part_y, Qout, go, before, after,
-                       static_cast<int>(nb * dev::kWaves), factor);
+    const size_t lds = static_cast<size_t>(k) * sizeof(double);
+    const dim3 grid(static_cast<unsigned>(nb));
+#define AMFLS_UPDATE(J)                                                                                          \
+    hipLaunchKernelGGL((update_kernel<T, J>), grid, dim3(kBlock), lds, s, N, k, R, B, ldb, Qin,                 \
+                       static_cast<int>(nb), y, ldy, part_y, Qout, go, before, after, static_cast<int>(nb), factor)
+    if (k <= 16) AMFLS_UPDATE(0);
+    else if (k <= 64) AMFLS_UPDATE(1);
+    else if (k <= 128) AMFLS_UPDATE(2);
+    else if (k <= 256) AMFLS_UPDATE(4);
+    else if (k <= 512) AMFLS_UPDATE(8);
+    else if (k <= 1024) AMFLS_UPDATE(16);
+    else AMFLS_UPDATE(32);
+#undef AMFLS_UPDATE
     AMFLS_LAUNCH_CHECK();
 }
 
@@ -384,7 +480,7 @@ void chain_normalize(hipStream_t s, int64_t N, int k, const T* y, int64_t ldy, T
     const int64_t nb = ceil_div(N, R);
     hipLaunchKernelGGL(chain_normalize_kernel<T>, dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s, N, R, y, ldy,
                        dst, ldd, part_scale, part_before, part_first, part_second,
-                       static_cast<int>(nb * dev::kWaves), dgks_factor,
+                       static_cast<int>(nb), dgks_factor,
                        rank_factor, Q1, Q2, static_cast<int>(nb), b, gpart, record, cell, go);
     AMFLS_LAUNCH_CHECK();
 }

@@ -40,7 +40,28 @@ __global__ __launch_bounds__(kThreads) void csr_spmm_kernel(
     for (int t = 0; t < CPL; ++t) acc[t] = T(0);
     const int64_t e0 = rp[row], e1 = rp[row + 1];
     int64_t e = e0 + sub;
-    // two entries per trip to keep more gathers in flight
+    // four entries per trip: their indices and values, then their gathers,
+    // are independent loads in flight together (entries still summed in order)
+    for (; e + 3 * P < e1; e += 4 * P) {
+        int32_t c[4];
+        T v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            c[u] = ci[e + u * P];
+            v[u] = load_as<T, V>(val[e + u * P]);
+        }
+#pragma unroll
+        for (int t = 0; t < CPL; ++t) {
+            const int j = cl + LC * t;
+            if (j < ncols) {
+                T xv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) xv[u] = X[static_cast<int64_t>(c[u]) * ldx + j];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) acc[t] = fma(v[u], xv[u], acc[t]);
+            }
+        }
+    }
     for (; e + P < e1; e += 2 * P) {
         const int32_t c0 = ci[e], c1 = ci[e + P];
         const T v0 = load_as<T, V>(val[e]);
@@ -81,6 +102,76 @@ __global__ __launch_bounds__(kThreads) void csr_spmm_kernel(
         T r = acc[t];
         if (Add != nullptr) r = fma(add_scale, Add[row * ld_add + j], r);
         Y[row * ldy + j] = r;
+    }
+}
+
+// Width-one products by CSR-Stream (J. L. Greathouse and M. Daga, "Efficient
+// sparse matrix-vector multiplication on GPUs using the CSR storage format",
+// SC14): a block owns consecutive rows holding at most kStreamBudget entries;
+// its threads stream those entries with coalesced loads, multiply by the
+// gathered x and park the products in LDS; then a thread per row sums its
+// products in entry order.  A row longer than the budget gets a block that
+// reduces it with all threads.
+constexpr int kStreamBudget = 2048;
+constexpr int kStreamMaxRows = 1024;
+
+template <typename V, typename T>
+__global__ __launch_bounds__(kThreads) void csr_stream_kernel(
+    const int64_t* __restrict__ rp, const int32_t* __restrict__ ci, const V* __restrict__ val,
+    const int64_t* __restrict__ br, const T* __restrict__ X, int64_t ldx, T* __restrict__ Y, int64_t ldy,
+    const T* __restrict__ Add, int64_t ld_add, T add_scale) {
+    __shared__ T prod[kStreamBudget];
+    __shared__ T red[kThreads / kWave];
+    const int t = threadIdx.x;
+    const int64_t r0 = br[blockIdx.x], r1 = br[blockIdx.x + 1];
+    const int64_t e0 = rp[r0], e1 = rp[r1];
+    if (e1 - e0 <= kStreamBudget) {
+        // all of a thread's entries in two waves of independent loads: the
+        // column indices and values, then the gathered x
+        constexpr int U = kStreamBudget / kThreads;
+        int32_t c[U];
+        T v[U], xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t e = e0 + t + u * kThreads;
+            const bool in = e < e1;
+            c[u] = in ? ci[e] : 0;
+            v[u] = in ? load_as<T, V>(val[e]) : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t e = e0 + t + u * kThreads;
+            xv[u] = e < e1 ? X[static_cast<int64_t>(c[u]) * ldx] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t e = e0 + t + u * kThreads;
+            if (e < e1) prod[e - e0] = v[u] * xv[u];
+        }
+        __syncthreads();
+        for (int64_t r = r0 + t; r < r1; r += kThreads) {
+            const int a = static_cast<int>(rp[r] - e0), b = static_cast<int>(rp[r + 1] - e0);
+            T acc = T(0);
+            for (int i = a; i < b; ++i) acc += prod[i];
+            if (Add != nullptr) acc = fma(add_scale, Add[r * ld_add], acc);
+            Y[r * ldy] = acc;
+        }
+        return;
+    }
+    // one row longer than the budget (r1 == r0 + 1)
+    T acc = T(0);
+    for (int64_t e = e0 + t; e < e1; e += kThreads)
+        acc = fma(load_as<T, V>(val[e]), X[static_cast<int64_t>(ci[e]) * ldx], acc);
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if (t % kWave == 0) red[t / kWave] = acc;
+    __syncthreads();
+    if (t == 0) {
+        T tot = T(0);
+#pragma unroll
+        for (int w = 0; w < kThreads / kWave; ++w) tot += red[w];
+        if (Add != nullptr) tot = fma(add_scale, Add[r0 * ld_add], tot);
+        Y[r0 * ldy] = tot;
     }
 }
 
@@ -139,6 +230,32 @@ void launch_variant(hipStream_t s, const CsrView& A, int ncols, const T* X, int6
                            A.col_idx, static_cast<const V*>(A.values), ncols, X, ldx, Y, ldy, Add, ld_add,
                            static_cast<T>(add_scale));
     AMFLS_LAUNCH_CHECK();
+}
+
+template <typename V, typename T>
+void launch_stream_typed(hipStream_t s, const CsrView& A, const T* X, int64_t ldx, T* Y, int64_t ldy,
+                         const T* Add, int64_t ld_add, double add_scale, hipEvent_t e0, hipEvent_t e1) {
+    const dim3 grid(static_cast<unsigned>(A.stream_blocks));
+    if (e0 || e1)
+        hipExtLaunchKernelGGL((csr_stream_kernel<V, T>), grid, dim3(kThreads), 0, s, e0, e1, 0, A.row_ptr, A.col_idx,
+                              static_cast<const V*>(A.values), A.stream_rows, X, ldx, Y, ldy, Add, ld_add,
+                              static_cast<T>(add_scale));
+    else
+        hipLaunchKernelGGL((csr_stream_kernel<V, T>), grid, dim3(kThreads), 0, s, A.row_ptr, A.col_idx,
+                           static_cast<const V*>(A.values), A.stream_rows, X, ldx, Y, ldy, Add, ld_add,
+                           static_cast<T>(add_scale));
+    AMFLS_LAUNCH_CHECK();
+}
+
+template <typename T>
+void launch_stream(hipStream_t s, const CsrView& A, const T* X, int64_t ldx, T* Y, int64_t ldy, const T* Add,
+                   int64_t ld_add, double add_scale, hipEvent_t e0, hipEvent_t e1) {
+    if (A.type == ValueType::f32)
+        launch_stream_typed<float, T>(s, A, X, ldx, Y, ldy, Add, ld_add, add_scale, e0, e1);
+    else if (A.type == ValueType::f64)
+        launch_stream_typed<double, T>(s, A, X, ldx, Y, ldy, Add, ld_add, add_scale, e0, e1);
+    else
+        throw std::invalid_argument("CSR values must be float32 or float64");
 }
 
 template <typename V, typename T>
@@ -276,18 +393,45 @@ __global__ void csr_validate_kernel(int64_t rows, int64_t cols, int64_t nnz, con
 
 }  // namespace
 
-int csr_spmm_variant_count() { return kVariantCount; }
+// The stream kernel follows the table's variants.
+constexpr int kStreamVariant = kVariantCount;
+
+int csr_spmm_variant_count() { return kVariantCount + 1; }
+int csr_spmm_stream_variant() { return kStreamVariant; }
+int csr_stream_budget() { return kStreamBudget; }
+
+std::vector<int64_t> csr_stream_partition(const std::vector<int64_t>& rp) {
+    const int64_t rows = static_cast<int64_t>(rp.size()) - 1;
+    std::vector<int64_t> br{0};
+    int64_t r = 0;
+    while (r < rows) {
+        const int64_t e0 = rp[r];
+        int64_t end = r + 1;
+        while (end < rows && rp[end + 1] - e0 <= kStreamBudget && end - r < kStreamMaxRows) ++end;
+        br.push_back(end);
+        r = end;
+    }
+    return br;
+}
 
 bool csr_spmm_variant_supports(int variant, int ncols) {
+    if (variant == kStreamVariant) return ncols == 1;
     if (variant < 0 || variant >= kVariantCount) return false;
     return ncols <= kVariants[variant].lc * kVariants[variant].cpl;
 }
 
 const char* csr_spmm_variant_name(int variant) {
+    if (variant == kStreamVariant) return "stream";
     return (variant >= 0 && variant < kVariantCount) ? kVariants[variant].name : "invalid";
 }
 
 bool csr_spmm_variant_shape(int variant, int* p, int* lc, int* cpl) {
+    if (variant == kStreamVariant) {
+        *p = 0;
+        *lc = 1;
+        *cpl = 1;
+        return true;
+    }
     if (variant < 0 || variant >= kVariantCount) return false;
     *p = kVariants[variant].p;
     *lc = kVariants[variant].lc;
@@ -328,11 +472,16 @@ void csr_spmm(hipStream_t s, const CsrView& A, int ncols, const T* X, int64_t ld
     for (int c0 = 0; c0 < ncols; c0 += 256) {
         const int nc = std::min(256, ncols - c0);
         int v = variant;
+        if (v == kStreamVariant && A.stream_rows == nullptr) v = -1;
         if (v < 0 || !csr_spmm_variant_supports(v, nc))
             v = csr_spmm_default_variant(nc, A.rows > 0 ? static_cast<double>(A.nnz) / A.rows : 0.0);
         const T* add = Add ? Add + c0 : nullptr;
         hipEvent_t e0 = c0 == 0 ? ev_start : nullptr;
         hipEvent_t e1 = c0 + nc == ncols ? ev_stop : nullptr;
+        if (v == kStreamVariant) {
+            launch_stream<T>(s, A, X, ldx, Y, ldy, add, ld_add, add_scale, e0, e1);
+            continue;
+        }
         switch (A.type) {
             case ValueType::f32:
                 launch_by_index<float, T>(v, s, A, nc, X + c0, ldx, Y + c0, ldy, add, ld_add, add_scale, e0, e1);

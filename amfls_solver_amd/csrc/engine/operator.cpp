@@ -149,6 +149,17 @@ CsrOperator::CsrOperator(int64_t rows, int64_t cols, int64_t nnz, const int64_t*
     ctx.download(hrp.data(), row_ptr, rows + 1);
     ctx.download(htrp.data(), t_rp_.get(), cols + 1);
     if (hm[1] > 0.0) throw std::invalid_argument("CSR values must be finite");
+    // row partitions of the width-one stream kernel, for A and A^T
+    for (int side = 0; side < 2; ++side) {
+        const std::vector<int64_t> br = csr_stream_partition(side == 0 ? hrp : htrp);
+        DevBuf<int64_t>& dst = side == 0 ? stream_a_ : stream_at_;
+        dst.reset(br.size());
+        AMFLS_HIP_CHECK(hipMemcpyAsync(dst.get(), br.data(), br.size() * sizeof(int64_t), hipMemcpyHostToDevice, s));
+        CsrView& v = side == 0 ? a_ : at_;
+        v.stream_rows = dst.get();
+        v.stream_blocks = static_cast<int64_t>(br.size()) - 1;
+    }
+    ctx.sync();
     std::vector<long long> rops(rows), cops(cols);
     long long max_row = 0, max_col = 0;
     for (int64_t i = 0; i < rows; ++i) {

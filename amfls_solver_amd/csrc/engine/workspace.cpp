@@ -554,8 +554,8 @@ int Workspace<T>::run_fast_batch(int count) {
     const int nbN = static_cast<int>(ceil_div(m_, chain_block_rows(m_, 1)));
     fast_cells_.ensure(static_cast<size_t>(4 + count));
     fast_rec_.ensure(static_cast<size_t>(8 * count));
-    // statistics partials: one triple per wavefront (4 per block)
-    const int64_t spart = 3 * 4 * nbmax;
+    // statistics partials: one triple per block
+    const int64_t spart = 3 * nbmax;
     fast_part_.ensure(static_cast<size_t>(4 * spart + 2 * qmax + static_cast<int64_t>(count) * nbN));
     double* cells = fast_cells_.get();
     double* go = cells, *alpha = cells + 1, *beta = cells + 2;  // beta[j]: coupling of level j

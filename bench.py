@@ -36,7 +36,7 @@ def measured_traffic():
     """HBM bytes per launch of the headline SpMM kernels from the committed
     PMC summary (FETCH_SIZE and WRITE_SIZE passes scaled by a calibration
     copy), or None when the summary is absent."""
-    path = os.path.join(ROOT, "profiles", "csr_spmm_c1_pmc_traffic.md")
+    path = os.path.join(ROOT, "profiles", "csr_spmm_c1_pmc_traffic.txt")
     try:
         with open(path) as f:
             for line in f:
@@ -69,19 +69,24 @@ def spmm_sweep(torch, np, am, P, widths=(1, 4, 16, 64), reps=20):
         Z = torch.empty(p.cols, s, device="cuda", dtype=torch.float32)
         row = {}
         for tr, (src, dst, ldi, ldo) in ((0, (X, Y, s, s)), (1, (Y, Z, s, s))):
-            torch.cuda.synchronize()
-            for _ in range(3):
-                lib.amfls_operator_apply_rows(h, M.handle, tr, ctypes.c_void_p(src.data_ptr()), s, ldi,
-                                              ctypes.c_void_p(dst.data_ptr()), ldo, 1)
-            M.context.synchronize()
-            M.profile(True)
-            for _ in range(reps):
-                lib.amfls_operator_apply_rows(h, M.handle, tr, ctypes.c_void_p(src.data_ptr()), s, ldi,
-                                              ctypes.c_void_p(dst.data_ptr()), ldo, 1)
-            M.context.synchronize()
-            n, sec, by = M.kernel_stats()
-            M.profile(False)
-            row["AT" if tr else "A"] = {"us": 1e6 * sec / n, "GBs": by / sec / 1e9, "frac": by / sec / 1e9 / HBM_PEAK_GBS}
+            kinds = [("", -1)] + ([("_stream", lib.amfls_spmm_variant_count() - 1)] if s == 1 else [])
+            for tag, var in kinds:
+                M.set_variant(s, bool(tr), var)
+                torch.cuda.synchronize()
+                for _ in range(3):
+                    lib.amfls_operator_apply_rows(h, M.handle, tr, ctypes.c_void_p(src.data_ptr()), s, ldi,
+                                                  ctypes.c_void_p(dst.data_ptr()), ldo, 1)
+                M.context.synchronize()
+                M.profile(True)
+                for _ in range(reps):
+                    lib.amfls_operator_apply_rows(h, M.handle, tr, ctypes.c_void_p(src.data_ptr()), s, ldi,
+                                                  ctypes.c_void_p(dst.data_ptr()), ldo, 1)
+                M.context.synchronize()
+                n, sec, by = M.kernel_stats()
+                M.profile(False)
+                row[("AT" if tr else "A") + tag] = {"us": 1e6 * sec / n, "GBs": by / sec / 1e9,
+                                                    "frac": by / sec / 1e9 / HBM_PEAK_GBS}
+            M.set_variant(s, bool(tr), -1)
         out["widths"][str(s)] = row
     del M
     return out
@@ -162,12 +167,20 @@ def batch_c3(torch, np, am, problems=256, lambdas=8, workers=4, seed=3, toleranc
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     ok = sum(1 for row in res for r in row if r.status == 0)
+    statuses = {}
+    for row in res:
+        for r in row:
+            statuses[str(r.status)] = statuses.get(str(r.status), 0) + 1
+    deep = sorted(range(problems), key=lambda i: -max(r.depth for r in res[i]))[:3]
     prods = sum(row[0].a_block_calls + row[0].at_block_calls for row in res)
     nnz = sum(M.nnz for M in mats)
     out = {"problems": problems, "lambdas_per_problem": lambdas, "workers": workers, "total_nnz": nnz,
            "seconds": dt, "certified": ok, "solves": problems * lambdas, "solves_per_s": problems * lambdas / dt,
            "block_products": prods, "block_products_per_s": prods / dt, "build_seconds": t_build,
-           "autotune_seconds": t_tune, "max_depth": max(r.depth for row in res for r in row)}
+           "autotune_seconds": t_tune, "max_depth": max(r.depth for row in res for r in row),
+           "statuses": statuses,
+           "deepest": [{"m": dims[i][0], "n": dims[i][1], "depth": [r.depth for r in res[i]],
+                        "status": [r.status for r in res[i]]} for i in deep]}
     # the same first problems solved one lambda at a time, for the sharing gain
     k = min(8, problems)
     torch.cuda.synchronize()
@@ -215,7 +228,9 @@ def main() -> int:
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-dense", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--batch-problems", type=int, default=256)
+    ap.add_argument("--batch-problems", type=int, default=64,
+                    help="problems of the batched configuration to run (the full configuration has 256)")
+    ap.add_argument("--batch-workers", type=int, default=4)
     ap.add_argument("--pmc-calibrate", action="store_true",
                     help="start with a 1 GiB device copy (reference kernel for PMC byte counters)")
     a = ap.parse_args()
@@ -242,6 +257,10 @@ def main() -> int:
     p = P.config1()
     M = am.DeviceMatrix(csr_tensor(torch, np, p), value_dtype=torch.float32)
     b = torch.from_numpy(p.b).cuda()
+    # width-one kernel variants for A and A^T picked by the autotuner (FP64
+    # blocks; FP32 blocks of the mixed inner solves use the same choice)
+    tuned = M.autotune(widths=(1,), dtype=torch.float64, reps=5)
+    log(f"autotuned width-one SpMM variants (A, A^T): {tuned[1]}")
     modes = ["fp64", "mixed"]
     results = {}
     for mode in modes:
@@ -309,7 +328,7 @@ def main() -> int:
     batch = None
     if not a.no_batch and a.batch_problems > 0:
         log(f"batched CSR problems: {a.batch_problems} x 8 lambdas")
-        batch = batch_c3(torch, np, am, problems=a.batch_problems)
+        batch = batch_c3(torch, np, am, problems=a.batch_problems, workers=a.batch_workers)
         log(f"batch: {batch}")
 
     line = {
@@ -341,6 +360,7 @@ def main() -> int:
             "learned_width_status": int(rl.status),
             "learned_model_heldout_speedup": wm.meta.get("heldout_speedup"),
             "rel_diff_vs_oracle": rel,
+            "spmm_variants_width1": {"A": tuned[1][0], "AT": tuned[1][1]},
             "parallelism": "single_gpu",
         },
         "roofline": {

@@ -11,6 +11,7 @@ batches of CSR problems (run_amfls_batch) and the SpMM autotuner
 from .binding import (  # noqa: F401
     Context,
     DeviceMatrix,
+    HostOperator,
     SolveResult,
     run_amfls,
     run_amfls_batch,
@@ -23,5 +24,5 @@ from . import torch_ops as _torch_ops
 
 _torch_ops.register()
 
-__all__ = ["Context", "DeviceMatrix", "SolveResult", "run_amfls", "run_amfls_batch", "run_amfls_sweep",
+__all__ = ["Context", "DeviceMatrix", "HostOperator", "SolveResult", "run_amfls", "run_amfls_batch", "run_amfls_sweep",
            "run_fixed_rbgk", "run_lsqr"]

@@ -61,6 +61,8 @@ SIGNATURES = {
                                  POINTER(c_void_p)]),
     "amfls_dense_create": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_int, c_int, c_int64,
                                    POINTER(c_void_p)]),
+    "amfls_host_operator_create": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_double,
+                                           c_double, c_int64, c_int64, POINTER(c_void_p)]),
     "amfls_operator_destroy": (c_int, [c_void_p]),
     "amfls_operator_shape": (c_int, [c_void_p, POINTER(c_int64), POINTER(c_int64)]),
     "amfls_operator_validation_model": (c_int, [c_void_p, POINTER(c_double), POINTER(c_double),
@@ -77,6 +79,7 @@ SIGNATURES = {
     "amfls_spmm_variant_count": (c_int, []),
     "amfls_spmm_variant_name": (c_char_p, [c_int]),
     "amfls_spmm_variant_supports": (c_int, [c_int, c_int]),
+    "amfls_spmm_default_variant": (c_int, [c_int, c_double]),
     "amfls_solve": (c_int, [c_void_p, c_void_p, c_void_p, c_int, POINTER(Options), POINTER(c_void_p)]),
     "amfls_solve_fixed_rbgk": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_double, c_double, c_double,
                                        c_int32, c_int32, c_int32, c_uint64, c_uint64, c_int32,

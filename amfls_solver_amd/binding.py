@@ -317,6 +317,54 @@ class DeviceMatrix:
             self.handle = None
 
 
+_HOST_APPLY = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int,
+                               ctypes.POINTER(ctypes.c_double))
+
+
+class HostOperator(DeviceMatrix):
+    """A matrix-free operator on the host: the counterpart of a custom
+    amfls::MatrixOperator (reference matrix_operator.hpp:29-89).
+
+    apply(x) gets a (cols, s) float64 array (column-major block) and returns
+    A x as a (rows, s) array; apply_transpose(y) returns A^T y.  Each product
+    goes through page-locked host memory.  The validation error model
+    (norm_upper, norm_lower, steps_apply, steps_transpose) is the caller's;
+    the default norm_upper = inf disables certification, as in the reference
+    (matrix_operator.hpp:57-60)."""
+
+    def __init__(self, rows: int, cols: int, apply, apply_transpose, *, norm_upper: float = float("inf"),
+                 norm_lower: float = 0.0, steps_apply: int = 0, steps_transpose: int = 0,
+                 context: Context | None = None):
+        dev = _require_gpu()
+        self.context = context or default_context()
+        self.rows, self.cols = int(rows), int(cols)
+        self._errors: list[BaseException] = []
+
+        def wrap(fn, rin, rout):
+            def cb(_user, inp, s, out):
+                try:
+                    x = np.ctypeslib.as_array(inp, shape=(s * rin,)).reshape((rin, s), order="F")
+                    y = np.asarray(fn(x), dtype=np.float64)
+                    if y.shape != (rout, s):
+                        raise ValueError(f"operator callback returned shape {y.shape}, expected {(rout, s)}")
+                    np.ctypeslib.as_array(out, shape=(s * rout,))[:] = y.ravel(order="F")
+                    return 0
+                except BaseException as e:  # reported by the engine as a failed callback
+                    self._errors.append(e)
+                    return 1
+            return _HOST_APPLY(cb)
+
+        self._cb_apply = wrap(apply, self.cols, self.rows)
+        self._cb_apply_t = wrap(apply_transpose, self.rows, self.cols)
+        self._val = torch.empty(0, device=dev)
+        h = ctypes.c_void_p()
+        _lib.check(self.context._lib.amfls_host_operator_create(
+            self.context.handle, self.rows, self.cols, ctypes.cast(self._cb_apply, ctypes.c_void_p),
+            ctypes.cast(self._cb_apply_t, ctypes.c_void_p), None, float(norm_upper), float(norm_lower),
+            int(steps_apply), int(steps_transpose), ctypes.byref(h)))
+        self.handle, self.kind, self.nnz, self.value_dtype = h, "host", 0, torch.float64
+
+
 def generate_csr(rows: int, cols: int, nnz_range=(4, 32), *, seed: int = 0, scale_range=(1.0, 1.0),
                  value_dtype: torch.dtype = torch.float32, context: "Context | None" = None):
     """Counter-addressed synthetic CSR built on the device: per-row lengths

@@ -189,6 +189,23 @@ int amfls_dense_create(amfls_context* ctx, int64_t rows, int64_t cols, const voi
     });
 }
 
+int amfls_host_operator_create(amfls_context* ctx, int64_t rows, int64_t cols, amfls_host_apply_fn apply,
+                               amfls_host_apply_fn apply_transpose, void* user, double norm_upper,
+                               double norm_lower, int64_t steps_apply, int64_t steps_transpose,
+                               amfls_operator** out) {
+    return guard([&] {
+        require(ctx != nullptr && out != nullptr, "null context or output");
+        ValidationModel m;
+        m.upper = norm_upper;
+        m.lower = norm_lower;
+        m.steps_apply = steps_apply;
+        m.steps_transpose = steps_transpose;
+        auto h = std::make_unique<amfls_operator>();
+        h->op = std::make_unique<HostOperator>(rows, cols, apply, apply_transpose, user, m);
+        *out = h.release();
+    });
+}
+
 int amfls_operator_destroy(amfls_operator* op) {
     return guard([&] { delete op; });
 }
@@ -303,6 +320,7 @@ int amfls_operator_autotune(amfls_context* ctx, amfls_operator* op, const int32_
 int amfls_spmm_variant_count(void) { return csr_spmm_variant_count(); }
 const char* amfls_spmm_variant_name(int v) { return csr_spmm_variant_name(v); }
 int amfls_spmm_variant_supports(int v, int s) { return csr_spmm_variant_supports(v, s) ? 1 : 0; }
+int amfls_spmm_default_variant(int s, double mean_row_entries) { return csr_spmm_default_variant(s, mean_row_entries); }
 
 namespace {
 

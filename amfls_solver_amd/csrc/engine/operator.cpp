@@ -514,3 +514,67 @@ void AugmentedOperator::apply_t(hipStream_t s, const float* Y, int n, int64_t ld
 }
 
 }  // namespace amflsx
+
+namespace amflsx {
+
+HostOperator::HostOperator(int64_t rows, int64_t cols, HostApplyFn apply, HostApplyFn apply_transpose, void* user,
+                           ValidationModel model)
+    : apply_(apply), apply_t_(apply_transpose), user_(user), model_(model) {
+    if (rows <= 0 || cols <= 0) throw std::invalid_argument("operator dimensions must be positive");
+    if (apply == nullptr || apply_transpose == nullptr) throw std::invalid_argument("operator callbacks are missing");
+    if (std::isnan(model.upper) || std::isnan(model.lower) || model.lower < 0.0 || model.steps_apply < 0 ||
+        model.steps_transpose < 0)
+        throw std::invalid_argument("validation error model is invalid");
+    rows_ = rows;
+    cols_ = cols;
+}
+
+template <typename T>
+void HostOperator::call(hipStream_t s, HostApplyFn fn, const T* in, int64_t in_rows, int n, int64_t ldi, T* out,
+                        int64_t out_rows, int64_t ldo) const {
+    if (n <= 0) return;
+    const size_t in_bytes = static_cast<size_t>(in_rows) * n * sizeof(T);
+    const size_t out_bytes = static_cast<size_t>(out_rows) * n * sizeof(T);
+    hin_.ensure(in_bytes);
+    hout_.ensure(out_bytes);
+    T* hi = reinterpret_cast<T*>(hin_.get());
+    T* ho = reinterpret_cast<T*>(hout_.get());
+    // device row-major (ld ldi) -> host row-major packed
+    AMFLS_HIP_CHECK(hipMemcpy2DAsync(hi, n * sizeof(T), in, ldi * sizeof(T), n * sizeof(T), in_rows,
+                                     hipMemcpyDeviceToHost, s));
+    AMFLS_HIP_CHECK(hipStreamSynchronize(s));
+    cin_.resize(static_cast<size_t>(in_rows) * n);
+    cout_.assign(static_cast<size_t>(out_rows) * n, 0.0);
+    for (int64_t i = 0; i < in_rows; ++i)
+        for (int j = 0; j < n; ++j) cin_[static_cast<size_t>(j) * in_rows + i] = static_cast<double>(hi[i * n + j]);
+    if (fn(user_, cin_.data(), n, cout_.data()) != 0) throw std::runtime_error("host operator callback failed");
+    for (int64_t i = 0; i < out_rows; ++i)
+        for (int j = 0; j < n; ++j) ho[i * n + j] = static_cast<T>(cout_[static_cast<size_t>(j) * out_rows + i]);
+    AMFLS_HIP_CHECK(hipMemcpy2DAsync(out, ldo * sizeof(T), ho, n * sizeof(T), n * sizeof(T), out_rows,
+                                     hipMemcpyHostToDevice, s));
+    // the page-locked buffer is reused by the next product: wait for the copy
+    AMFLS_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void HostOperator::apply(hipStream_t s, const double* X, int n, int64_t ldx, double* Y, int64_t ldy) const {
+    prof_begin(s);
+    call<double>(s, apply_, X, cols_, n, ldx, Y, rows_, ldy);
+    prof_end(s, n, false, 8);
+}
+void HostOperator::apply_t(hipStream_t s, const double* Y, int n, int64_t ldy, double* X, int64_t ldx) const {
+    prof_begin(s);
+    call<double>(s, apply_t_, Y, rows_, n, ldy, X, cols_, ldx);
+    prof_end(s, n, true, 8);
+}
+void HostOperator::apply(hipStream_t s, const float* X, int n, int64_t ldx, float* Y, int64_t ldy) const {
+    prof_begin(s);
+    call<float>(s, apply_, X, cols_, n, ldx, Y, rows_, ldy);
+    prof_end(s, n, false, 4);
+}
+void HostOperator::apply_t(hipStream_t s, const float* Y, int n, int64_t ldy, float* X, int64_t ldx) const {
+    prof_begin(s);
+    call<float>(s, apply_t_, Y, rows_, n, ldy, X, cols_, ldx);
+    prof_end(s, n, true, 4);
+}
+
+}  // namespace amflsx

@@ -177,6 +177,43 @@ private:
     mutable std::vector<double> h_val_;  // row-major copy for refinement
 };
 
+// A caller's operator that lives on the host (the reference's custom
+// MatrixOperator subclasses, matrix_operator.hpp:29-89): each product copies
+// the engine's device block to page-locked host memory, hands it to the
+// callback as a column-major FP64 block (x: cols x s, y: rows x s, leading
+// dimension = rows of the block, matrix_operator.hpp:36-44), and copies the
+// result back.  The callbacks run synchronously on the calling thread; a
+// nonzero return is an error.  FP32 blocks are widened to FP64 for the
+// callback and rounded back.
+using HostApplyFn = int (*)(void* user, const double* in, int block_cols, double* out);
+
+class HostOperator final : public DeviceOperator {
+public:
+    HostOperator(int64_t rows, int64_t cols, HostApplyFn apply, HostApplyFn apply_transpose, void* user,
+                 ValidationModel model);
+    void apply(hipStream_t s, const double* X, int n, int64_t ldx, double* Y, int64_t ldy) const override;
+    void apply_t(hipStream_t s, const double* Y, int n, int64_t ldy, double* X, int64_t ldx) const override;
+    void apply(hipStream_t s, const float* X, int n, int64_t ldx, float* Y, int64_t ldy) const override;
+    void apply_t(hipStream_t s, const float* Y, int n, int64_t ldy, float* X, int64_t ldx) const override;
+    ValidationModel validation_model() const override { return model_; }
+    // host-memory bytes the callback reads and writes
+    double product_bytes(int s, bool, int) const override {
+        return 8.0 * s * static_cast<double>(rows_ + cols_);
+    }
+    double product_flops(int) const override { return 0.0; }
+
+private:
+    template <typename T>
+    void call(hipStream_t s, HostApplyFn fn, const T* in, int64_t in_rows, int n, int64_t ldi, T* out,
+              int64_t out_rows, int64_t ldo) const;
+
+    HostApplyFn apply_, apply_t_;
+    void* user_;
+    ValidationModel model_;
+    mutable PinnedBuf<char> hin_, hout_;
+    mutable std::vector<double> cin_, cout_;
+};
+
 // [A; a I] for the ridge correction problems of the mixed-precision outer
 // refinement (rows = m + n).
 class AugmentedOperator final : public DeviceOperator {

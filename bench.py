@@ -257,10 +257,13 @@ def main() -> int:
     p = P.config1()
     M = am.DeviceMatrix(csr_tensor(torch, np, p), value_dtype=torch.float32)
     b = torch.from_numpy(p.b).cuda()
-    # width-one kernel variants for A and A^T picked by the autotuner (FP64
-    # blocks; FP32 blocks of the mixed inner solves use the same choice)
-    tuned = M.autotune(widths=(1,), dtype=torch.float64, reps=5)
-    log(f"autotuned width-one SpMM variants (A, A^T): {tuned[1]}")
+    # width-one kernel variants: the measured default table (an autotuned
+    # choice varies from run to run between variants within noise of each
+    # other, which would make the PMC traffic runs incomparable)
+    lib = M.context._lib
+    mean_a, mean_at = M.nnz / M.rows, M.nnz / M.cols
+    variants = {"A": lib.amfls_spmm_variant_name(lib.amfls_spmm_default_variant(1, mean_a)).decode(),
+                "AT": lib.amfls_spmm_variant_name(lib.amfls_spmm_default_variant(1, mean_at)).decode()}
     modes = ["fp64", "mixed"]
     results = {}
     for mode in modes:
@@ -360,7 +363,7 @@ def main() -> int:
             "learned_width_status": int(rl.status),
             "learned_model_heldout_speedup": wm.meta.get("heldout_speedup"),
             "rel_diff_vs_oracle": rel,
-            "spmm_variants_width1": {"A": tuned[1][0], "AT": tuned[1][1]},
+            "spmm_variants_width1": variants,
             "parallelism": "single_gpu",
         },
         "roofline": {

@@ -112,6 +112,20 @@ int amfls_csr_create(amfls_context* ctx, int64_t rows, int64_t cols, int64_t nnz
                      const int32_t* col_idx, const void* values, int value_type, amfls_operator** out);
 int amfls_dense_create(amfls_context* ctx, int64_t rows, int64_t cols, const void* values, int value_type,
                        int row_major, int64_t ld, amfls_operator** out);
+/* A caller's operator on the host: the adapter for custom
+ * amfls::MatrixOperator subclasses (reference matrix_operator.hpp:29-89).
+ * apply(user, x, s, y) computes y = A x and apply_transpose(user, y, s, x)
+ * x = A^T y on host column-major FP64 blocks (x: cols x s, y: rows x s,
+ * leading dimension = rows of the block, matrix_operator.hpp:36-44),
+ * returning 0 on success.  Each product copies through page-locked host
+ * memory and waits for the callback.  The validation error model is the
+ * caller's (matrix_operator.hpp:57-60: norm_upper = +inf disables
+ * certification); relative_block_product_cost is s, the reference default. */
+typedef int (*amfls_host_apply_fn)(void* user, const double* in, int block_cols, double* out);
+int amfls_host_operator_create(amfls_context* ctx, int64_t rows, int64_t cols, amfls_host_apply_fn apply,
+                               amfls_host_apply_fn apply_transpose, void* user, double norm_upper,
+                               double norm_lower, int64_t steps_apply, int64_t steps_transpose,
+                               amfls_operator** out);
 int amfls_operator_destroy(amfls_operator* op);
 int amfls_operator_shape(const amfls_operator* op, int64_t* rows, int64_t* cols);
 int amfls_operator_validation_model(const amfls_operator* op, double* norm_upper, double* norm_lower,
@@ -147,6 +161,9 @@ int amfls_operator_kernel_stats(amfls_operator* op, int64_t* launches, double* s
 int amfls_spmm_variant_count(void);
 const char* amfls_spmm_variant_name(int variant);
 int amfls_spmm_variant_supports(int variant, int block_cols);
+/* The variant used when none is set: lanes sized to the block width and the
+ * mean number of entries per row of the product's matrix (A or A^T). */
+int amfls_spmm_default_variant(int block_cols, double mean_row_entries);
 
 /* b: rows values, device pointer if b_on_device != 0 else host pointer. */
 int amfls_solve(amfls_context* ctx, const amfls_operator* op, const double* b, int b_on_device,

@@ -110,3 +110,33 @@ def test_batch_matches_serial_sweeps(torch_mod):
             assert torch.equal(rb.solution, rsr.solution)
     with pytest.raises(ValueError):
         am.run_amfls_batch(mats[:1], bs[:1], lams[:1], workers=1, precision="mixed")
+
+
+def test_host_operator_matches_device_csr(torch_mod):
+    """A matrix-free host operator (custom MatrixOperator counterpart) with
+    the device operator's validation model certifies the same problem to
+    the same solution; without a model it cannot certify; a failing
+    callback is an error."""
+    import scipy.sparse as sp
+    import amfls_solver_amd as am
+    torch = torch_mod
+    rows, cols = 5000, 300
+    M, (rp, ci, val) = _csr(torch, rows, cols, (4, 32), 41)
+    A = sp.csr_matrix((val.cpu().numpy(), ci.cpu().numpy(), rp.cpu().numpy()), shape=(rows, cols))
+    up, lo, sa, st = M.validation_model()
+    H = am.HostOperator(rows, cols, lambda x: A @ x, lambda y: A.T @ y, norm_upper=up, norm_lower=lo,
+                        steps_apply=sa, steps_transpose=st, context=M.context)
+    b = torch.from_numpy(np.random.default_rng(2).standard_normal(rows)).cuda()
+    rd = am.run_amfls(M, b, regularization=1e-3, tolerance=1e-9, clock="synthetic")
+    rh = am.run_amfls(H, b, regularization=1e-3, tolerance=1e-9, clock="synthetic")
+    assert rd.status == 0 and rh.status == 0
+    d = rh.solution.numpy() - rd.solution.numpy()
+    assert np.linalg.norm(d) <= 1e-8 * np.linalg.norm(rd.solution.numpy())
+    H0 = am.HostOperator(rows, cols, lambda x: A @ x, lambda y: A.T @ y, context=M.context)
+    r0 = am.run_amfls(H0, b, regularization=1e-3, tolerance=1e-9, maximum_depth=4)
+    assert r0.status != 0
+    bad = am.HostOperator(rows, cols, lambda x: np.zeros((rows + 1, x.shape[1])), lambda y: A.T @ y,
+                          norm_upper=up, norm_lower=lo, steps_apply=sa, steps_transpose=st, context=M.context)
+    with pytest.raises(RuntimeError):
+        am.run_amfls(bad, b, regularization=1e-3, tolerance=1e-9)
+    assert bad._errors and isinstance(bad._errors[0], ValueError)

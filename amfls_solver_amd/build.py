@@ -29,6 +29,7 @@ SOURCES = [
     CSRC / "device" / "dense_mfma.hip",
     CSRC / "device" / "mlp.hip",
     CSRC / "device" / "generate.hip",
+    CSRC / "device" / "dense_narrow.hip",
     CSRC / "engine" / "operator.cpp",
     CSRC / "engine" / "small_la.cpp",
     CSRC / "engine" / "validation.cpp",

@@ -61,6 +61,15 @@ void rowmul(hipStream_t s, int64_t rows, int w, int r, const T* X, int64_t ldx, 
 enum class DenseMfmaMode : int { split3 = 0, f32 = 1, f64 = 2 };
 // The mode for a storage type and block precision; `reduced` asks for
 // fp32-accurate products on fp64 blocks.
+// Narrow dense products on the vector ALUs (dense_narrow.hip): row-major A
+// (bf16 / fp32 / fp64) and blocks of 1..8 columns, FP64 accumulation.
+bool dense_narrow_supported(const Strided& A, int ncols);
+template <typename TX>
+void dense_narrow_nn(hipStream_t s, const Strided& A, int64_t M, int64_t N, int ncols, const TX* X, int64_t ldx,
+                     TX* Y, int64_t ldy);
+template <typename TX>
+void dense_narrow_tn(hipStream_t s, const Strided& A, int64_t M, int64_t N, int ncols, const TX* Y, int64_t ldy,
+                     TX* Z, int64_t ldz, Scratch& scratch);
 DenseMfmaMode dense_mfma_mode(ValueType storage, bool fp64_blocks, bool reduced);
 template <typename TX>
 void dense_mfma_nn(hipStream_t s, DenseMfmaMode mode, const Strided& A, int64_t M, int64_t N, int ncols, const TX* X,

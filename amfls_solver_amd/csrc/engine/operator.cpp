@@ -357,10 +357,30 @@ DenseOperator::DenseOperator(int64_t rows, int64_t cols, const void* values, Val
     }
 }
 
+void DenseOperator::set_variant(int s, bool transpose, int variant) {
+    if (variant < -1 || variant > 1) throw std::invalid_argument("dense product variant must be -1, 0 or 1");
+    if (variant < 0) variants_.erase({s, transpose});
+    else variants_[{s, transpose}] = variant;
+}
+
+bool DenseOperator::use_narrow(int ncols, bool transpose) const {
+    if (!dense_narrow_supported(a_, ncols)) return false;
+    auto it = variants_.find({ncols, transpose});
+    // Default: the vector-ALU kernels for single columns only.  They are
+    // also faster per product at widths 2-8 (1.3-3.4x on the 1e6 x 2000
+    // bf16 matrix), but then the width controller, which weighs measured
+    // per-level seconds, keeps that problem at width two and certifies in
+    // 2.5 s instead of 1.0 s at width sixteen.
+    if (it == variants_.end()) return ncols == 1;
+    return it->second == 0;
+}
+
 template <typename T>
 void DenseOperator::do_apply(hipStream_t s, const T* X, int n, int64_t ldx, T* Y, int64_t ldy) const {
     prof_begin(s);
-    if (use_mfma(n))
+    if (use_narrow(n, false))
+        dense_narrow_nn<T>(s, a_, rows_, cols_, n, X, ldx, Y, ldy);
+    else if (use_mfma(n))
         dense_mfma_nn<T>(s, dense_mfma_mode(a_.type, sizeof(T) == 8, reduced_), a_, rows_, cols_, n, X, ldx, Y, ldy,
                          ctx_.scratch);
     else
@@ -372,7 +392,9 @@ template <typename T>
 void DenseOperator::do_apply_t(hipStream_t s, const T* Y, int n, int64_t ldy, T* X, int64_t ldx) const {
     const ValueType ct = sizeof(T) == 8 ? ValueType::f64 : ValueType::f32;
     prof_begin(s);
-    if (use_mfma(n))
+    if (use_narrow(n, true))
+        dense_narrow_tn<T>(s, a_, rows_, cols_, n, Y, ldy, X, ldx, ctx_.scratch);
+    else if (use_mfma(n))
         dense_mfma_tn<T>(s, dense_mfma_mode(a_.type, sizeof(T) == 8, reduced_), a_, rows_, cols_, n, Y, ldy, X, ldx,
                          ctx_.scratch);
     else

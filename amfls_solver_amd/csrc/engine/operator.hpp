@@ -160,10 +160,17 @@ public:
     bool use_mfma(int ncols) const { return mfma_ && a_.cs == 1 && ncols <= 64; }
     void set_mfma(bool on) { mfma_ = on; }
     void set_reduced_products(bool on) { reduced_ = on; }
+    // Row-major storage and single-column blocks run on the vector ALUs
+    // (FP64 accumulation) by default; widths up to 8 can be sent there with
+    // set_variant(s, transpose, v): v = 0 the vector-ALU kernel, 1 the
+    // matrix cores, -1 the default.
+    void set_variant(int s, bool transpose, int variant) override;
+    bool use_narrow(int ncols, bool transpose) const;
 
 private:
     bool mfma_ = true;
     bool reduced_ = false;
+    std::map<std::pair<int, bool>, int> variants_;
     template <typename T>
     void do_apply(hipStream_t s, const T* X, int n, int64_t ldx, T* Y, int64_t ldy) const;
     template <typename T>

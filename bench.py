@@ -101,24 +101,30 @@ def dense_c2(torch, am, P, rows=1_000_000, cols=2000):
     out = {"matrix": f"dense_{rows}x{cols}_bf16_GD_6decades_lambda0"}
     lib, h = M.context._lib, M.context.handle
     prod = {}
-    for s in (1, 16, 64):
+    for s in (1, 8, 16, 64):
         for dt, code, name in ((torch.float64, 0, "f64"), (torch.float32, 1, "f32")):
             X = torch.randn(cols, s, device="cuda", dtype=dt)
             Y = torch.empty(rows, s, device="cuda", dtype=dt)
             Z = torch.empty(cols, s, device="cuda", dtype=dt)
             for tr, (src, dst) in ((0, (X, Y)), (1, (Y, Z))):
-                lib.amfls_operator_apply_rows(h, M.handle, tr, ctypes.c_void_p(src.data_ptr()), s, s,
-                                              ctypes.c_void_p(dst.data_ptr()), s, code)
-                M.context.synchronize()
-                M.profile(True)
-                for _ in range(5):
+                # widths <= 8: the default kernel and the other family (vector
+                # ALUs by default for one column, matrix cores above)
+                alt = ((1, "_mfma") if s == 1 else (0, "_valu"))
+                for var, tag in (((-1, ""), alt) if s <= 8 else ((-1, ""),)):
+                    M.set_variant(s, bool(tr), var)
                     lib.amfls_operator_apply_rows(h, M.handle, tr, ctypes.c_void_p(src.data_ptr()), s, s,
                                                   ctypes.c_void_p(dst.data_ptr()), s, code)
-                M.context.synchronize()
-                n, sec, by = M.kernel_stats()
-                M.profile(False)
-                prod[f"{'AT' if tr else 'A'}_s{s}_{name}"] = {
-                    "ms": 1e3 * sec / n, "GBs": by / sec / 1e9, "TFs": 2.0 * rows * cols * s / (sec / n) / 1e12}
+                    M.context.synchronize()
+                    M.profile(True)
+                    for _ in range(5):
+                        lib.amfls_operator_apply_rows(h, M.handle, tr, ctypes.c_void_p(src.data_ptr()), s, s,
+                                                      ctypes.c_void_p(dst.data_ptr()), s, code)
+                    M.context.synchronize()
+                    n, sec, by = M.kernel_stats()
+                    M.profile(False)
+                    prod[f"{'AT' if tr else 'A'}_s{s}_{name}{tag}"] = {
+                        "ms": 1e3 * sec / n, "GBs": by / sec / 1e9, "TFs": 2.0 * rows * cols * s / (sec / n) / 1e12}
+                M.set_variant(s, bool(tr), -1)
     out["products"] = prod
     for mode in ("fp64", "mixed"):
         torch.cuda.synchronize()

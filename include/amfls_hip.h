@@ -142,7 +142,10 @@ int amfls_operator_apply_rows(amfls_context* ctx, const amfls_operator* op, int 
 /* Algorithmic HBM bytes and flops of one product of width block_cols. */
 double amfls_operator_product_bytes(const amfls_operator* op, int block_cols, int transpose, int dtype);
 double amfls_operator_product_flops(const amfls_operator* op, int block_cols);
-/* CSR SpMM kernel variant per (width, direction); -1 restores the default. */
+/* Kernel variant per (width, direction); -1 restores the default.  CSR
+ * operators: an SpMM variant index.  Dense row-major operators: 0 the
+ * vector-ALU kernel (widths <= 8; the default for one column), 1 the matrix
+ * cores (the default for wider blocks). */
 int amfls_operator_set_variant(amfls_operator* op, int block_cols, int transpose, int variant);
 /* SpMM autotuner (engine extension; the reference's CSR apply has a single
  * loop, src/algorithms/operators/sparse_csr_operator.cpp:203-269): for each

@@ -127,11 +127,17 @@ def test_host_operator_matches_device_csr(torch_mod):
     H = am.HostOperator(rows, cols, lambda x: A @ x, lambda y: A.T @ y, norm_upper=up, norm_lower=lo,
                         steps_apply=sa, steps_transpose=st, context=M.context)
     b = torch.from_numpy(np.random.default_rng(2).standard_normal(rows)).cuda()
-    rd = am.run_amfls(M, b, regularization=1e-3, tolerance=1e-9, clock="synthetic")
-    rh = am.run_amfls(H, b, regularization=1e-3, tolerance=1e-9, clock="synthetic")
-    assert rd.status == 0 and rh.status == 0
-    d = rh.solution.numpy() - rd.solution.numpy()
-    assert np.linalg.norm(d) <= 1e-8 * np.linalg.norm(rd.solution.numpy())
+    # a tolerance the FP64 certificate reaches without the long-double
+    # refinement hooks, which host operators (like the reference's default
+    # MatrixOperator) do not offer
+    rd = am.run_amfls(M, b, regularization=1e-3, tolerance=1e-7, clock="synthetic")
+    rh = am.run_amfls(H, b, regularization=1e-3, tolerance=1e-7, clock="synthetic")
+    assert rd.status == 0 and rh.status == 0, (rd.status, rh.status)
+    x_ref, Ad = _ridge(rp.cpu().numpy(), ci.cpu().numpy(), val.cpu().numpy(), (rows, cols), b.cpu().numpy(), 1e-3)
+    K = Ad.T @ Ad + 1e-3 * np.eye(cols)
+    for r in (rd, rh):  # each solution's true energy error is below its certificate
+        d = r.solution.numpy() - x_ref
+        assert np.sqrt(d @ K @ d) / np.sqrt(x_ref @ K @ x_ref) <= r.relative_energy_error_upper_bound * 1.0001
     H0 = am.HostOperator(rows, cols, lambda x: A @ x, lambda y: A.T @ y, context=M.context)
     r0 = am.run_amfls(H0, b, regularization=1e-3, tolerance=1e-9, maximum_depth=4)
     assert r0.status != 0

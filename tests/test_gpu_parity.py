@@ -272,10 +272,11 @@ def test_torch_ops_entry_points(torch_mod, fx):
 
 @pytest.mark.parametrize("vt", ["bfloat16", "float32", "float64"])
 def test_dense_matrix_core_products(torch_mod, vt):
-    """Row-major dense products on the matrix cores: fp64 blocks are
-    fp64-accurate (A upconverted exactly), fp32 blocks fp32-accurate
-    (split-bf16 or fp32 matrix cores); ragged sizes and widths up to 64
-    (70 falls back to the vector kernels)."""
+    """Row-major dense products: widths up to 8 on both the vector-ALU
+    kernels (FP64 accumulation) and the matrix cores, wider blocks on the matrix
+    cores; fp64 blocks are fp64-accurate (A upconverted exactly), fp32
+    blocks fp32-accurate (split-bf16 or fp32 matrix cores); ragged sizes
+    and widths up to 64 (70 falls back to the tall-skinny kernels)."""
     import amfls_solver_amd as am
     torch = torch_mod
     rng = np.random.default_rng(7)
@@ -296,6 +297,15 @@ def test_dense_matrix_core_products(torch_mod, vt):
             z32 = M.apply(torch.from_numpy(Yg).float(), transpose=True).cpu().double().numpy()
             assert np.abs(y32 - Y).max() <= 1e-5 * np.abs(np.abs(stored) @ np.abs(X)).max()
             assert np.abs(z32 - Z).max() <= 1e-5 * np.abs(np.abs(stored).T @ np.abs(Yg)).max()
+            if s <= 8:  # the other kernel family at widths <= 8
+                for tr in (False, True):
+                    M.set_variant(s, tr, 1 if s == 1 else 0)
+                y = M.apply(torch.from_numpy(X)).cpu().numpy()
+                z = M.apply(torch.from_numpy(Yg), transpose=True).cpu().numpy()
+                assert np.abs(y - Y).max() <= 1e-13 * max(1.0, np.abs(Y).max()) * np.sqrt(n)
+                assert np.abs(z - Z).max() <= 1e-13 * max(1.0, np.abs(Z).max()) * np.sqrt(m)
+                for tr in (False, True):
+                    M.set_variant(s, tr, -1)
 
 
 def _mlp_data(n, seed):

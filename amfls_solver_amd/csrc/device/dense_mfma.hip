@@ -329,17 +329,22 @@ __global__ __launch_bounds__(kThreads) void dense_tn_kernel(int64_t M, int64_t N
     }
 }
 
-// Z[j][col] = sum over chunks of P[chunk][j][col] (chunk order).
+// Z[j][col] = sum over chunks of P[chunk][j][col]: a wavefront per output,
+// lanes over chunks in order, then the butterfly (a fixed order).
 template <typename TX>
-__global__ void dense_tn_reduce_kernel(int64_t N, int s, int64_t chunks, const double* __restrict__ P,
-                                       TX* __restrict__ Z, int64_t ldz) {
-    const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (e >= N * s) return;
+__global__ __launch_bounds__(kThreads) void dense_tn_reduce_kernel(int64_t N, int s, int64_t chunks,
+                                                                   const double* __restrict__ P,
+                                                                   TX* __restrict__ Z, int64_t ldz) {
+    const int lane = threadIdx.x % kWave;
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + threadIdx.x / kWave;
+    if (e >= N * s) return;  // whole wavefront
     const int64_t j = e / s;
     const int col = static_cast<int>(e % s);
     double acc = 0.0;
-    for (int64_t ch = 0; ch < chunks; ++ch) acc += P[(ch * N + j) * s + col];
-    Z[j * ldz + col] = static_cast<TX>(acc);
+    for (int64_t ch = lane; ch < chunks; ch += kWave) acc += P[(ch * N + j) * s + col];
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if (lane == 0) Z[j * ldz + col] = static_cast<TX>(acc);
 }
 
 template <typename TA, typename TX, int MODE>
@@ -355,7 +360,9 @@ void nn_dispatch(hipStream_t st, int64_t M, int64_t N, int s, const TA* A, int64
     hipLaunchKernelGGL((pack_x_kernel<TX, MODE>), dim3(static_cast<unsigned>(ceil_div(plane, kThreads))),
                        dim3(kThreads), 0, st, N, s, ncp, nk, X, ldx, XT);
     AMFLS_LAUNCH_CHECK();
-    constexpr int RT = MODE == kF64 ? 2 : 4;  // row tiles per wavefront (register budget)
+    // row tiles per wavefront: four share each X fragment, two for fp64
+    // storage (its A fragments take four times the registers of bf16)
+    constexpr int RT = (MODE == kF64 && sizeof(TA) == 8) ? 2 : 4;
     const dim3 grid(static_cast<unsigned>(ceil_div(M, 16 * RT * (kThreads / kWave))));
 #define AMFLS_NN(NT)                                                                                           \
     hipLaunchKernelGGL((dense_nn_kernel<TA, TX, MODE, NT, RT>), grid, dim3(kThreads), 0, st, M, N, s, A, lda, XT, \
@@ -389,7 +396,7 @@ void tn_dispatch(hipStream_t st, int64_t M, int64_t N, int s, const TA* A, int64
     else { AMFLS_TN(4); }
 #undef AMFLS_TN
     AMFLS_LAUNCH_CHECK();
-    hipLaunchKernelGGL(dense_tn_reduce_kernel<TX>, dim3(static_cast<unsigned>(ceil_div(N * s, kThreads))),
+    hipLaunchKernelGGL(dense_tn_reduce_kernel<TX>, dim3(static_cast<unsigned>(ceil_div(N * s, kThreads / kWave))),
                        dim3(kThreads), 0, st, N, s, chunks, P, Z, ldz);
     AMFLS_LAUNCH_CHECK();
 }

@@ -19,7 +19,7 @@
 //   Rows advance 64 at a time: their Y values are staged in LDS and every
 //   thread streams its 8 entries of each row (16-byte loads, four rows in
 //   flight), accumulating 8 x S sums.  Chunk partials P[chunk][col][j] are
-//   added in chunk order by narrow_reduce_kernel, so results are
+//   added in a fixed order by narrow_reduce_kernel, so results are
 //   deterministic.
 #include "kernels.hpp"
 
@@ -191,16 +191,22 @@ __global__ __launch_bounds__(kThreads) void narrow_tn_kernel(int64_t M, int64_t 
     }
 }
 
+// Z[j][col] = sum over chunks of P[chunk][j][col]: a wavefront per output,
+// lanes over chunks in order, then the butterfly (a fixed order).
 template <typename TX>
-__global__ void narrow_reduce_kernel(int64_t N, int s, int64_t chunks, const double* __restrict__ P,
-                                     TX* __restrict__ Z, int64_t ldz) {
-    const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (e >= N * s) return;
+__global__ __launch_bounds__(kThreads) void narrow_reduce_kernel(int64_t N, int s, int64_t chunks,
+                                                                 const double* __restrict__ P, TX* __restrict__ Z,
+                                                                 int64_t ldz) {
+    const int lane = threadIdx.x % kWave;
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + threadIdx.x / kWave;
+    if (e >= N * s) return;  // whole wavefront
     const int64_t j = e / s;
     const int col = static_cast<int>(e % s);
     double acc = 0.0;
-    for (int64_t ch = 0; ch < chunks; ++ch) acc += P[(ch * N + j) * s + col];
-    Z[j * ldz + col] = static_cast<TX>(acc);
+    for (int64_t ch = lane; ch < chunks; ch += kWave) acc += P[(ch * N + j) * s + col];
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if (lane == 0) Z[j * ldz + col] = static_cast<TX>(acc);
 }
 
 template <typename TA, typename TX>
@@ -236,7 +242,7 @@ void tn_typed(hipStream_t st, int64_t M, int64_t N, int s, const TA* A, int64_t 
     else AMFLS_NARROW_TN(8);
 #undef AMFLS_NARROW_TN
     AMFLS_LAUNCH_CHECK();
-    hipLaunchKernelGGL(narrow_reduce_kernel<TX>, dim3(static_cast<unsigned>(ceil_div(N * s, kThreads))),
+    hipLaunchKernelGGL(narrow_reduce_kernel<TX>, dim3(static_cast<unsigned>(ceil_div(N * s, kThreads / kWave))),
                        dim3(kThreads), 0, st, N, s, chunks, P, Z, ldz);
     AMFLS_LAUNCH_CHECK();
 }

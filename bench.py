@@ -277,7 +277,6 @@ def main() -> int:
             am.run_amfls(M, b, regularization=p.regularization, tolerance=1e-8, precision=mode)
         log(f"warm {mode}")
     for mode in modes:
-        M.profile(mode == a.headline)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         last = None
@@ -286,9 +285,19 @@ def main() -> int:
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / a.steps
         if mode == a.headline:
+            # the same K solves again with every product's kernel timed by its
+            # dispatch timestamps (kept out of the headline timing: recording
+            # and resolving the event pairs costs host time)
+            M.profile(True)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            for _ in range(a.steps):
+                am.run_amfls(M, b, regularization=p.regularization, tolerance=1e-8, precision=mode)
+            torch.cuda.synchronize()
+            dtp = (time.perf_counter() - t1) / a.steps
             n, sec, by = M.kernel_stats()
             M.profile(False)
-            results["kernel"] = (n, sec, by)
+            results["kernel"] = (n, sec, by, dtp)
         results[mode] = (dt, last)
         log(f"{mode}: {dt:.4f} s/solve status {last.status} cert {last.relative_energy_error_upper_bound:.3e} "
             f"depth {last.depth} width {last.auxiliary_width} rank {last.basis_rank} refine {last.refinement_steps}")
@@ -308,7 +317,7 @@ def main() -> int:
 
     dt, r = results[a.headline]
     other = "mixed" if a.headline == "fp64" else "fp64"
-    n, sec, by = results["kernel"]
+    n, sec, by, dt_profiled = results["kernel"]
     achieved = by / sec / 1e9 if sec > 0 else 0.0
     sol = {m: results[m][1].solution.numpy() for m in modes}
 
@@ -382,6 +391,8 @@ def main() -> int:
             "kernel": "csr_spmm (A and A^T products of the timed solves)",
             "launches": n,
             "avg_us": 1e6 * sec / max(n, 1),
+            "timed_pass": f"{a.steps} further solves of the workload with per-product dispatch-timestamp events "
+                          f"({1e3 * dt_profiled:.2f} ms per solve with the events)",
             "note": "working set fits the 256 MB last-level cache; see spmm_sweep for an HBM-resident matrix",
         },
         "cpu_baseline": cpu,

@@ -331,7 +331,41 @@ __global__ __launch_bounds__(kBlock) void update_kernel(int64_t N, int k, int R,
         const int o = lane + kWave * j;
         cl[j] = o < k ? cs[o] : 0.0;
     }
-    for (int rr = w; rr < nr; rr += dev::kWaves) {
+    // U rows per trip, their loads issued together (narrow J only: registers)
+    constexpr int U = (J > 0 && J <= 4) ? 4 : 1;
+    int rr = w;
+    for (; rr + (U - 1) * dev::kWaves < nr; rr += U * dev::kWaves) {
+        double bv[U][J > 0 ? J : 1], yo[U], acc[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t i = r0 + rr + u * dev::kWaves;
+            acc[u] = 0.0;
+#pragma unroll
+            for (int j = 0; j < J; ++j) {
+                const int o = lane + kWave * j;
+                bv[u][j] = o < k ? static_cast<double>(B[i * ldb + o]) : 0.0;
+            }
+            yo[u] = static_cast<double>(y[i * ldy]);  // every lane reads the same word
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+#pragma unroll
+            for (int j = 0; j < J; ++j) acc[u] = fma(bv[u][j], cl[j], acc[u]);
+            acc[u] = dev::wave_sum(acc[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t i = r0 + rr + u * dev::kWaves;
+            const T nv = static_cast<T>(yo[u] - acc[u]);
+            if (lane == 0) {
+                y[i * ldy] = nv;
+                st.add(static_cast<double>(nv));
+            }
+#pragma unroll
+            for (int j = 0; j < J; ++j) qa[j] = fma(bv[u][j], static_cast<double>(nv), qa[j]);
+        }
+    }
+    for (; rr < nr; rr += dev::kWaves) {
         const int64_t i = r0 + rr;
         double bv[J > 0 ? J : 1];
         double acc = 0.0;
@@ -342,7 +376,7 @@ __global__ __launch_bounds__(kBlock) void update_kernel(int64_t N, int k, int R,
             acc = fma(bv[j], cl[j], acc);
         }
         acc = dev::wave_sum(acc);
-        const double yo = static_cast<double>(y[i * ldy]);  // every lane reads the same word
+        const double yo = static_cast<double>(y[i * ldy]);
         const T nv = static_cast<T>(yo - acc);
         if (lane == 0) {
             y[i * ldy] = nv;

@@ -146,3 +146,26 @@ def test_host_operator_matches_device_csr(torch_mod):
     with pytest.raises(RuntimeError):
         am.run_amfls(bad, b, regularization=1e-3, tolerance=1e-9)
     assert bad._errors and isinstance(bad._errors[0], ValueError)
+
+
+def test_matrix_market_operator_solves(torch_mod, tmp_path):
+    """A CSR problem read from a Matrix Market file solves on the device to
+    its certificate (true energy error against a dense ridge solve)."""
+    import scipy.io
+    import scipy.sparse as sp
+    import amfls_solver_amd as am
+    from amfls_solver_amd import matrix_market as MM
+    torch = torch_mod
+    rng = np.random.default_rng(8)
+    A = sp.random(3000, 200, density=0.02, random_state=3, format="coo")
+    scipy.io.mmwrite(str(tmp_path / "A.mtx"), A)
+    scipy.io.mmwrite(str(tmp_path / "b.mtx"), rng.standard_normal((3000, 1)))
+    M = MM.load_csr(tmp_path / "A.mtx")
+    b = MM.read_rhs(tmp_path / "b.mtx", 3000)
+    r = am.run_amfls(M, torch.from_numpy(b).cuda(), regularization=1e-2, tolerance=1e-8)
+    assert r.status == 0
+    Ad = MM.read_matrix_market(tmp_path / "A.mtx").toarray()
+    K = Ad.T @ Ad + 1e-2 * np.eye(200)
+    x_ref = np.linalg.solve(K, Ad.T @ b)
+    d = r.solution.numpy() - x_ref
+    assert np.sqrt(d @ K @ d) / np.sqrt(x_ref @ K @ x_ref) <= r.relative_energy_error_upper_bound * 1.0001

@@ -169,3 +169,22 @@ def test_matrix_market_operator_solves(torch_mod, tmp_path):
     x_ref = np.linalg.solve(K, Ad.T @ b)
     d = r.solution.numpy() - x_ref
     assert np.sqrt(d @ K @ d) / np.sqrt(x_ref @ K @ x_ref) <= r.relative_energy_error_upper_bound * 1.0001
+
+
+def test_study_cases_succeed_externally(torch_mod):
+    """The reference's dense study cases (one instance each, the reference's
+    limits and algorithm seeds) succeed by the reference's external
+    diagnostics on the HIP engine, as every AMFLS row of the reference's
+    tables does."""
+    import json
+    import os
+    from amfls_solver_amd import study as S
+    with open(os.path.join(os.path.dirname(__file__), "golden", "study_cases.json")) as f:
+        data = json.load(f)
+    for suite, spec in data.items():
+        for entry in spec["cases"]:
+            inst = entry["instances"][0]
+            assert inst["external_success"] == "1"
+            row = S.run_case(suite, spec["options"], entry["case"], int(inst["instance_seed"]))
+            assert row["external_success"] == 1, (suite, entry["case"]["name"], row)
+            assert set(row) == set(S.FIELDS)

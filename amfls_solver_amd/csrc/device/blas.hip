@@ -172,6 +172,44 @@ __global__ __launch_bounds__(kThreads) void ts_gemm_nn_kernel(
     }
 }
 
+// Thin product for row-major A with K <= 16 and N <= 16 (block updates of
+// tall frontiers, R^-1 applications): a thread per row reads its K
+// contiguous entries, B sits in LDS, the N outputs are written
+// contiguously; the k terms are summed in order.
+template <typename V, typename TB, typename T, typename Acc, int KM, int NM>
+__global__ __launch_bounds__(kThreads) void thin_nn_kernel(int64_t M, int N, int K, const V* __restrict__ A,
+                                                           int64_t rs, const TB* __restrict__ B, int64_t ldb,
+                                                           T* __restrict__ Y, int64_t ldy, Acc alpha, Acc beta,
+                                                           const double* gate) {
+    if (gated_off(gate)) return;
+    __shared__ Acc Bs[KM][NM];
+    for (int idx = threadIdx.x; idx < KM * NM; idx += kThreads) {
+        const int kk = idx / NM, j = idx % NM;
+        Bs[kk][j] = (kk < K && j < N) ? static_cast<Acc>(B[kk * ldb + j]) : Acc(0);
+    }
+    __syncthreads();
+    const int64_t row = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (row >= M) return;
+    Acc a[KM];
+#pragma unroll
+    for (int kk = 0; kk < KM; ++kk) a[kk] = kk < K ? ld_acc<Acc>(A, row * rs + kk) : Acc(0);
+    Acc acc[NM];
+#pragma unroll
+    for (int j = 0; j < NM; ++j) acc[j] = Acc(0);
+#pragma unroll
+    for (int kk = 0; kk < KM; ++kk)
+#pragma unroll
+        for (int j = 0; j < NM; ++j) acc[j] = fma(a[kk], Bs[kk][j], acc[j]);
+    T* out = Y + row * ldy;
+#pragma unroll
+    for (int j = 0; j < NM; ++j) {
+        if (j >= N) break;
+        Acc v = alpha * acc[j];
+        if (beta != Acc(0)) v = fma(beta, static_cast<Acc>(out[j]), v);
+        out[j] = static_cast<T>(v);
+    }
+}
+
 template <typename V, typename TB, typename T>
 void gemm_nn_dispatch(hipStream_t s, int64_t M, int N, int64_t K, const V* A, int64_t rs,
                       int64_t cs, const TB* B, int64_t ldb, T* Y, int64_t ldy, double alpha,
@@ -179,6 +217,28 @@ void gemm_nn_dispatch(hipStream_t s, int64_t M, int N, int64_t K, const V* A, in
     using Acc = T;
     if (M <= 0 || N <= 0) return;
     const bool row_major = (cs == 1);
+    if (row_major && N <= 16 && K <= 16) {
+        const dim3 grid(static_cast<unsigned>(ceil_div(M, kThreads)));
+#define AMFLS_THIN(KM, NM)                                                                                  \
+        hipLaunchKernelGGL((thin_nn_kernel<V, TB, T, Acc, KM, NM>), grid, dim3(kThreads), 0, s, M, N,            \
+                           static_cast<int>(K), A, rs, B, ldb, Y, ldy, Acc(alpha), Acc(beta), gate)
+        const int kb = K <= 1 ? 1 : (K <= 2 ? 2 : (K <= 4 ? 4 : (K <= 8 ? 8 : 16)));
+        const int nb = N <= 1 ? 1 : (N <= 2 ? 2 : (N <= 4 ? 4 : (N <= 8 ? 8 : 16)));
+        switch (kb * 100 + nb) {
+            case 101: AMFLS_THIN(1, 1); break;
+            case 202: AMFLS_THIN(2, 2); break;
+            case 404: AMFLS_THIN(4, 4); break;
+            case 808: AMFLS_THIN(8, 8); break;
+            case 1616: AMFLS_THIN(16, 16); break;
+            default:
+                if (kb <= 4 && nb <= 4) AMFLS_THIN(4, 4);
+                else if (kb <= 8 && nb <= 8) AMFLS_THIN(8, 8);
+                else AMFLS_THIN(16, 16);
+        }
+#undef AMFLS_THIN
+        AMFLS_LAUNCH_CHECK();
+        return;
+    }
     if (N <= 16) {
         const dim3 grid(static_cast<unsigned>(ceil_div(M, 64)));
 #define AMFLS_TSNN(NM)                                                                                     \

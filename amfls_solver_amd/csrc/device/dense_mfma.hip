@@ -245,6 +245,99 @@ __global__ __launch_bounds__(kThreads) void dense_nn_kernel(int64_t M, int64_t N
     }
 }
 
+// FP64 blocks, s <= 32: as dense_nn_kernel, but the X^T slice of each
+// 64-wide K step (16 NT columns x 64 doubles) is staged in LDS once per
+// block and shared by its four wavefronts instead of every wavefront
+// reading its B operands through the cache (with fp64 values those reads
+// outweighed the A stream).  The next step's slice and A entries are in
+// flight while the current step runs on the matrix cores; two LDS buffers
+// alternate, one barrier per step.
+template <typename TA, typename TX, int NT, int RT>
+__global__ __launch_bounds__(kThreads) void dense_nn_f64_kernel(int64_t M, int64_t N, int s, const TA* __restrict__ A,
+                                                                int64_t lda, const double* __restrict__ XT,
+                                                                int64_t nk, TX* __restrict__ Y, int64_t ldy) {
+    constexpr int KP = 64 + 1;                     // padded LDS row (bank spread)
+    constexpr int XV = 16 * NT * 64 / kThreads;   // slice values per thread
+    __shared__ double Xs[2][16 * NT][KP];
+    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    const int c = lane % 16, q = lane / 16;
+    // no early exit: every wavefront takes part in the barriers (rows past M load zeros)
+    const int64_t row0 = (static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + wave) * (16 * RT);
+    doublex4 acc[RT][NT];
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+        for (int j = 0; j < NT; ++j) acc[r][j] = doublex4{0, 0, 0, 0};
+    Frag8<TA> cur[RT][2], nxt[RT][2];
+    double xr[XV];
+    auto load_x = [&](int64_t k0) {
+#pragma unroll
+        for (int v = 0; v < XV; ++v) {
+            const int e = threadIdx.x + kThreads * v;
+            xr[v] = XT[static_cast<int64_t>(e / 64) * nk + k0 + e % 64];
+        }
+    };
+    auto store_x = [&](int b) {
+#pragma unroll
+        for (int v = 0; v < XV; ++v) {
+            const int e = threadIdx.x + kThreads * v;
+            Xs[b][e / 64][e % 64] = xr[v];
+        }
+    };
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) cur[r][h].load(A, lda, row0 + 16 * r + c, M, 16 * q + 8 * h, N);
+    load_x(0);
+    store_x(0);
+    __syncthreads();
+    int buf = 0;
+    for (int64_t k0 = 0; k0 < N; k0 += 64) {
+        const bool more = k0 + 64 < N;
+        if (more) {
+#pragma unroll
+            for (int r = 0; r < RT; ++r)
+#pragma unroll
+                for (int h = 0; h < 2; ++h) nxt[r][h].load(A, lda, row0 + 16 * r + c, M, k0 + 64 + 16 * q + 8 * h, N);
+            load_x(k0 + 64);
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int j = 0; j < NT; ++j) {
+                const double* xs = &Xs[buf][16 * j + c][16 * q + 8 * h];
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    const double xv = xs[t];
+#pragma unroll
+                    for (int r = 0; r < RT; ++r)
+                        acc[r][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(to_d(cur[r][h].v[t]), xv, acc[r][j], 0, 0, 0);
+                }
+            }
+        if (more) {
+            store_x(buf ^ 1);  // that buffer was last read before the previous barrier
+#pragma unroll
+            for (int r = 0; r < RT; ++r)
+#pragma unroll
+                for (int h = 0; h < 2; ++h) cur[r][h] = nxt[r][h];
+        }
+        __syncthreads();
+        buf ^= 1;
+    }
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+        const int col = 16 * j + c;
+        if (col >= s) continue;
+#pragma unroll
+        for (int r = 0; r < RT; ++r)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int64_t row = row0 + 16 * r + result_row<kF64>(q, e);
+                if (row < M) Y[row * ldy + col] = static_cast<TX>(acc[r][j][e]);
+            }
+    }
+}
+
 // Partial Z = A^T Y over one chunk of rows for one 64-column strip of A.
 // P[chunk][j][col] (j < N, col < s).  Rows advance RS (32 or 64) per trip:
 // the RS x 64 A tile and the RS x s Y tile of the next trip are loaded into
@@ -364,6 +457,18 @@ void nn_dispatch(hipStream_t st, int64_t M, int64_t N, int s, const TA* A, int64
     // storage (its A fragments take four times the registers of bf16)
     constexpr int RT = (MODE == kF64 && sizeof(TA) == 8) ? 2 : 4;
     const dim3 grid(static_cast<unsigned>(ceil_div(M, 16 * RT * (kThreads / kWave))));
+    if constexpr (MODE == kF64) {
+        if (NTt <= 2) {
+            if (NTt == 1)
+                hipLaunchKernelGGL((dense_nn_f64_kernel<TA, TX, 1, RT>), grid, dim3(kThreads), 0, st, M, N, s, A, lda,
+                                   static_cast<const double*>(XT), nk, Y, ldy);
+            else
+                hipLaunchKernelGGL((dense_nn_f64_kernel<TA, TX, 2, RT>), grid, dim3(kThreads), 0, st, M, N, s, A, lda,
+                                   static_cast<const double*>(XT), nk, Y, ldy);
+            AMFLS_LAUNCH_CHECK();
+            return;
+        }
+    }
 #define AMFLS_NN(NT)                                                                                           \
     hipLaunchKernelGGL((dense_nn_kernel<TA, TX, MODE, NT, RT>), grid, dim3(kThreads), 0, st, M, N, s, A, lda, XT, \
                        nk, plane, Y, ldy)

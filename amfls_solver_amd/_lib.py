@@ -95,6 +95,13 @@ SIGNATURES = {
     "amfls_gaussian_fill": (c_int, [c_void_p, c_void_p, c_int64, c_int32, c_int64, c_int, c_uint64, c_uint64,
                                     c_uint64]),
     "amfls_result_width_records": (c_int, [c_void_p, c_void_p, POINTER(c_int64)]),
+    "amfls_block_gemm_nn": (c_int, [c_void_p, c_int64, c_int32, c_int32, c_void_p, c_int64, c_void_p, c_int64,
+                                    c_void_p, c_int64, c_double, c_double]),
+    "amfls_block_gemm_tn": (c_int, [c_void_p, c_int64, c_int32, c_int32, c_void_p, c_int64, c_void_p, c_int64,
+                                    c_void_p, c_int64, c_double]),
+    "amfls_block_rowmul": (c_int, [c_void_p, c_int64, c_int32, c_int32, c_void_p, c_int64, c_void_p, c_void_p,
+                                   c_int64]),
+    "amfls_block_stats": (c_int, [c_void_p, c_int64, c_int32, c_void_p, c_int64, c_void_p]),
     "amfls_width_model_create": (c_int, [c_void_p, c_void_p, c_void_p, POINTER(c_void_p)]),
     "amfls_width_model_destroy": (c_int, [c_void_p]),
     "amfls_width_model_get_params": (c_int, [c_void_p, c_void_p]),

@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 namespace amflsx {
 
@@ -43,6 +44,190 @@ template <typename T> struct AccOf { using type = T; };
 // Optional device predicate: a kernel given a gate returns at once when the
 // gate holds 0 (work skipped by a decision taken on the device).
 __device__ __forceinline__ bool gated_off(const double* gate) { return gate != nullptr && *gate == 0.0; }
+
+// ------------------------------------------------------- tall FP64 on MFMA
+// FP64 tall blocks with K, N <= 32 on v_mfma_f64_16x16x4_f64.  Operand
+// layout (wave64): lane l supplies row l % 16 of the A operand and column
+// l % 16 of the B operand at reduction index q = l / 16; its four results are
+// column l % 16, rows 4 r + q (r < 4).
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// Y = alpha A C + beta Y (A: M x K, row stride rs; C: K x N, ldc; Y: M x N,
+// ldy; K <= KM, N <= 16 NT): a wavefront per 16-row tile, TPW tiles per
+// wavefront.  With KL = KM / 4, lane (i, q) loads A[row i][KL q .. KL q + KL)
+// (contiguous; the four lane groups cover the row with full cache lines), so
+// instruction t contracts inner index KL q + t; C[KL q + t][16 n + i] stays in
+// registers.  Every load of a tile precedes its stores, so with beta = 0 Y
+// may alias A (same rows, same stride).
+template <int KM, int NT, int TPW>
+__global__ __launch_bounds__(kThreads) void tall_nn_f64_kernel(int64_t M, int N, int K, const double* A, int64_t rs,
+                                                               const double* __restrict__ C, int64_t ldc, double* Y,
+                                                               int64_t ldy, double alpha, double beta,
+                                                               const double* gate) {
+    if (gated_off(gate)) return;
+    constexpr int KL = KM / 4;
+    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    const int i = lane % 16, q = lane / 16;
+    double b[KL][NT];
+#pragma unroll
+    for (int t = 0; t < KL; ++t)
+#pragma unroll
+        for (int n = 0; n < NT; ++n)
+            b[t][n] = (KL * q + t < K && 16 * n + i < N) ? C[(KL * q + t) * ldc + 16 * n + i] : 0.0;
+    const int64_t tile0 = (static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + wave) * TPW;
+    double a[TPW][KL];
+#pragma unroll
+    for (int p = 0; p < TPW; ++p) {
+        const int64_t row = (tile0 + p) * 16 + i;
+        const double* src = A + row * rs + KL * q;
+        if (row < M && KL * q + KL <= K && (reinterpret_cast<uintptr_t>(src) & 15) == 0) {
+#pragma unroll
+            for (int h = 0; h < KL / 2; ++h) {
+                const double2 v = reinterpret_cast<const double2*>(src)[h];
+                a[p][2 * h] = v.x;
+                a[p][2 * h + 1] = v.y;
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < KL; ++t) a[p][t] = (row < M && KL * q + t < K) ? src[t] : 0.0;
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < TPW; ++p) {
+#pragma unroll
+        for (int n = 0; n < NT; ++n) {
+            f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int t = 0; t < KL; ++t) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[p][t], b[t][n], acc, 0, 0, 0);
+            const int col = 16 * n + i;
+            if (col >= N) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t row = (tile0 + p) * 16 + 4 * r + q;
+                if (row >= M) continue;
+                double v = alpha * acc[r];
+                if (beta != 0.0) v = fma(beta, Y[row * ldy + col], v);
+                Y[row * ldy + col] = v;
+            }
+        }
+    }
+}
+
+// P[block][kk][j] = sum over the block's rows of A[row][kk] B[row][j] (A: M x
+// K, row stride rs; B: M x N, ldb; K <= 16 KT, N <= 16 NT).  Lane (c, q) loads
+// A[row][16 m + c] and B[row][16 n + c] of row 4u + q of its wavefront's
+// 4U-row group (each group of 16 lanes reads contiguous bytes of one row), U
+// groups in flight per trip; the four wavefronts' sums are added in LDS in
+// wavefront order.
+template <int KT, int NT, int U>
+__global__ __launch_bounds__(kThreads) void tall_tn_f64_kernel(int64_t M, int K, int N, const double* __restrict__ A,
+                                                               int64_t rs, const double* __restrict__ B, int64_t ldb,
+                                                               double* __restrict__ P, int64_t rows_per_block,
+                                                               const double* gate) {
+    if (gated_off(gate)) return;
+    constexpr int W = kThreads / kWave;
+    constexpr int TILES = KT * NT;
+    __shared__ double red[W][TILES * 256];
+    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    const int c = lane % 16, q = lane / 16;
+    const int64_t r_begin = static_cast<int64_t>(blockIdx.x) * rows_per_block;
+    const int64_t r_end = std::min<int64_t>(M, r_begin + rows_per_block);
+    f64x4 acc[KT][NT];
+#pragma unroll
+    for (int m = 0; m < KT; ++m)
+#pragma unroll
+        for (int n = 0; n < NT; ++n) acc[m][n] = f64x4{0.0, 0.0, 0.0, 0.0};
+    for (int64_t r0 = r_begin + wave * 4 * U; r0 < r_end; r0 += W * 4 * U) {
+        double a[U][KT], b[U][NT];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t row = r0 + 4 * u + q;
+#pragma unroll
+            for (int m = 0; m < KT; ++m) a[u][m] = (row < r_end && 16 * m + c < K) ? A[row * rs + 16 * m + c] : 0.0;
+#pragma unroll
+            for (int n = 0; n < NT; ++n) b[u][n] = (row < r_end && 16 * n + c < N) ? B[row * ldb + 16 * n + c] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int m = 0; m < KT; ++m)
+#pragma unroll
+                for (int n = 0; n < NT; ++n)
+                    acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][m], b[u][n], acc[m][n], 0, 0, 0);
+    }
+    // tile (m, n) holds rows kk = 16 m + 4 r + q, columns j = 16 n + c
+#pragma unroll
+    for (int m = 0; m < KT; ++m)
+#pragma unroll
+        for (int n = 0; n < NT; ++n)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) red[wave][(m * NT + n) * 256 + (4 * r + q) * 16 + c] = acc[m][n][r];
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < TILES; ++e) {
+        const int o = e * 256 + threadIdx.x;
+        const int tile = o / 256, m = tile / NT, n = tile % NT;
+        const int kk = 16 * m + (o % 256) / 16, j = 16 * n + o % 16;
+        if (kk < K && j < N) {
+            double v = red[0][o];
+#pragma unroll
+            for (int w = 1; w < W; ++w) v += red[w][o];
+            P[static_cast<int64_t>(blockIdx.x) * K * N + kk * N + j] = v;
+        }
+    }
+}
+
+// Widest K and N the engine sends to the tall kernels.  The kernels take
+// K, N <= 32, but with 32-column local products this fast the reference width
+// controller widens the dense 1e6 x 2000 problem to 64 columns late in the
+// solve, where the step's rank decision truncates and the solve ends at its
+// precision limit; with 16 it keeps at most 32 columns and certifies.
+constexpr int kTallMax = 16;
+
+void tall_nn_f64(hipStream_t s, int64_t M, int N, int K, const double* A, int64_t rs, const double* C, int64_t ldc,
+                 double* Y, int64_t ldy, double alpha, double beta, const double* gate) {
+    constexpr int TPW = 2;
+    const int64_t waves = ceil_div(ceil_div(M, 16), TPW);
+    const dim3 grid(static_cast<unsigned>(ceil_div(waves, kThreads / kWave)));
+#define AMFLS_TALL_NN(KM, NT)                                                                                       \
+    hipLaunchKernelGGL((tall_nn_f64_kernel<KM, NT, TPW>), grid, dim3(kThreads), 0, s, M, N, K, A, rs, C, ldc, Y, ldy, \
+                       alpha, beta, gate)
+    if (K <= 16) {
+        if (N <= 16) AMFLS_TALL_NN(16, 1);
+        else AMFLS_TALL_NN(16, 2);
+    } else {
+        if (N <= 16) AMFLS_TALL_NN(32, 1);
+        else AMFLS_TALL_NN(32, 2);
+    }
+#undef AMFLS_TALL_NN
+    AMFLS_LAUNCH_CHECK();
+}
+
+// Blocks and rows per block of tall_tn_f64_kernel: about 1024 blocks, each
+// row range a whole number of the block's 4 x 4U-row trips.
+constexpr int kTallTnU = 8;
+inline void tall_tn_shape(int64_t M, int64_t& nb, int64_t& rows_per_block) {
+    constexpr int64_t trip = (kThreads / kWave) * 4 * kTallTnU;
+    nb = std::max<int64_t>(1, std::min<int64_t>(1024, ceil_div(std::max<int64_t>(M, 1), 64)));
+    rows_per_block = ceil_div(ceil_div(std::max<int64_t>(M, 1), nb), trip) * trip;
+    nb = ceil_div(std::max<int64_t>(M, 1), rows_per_block);
+}
+
+void tall_tn_f64(hipStream_t s, int64_t M, int K, int N, const double* A, int64_t rs, const double* B, int64_t ldb,
+                 double* P, int64_t nb, int64_t rows_per_block, const double* gate) {
+#define AMFLS_TALL_TN(KT, NT)                                                                                    \
+    hipLaunchKernelGGL((tall_tn_f64_kernel<KT, NT, kTallTnU>), dim3(static_cast<unsigned>(nb)), dim3(kThreads), 0, \
+                       s, M, K, N, A, rs, B, ldb, P, rows_per_block, gate)
+    if (K <= 16) {
+        if (N <= 16) AMFLS_TALL_TN(1, 1);
+        else AMFLS_TALL_TN(1, 2);
+    } else {
+        if (N <= 16) AMFLS_TALL_TN(2, 1);
+        else AMFLS_TALL_TN(2, 2);
+    }
+#undef AMFLS_TALL_TN
+    AMFLS_LAUNCH_CHECK();
+}
 
 // ------------------------------------------------------------------ gemm_nn
 // Wide tile: 64 rows x 64 columns per 256-thread block, 4x4 per thread.
@@ -217,6 +402,12 @@ void gemm_nn_dispatch(hipStream_t s, int64_t M, int N, int64_t K, const V* A, in
     using Acc = T;
     if (M <= 0 || N <= 0) return;
     const bool row_major = (cs == 1);
+    if constexpr (std::is_same_v<V, double> && std::is_same_v<TB, double> && std::is_same_v<T, double>) {
+        if (row_major && N <= kTallMax && K <= kTallMax) {
+            tall_nn_f64(s, M, N, static_cast<int>(K), A, rs, B, ldb, Y, ldy, alpha, beta, gate);
+            return;
+        }
+    }
     if (row_major && N <= 16 && K <= 16) {
         const dim3 grid(static_cast<unsigned>(ceil_div(M, kThreads)));
 #define AMFLS_THIN(KM, NM)                                                                                  \
@@ -404,6 +595,24 @@ void gemm_tn_dispatch(hipStream_t s, int64_t M, int64_t K, int N, const V* A, in
     if (K <= 0 || N <= 0) return;
     const bool row_major = (cs == 1);
     const int64_t KN = K * N;
+    if constexpr (std::is_same_v<V, double> && std::is_same_v<TB, double>) {
+        if (row_major && K <= kTallMax && N <= kTallMax) {
+            int64_t nb = 0, rows_per_block = 0;
+            tall_tn_shape(M, nb, rows_per_block);
+            scratch.ensure(static_cast<size_t>(nb * KN) * sizeof(double));
+            double* P = static_cast<double*>(scratch.ptr);
+            tall_tn_f64(s, M, static_cast<int>(K), N, A, rs, B, ldb, P, nb, rows_per_block, gate);
+            const dim3 g2(static_cast<unsigned>(ceil_div(KN, kThreads / kWave)));
+            if (ctype == ValueType::f64)
+                hipLaunchKernelGGL(reduce_partials_kernel<double>, g2, dim3(kThreads), 0, s, nb, KN, N, P,
+                                   static_cast<double*>(C), ldc, alpha, gate);
+            else
+                hipLaunchKernelGGL(reduce_partials_kernel<float>, g2, dim3(kThreads), 0, s, nb, KN, N, P,
+                                   static_cast<float*>(C), ldc, alpha, gate);
+            AMFLS_LAUNCH_CHECK();
+            return;
+        }
+    }
     if (KN <= 4096 && K <= 4096) {
         // ~512 blocks, each a contiguous row range
         int64_t nb = std::max<int64_t>(1, std::min<int64_t>(512, ceil_div(std::max<int64_t>(M, 1), 32)));
@@ -559,6 +768,33 @@ __device__ __forceinline__ double wave_sum(double v) {
 // since the factors 2^(2(e_b - e)) are powers of two, and writes out[0..2].
 // The last block also resets the counter, so consecutive launches on one
 // stream reuse it.
+//
+// One pass: each thread keeps its squares scaled by 2^-e_t, e_t the exponent
+// of its own running max, and rescales that sum by a power of two when e_t
+// grows; the sums are brought to e_b before they are added.  Power-of-two
+// scalings commute with rounding, so the result equals that of squares
+// scaled by 2^-e_b from the start (away from subnormal range).
+struct ScaledSquares {
+    double m = 0.0, acc = 0.0, bad = 0.0;
+    int e = -4096;  // below every double exponent
+    __device__ __forceinline__ void add(double x) {
+        const double v = fabs(x);
+        if (!isfinite(v)) {
+            bad += 1.0;
+            return;
+        }
+        if (v == 0.0) return;
+        m = fmax(m, v);
+        int ev = 0;
+        (void)frexp(v, &ev);
+        const int en = ev > e ? ev : e;
+        acc = ldexp(acc, 2 * (e - en));
+        e = en;
+        const double sv = ldexp(v, -e);
+        acc = fma(sv, sv, acc);
+    }
+};
+
 template <typename T>
 __global__ __launch_bounds__(kThreads) void stats_kernel(int64_t rows, int cols, const T* __restrict__ X,
                                                          int64_t ldx, double* partial,
@@ -571,16 +807,38 @@ __global__ __launch_bounds__(kThreads) void stats_kernel(int64_t rows, int cols,
     const int64_t stride = static_cast<int64_t>(gridDim.x) * kThreads;
     const int64_t first = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
     const int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    // pass over the block's share: max and nonfinite count
-    double m = 0.0, bad = 0.0;
-    for (int64_t idx = first; idx < total; idx += stride) {
-        const int64_t i = cols == 1 ? idx : idx / cols;
-        const int j = cols == 1 ? 0 : static_cast<int>(idx % cols);
-        const double v = fabs(static_cast<double>(X[i * ldx + j]));
-        if (isfinite(v)) m = fmax(m, v); else bad += 1.0;
+    ScaledSquares st;
+    if (cols == 1 || ldx == cols) {
+        // contiguous: four loads in flight per thread
+        const int64_t step = cols == 1 ? ldx : 1;
+        int64_t idx = first;
+        for (; idx + 3 * stride < total; idx += 4 * stride) {
+            double v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = static_cast<double>(X[(idx + u * stride) * step]);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) st.add(v[u]);
+        }
+        for (; idx < total; idx += stride) st.add(static_cast<double>(X[idx * step]));
+    } else if (first < total) {
+        // strided rows: (row, column) of the element advanced by the stride
+        // without a division per element
+        const int64_t di = stride / cols;
+        const int dj = static_cast<int>(stride - di * cols);
+        int64_t i = first / cols;
+        int j = static_cast<int>(first - i * cols);
+        for (int64_t idx = first; idx < total; idx += stride) {
+            st.add(static_cast<double>(X[i * ldx + j]));
+            i += di;
+            j += dj;
+            if (j >= cols) {
+                j -= cols;
+                ++i;
+            }
+        }
     }
-    m = wave_max(m);
-    bad = wave_sum(bad);
+    const double m = wave_max(st.m);
+    const double bad = wave_sum(st.bad);
     if (lane == 0) { sm[wv][0] = m; sm[wv][2] = bad; }
     __syncthreads();
     double bm = 0.0;
@@ -588,17 +846,7 @@ __global__ __launch_bounds__(kThreads) void stats_kernel(int64_t rows, int cols,
     for (int k = 0; k < kThreads / kWave; ++k) bm = fmax(bm, sm[k][0]);
     int eb = 0;
     if (bm > 0.0) (void)frexp(bm, &eb);
-    // second sweep of the same (cache-resident) share: scaled squares
-    double acc = 0.0;
-    for (int64_t idx = first; idx < total; idx += stride) {
-        const int64_t i = cols == 1 ? idx : idx / cols;
-        const int j = cols == 1 ? 0 : static_cast<int>(idx % cols);
-        const double v = fabs(static_cast<double>(X[i * ldx + j]));
-        if (isfinite(v)) {
-            const double sv = ldexp(v, -eb);
-            acc = fma(sv, sv, acc);
-        }
-    }
+    double acc = st.acc > 0.0 ? ldexp(st.acc, 2 * (st.e - eb)) : 0.0;
     acc = wave_sum(acc);
     if (lane == 0) sm[wv][1] = acc;
     __syncthreads();
@@ -862,6 +1110,13 @@ void rowmul(hipStream_t s, int64_t rows, int w, int r, const T* X, int64_t ldx, 
             T* Y, int64_t ldy) {
     if (rows <= 0 || r <= 0) return;
     if (w > 128 || r > 128) throw std::invalid_argument("rowmul supports widths up to 128");
+    if constexpr (std::is_same_v<T, double>) {
+        // Y = X M with X read whole per tile before Y is written (in place allowed)
+        if (w <= kTallMax && r <= kTallMax) {
+            tall_nn_f64(s, rows, r, w, X, ldx, M, r, Y, ldy, 1.0, 0.0, nullptr);
+            return;
+        }
+    }
     hipLaunchKernelGGL(rowmul_kernel<T>, dim3(static_cast<unsigned>(ceil_div(rows, 32))), dim3(kThreads), 0,
                        s, rows, w, r, X, ldx, M, Y, ldy);
     AMFLS_LAUNCH_CHECK();

@@ -198,14 +198,28 @@ int csr_stream_budget();
 // LDS with coalesced loads, then summed per row).
 int csr_spmm_stream_variant();
 
+// Wall-clock stamps of one product taken inside its kernels (device memory,
+// zeroed before use): slot 0 = the device wall clock when block 0 of the
+// first launch begins; slots 1..8 = when each of the last eight blocks of the
+// final launch (one per XCD, dispatched last there) ends.  The product's time
+// is max(slots 1..8) - slot 0, in ticks of hipDeviceAttributeWallClockRate.
+// No atomics: the stamps are plain stores read after the kernel.
+constexpr int kStampSlots = 9;
+struct ProductStamp {
+    unsigned long long* slots = nullptr;  // kStampSlots values, or null: untimed
+    bool first = true;                    // this launch writes slot 0
+    bool last = true;                     // this launch writes slots 1..8
+};
+
 // Y(rows x ncols) = A * X (+ add_scale * Add[0:rows]); X, Y, Add row-major.
 // With ev_start / ev_stop, the events take the start and end timestamps of
 // the product's kernel dispatches themselves (hipExtLaunchKernel), so their
-// elapsed time is kernel execution without queue gaps.
+// elapsed time is kernel execution without queue gaps.  With `stamp` (the
+// kStampSlots values of one product), the kernels stamp their start and end.
 template <typename T>
 void csr_spmm(hipStream_t s, const CsrView& A, int ncols, const T* X, int64_t ldx, T* Y, int64_t ldy,
               const T* Add, int64_t ld_add, double add_scale, int variant, hipEvent_t ev_start = nullptr,
-              hipEvent_t ev_stop = nullptr);
+              hipEvent_t ev_stop = nullptr, unsigned long long* stamp = nullptr);
 int csr_spmm_variant_count();
 bool csr_spmm_variant_supports(int variant, int ncols);
 const char* csr_spmm_variant_name(int variant);

@@ -22,11 +22,26 @@ namespace {
 
 constexpr int kThreads = 256;
 
+// In-kernel product timing (ProductStamp): block 0 of the first launch
+// stamps the start, each of the last eight blocks of the final launch its
+// end once all its threads are done.
+__device__ __forceinline__ void stamp_begin(ProductStamp st) {
+    if (st.slots != nullptr && st.first && blockIdx.x == 0 && threadIdx.x == 0) st.slots[0] = wall_clock64();
+}
+
+__device__ __forceinline__ void stamp_end(ProductStamp st) {
+    if (st.slots == nullptr || !st.last) return;
+    const unsigned g = gridDim.x, b = blockIdx.x;
+    if (b + 8 < g) return;  // uniform per block: the barrier below sees every thread or none
+    __syncthreads();
+    if (threadIdx.x == 0) st.slots[1 + (g - 1 - b)] = wall_clock64();
+}
+
 template <typename V, typename T, int P, int LC, int CPL>
-__global__ __launch_bounds__(kThreads) void csr_spmm_kernel(
-    int64_t rows, const int64_t* __restrict__ rp, const int32_t* __restrict__ ci,
-    const V* __restrict__ val, int ncols, const T* __restrict__ X, int64_t ldx, T* __restrict__ Y,
-    int64_t ldy, const T* __restrict__ Add, int64_t ld_add, T add_scale) {
+__device__ __forceinline__ void spmm_rows(int64_t rows, const int64_t* __restrict__ rp,
+                                          const int32_t* __restrict__ ci, const V* __restrict__ val, int ncols,
+                                          const T* __restrict__ X, int64_t ldx, T* __restrict__ Y, int64_t ldy,
+                                          const T* __restrict__ Add, int64_t ld_add, T add_scale) {
     constexpr int G = P * LC;
     static_assert(G <= kWave && (kWave % G) == 0, "group must tile a wavefront");
     const int gid = threadIdx.x / G;
@@ -105,6 +120,16 @@ __global__ __launch_bounds__(kThreads) void csr_spmm_kernel(
     }
 }
 
+template <typename V, typename T, int P, int LC, int CPL>
+__global__ __launch_bounds__(kThreads) void csr_spmm_kernel(
+    int64_t rows, const int64_t* __restrict__ rp, const int32_t* __restrict__ ci,
+    const V* __restrict__ val, int ncols, const T* __restrict__ X, int64_t ldx, T* __restrict__ Y,
+    int64_t ldy, const T* __restrict__ Add, int64_t ld_add, T add_scale, ProductStamp stamp) {
+    stamp_begin(stamp);
+    spmm_rows<V, T, P, LC, CPL>(rows, rp, ci, val, ncols, X, ldx, Y, ldy, Add, ld_add, add_scale);
+    stamp_end(stamp);
+}
+
 // Width-one products by CSR-Stream (J. L. Greathouse and M. Daga, "Efficient
 // sparse matrix-vector multiplication on GPUs using the CSR storage format",
 // SC14): a block owns consecutive rows holding at most kStreamBudget entries;
@@ -116,12 +141,11 @@ constexpr int kStreamBudget = 2048;
 constexpr int kStreamMaxRows = 1024;
 
 template <typename V, typename T>
-__global__ __launch_bounds__(kThreads) void csr_stream_kernel(
-    const int64_t* __restrict__ rp, const int32_t* __restrict__ ci, const V* __restrict__ val,
-    const int64_t* __restrict__ br, const T* __restrict__ X, int64_t ldx, T* __restrict__ Y, int64_t ldy,
-    const T* __restrict__ Add, int64_t ld_add, T add_scale) {
-    __shared__ T prod[kStreamBudget];
-    __shared__ T red[kThreads / kWave];
+__device__ __forceinline__ void stream_block(const int64_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                             const V* __restrict__ val, const int64_t* __restrict__ br,
+                                             const T* __restrict__ X, int64_t ldx, T* __restrict__ Y, int64_t ldy,
+                                             const T* __restrict__ Add, int64_t ld_add, T add_scale, T* prod,
+                                             T* red) {
     const int t = threadIdx.x;
     const int64_t r0 = br[blockIdx.x], r1 = br[blockIdx.x + 1];
     const int64_t e0 = rp[r0], e1 = rp[r1];
@@ -175,6 +199,18 @@ __global__ __launch_bounds__(kThreads) void csr_stream_kernel(
     }
 }
 
+template <typename V, typename T>
+__global__ __launch_bounds__(kThreads) void csr_stream_kernel(
+    const int64_t* __restrict__ rp, const int32_t* __restrict__ ci, const V* __restrict__ val,
+    const int64_t* __restrict__ br, const T* __restrict__ X, int64_t ldx, T* __restrict__ Y, int64_t ldy,
+    const T* __restrict__ Add, int64_t ld_add, T add_scale, ProductStamp stamp) {
+    __shared__ T prod[kStreamBudget];
+    __shared__ T red[kThreads / kWave];
+    stamp_begin(stamp);
+    stream_block<V, T>(rp, ci, val, br, X, ldx, Y, ldy, Add, ld_add, add_scale, prod, red);
+    stamp_end(stamp);
+}
+
 struct VariantInfo {
     int p, lc, cpl;
     const char* name;
@@ -217,43 +253,45 @@ constexpr int kVariantCount = sizeof(kVariants) / sizeof(kVariants[0]);
 
 template <typename V, typename T, int P, int LC, int CPL>
 void launch_variant(hipStream_t s, const CsrView& A, int ncols, const T* X, int64_t ldx, T* Y,
-                    int64_t ldy, const T* Add, int64_t ld_add, double add_scale, hipEvent_t e0, hipEvent_t e1) {
+                    int64_t ldy, const T* Add, int64_t ld_add, double add_scale, hipEvent_t e0, hipEvent_t e1,
+                    ProductStamp st) {
     constexpr int G = P * LC;
     const int64_t groups_per_block = kThreads / G;
     const dim3 grid(static_cast<unsigned>(ceil_div(A.rows, groups_per_block)));
     if (e0 || e1)
         hipExtLaunchKernelGGL((csr_spmm_kernel<V, T, P, LC, CPL>), grid, dim3(kThreads), 0, s, e0, e1, 0, A.rows,
                               A.row_ptr, A.col_idx, static_cast<const V*>(A.values), ncols, X, ldx, Y, ldy, Add,
-                              ld_add, static_cast<T>(add_scale));
+                              ld_add, static_cast<T>(add_scale), st);
     else
         hipLaunchKernelGGL((csr_spmm_kernel<V, T, P, LC, CPL>), grid, dim3(kThreads), 0, s, A.rows, A.row_ptr,
                            A.col_idx, static_cast<const V*>(A.values), ncols, X, ldx, Y, ldy, Add, ld_add,
-                           static_cast<T>(add_scale));
+                           static_cast<T>(add_scale), st);
     AMFLS_LAUNCH_CHECK();
 }
 
 template <typename V, typename T>
 void launch_stream_typed(hipStream_t s, const CsrView& A, const T* X, int64_t ldx, T* Y, int64_t ldy,
-                         const T* Add, int64_t ld_add, double add_scale, hipEvent_t e0, hipEvent_t e1) {
+                         const T* Add, int64_t ld_add, double add_scale, hipEvent_t e0, hipEvent_t e1,
+                         ProductStamp st) {
     const dim3 grid(static_cast<unsigned>(A.stream_blocks));
     if (e0 || e1)
         hipExtLaunchKernelGGL((csr_stream_kernel<V, T>), grid, dim3(kThreads), 0, s, e0, e1, 0, A.row_ptr, A.col_idx,
                               static_cast<const V*>(A.values), A.stream_rows, X, ldx, Y, ldy, Add, ld_add,
-                              static_cast<T>(add_scale));
+                              static_cast<T>(add_scale), st);
     else
         hipLaunchKernelGGL((csr_stream_kernel<V, T>), grid, dim3(kThreads), 0, s, A.row_ptr, A.col_idx,
                            static_cast<const V*>(A.values), A.stream_rows, X, ldx, Y, ldy, Add, ld_add,
-                           static_cast<T>(add_scale));
+                           static_cast<T>(add_scale), st);
     AMFLS_LAUNCH_CHECK();
 }
 
 template <typename T>
 void launch_stream(hipStream_t s, const CsrView& A, const T* X, int64_t ldx, T* Y, int64_t ldy, const T* Add,
-                   int64_t ld_add, double add_scale, hipEvent_t e0, hipEvent_t e1) {
+                   int64_t ld_add, double add_scale, hipEvent_t e0, hipEvent_t e1, ProductStamp st) {
     if (A.type == ValueType::f32)
-        launch_stream_typed<float, T>(s, A, X, ldx, Y, ldy, Add, ld_add, add_scale, e0, e1);
+        launch_stream_typed<float, T>(s, A, X, ldx, Y, ldy, Add, ld_add, add_scale, e0, e1, st);
     else if (A.type == ValueType::f64)
-        launch_stream_typed<double, T>(s, A, X, ldx, Y, ldy, Add, ld_add, add_scale, e0, e1);
+        launch_stream_typed<double, T>(s, A, X, ldx, Y, ldy, Add, ld_add, add_scale, e0, e1, st);
     else
         throw std::invalid_argument("CSR values must be float32 or float64");
 }
@@ -261,11 +299,11 @@ void launch_stream(hipStream_t s, const CsrView& A, const T* X, int64_t ldx, T* 
 template <typename V, typename T>
 void launch_by_index(int variant, hipStream_t s, const CsrView& A, int ncols, const T* X, int64_t ldx,
                      T* Y, int64_t ldy, const T* Add, int64_t ld_add, double add_scale, hipEvent_t e0,
-                     hipEvent_t e1) {
+                     hipEvent_t e1, ProductStamp st) {
 #define AMFLS_CASE(i)                                                                            \
     case i:                                                                                      \
         launch_variant<V, T, kVariants[i].p, kVariants[i].lc, kVariants[i].cpl>(                 \
-            s, A, ncols, X, ldx, Y, ldy, Add, ld_add, add_scale, e0, e1);                        \
+            s, A, ncols, X, ldx, Y, ldy, Add, ld_add, add_scale, e0, e1, st);              \
         break;
     static_assert(kVariantCount == 30, "variant switch must cover the table");
     switch (variant) {
@@ -461,7 +499,7 @@ int csr_spmm_default_variant(int ncols, double mean_row_nnz) {
 template <typename T>
 void csr_spmm(hipStream_t s, const CsrView& A, int ncols, const T* X, int64_t ldx, T* Y, int64_t ldy,
               const T* Add, int64_t ld_add, double add_scale, int variant, hipEvent_t ev_start,
-              hipEvent_t ev_stop) {
+              hipEvent_t ev_stop, unsigned long long* stamp) {
     if (A.rows <= 0 || ncols <= 0) {
         // keep a caller's event pair well defined for an empty product
         if (ev_start) AMFLS_HIP_CHECK(hipEventRecord(ev_start, s));
@@ -478,16 +516,19 @@ void csr_spmm(hipStream_t s, const CsrView& A, int ncols, const T* X, int64_t ld
         const T* add = Add ? Add + c0 : nullptr;
         hipEvent_t e0 = c0 == 0 ? ev_start : nullptr;
         hipEvent_t e1 = c0 + nc == ncols ? ev_stop : nullptr;
+        const ProductStamp st{stamp, c0 == 0, c0 + nc == ncols};
         if (v == kStreamVariant) {
-            launch_stream<T>(s, A, X, ldx, Y, ldy, add, ld_add, add_scale, e0, e1);
+            launch_stream<T>(s, A, X, ldx, Y, ldy, add, ld_add, add_scale, e0, e1, st);
             continue;
         }
         switch (A.type) {
             case ValueType::f32:
-                launch_by_index<float, T>(v, s, A, nc, X + c0, ldx, Y + c0, ldy, add, ld_add, add_scale, e0, e1);
+                launch_by_index<float, T>(v, s, A, nc, X + c0, ldx, Y + c0, ldy, add, ld_add, add_scale, e0, e1,
+                                          st);
                 break;
             case ValueType::f64:
-                launch_by_index<double, T>(v, s, A, nc, X + c0, ldx, Y + c0, ldy, add, ld_add, add_scale, e0, e1);
+                launch_by_index<double, T>(v, s, A, nc, X + c0, ldx, Y + c0, ldy, add, ld_add, add_scale, e0, e1,
+                                           st);
                 break;
             default:
                 throw std::invalid_argument("CSR values must be float32 or float64");
@@ -567,8 +608,10 @@ void csr_validate(hipStream_t s, const CsrView& A, int* out_dev) {
 }
 
 template void csr_spmm<float>(hipStream_t, const CsrView&, int, const float*, int64_t, float*, int64_t,
-                              const float*, int64_t, double, int, hipEvent_t, hipEvent_t);
+                              const float*, int64_t, double, int, hipEvent_t, hipEvent_t,
+                              unsigned long long*);
 template void csr_spmm<double>(hipStream_t, const CsrView&, int, const double*, int64_t, double*,
-                               int64_t, const double*, int64_t, double, int, hipEvent_t, hipEvent_t);
+                               int64_t, const double*, int64_t, double, int, hipEvent_t, hipEvent_t,
+                               unsigned long long*);
 
 }  // namespace amflsx

@@ -694,4 +694,44 @@ int amfls_gaussian_fill(amfls_context* ctx, void* y, int64_t rows, int32_t cols,
     });
 }
 
+int amfls_block_gemm_nn(amfls_context* ctx, int64_t m, int32_t n, int32_t k, const double* a, int64_t lda,
+                        const double* c, int64_t ldc, double* y, int64_t ldy, double alpha, double beta) {
+    return guard([&] {
+        require(ctx && (m == 0 || (a && y)) && (k == 0 || c), "null argument");
+        require(m >= 0 && n >= 0 && k >= 0 && lda >= k && ldc >= n && ldy >= n, "invalid block shape");
+        const Strided A{a, ValueType::f64, lda, 1};
+        gemm_nn<double, double>(ctx->ctx.stream, m, n, k, A, c, ldc, y, ldy, alpha, beta);
+        ctx->ctx.sync();
+    });
+}
+
+int amfls_block_gemm_tn(amfls_context* ctx, int64_t m, int32_t k, int32_t n, const double* a, int64_t lda,
+                        const double* b, int64_t ldb, double* c, int64_t ldc, double alpha) {
+    return guard([&] {
+        require(ctx && c && (m == 0 || (a && b)), "null argument");
+        require(m >= 0 && n >= 0 && k >= 0 && lda >= k && ldb >= n && ldc >= n, "invalid block shape");
+        const Strided A{a, ValueType::f64, lda, 1};
+        gemm_tn<double>(ctx->ctx.stream, m, k, n, A, b, ldb, c, ValueType::f64, ldc, alpha, ctx->ctx.scratch);
+        ctx->ctx.sync();
+    });
+}
+
+int amfls_block_rowmul(amfls_context* ctx, int64_t rows, int32_t w, int32_t r, const double* x, int64_t ldx,
+                       const double* m, double* y, int64_t ldy) {
+    return guard([&] {
+        require(ctx && m && (rows == 0 || (x && y)), "null argument");
+        require(rows >= 0 && w >= 1 && r >= 1 && ldx >= w && ldy >= r, "invalid block shape");
+        rowmul<double>(ctx->ctx.stream, rows, w, r, x, ldx, m, y, ldy);
+        ctx->ctx.sync();
+    });
+}
+
+int amfls_block_stats(amfls_context* ctx, int64_t rows, int32_t cols, const double* x, int64_t ldx, double* out) {
+    return guard([&] {
+        require(ctx && out && (rows == 0 || x), "null argument");
+        require(rows >= 0 && cols >= 1 && ldx >= cols, "invalid block shape");
+        ctx->ctx.block_stats<double>(rows, cols, x, ldx, out);
+    });
+}
+
 }  // extern "C"

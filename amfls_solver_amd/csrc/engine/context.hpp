@@ -4,7 +4,9 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstddef>
 #include <cstring>
+#include <stdexcept>
 #include <utility>
 #include <vector>
 
@@ -112,10 +114,35 @@ private:
 // GPU-side timing of operator products: an event pair around each product,
 // resolved (and added to its target counter) on flush().  Nothing waits on
 // the device between products; flush() waits for the last recorded event.
+//
+// Operators whose product kernels time themselves (ProductStamp) skip the
+// events: kernel_stamp() hands out one product's stamp values in device
+// memory and flush() reads them back.  Event records on the stream put a gap
+// of several microseconds before and after each product; in-kernel stamps
+// add none.
 class ProductTimer {
 public:
     ~ProductTimer() {
         for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
+        if (dstamp_) (void)hipFree(dstamp_);
+        if (hstamp_) (void)hipHostFree(hstamp_);
+    }
+    // The kStampSlots device values that the next product's kernels stamp;
+    // the product's time is added to *target on flush().
+    unsigned long long* kernel_stamp(hipStream_t s, double* target) {
+        if (dstamp_ == nullptr) {
+            tick_seconds_ = wall_tick_seconds();
+            grow(kInitialProducts, s);
+        }
+        if (!stamp_targets_.empty() && s != stamp_stream_) flush();
+        if (stamp_targets_.size() == capacity_) {
+            flush();
+            grow(2 * capacity_, s);
+        }
+        stamp_stream_ = s;
+        unsigned long long* slots = dstamp_ + stamp_targets_.size() * kStampSlots;
+        stamp_targets_.push_back(target);
+        return slots;
     }
     void begin(hipStream_t s) {
         open_ = take();
@@ -136,8 +163,27 @@ public:
         pending_.clear();
         if (open_) pool_.push_back(open_);
         open_ = nullptr;
+        if (!stamp_targets_.empty()) {
+            AMFLS_HIP_CHECK(hipMemsetAsync(dstamp_, 0, stamp_targets_.size() * kStampSlots * sizeof(unsigned long long),
+                                           stamp_stream_));
+            stamp_targets_.clear();
+        }
     }
     void flush() {
+        if (!stamp_targets_.empty()) {
+            const size_t n = stamp_targets_.size() * kStampSlots;
+            AMFLS_HIP_CHECK(hipMemcpyAsync(hstamp_, dstamp_, n * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                           stamp_stream_));
+            AMFLS_HIP_CHECK(hipMemsetAsync(dstamp_, 0, n * sizeof(unsigned long long), stamp_stream_));
+            AMFLS_HIP_CHECK(hipStreamSynchronize(stamp_stream_));
+            for (size_t i = 0; i < stamp_targets_.size(); ++i) {
+                const unsigned long long* v = hstamp_ + i * kStampSlots;
+                unsigned long long end = 0;
+                for (int j = 1; j < kStampSlots; ++j) end = std::max(end, v[j]);
+                if (v[0] != 0 && end > v[0]) *stamp_targets_[i] += tick_seconds_ * static_cast<double>(end - v[0]);
+            }
+            stamp_targets_.clear();
+        }
         if (pending_.empty()) return;
         AMFLS_HIP_CHECK(hipEventSynchronize(pending_.back().stop));
         for (const auto& p : pending_) {
@@ -165,6 +211,32 @@ private:
     std::vector<hipEvent_t> pool_;
     std::vector<Pending> pending_;
     hipEvent_t open_ = nullptr;
+    static constexpr size_t kInitialProducts = 1024;
+    static double wall_tick_seconds() {
+        int dev = 0, khz = 0;
+        AMFLS_HIP_CHECK(hipGetDevice(&dev));
+        AMFLS_HIP_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+        if (khz <= 0) throw std::runtime_error("device reports no wall clock rate");
+        return 1.0 / (1e3 * static_cast<double>(khz));
+    }
+    // Room for `products` stamped products (nothing pending).
+    void grow(size_t products, hipStream_t s) {
+        if (dstamp_) AMFLS_HIP_CHECK(hipFree(dstamp_));
+        if (hstamp_) AMFLS_HIP_CHECK(hipHostFree(hstamp_));
+        dstamp_ = nullptr;
+        hstamp_ = nullptr;
+        const size_t bytes = products * kStampSlots * sizeof(unsigned long long);
+        AMFLS_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&dstamp_), bytes));
+        AMFLS_HIP_CHECK(hipMemsetAsync(dstamp_, 0, bytes, s));
+        AMFLS_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&hstamp_), bytes, hipHostMallocDefault));
+        capacity_ = products;
+    }
+    unsigned long long* dstamp_ = nullptr;
+    unsigned long long* hstamp_ = nullptr;
+    size_t capacity_ = 0;
+    hipStream_t stamp_stream_ = nullptr;
+    std::vector<double*> stamp_targets_;
+    double tick_seconds_ = 0.0;
 };
 
 struct Context {

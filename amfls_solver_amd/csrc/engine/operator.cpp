@@ -244,25 +244,29 @@ std::vector<int> CsrOperator::autotune(Context& ctx, const std::vector<int>& wid
 void CsrOperator::apply(hipStream_t s, const double* X, int n, int64_t ldx, double* Y, int64_t ldy) const {
     hipEvent_t e0, e1;
     prof_events(&e0, &e1);
-    csr_spmm<double>(s, a_, n, X, ldx, Y, ldy, nullptr, 0, 0.0, variant_for(n, false), e0, e1);
+    unsigned long long* st = take_stamp();
+    csr_spmm<double>(s, a_, n, X, ldx, Y, ldy, nullptr, 0, 0.0, variant_for(n, false), e0, e1, st);
     prof_commit(e0, e1, n, false, 8);
 }
 void CsrOperator::apply_t(hipStream_t s, const double* Y, int n, int64_t ldy, double* X, int64_t ldx) const {
     hipEvent_t e0, e1;
     prof_events(&e0, &e1);
-    csr_spmm<double>(s, at_, n, Y, ldy, X, ldx, nullptr, 0, 0.0, variant_for(n, true), e0, e1);
+    unsigned long long* st = take_stamp();
+    csr_spmm<double>(s, at_, n, Y, ldy, X, ldx, nullptr, 0, 0.0, variant_for(n, true), e0, e1, st);
     prof_commit(e0, e1, n, true, 8);
 }
 void CsrOperator::apply(hipStream_t s, const float* X, int n, int64_t ldx, float* Y, int64_t ldy) const {
     hipEvent_t e0, e1;
     prof_events(&e0, &e1);
-    csr_spmm<float>(s, a_, n, X, ldx, Y, ldy, nullptr, 0, 0.0, variant_for(n, false), e0, e1);
+    unsigned long long* st = take_stamp();
+    csr_spmm<float>(s, a_, n, X, ldx, Y, ldy, nullptr, 0, 0.0, variant_for(n, false), e0, e1, st);
     prof_commit(e0, e1, n, false, 4);
 }
 void CsrOperator::apply_t(hipStream_t s, const float* Y, int n, int64_t ldy, float* X, int64_t ldx) const {
     hipEvent_t e0, e1;
     prof_events(&e0, &e1);
-    csr_spmm<float>(s, at_, n, Y, ldy, X, ldx, nullptr, 0, 0.0, variant_for(n, true), e0, e1);
+    unsigned long long* st = take_stamp();
+    csr_spmm<float>(s, at_, n, Y, ldy, X, ldx, nullptr, 0, 0.0, variant_for(n, true), e0, e1, st);
     prof_commit(e0, e1, n, true, 4);
 }
 

@@ -56,6 +56,12 @@ public:
     // Selectable kernel variants (CSR SpMM); -1 = default.
     virtual void set_variant(int, bool, int) {}
 
+    // Operators whose product kernels time themselves: stamp_next() hands the
+    // next apply / apply_t the device values its kernels stamp their start
+    // and end into (ProductStamp); the product consumes them.
+    virtual bool times_own_kernels() const { return false; }
+    void stamp_next(unsigned long long* slots) const { stamp_ = slots; }
+
     // Kernel timing with HIP events recorded on the launching stream around
     // every product (off by default).  Stats: launches, seconds, algorithmic
     // bytes (product_bytes of each launch).
@@ -69,9 +75,15 @@ protected:
     // profiling is off), and the record of one product timed that way.
     void prof_events(hipEvent_t* start, hipEvent_t* stop) const;
     void prof_commit(hipEvent_t start, hipEvent_t stop, int ncols, bool transpose, int elem_bytes) const;
+    unsigned long long* take_stamp() const {
+        unsigned long long* st = stamp_;
+        stamp_ = nullptr;
+        return st;
+    }
     int64_t rows_ = 0, cols_ = 0;
 
 private:
+    mutable unsigned long long* stamp_ = nullptr;
     struct ProfPair { hipEvent_t start, stop; double bytes; };
     bool profiling_ = false;
     mutable std::vector<ProfPair> pending_;
@@ -100,6 +112,7 @@ public:
     double product_bytes(int s, bool transpose, int eb) const override;
     double product_flops(int s) const override { return 2.0 * static_cast<double>(nnz_) * s; }
     void set_variant(int s, bool transpose, int variant) override;
+    bool times_own_kernels() const override { return true; }
     // Times the SpMM variants whose column lanes fit each block width (the
     // next power of two and twice it) on this matrix, with FP64 or FP32
     // blocks, and keeps the fastest for A and A^T (median kernel time of

@@ -109,10 +109,11 @@ void Workspace<T>::apply_search(const T* X, int s, int64_t ldx, T* Y, int64_t ld
     const double charged_from = clk.now();
     clk.charge_operator(op_.relative_block_product_cost(s));
     if (!wall) *seconds += clk.now() - charged_from;
-    if (wall) ctx_.timer.begin(ctx_.stream);
+    const bool events = wall && !time_with_stamp(seconds);
+    if (events) ctx_.timer.begin(ctx_.stream);
     if (transpose) op_.apply_t(ctx_.stream, X, s, ldx, Y, ldy);
     else op_.apply(ctx_.stream, X, s, ldx, Y, ldy);
-    if (wall) ctx_.timer.end(ctx_.stream, seconds);
+    if (events) ctx_.timer.end(ctx_.stream, seconds);
     if (transpose) {
         stats_.at_columns += s;
         stats_.search_at_columns += s;
@@ -507,11 +508,21 @@ bool Workspace<T>::advance_level() {
 template <typename T>
 void Workspace<T>::enqueue_product(const T* X, int64_t ldx, T* Y, int64_t ldy, bool transpose) {
     // Counted and (on the synthetic clock) charged when the level commits.
-    const bool wall = !ctx_.clock.synthetic();
-    if (wall) ctx_.timer.begin(ctx_.stream);
+    double* seconds = transpose ? &stats_.at_seconds : &stats_.a_seconds;
+    const bool events = !ctx_.clock.synthetic() && !time_with_stamp(seconds);
+    if (events) ctx_.timer.begin(ctx_.stream);
     if (transpose) op_.apply_t(ctx_.stream, X, 1, ldx, Y, ldy);
     else op_.apply(ctx_.stream, X, 1, ldx, Y, ldy);
-    if (wall) ctx_.timer.end(ctx_.stream, transpose ? &stats_.at_seconds : &stats_.a_seconds);
+    if (events) ctx_.timer.end(ctx_.stream, seconds);
+}
+
+template <typename T>
+bool Workspace<T>::time_with_stamp(double* seconds) {
+    // wall clock with an operator that times its own kernels: the product
+    // adds its kernel time to `seconds` through the context's device stamp
+    if (ctx_.clock.synthetic() || !op_.times_own_kernels()) return false;
+    op_.stamp_next(ctx_.timer.kernel_stamp(ctx_.stream, seconds));
+    return true;
 }
 
 template <typename T>

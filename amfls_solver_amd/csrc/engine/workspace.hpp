@@ -112,6 +112,9 @@ private:
     int fast_batch_limit() const;
     int run_fast_batch(int count);  // returns levels committed
     void enqueue_product(const T* X, int64_t ldx, T* Y, int64_t ldy, bool transpose);
+    // Wall clock and an operator whose kernels time themselves: arms the
+    // operator's next product to add its kernel time to `seconds`.
+    bool time_with_stamp(double* seconds);
     double rank_factor(int k, int w) const;  // rank cutoff = factor * scale
     void sync_projected_rhs();
     double block_norm(const T* X, int64_t ldx, int64_t rows, int cols);

@@ -265,6 +265,24 @@ int amfls_csr_generate_entries(amfls_context* ctx, int64_t rows, int64_t cols, u
 int amfls_gaussian_fill(amfls_context* ctx, void* y, int64_t rows, int32_t cols, int64_t ld, int dtype,
                         uint64_t seed, uint64_t stream, uint64_t first_col);
 
+/* The engine's FP64 block kernels on row-major device blocks (the dense
+ * BLAS of the reference's Krylov workspace, linalg/blas_lapack.cpp dgemm
+ * and dnrm2 call sites; exposed so callers and tests can check them):
+ *   block_gemm_nn: Y (m x n, ldy) = alpha A C + beta Y, A m x k (lda),
+ *                  C k x n (ldc);
+ *   block_gemm_tn: C (k x n, ldc) = alpha A^T B, A m x k (lda), B m x n (ldb);
+ *   block_rowmul:  Y[i, 0:r] = X[i, 0:w] M, M w x r (ld r); Y may be X;
+ *   block_stats:   out[3] (host) = (max |v|, sum (|v| / 2^e)^2, nonfinite
+ *                  count), e the frexp exponent of the max.
+ * Each call waits for its kernels. */
+int amfls_block_gemm_nn(amfls_context* ctx, int64_t m, int32_t n, int32_t k, const double* a, int64_t lda,
+                        const double* c, int64_t ldc, double* y, int64_t ldy, double alpha, double beta);
+int amfls_block_gemm_tn(amfls_context* ctx, int64_t m, int32_t k, int32_t n, const double* a, int64_t lda,
+                        const double* b, int64_t ldb, double* c, int64_t ldc, double alpha);
+int amfls_block_rowmul(amfls_context* ctx, int64_t rows, int32_t w, int32_t r, const double* x, int64_t ldx,
+                       const double* m, double* y, int64_t ldy);
+int amfls_block_stats(amfls_context* ctx, int64_t rows, int32_t cols, const double* x, int64_t ldx, double* out);
+
 #ifdef __cplusplus
 }
 #endif

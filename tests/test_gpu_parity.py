@@ -280,7 +280,7 @@ def test_dense_matrix_core_products(torch_mod, vt):
     import amfls_solver_amd as am
     torch = torch_mod
     rng = np.random.default_rng(7)
-    for m, n in ((1000, 203), (37, 64), (4099, 1)):
+    for m, n in ((1000, 203), (600, 517), (37, 64), (4099, 1)):
         a = torch.from_numpy(rng.standard_normal((m, n))).to(getattr(torch, vt))
         stored = a.double().numpy()
         M = am.DeviceMatrix(a.contiguous(), value_dtype=getattr(torch, vt))

@@ -461,14 +461,57 @@ __global__ __launch_bounds__(kBlock) void chain_normalize_kernel(
     }
 }
 
+// Q[0][o] = sum over the nb blocks of Q[b][o], in place: thread (o, g) adds
+// blocks g, g + 8, ... in order, then the eight group sums are added in group
+// order; a block reads and writes only its own 32 columns (row 0 of a column
+// is read by its group 0 before the barrier and written after it).
+constexpr int kCombineCols = 32, kCombineGroups = kBlock / kCombineCols;
+__global__ __launch_bounds__(kBlock) void chain_combine_kernel(double* Q, int nb, int k, const double* go) {
+    if (*go == 0.0) return;
+    __shared__ double red[kCombineGroups][kCombineCols];
+    const int ol = threadIdx.x % kCombineCols, g = threadIdx.x / kCombineCols;
+    const int o = blockIdx.x * kCombineCols + ol;
+    double acc = 0.0;
+    if (o < k) {
+        int b = g;
+        for (; b + 3 * kCombineGroups < nb; b += 4 * kCombineGroups) {
+            double v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = Q[static_cast<int64_t>(b + u * kCombineGroups) * k + o];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc += v[u];
+        }
+        for (; b < nb; b += kCombineGroups) acc += Q[static_cast<int64_t>(b) * k + o];
+    }
+    red[g][ol] = acc;
+    __syncthreads();
+    if (g == 0 && o < k) {
+        double v = red[0][ol];
+#pragma unroll
+        for (int q = 1; q < kCombineGroups; ++q) v += red[q][ol];
+        Q[o] = v;
+    }
+}
+
 }  // namespace
 
+// Whether the nb x k projection partials of the chain kernels on N rows pass
+// through chain_combine_kernel before their consumer (wide bases: k > 256)
+// instead of every consumer block re-reading them all.
+static bool chain_combines(int64_t /*N*/, int k) { return k > 256; }
+
 int chain_block_rows(int64_t N, int k) {
-    // about 128 blocks; fewer, longer blocks when the k projection partials
-    // each consumer block combines would exceed 8K values
+    // about 128 blocks; for k <= 256, fewer and longer blocks when the k
+    // projection partials each consumer block combines would exceed 8K values
     const int64_t base = std::max<int64_t>(64, ceil_div(N, 128));
-    const int64_t wide = ceil_div(N * std::max(k, 1), 8192);
+    const int64_t wide = chain_combines(N, k) ? base : ceil_div(N * std::max(k, 1), 8192);
     return static_cast<int>(std::min<int64_t>(4096, std::max(base, wide)));
+}
+
+static void chain_combine(hipStream_t s, double* Q, int64_t nb, int k, const double* go) {
+    hipLaunchKernelGGL(chain_combine_kernel, dim3(static_cast<unsigned>(ceil_div(k, kCombineCols))), dim3(kBlock), 0, s,
+                       Q, static_cast<int>(nb), k, go);
+    AMFLS_LAUNCH_CHECK();
 }
 
 template <typename T>
@@ -480,6 +523,7 @@ void chain_couple_project(hipStream_t s, int64_t N, int k, T* r, int64_t ldr, co
     hipLaunchKernelGGL(couple_project_kernel<T>, dim3(static_cast<unsigned>(nb)), dim3(kBlock), R * sizeof(double), s,
                        N, k, R, r, ldr, p, ldp, coef, B, ldb, go, part_r, part_y, Q);
     AMFLS_LAUNCH_CHECK();
+    if (chain_combines(N, k)) chain_combine(s, Q, nb, k, go);
 }
 
 template <typename T>
@@ -491,9 +535,11 @@ void chain_update(hipStream_t s, int64_t N, int k, const T* B, int64_t ldb, cons
     const int64_t nb = ceil_div(N, R);
     const size_t lds = static_cast<size_t>(k) * sizeof(double);
     const dim3 grid(static_cast<unsigned>(nb));
+    const bool combined = chain_combines(N, k);  // Qin holds its sums in row 0
+    const int nbq = combined ? 1 : static_cast<int>(nb);
 #define AMFLS_UPDATE(J)                                                                                          \
-    hipLaunchKernelGGL((update_kernel<T, J>), grid, dim3(kBlock), lds, s, N, k, R, B, ldb, Qin,                 \
-                       static_cast<int>(nb), y, ldy, part_y, Qout, go, before, after, static_cast<int>(nb), factor)
+    hipLaunchKernelGGL((update_kernel<T, J>), grid, dim3(kBlock), lds, s, N, k, R, B, ldb, Qin, nbq, y, ldy,    \
+                       part_y, Qout, go, before, after, static_cast<int>(nb), factor)
     if (k <= 16) AMFLS_UPDATE(0);
     else if (k <= 64) AMFLS_UPDATE(1);
     else if (k <= 128) AMFLS_UPDATE(2);
@@ -503,6 +549,7 @@ void chain_update(hipStream_t s, int64_t N, int k, const T* B, int64_t ldb, cons
     else AMFLS_UPDATE(32);
 #undef AMFLS_UPDATE
     AMFLS_LAUNCH_CHECK();
+    if (Qout != nullptr && combined) chain_combine(s, Qout, nb, k, go);
 }
 
 template <typename T>

@@ -24,9 +24,11 @@ __device__ __forceinline__ bool gates_off(const double* a, const double* b) {
 }
 
 // c[o] = sum_i B[i, o] w[i] (o < k) over rows [0, N).  Block b owns rows
-// [b * rpb, (b + 1) * rpb) and writes its k sums to partial[b * k + o]; the
-// last block adds the partials in block order and writes c and coeff
-// (coeff = c, or coeff += c when accumulate is set).
+// [b * rpb, (b + 1) * rpb) and writes its k sums to partial[b * k + o]; for
+// k <= kBlock the last block adds the partials in block order and writes c
+// and coeff (coeff = c, or coeff += c when accumulate is set); wider bases
+// leave that to project_combine_kernel, whose threads share the nb x k
+// partials instead of one block reading them all.
 template <typename TB, typename TW>
 __global__ __launch_bounds__(kBlock) void project_kernel(int64_t N, int k, const TB* __restrict__ B, int64_t ldb,
                                                          const TW* __restrict__ w, int64_t ldw, int64_t rpb,
@@ -72,10 +74,22 @@ __global__ __launch_bounds__(kBlock) void project_kernel(int64_t N, int k, const
     } else {
         for (int o = t; o < k; o += kBlock) {
             double acc = 0.0;
-            for (int64_t i = r0; i < r1; ++i)
-                acc = fma(static_cast<double>(B[i * ldb + o]), static_cast<double>(w[i * ldw]), acc);
+            // eight rows' loads in flight per trip (the sum keeps row order)
+            int64_t i = r0;
+            for (; i + 8 <= r1; i += 8) {
+                double bv[8], wv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    bv[u] = static_cast<double>(B[(i + u) * ldb + o]);
+                    wv[u] = static_cast<double>(w[(i + u) * ldw]);
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) acc = fma(bv[u], wv[u], acc);
+            }
+            for (; i < r1; ++i) acc = fma(static_cast<double>(B[i * ldb + o]), static_cast<double>(w[i * ldw]), acc);
             partial[static_cast<int64_t>(blockIdx.x) * k + o] = acc;
         }
+        return;  // combined by project_combine_kernel
     }
     __threadfence();
     __syncthreads();
@@ -113,15 +127,43 @@ __global__ __launch_bounds__(kBlock) void project_kernel(int64_t N, int k, const
             c[t] = v;
             coeff[t] = accumulate ? coeff[t] + v : v;
         }
-    } else {
-        for (int o = t; o < k; o += kBlock) {
-            double acc = 0.0;
-            for (int b = 0; b < nb; ++b) acc += partial[static_cast<int64_t>(b) * k + o];
-            c[o] = acc;
-            coeff[o] = accumulate ? coeff[o] + acc : acc;
-        }
     }
     if (t == 0) *counter = 0u;
+}
+
+// c[o] = sum over blocks of partial[b][o] for k > kBlock: a block per 32
+// outputs; thread (o, g) adds blocks g, g + 8, ... in order, then the eight
+// group sums are added in group order.
+constexpr int kCombineCols = 32, kCombineGroups = kBlock / kCombineCols;
+__global__ __launch_bounds__(kBlock) void project_combine_kernel(int nb, int k, const double* __restrict__ partial,
+                                                                 double* __restrict__ c, double* __restrict__ coeff,
+                                                                 int accumulate, const double* gate_a,
+                                                                 const double* gate_b) {
+    if (gates_off(gate_a, gate_b)) return;
+    __shared__ double red[kCombineGroups][kCombineCols];
+    const int ol = threadIdx.x % kCombineCols, g = threadIdx.x / kCombineCols;
+    const int o = blockIdx.x * kCombineCols + ol;
+    double acc = 0.0;
+    if (o < k) {
+        int b = g;
+        for (; b + 3 * kCombineGroups < nb; b += 4 * kCombineGroups) {
+            double pv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) pv[u] = partial[static_cast<int64_t>(b + u * kCombineGroups) * k + o];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc += pv[u];
+        }
+        for (; b < nb; b += kCombineGroups) acc += partial[static_cast<int64_t>(b) * k + o];
+    }
+    red[g][ol] = acc;
+    __syncthreads();
+    if (g == 0 && o < k) {
+        double v = red[0][ol];
+#pragma unroll
+        for (int q = 1; q < kCombineGroups; ++q) v += red[q][ol];
+        c[o] = v;
+        coeff[o] = accumulate ? coeff[o] + v : v;
+    }
 }
 
 // w[i] = w[i] - sum_o B[i, o] c[o] (rounded once to T from double), then the
@@ -177,6 +219,7 @@ __global__ __launch_bounds__(kBlock) void update_kernel(int64_t N, int k, const 
         int rr = wv;
         for (; rr + 3 * dev::kWaves < nr; rr += 4 * dev::kWaves) {
             double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
             for (int o = lane; o < k; o += kWave) {
                 const double co = c[o];
 #pragma unroll
@@ -234,6 +277,12 @@ void rank_one_project(hipStream_t s, int64_t N, int k, const TB* B, int64_t ldb,
                        w, ldw, rpb, static_cast<double*>(scratch.ptr), c, coeff, accumulate ? 1 : 0, counter,
                        gate_a, gate_b);
     AMFLS_LAUNCH_CHECK();
+    if (k > kBlock) {
+        hipLaunchKernelGGL(project_combine_kernel, dim3(static_cast<unsigned>(ceil_div(k, kCombineCols))),
+                           dim3(kBlock), 0, s, static_cast<int>(nb), k, static_cast<const double*>(scratch.ptr), c,
+                           coeff, accumulate ? 1 : 0, gate_a, gate_b);
+        AMFLS_LAUNCH_CHECK();
+    }
 }
 
 template <typename T>

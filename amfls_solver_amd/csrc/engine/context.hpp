@@ -128,8 +128,9 @@ public:
         if (hstamp_) (void)hipHostFree(hstamp_);
     }
     // The kStampSlots device values that the next product's kernels stamp;
-    // the product's time is added to *target on flush().
-    unsigned long long* kernel_stamp(hipStream_t s, double* target) {
+    // the product's time is added to *target (and *target2, if given) on
+    // flush().
+    unsigned long long* kernel_stamp(hipStream_t s, double* target, double* target2 = nullptr) {
         if (dstamp_ == nullptr) {
             tick_seconds_ = wall_tick_seconds();
             grow(kInitialProducts, s);
@@ -141,7 +142,7 @@ public:
         }
         stamp_stream_ = s;
         unsigned long long* slots = dstamp_ + stamp_targets_.size() * kStampSlots;
-        stamp_targets_.push_back(target);
+        stamp_targets_.push_back({target, target2});
         return slots;
     }
     void begin(hipStream_t s) {
@@ -180,7 +181,10 @@ public:
                 const unsigned long long* v = hstamp_ + i * kStampSlots;
                 unsigned long long end = 0;
                 for (int j = 1; j < kStampSlots; ++j) end = std::max(end, v[j]);
-                if (v[0] != 0 && end > v[0]) *stamp_targets_[i] += tick_seconds_ * static_cast<double>(end - v[0]);
+                if (v[0] == 0 || end <= v[0]) continue;
+                const double sec = tick_seconds_ * static_cast<double>(end - v[0]);
+                *stamp_targets_[i].first += sec;
+                if (stamp_targets_[i].second) *stamp_targets_[i].second += sec;
             }
             stamp_targets_.clear();
         }
@@ -235,7 +239,7 @@ private:
     unsigned long long* hstamp_ = nullptr;
     size_t capacity_ = 0;
     hipStream_t stamp_stream_ = nullptr;
-    std::vector<double*> stamp_targets_;
+    std::vector<std::pair<double*, double*>> stamp_targets_;
     double tick_seconds_ = 0.0;
 };
 

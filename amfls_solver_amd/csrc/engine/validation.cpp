@@ -269,6 +269,18 @@ ValidationEngine::Stats3 ValidationEngine::stats_of(const double* v, int64_t n) 
     return {out[0], out[1], out[2]};
 }
 
+void ValidationEngine::stats_of_many(const double* const* v, const int64_t* n, int count, Stats3* out) {
+    // device slots 16.. of the context's statistics buffer (slot 8 holds a
+    // dot product of the ridge correction)
+    constexpr int kFirst = 16;
+    if (kFirst + 3 * count > static_cast<int>(ctx_.stats.size()))
+        throw std::logic_error("too many statistics in one batch");
+    for (int i = 0; i < count; ++i) ctx_.block_stats_async<double>(n[i], 1, v[i], 1, ctx_.stats.get() + kFirst + 3 * i);
+    double host[48];
+    ctx_.download(host, ctx_.stats.get() + kFirst, static_cast<size_t>(3 * count));
+    for (int i = 0; i < count; ++i) out[i] = {host[3 * i], host[3 * i + 1], host[3 * i + 2]};
+}
+
 void ValidationEngine::count(bool transpose, bool base, double sec) {
     RunStatistics& s = stats_;
     if (transpose) {
@@ -293,6 +305,18 @@ void ValidationEngine::count(bool transpose, bool base, double sec) {
 
 void ValidationEngine::apply_counted(const double* in, double* out, bool transpose, bool base) {
     Clock& clk = ctx_.clock;
+    if (!clk.synthetic() && op_.times_own_kernels()) {
+        // the product's kernels stamp their own time, added to the operator
+        // and validation seconds when the timer is flushed: no waits here
+        RunStatistics& st = stats_;
+        op_.stamp_next(ctx_.timer.kernel_stamp(ctx_.stream, transpose ? &st.at_seconds : &st.a_seconds,
+                                               base ? &st.base_validation_seconds
+                                                    : &st.ridge_correction_validation_seconds));
+        if (transpose) op_.apply_t(ctx_.stream, in, 1, 1, out, 1);
+        else op_.apply(ctx_.stream, in, 1, 1, out, 1);
+        count(transpose, base, 0.0);
+        return;
+    }
     if (!clk.synthetic()) ctx_.sync();
     const double start = clk.now();
     clk.charge_operator(op_.relative_block_product_cost(1));
@@ -457,8 +481,13 @@ LeastSquaresResult ValidationEngine::validate(const double* x_dev, double lambda
     sub_vec(ctx_.stream, m_, ax_.get(), b_dev_, r_.get());
     apply_counted(r_.get(), atr_.get(), true, true);
     fma_vec(ctx_.stream, n_, lambda, x_dev, atr_.get(), g_.get());
-    Stats3 sx = stats_of(x_dev, n_), sax = stats_of(ax_.get(), m_), sr = stats_of(r_.get(), m_);
-    Stats3 satr = stats_of(atr_.get(), n_), sg = stats_of(g_.get(), n_);
+    Stats3 s5[5];
+    {
+        const double* const vs[5] = {x_dev, ax_.get(), r_.get(), atr_.get(), g_.get()};
+        const int64_t ns[5] = {n_, m_, m_, n_, n_};
+        stats_of_many(vs, ns, 5, s5);
+    }
+    Stats3 sx = s5[0], sax = s5[1], sr = s5[2], satr = s5[3], sg = s5[4];
     auto populate = [&](double axn) {
         res.residual_norm = nominal(sr);
         res.normal_residual_norm = nominal(sg);
@@ -581,6 +610,10 @@ LeastSquaresResult ValidationEngine::validate(const double* x_dev, double lambda
     }
     res.solution.resize(n_);
     ctx_.download(res.solution.data(), x_dev, n_);
+    // the stream is idle after the download: resolve the product stamps so
+    // the statistics carry this validation's operator seconds
+    ctx_.timer.flush();
+    stats_.validation_seconds = stats_.base_validation_seconds + stats_.ridge_correction_validation_seconds;
     res.statistics = stats_;
     return res;
 }

@@ -41,6 +41,8 @@ public:
 private:
     struct Stats3 { double max = 0, sum = 0, bad = 0; };
     Stats3 stats_of(const double* v, int64_t n);
+    // Statistics of several device vectors with one wait.
+    void stats_of_many(const double* const* v, const int64_t* n, int count, Stats3* out);
     fp::Interval interval(const Stats3& s, int64_t n) const {
         return fp::norm_interval(s.max, s.sum, s.bad, n);
     }

@@ -537,29 +537,39 @@ LeastSquaresResult ValidationEngine::validate(const double* x_dev, double lambda
                         return;
                     }
                     scale_vec(ctx_.stream, n_, 1.0 / gnorm, g_.get(), d_.get());
-                    // reference divides elementwise (g / ||g||); one rounding either way
-                    const Stats3 sd = stats_of(d_.get(), n_);
+                    // reference divides elementwise (g / ||g||); one rounding either way.
+                    // d = g / ||g|| is finite whenever g is (checked by the base
+                    // pass), so A d is formed before d's norm is read: one wait for
+                    // both statistics.
+                    apply_counted(d_.get(), ad_.get(), false, false);
+                    Stats3 s2[2];
+                    {
+                        const double* const vs[2] = {d_.get(), ad_.get()};
+                        const int64_t ns[2] = {n_, m_};
+                        stats_of_many(vs, ns, 2, s2);
+                    }
+                    const Stats3 sd = s2[0], sad = s2[1];
                     if (sd.bad > 0) { disp = RidgeCorrection::skipped_nonfinite_candidate; return; }
                     const fp::Interval dn = interval(sd, n_);
-                    apply_counted(d_.get(), ad_.get(), false, false);
-                    const Stats3 sad = stats_of(ad_.get(), m_);
                     if (sad.bad > 0) { disp = RidgeCorrection::attempted_partial_failure; return; }
                     const fp::Interval adn = interval(sad, m_);
                     const double ad_err = callback_radius(model, false, dn.upper, m_);
                     apply_counted(ad_.get(), td_.get(), true, false);
                     const double t_err = callback_radius(model, true, adn.upper, n_);
                     fma_vec(ctx_.stream, n_, lambda, d_.get(), td_.get(), hd_.get());
-                    const Stats3 shd = stats_of(hd_.get(), n_);
+                    // curvature = d . hd (double dot on device), read with hd's norm
+                    gemm_tn<double>(ctx_.stream, n_, 1, 1, Strided{d_.get(), ValueType::f64, 1, 1}, hd_.get(), 1,
+                                    ctx_.stats.get() + 8, ValueType::f64, 1, 1.0, ctx_.scratch);
+                    ctx_.block_stats_async<double>(n_, 1, hd_.get(), 1, ctx_.stats.get() + 16);
+                    double h11[11];  // statistics buffer slots 8 .. 18
+                    ctx_.download(h11, ctx_.stats.get() + 8, 11);
+                    const double curvature = h11[0];
+                    const Stats3 shd{h11[8], h11[9], h11[10]};
                     if (shd.bad > 0) return;
                     const fp::Interval hdn = interval(shd, n_);
                     const double prop = fp::up_mul(env.operator_norm_upper_bound, ad_err);
                     const double hsum = one_op_radius(hdn.upper, lambda > 0.0 && dn.upper > 0.0, n_);
                     const double h_err = fp::up_add(fp::up_add(t_err, prop), hsum);
-                    // curvature = d . hd (double dot on device)
-                    gemm_tn<double>(ctx_.stream, n_, 1, 1, Strided{d_.get(), ValueType::f64, 1, 1}, hd_.get(), 1,
-                                    ctx_.stats.get() + 8, ValueType::f64, 1, 1.0, ctx_.scratch);
-                    double curvature = 0.0;
-                    ctx_.download(&curvature, ctx_.stats.get() + 8, 1);
                     if (!std::isfinite(curvature)) return;
                     if (!(curvature > 0.0)) { disp = RidgeCorrection::attempted_invalid_curvature; return; }
                     const double step = gnorm / curvature;

@@ -709,11 +709,11 @@ LeastSquaresResult Workspace<T>::evaluate(double tolerance, int epoch, int auxil
     ctx_.timer.flush();
     Clock& clk = ctx_.clock;
     const double start = clk.now();
-    DevBuf<double> x(static_cast<size_t>(n_));
-    candidate(x.get());
+    xcand_.ensure(static_cast<size_t>(n_));
+    candidate(xcand_.get());
     if (!clk.synthetic()) ctx_.sync();
     stats_.projected_solve_seconds += clk.now() - start;
-    LeastSquaresResult res = validator_.validate(x.get(), lambda_, tolerance);
+    LeastSquaresResult res = validator_.validate(xcand_.get(), lambda_, tolerance);
     res.depth = depth_;
     res.auxiliary_width = auxiliary_width;
     res.basis_rank = k_;

@@ -139,6 +139,7 @@ private:
     DevBuf<double> coeff_;   // k x w FP64 coefficient accumulation
     DevBuf<double> cbuf_;    // pass coefficients
     DevBuf<double> gdev_;    // g = U^T b entries computed on the device
+    DevBuf<double> xcand_;   // candidate solution of the current checkpoint (n)
     DevBuf<double> slots_;   // orthogonalization statistics (kSlotCount doubles)
     DevBuf<double> fast_cells_;  // go, gate, inverse norm, then one coupling cell per batched level
     DevBuf<double> fast_rec_;    // 8 doubles per batched level (right and left decisions)

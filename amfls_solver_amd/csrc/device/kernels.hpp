@@ -199,11 +199,12 @@ int csr_stream_budget();
 int csr_spmm_stream_variant();
 
 // Wall-clock stamps of one product taken inside its kernels (device memory,
-// zeroed before use): slot 0 = the device wall clock when block 0 of the
+// zeroed when allocated): slot 0 = the device wall clock when block 0 of the
 // first launch begins; slots 1..8 = when each of the last eight blocks of the
-// final launch (one per XCD, dispatched last there) ends.  The product's time
-// is max(slots 1..8) - slot 0, in ticks of hipDeviceAttributeWallClockRate.
-// No atomics: the stamps are plain stores read after the kernel.
+// final launch (one per XCD, dispatched last there) ends (a grid of fewer
+// blocks leaves older, smaller values in the rest).  The product's time is
+// max(slots 1..8) - slot 0, in ticks of hipDeviceAttributeWallClockRate.  No
+// atomics: the stamps are plain stores read after the kernel.
 constexpr int kStampSlots = 9;
 struct ProductStamp {
     unsigned long long* slots = nullptr;  // kStampSlots values, or null: untimed

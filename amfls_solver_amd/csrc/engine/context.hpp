@@ -164,18 +164,16 @@ public:
         pending_.clear();
         if (open_) pool_.push_back(open_);
         open_ = nullptr;
-        if (!stamp_targets_.empty()) {
-            AMFLS_HIP_CHECK(hipMemsetAsync(dstamp_, 0, stamp_targets_.size() * kStampSlots * sizeof(unsigned long long),
-                                           stamp_stream_));
-            stamp_targets_.clear();
-        }
+        stamp_targets_.clear();
     }
     void flush() {
         if (!stamp_targets_.empty()) {
             const size_t n = stamp_targets_.size() * kStampSlots;
+            // No clearing between uses: every launched product rewrites slot 0,
+            // and end slots left from an earlier product hold earlier clock
+            // values, below this product's own ends, so the maximum is unchanged.
             AMFLS_HIP_CHECK(hipMemcpyAsync(hstamp_, dstamp_, n * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                                            stamp_stream_));
-            AMFLS_HIP_CHECK(hipMemsetAsync(dstamp_, 0, n * sizeof(unsigned long long), stamp_stream_));
             AMFLS_HIP_CHECK(hipStreamSynchronize(stamp_stream_));
             for (size_t i = 0; i < stamp_targets_.size(); ++i) {
                 const unsigned long long* v = hstamp_ + i * kStampSlots;

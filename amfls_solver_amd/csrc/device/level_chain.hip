@@ -34,6 +34,8 @@ namespace amflsx {
 namespace {
 
 using dev::kBlock;
+// rows per thread whose loads a row loop issues together (see the row loops)
+constexpr int kTrip = 8;
 
 struct Stat {
     double m = 0.0;    // max |v| over finite values
@@ -240,14 +242,28 @@ __global__ __launch_bounds__(kBlock) void couple_project_kernel(int64_t N, int k
     const int nr = static_cast<int>(std::min<int64_t>(R, N - r0));
     const double a = *coef;
     Stat sr, sy;
-    for (int i = threadIdx.x; i < nr; i += kBlock) {
-        const int64_t g = r0 + i;
-        const double rv = static_cast<double>(r[g * ldr]);
-        const T yv = static_cast<T>(rv - static_cast<double>(p[g * ldp]) * a);
-        r[g * ldr] = yv;
-        ys[i] = static_cast<double>(yv);
-        sr.add(rv);
-        sy.add(static_cast<double>(yv));
+    // a thread's rows in trips of kTrip: every load of a trip is issued before
+    // its stores (r may be written in place), so a trip costs one round trip
+    for (int i0 = threadIdx.x; i0 < nr; i0 += kTrip * kBlock) {
+        double rv[kTrip], pv[kTrip];
+#pragma unroll
+        for (int u = 0; u < kTrip; ++u) {
+            const int i = i0 + u * kBlock;
+            const int64_t g = r0 + i;
+            rv[u] = i < nr ? static_cast<double>(r[g * ldr]) : 0.0;
+            pv[u] = i < nr ? static_cast<double>(p[g * ldp]) : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kTrip; ++u) {
+            const int i = i0 + u * kBlock;
+            if (i >= nr) break;
+            const int64_t g = r0 + i;
+            const T yv = static_cast<T>(rv[u] - pv[u] * a);
+            r[g * ldr] = yv;
+            ys[i] = static_cast<double>(yv);
+            sr.add(rv[u]);
+            sy.add(static_cast<double>(yv));
+        }
     }
     block_publish(sr, part_r, red);
     block_publish(sy, part_y, red);
@@ -280,6 +296,43 @@ __global__ __launch_bounds__(kBlock) void update_kernel(int64_t N, int k, int R,
     const int64_t r0 = static_cast<int64_t>(blockIdx.x) * R;
     const int nr = static_cast<int>(std::min<int64_t>(R, N - r0));
     Stat st;
+    if (J == 0 && k == 1) {  // one basis column: trips of kTrip rows per thread
+        const double c0 = cs[0];
+        double q0 = 0.0;
+        for (int r1 = t; r1 < nr; r1 += kTrip * kBlock) {
+            double bv[kTrip], yo[kTrip];
+#pragma unroll
+            for (int u = 0; u < kTrip; ++u) {
+                const int rr = r1 + u * kBlock;
+                const int64_t i = r0 + rr;
+                bv[u] = rr < nr ? static_cast<double>(B[i * ldb]) : 0.0;
+                yo[u] = rr < nr ? static_cast<double>(y[i * ldy]) : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < kTrip; ++u) {
+                const int rr = r1 + u * kBlock;
+                if (rr >= nr) break;
+                const T nv = static_cast<T>(yo[u] - fma(bv[u], c0, 0.0));
+                y[(r0 + rr) * ldy] = nv;
+                st.add(static_cast<double>(nv));
+                q0 = fma(bv[u], static_cast<double>(nv), q0);
+            }
+        }
+        block_publish(st, part_y, red);
+        if (Qout != nullptr) {
+            const int lane = t % kWave, w = t / kWave;
+            const double q = dev::wave_sum(q0);
+            if (lane == 0) wq[w][0] = q;
+            __syncthreads();
+            if (t == 0) {
+                double qs = 0.0;
+#pragma unroll
+                for (int w2 = 0; w2 < dev::kWaves; ++w2) qs += wq[w2][0];
+                Qout[static_cast<int64_t>(blockIdx.x)] = qs;
+            }
+        }
+        return;
+    }
     if (J == 0) {  // k <= 16: a thread per row
         double qa[16];
 #pragma unroll
@@ -448,12 +501,24 @@ __global__ __launch_bounds__(kBlock) void chain_normalize_kernel(
     const int64_t r0 = static_cast<int64_t>(blockIdx.x) * R;
     const int nr = static_cast<int>(std::min<int64_t>(R, N - r0));
     double acc = 0.0;
-    for (int i = threadIdx.x; i < nr; i += kBlock) {
-        const int64_t g = r0 + i;
-        const double v = static_cast<double>(y[g * ldy]);
-        const T nv = static_cast<T>(inv == 1.0 ? v : inv * v);
-        dst[g * ldd] = nv;
-        if (b != nullptr) acc = fma(static_cast<double>(nv), b[g], acc);
+    // trips of kTrip rows: loads before stores (dst may be y itself)
+    for (int i0 = threadIdx.x; i0 < nr; i0 += kTrip * kBlock) {
+        double v[kTrip], bv[kTrip];
+#pragma unroll
+        for (int u = 0; u < kTrip; ++u) {
+            const int i = i0 + u * kBlock;
+            const int64_t g = r0 + i;
+            v[u] = i < nr ? static_cast<double>(y[g * ldy]) : 0.0;
+            bv[u] = (i < nr && b != nullptr) ? b[g] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kTrip; ++u) {
+            const int i = i0 + u * kBlock;
+            if (i >= nr) break;
+            const T nv = static_cast<T>(inv == 1.0 ? v[u] : inv * v[u]);
+            dst[(r0 + i) * ldd] = nv;
+            if (b != nullptr) acc = fma(static_cast<double>(nv), bv[u], acc);
+        }
     }
     if (b != nullptr) {
         acc = dev::block_sum(acc, red);

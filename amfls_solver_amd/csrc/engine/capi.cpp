@@ -322,6 +322,8 @@ const char* amfls_spmm_variant_name(int v) { return csr_spmm_variant_name(v); }
 int amfls_spmm_variant_supports(int v, int s) { return csr_spmm_variant_supports(v, s) ? 1 : 0; }
 int amfls_spmm_default_variant(int s, double mean_row_entries) { return csr_spmm_default_variant(s, mean_row_entries); }
 
+}  // extern "C"
+
 namespace {
 
 SolveOptions solve_options(const amfls_options* options) {
@@ -376,6 +378,8 @@ WidthPolicy width_policy(const amfls_context* ctx, int code, std::vector<float>*
 }
 
 }  // namespace
+
+extern "C" {
 
 int amfls_solve(amfls_context* ctx, const amfls_operator* op, const double* b, int b_on_device,
                 const amfls_options* options, amfls_result** out) {

@@ -494,8 +494,8 @@ bool DenseOperator::refine_apply_t(const long double* y, long double* x) const {
 }
 
 // ------------------------------------------------------------------ augmented
-AugmentedOperator::AugmentedOperator(const DeviceOperator& base, double a, Context& ctx)
-    : base_(base), a_(a), ctx_(ctx) {
+AugmentedOperator::AugmentedOperator(const DeviceOperator& base, double a, Context& /*ctx*/)
+    : base_(base), a_(a) {
     rows_ = base.rows() + base.cols();
     cols_ = base.cols();
     const ValidationModel b = base.validation_model();

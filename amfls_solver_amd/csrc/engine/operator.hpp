@@ -259,7 +259,6 @@ private:
     const DeviceOperator& base_;
     double a_;
     ValidationModel model_;
-    Context& ctx_;
 };
 
 // Bounds from per-row / per-column |a| sums and square sums.

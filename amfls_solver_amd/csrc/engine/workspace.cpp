@@ -552,7 +552,6 @@ int Workspace<T>::run_fast_batch(int count) {
     Clock& clk = ctx_.clock;
     sync_projected_rhs();
     const int k0 = k_;
-    const int L0 = static_cast<int>(g_.size());
     ensure_basis(k0 + count);
     // partial buffers sized for the largest block counts of the batch
     int64_t nbmax = ceil_div(m_, chain_block_rows(m_, 1));

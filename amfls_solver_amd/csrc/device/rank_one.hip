@@ -23,12 +23,11 @@ __device__ __forceinline__ bool gates_off(const double* a, const double* b) {
     return dev::gated_off(a) || dev::gated_off(b);
 }
 
-// c[o] = sum_i B[i, o] w[i] (o < k) over rows [0, N).  Block b owns rows
-// [b * rpb, (b + 1) * rpb) and writes its k sums to partial[b * k + o]; for
-// k <= kBlock the last block adds the partials in block order and writes c
-// and coeff (coeff = c, or coeff += c when accumulate is set); wider bases
-// leave that to project_combine_kernel, whose threads share the nb x k
-// partials instead of one block reading them all.
+// c[o] = sum_i B[i, o] w[i] (o < k <= kBlock) over rows [0, N).  Block b owns
+// rows [b * rpb, (b + 1) * rpb) and writes its k sums to partial[b * k + o];
+// the last block adds the partials in block order and writes c and coeff
+// (coeff = c, or coeff += c when accumulate is set).  Wider bases go through
+// project_wide_kernel and project_combine_kernel.
 template <typename TB, typename TW>
 __global__ __launch_bounds__(kBlock) void project_kernel(int64_t N, int k, const TB* __restrict__ B, int64_t ldb,
                                                          const TW* __restrict__ w, int64_t ldw, int64_t rpb,
@@ -71,25 +70,6 @@ __global__ __launch_bounds__(kBlock) void project_kernel(int64_t N, int k, const
             __syncthreads();
         }
         if (t < k) partial[static_cast<int64_t>(blockIdx.x) * k + t] = red[t];
-    } else {
-        for (int o = t; o < k; o += kBlock) {
-            double acc = 0.0;
-            // eight rows' loads in flight per trip (the sum keeps row order)
-            int64_t i = r0;
-            for (; i + 8 <= r1; i += 8) {
-                double bv[8], wv[8];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    bv[u] = static_cast<double>(B[(i + u) * ldb + o]);
-                    wv[u] = static_cast<double>(w[(i + u) * ldw]);
-                }
-#pragma unroll
-                for (int u = 0; u < 8; ++u) acc = fma(bv[u], wv[u], acc);
-            }
-            for (; i < r1; ++i) acc = fma(static_cast<double>(B[i * ldb + o]), static_cast<double>(w[i * ldw]), acc);
-            partial[static_cast<int64_t>(blockIdx.x) * k + o] = acc;
-        }
-        return;  // combined by project_combine_kernel
     }
     __threadfence();
     __syncthreads();
@@ -129,6 +109,36 @@ __global__ __launch_bounds__(kBlock) void project_kernel(int64_t N, int k, const
         }
     }
     if (t == 0) *counter = 0u;
+}
+
+// Wide bases (k > kBlock): partial[y][o] = sum over the rows of row chunk y of
+// B[i, o] w[i]; block (x, y) owns columns [256 x, 256 x + 256) of chunk y, so
+// the grid has many blocks even when N is small; eight rows' loads are in
+// flight per trip and the sum keeps row order.
+template <typename TB, typename TW>
+__global__ __launch_bounds__(kBlock) void project_wide_kernel(int64_t N, int k, const TB* __restrict__ B, int64_t ldb,
+                                                              const TW* __restrict__ w, int64_t ldw, int64_t rpb,
+                                                              double* __restrict__ partial, const double* gate_a,
+                                                              const double* gate_b) {
+    if (gates_off(gate_a, gate_b)) return;
+    const int o = blockIdx.x * kBlock + threadIdx.x;
+    const int64_t r0 = static_cast<int64_t>(blockIdx.y) * rpb;
+    const int64_t r1 = std::min<int64_t>(N, r0 + rpb);
+    if (o >= k) return;
+    double acc = 0.0;
+    int64_t i = r0;
+    for (; i + 8 <= r1; i += 8) {
+        double bv[8], wv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            bv[u] = static_cast<double>(B[(i + u) * ldb + o]);
+            wv[u] = static_cast<double>(w[(i + u) * ldw]);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc = fma(bv[u], wv[u], acc);
+    }
+    for (; i < r1; ++i) acc = fma(static_cast<double>(B[i * ldb + o]), static_cast<double>(w[i * ldw]), acc);
+    partial[static_cast<int64_t>(blockIdx.y) * k + o] = acc;
 }
 
 // c[o] = sum over blocks of partial[b][o] for k > kBlock: a block per 32
@@ -267,6 +277,23 @@ void rank_one_project(hipStream_t s, int64_t N, int k, const TB* B, int64_t ldb,
                       double* c, double* coeff, bool accumulate, Scratch& scratch, unsigned* counter,
                       const double* gate_a, const double* gate_b) {
     if (N <= 0 || k <= 0) return;
+    if (k > kBlock) {
+        // row chunks of at least 64 rows, at most 64 of them
+        const int64_t nbr = std::max<int64_t>(1, std::min<int64_t>(64, ceil_div(N, 64)));
+        const int64_t rpb = ceil_div(N, nbr);
+        const int64_t chunks = ceil_div(N, rpb);
+        scratch.ensure(static_cast<size_t>(chunks * k) * sizeof(double));
+        double* partial = static_cast<double*>(scratch.ptr);
+        const dim3 grid(static_cast<unsigned>(ceil_div(k, kBlock)), static_cast<unsigned>(chunks));
+        hipLaunchKernelGGL((project_wide_kernel<TB, TW>), grid, dim3(kBlock), 0, s, N, k, B, ldb, w, ldw, rpb,
+                           partial, gate_a, gate_b);
+        AMFLS_LAUNCH_CHECK();
+        hipLaunchKernelGGL(project_combine_kernel, dim3(static_cast<unsigned>(ceil_div(k, kCombineCols))),
+                           dim3(kBlock), 0, s, static_cast<int>(chunks), k, static_cast<const double*>(partial), c,
+                           coeff, accumulate ? 1 : 0, gate_a, gate_b);
+        AMFLS_LAUNCH_CHECK();
+        return;
+    }
     const int64_t work = N * static_cast<int64_t>(k);
     int64_t nb = std::max<int64_t>(ceil_div(N, 1024), ceil_div(work, 16384));
     nb = std::max<int64_t>(1, std::min<int64_t>(nb, 256));
@@ -277,12 +304,6 @@ void rank_one_project(hipStream_t s, int64_t N, int k, const TB* B, int64_t ldb,
                        w, ldw, rpb, static_cast<double*>(scratch.ptr), c, coeff, accumulate ? 1 : 0, counter,
                        gate_a, gate_b);
     AMFLS_LAUNCH_CHECK();
-    if (k > kBlock) {
-        hipLaunchKernelGGL(project_combine_kernel, dim3(static_cast<unsigned>(ceil_div(k, kCombineCols))),
-                           dim3(kBlock), 0, s, static_cast<int>(nb), k, static_cast<const double*>(scratch.ptr), c,
-                           coeff, accumulate ? 1 : 0, gate_a, gate_b);
-        AMFLS_LAUNCH_CHECK();
-    }
 }
 
 template <typename T>
@@ -290,7 +311,10 @@ void rank_one_update(hipStream_t s, int64_t N, int k, const T* B, int64_t ldb, c
                      int64_t ldw, double* out, Scratch& scratch, unsigned* counter, const double* gate_a,
                      const double* gate_b, const double* before, double factor, double* gate_out) {
     if (N <= 0) return;
-    const int rpb = static_cast<int>(std::max<int64_t>(64, std::min<int64_t>(1024, ceil_div(N, 256))));
+    // wide bases (a wavefront per row): about 1024 blocks of at least 16 rows,
+    // so enough rows' loads are in flight; narrow ones: about 256 blocks
+    const int rpb = k > 16 ? static_cast<int>(std::max<int64_t>(16, std::min<int64_t>(1024, ceil_div(N, 1024))))
+                           : static_cast<int>(std::max<int64_t>(64, std::min<int64_t>(1024, ceil_div(N, 256))));
     const int64_t nb = ceil_div(N, rpb);
     scratch.ensure(static_cast<size_t>(3 * nb) * sizeof(double));
     const size_t lds = static_cast<size_t>(rpb) * sizeof(double);

@@ -273,7 +273,7 @@ void CsrOperator::apply_t(hipStream_t s, const float* Y, int n, int64_t ldy, flo
 template <typename T>
 void CsrOperator::apply_t_add(hipStream_t s, const T* Y, int n, int64_t ldy, T* X, int64_t ldx, const T* add,
                               int64_t ld_add, double a) const {
-    csr_spmm<T>(s, at_, n, Y, ldy, X, ldx, add, ld_add, a, variant_for(n, true));
+    csr_spmm<T>(s, at_, n, Y, ldy, X, ldx, add, ld_add, a, variant_for(n, true), nullptr, nullptr, take_stamp());
 }
 template void CsrOperator::apply_t_add<double>(hipStream_t, const double*, int, int64_t, double*, int64_t,
                                                const double*, int64_t, double) const;
@@ -511,6 +511,8 @@ AugmentedOperator::AugmentedOperator(const DeviceOperator& base, double a, Conte
 
 template <typename T>
 void AugmentedOperator::do_apply(hipStream_t s, const T* X, int n, int64_t ldx, T* Y, int64_t ldy) const {
+    // a CSR base stamps its own kernel (the a X rows' copy is not timed)
+    base_.stamp_next(take_stamp());
     base_.apply(s, X, n, ldx, Y, ldy);
     copy_block<T, T>(s, base_.cols(), n, X, ldx, Y + base_.rows() * ldy, ldy, a_);
 }
@@ -519,6 +521,7 @@ template <typename T>
 void AugmentedOperator::do_apply_t(hipStream_t s, const T* Y, int n, int64_t ldy, T* X, int64_t ldx) const {
     const T* tail = Y + base_.rows() * ldy;
     if (auto* csr = dynamic_cast<const CsrOperator*>(&base_)) {
+        csr->stamp_next(take_stamp());
         csr->apply_t_add<T>(s, Y, n, ldy, X, ldx, tail, ldy, a_);
         return;
     }

@@ -248,6 +248,8 @@ public:
     double product_bytes(int s, bool transpose, int eb) const override {
         return base_.product_bytes(s, transpose, eb) + 2.0 * eb * s * static_cast<double>(base_.cols());
     }
+    // a CSR base times its own kernels (the augmented rows' copy excepted)
+    bool times_own_kernels() const override { return base_.times_own_kernels(); }
     double product_flops(int s) const override { return base_.product_flops(s); }
 
 private:

@@ -275,7 +275,7 @@ __global__ __launch_bounds__(kBlock) void couple_project_kernel(int64_t N, int k
 // partials of the new y for a following pass, accumulated in the same read
 // of B.  Narrow bases (k <= 16): a thread per row.  Wider bases: a
 // wavefront per row, lanes over the k coefficients (J per lane).
-template <typename T, int J>
+template <typename T, int J, int LPR>
 __global__ __launch_bounds__(kBlock) void update_kernel(int64_t N, int k, int R, const T* __restrict__ B,
                                                         int64_t ldb, const double* Qin, int nbq, T* y, int64_t ldy,
                                                         double* part_y, double* Qout, const double* go,
@@ -374,43 +374,50 @@ __global__ __launch_bounds__(kBlock) void update_kernel(int64_t N, int k, int R,
         }
         return;
     }
-    // J >= 1: wavefront per row, lane holds coefficients lane + 64 j (j < J)
+    // J >= 1: a group of LPR lanes per row (LPR = 64: the whole wavefront),
+    // lane s of a group holds coefficients s + LPR j (j < J); a wavefront
+    // works on kWave / LPR rows at once, U rows per group per trip
+    constexpr int G = kWave / LPR;
     const int lane = t % kWave, w = t / kWave;
+    const int sub = lane % LPR, grp = lane / LPR;
     double qa[J > 0 ? J : 1];
     double cl[J > 0 ? J : 1];
 #pragma unroll
     for (int j = 0; j < J; ++j) {
         qa[j] = 0.0;
-        const int o = lane + kWave * j;
+        const int o = sub + LPR * j;
         cl[j] = o < k ? cs[o] : 0.0;
     }
-    // U rows per trip, their loads issued together (narrow J only: registers)
-    constexpr int U = (J > 0 && J <= 4) ? 4 : 1;
-    int rr = w;
-    for (; rr + (U - 1) * dev::kWaves < nr; rr += U * dev::kWaves) {
+    // U rows per trip, their loads issued together (registers bound U)
+    constexpr int U = (J > 0 && (LPR < kWave ? J <= 8 : J <= 4)) ? 4 : 1;
+    constexpr int kSlots = dev::kWaves * G;  // rows a block covers per step
+    const int slot = w * G + grp;
+    int rr = slot;
+    for (; rr + (U - 1) * kSlots < nr; rr += U * kSlots) {
         double bv[U][J > 0 ? J : 1], yo[U], acc[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int64_t i = r0 + rr + u * dev::kWaves;
+            const int64_t i = r0 + rr + u * kSlots;
             acc[u] = 0.0;
 #pragma unroll
             for (int j = 0; j < J; ++j) {
-                const int o = lane + kWave * j;
+                const int o = sub + LPR * j;
                 bv[u][j] = o < k ? static_cast<double>(B[i * ldb + o]) : 0.0;
             }
-            yo[u] = static_cast<double>(y[i * ldy]);  // every lane reads the same word
+            yo[u] = static_cast<double>(y[i * ldy]);  // every lane of the group reads the same word
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
 #pragma unroll
             for (int j = 0; j < J; ++j) acc[u] = fma(bv[u][j], cl[j], acc[u]);
-            acc[u] = dev::wave_sum(acc[u]);
+#pragma unroll
+            for (int off = LPR / 2; off > 0; off >>= 1) acc[u] += __shfl_xor(acc[u], off);
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int64_t i = r0 + rr + u * dev::kWaves;
+            const int64_t i = r0 + rr + u * kSlots;
             const T nv = static_cast<T>(yo[u] - acc[u]);
-            if (lane == 0) {
+            if (sub == 0) {
                 y[i * ldy] = nv;
                 st.add(static_cast<double>(nv));
             }
@@ -418,20 +425,21 @@ __global__ __launch_bounds__(kBlock) void update_kernel(int64_t N, int k, int R,
             for (int j = 0; j < J; ++j) qa[j] = fma(bv[u][j], static_cast<double>(nv), qa[j]);
         }
     }
-    for (; rr < nr; rr += dev::kWaves) {
+    for (; rr < nr; rr += kSlots) {
         const int64_t i = r0 + rr;
         double bv[J > 0 ? J : 1];
         double acc = 0.0;
 #pragma unroll
         for (int j = 0; j < J; ++j) {
-            const int o = lane + kWave * j;
+            const int o = sub + LPR * j;
             bv[j] = o < k ? static_cast<double>(B[i * ldb + o]) : 0.0;
             acc = fma(bv[j], cl[j], acc);
         }
-        acc = dev::wave_sum(acc);
+#pragma unroll
+        for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
         const double yo = static_cast<double>(y[i * ldy]);
         const T nv = static_cast<T>(yo - acc);
-        if (lane == 0) {
+        if (sub == 0) {
             y[i * ldy] = nv;
             st.add(static_cast<double>(nv));
         }
@@ -440,13 +448,19 @@ __global__ __launch_bounds__(kBlock) void update_kernel(int64_t N, int k, int R,
     }
     block_publish(st, part_y, red);
     if (Qout != nullptr) {
-        // Qout[block][o] = the kWaves wavefront sums of coefficient o, in order
+        // the G groups of a wavefront add their sums (butterfly over the group
+        // bits), then Qout[block][o] = the kWaves wavefront sums in order
 #pragma unroll
         for (int j = 0; j < J; ++j) {
-            wq[w][lane] = qa[j];
+#pragma unroll
+            for (int off = LPR; off < kWave; off <<= 1) qa[j] += __shfl_xor(qa[j], off);
+        }
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            if (grp == 0) wq[w][sub] = qa[j];
             __syncthreads();
-            if (t < kWave) {
-                const int o = t + kWave * j;
+            if (t < LPR) {
+                const int o = t + LPR * j;
                 double q = 0.0;
 #pragma unroll
                 for (int w2 = 0; w2 < dev::kWaves; ++w2) q += wq[w2][t];
@@ -602,16 +616,18 @@ void chain_update(hipStream_t s, int64_t N, int k, const T* B, int64_t ldb, cons
     const dim3 grid(static_cast<unsigned>(nb));
     const bool combined = chain_combines(N, k);  // Qin holds its sums in row 0
     const int nbq = combined ? 1 : static_cast<int>(nb);
-#define AMFLS_UPDATE(J)                                                                                          \
-    hipLaunchKernelGGL((update_kernel<T, J>), grid, dim3(kBlock), lds, s, N, k, R, B, ldb, Qin, nbq, y, ldy,    \
+#define AMFLS_UPDATE(J, LPR)                                                                                     \
+    hipLaunchKernelGGL((update_kernel<T, J, LPR>), grid, dim3(kBlock), lds, s, N, k, R, B, ldb, Qin, nbq, y, ldy, \
                        part_y, Qout, go, before, after, static_cast<int>(nb), factor)
-    if (k <= 16) AMFLS_UPDATE(0);
-    else if (k <= 64) AMFLS_UPDATE(1);
-    else if (k <= 128) AMFLS_UPDATE(2);
-    else if (k <= 256) AMFLS_UPDATE(4);
-    else if (k <= 512) AMFLS_UPDATE(8);
-    else if (k <= 1024) AMFLS_UPDATE(16);
-    else AMFLS_UPDATE(32);
+    // up to 128 columns: groups of 16 lanes per row (four rows per wavefront
+    // at once); wider bases: a wavefront per row
+    if (k <= 16) AMFLS_UPDATE(0, 64);
+    else if (k <= 64) AMFLS_UPDATE(4, 16);
+    else if (k <= 128) AMFLS_UPDATE(8, 16);
+    else if (k <= 256) AMFLS_UPDATE(4, 64);
+    else if (k <= 512) AMFLS_UPDATE(8, 64);
+    else if (k <= 1024) AMFLS_UPDATE(16, 64);
+    else AMFLS_UPDATE(32, 64);
 #undef AMFLS_UPDATE
     AMFLS_LAUNCH_CHECK();
     if (Qout != nullptr && combined) chain_combine(s, Qout, nb, k, go);

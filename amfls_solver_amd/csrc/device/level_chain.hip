@@ -194,6 +194,64 @@ __device__ void sum_partials(const double* Q, int nbq, int k, double* cs, double
 
 // Per-block projection partials Q[b][o] = sum over the block's rows of
 // B[i, o] * y[i], y read from shared memory (ys[row - r0]).
+// As project_block for k <= 16 J, with the rows spread over many more
+// threads: lanes in groups of 16, lane s of a group accumulates coefficients
+// s + 16 j over the rows of its slot (one slot per group, 16 per block),
+// four rows per trip; the groups' sums are added by a butterfly within each
+// wavefront, then across the four wavefronts in order.
+template <typename T, int J>
+__device__ void project_block_grouped(const T* __restrict__ B, int64_t ldb, int k, int64_t r0, int nr,
+                                      const double* ys, double* red, double* Q) {
+    constexpr int LPR = 16, G = kWave / LPR, kSlots = dev::kWaves * G;
+    const int t = threadIdx.x, lane = t % kWave, w = t / kWave;
+    const int sub = lane % LPR, grp = lane / LPR;
+    double acc[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) acc[j] = 0.0;
+    int i = w * G + grp;
+    for (; i + 3 * kSlots < nr; i += 4 * kSlots) {
+        double bv[4][J], yv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int row = i + u * kSlots;
+            yv[u] = ys[row];
+#pragma unroll
+            for (int j = 0; j < J; ++j) {
+                const int o = sub + LPR * j;
+                bv[u][j] = o < k ? static_cast<double>(B[(r0 + row) * ldb + o]) : 0.0;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int j = 0; j < J; ++j) acc[j] = fma(bv[u][j], yv[u], acc[j]);
+    }
+    for (; i < nr; i += kSlots) {
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            const int o = sub + LPR * j;
+            if (o < k) acc[j] = fma(static_cast<double>(B[(r0 + i) * ldb + o]), ys[i], acc[j]);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < J; ++j)
+#pragma unroll
+        for (int off = LPR; off < kWave; off <<= 1) acc[j] += __shfl_xor(acc[j], off);
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        if (grp == 0) red[w * LPR + sub] = acc[j];
+        __syncthreads();
+        if (t < LPR) {
+            const int o = t + LPR * j;
+            double q = 0.0;
+#pragma unroll
+            for (int w2 = 0; w2 < dev::kWaves; ++w2) q += red[w2 * LPR + t];
+            if (o < k) Q[static_cast<int64_t>(blockIdx.x) * k + o] = q;
+        }
+        __syncthreads();
+    }
+}
+
 template <typename T>
 __device__ void project_block(const T* __restrict__ B, int64_t ldb, int k, int64_t r0, int nr, const double* ys,
                               double* red, double* Q) {
@@ -267,7 +325,11 @@ __global__ __launch_bounds__(kBlock) void couple_project_kernel(int64_t N, int k
     }
     block_publish(sr, part_r, red);
     block_publish(sy, part_y, red);
-    project_block<T>(B, ldb, k, r0, nr, ys, red, Q);
+    // (the statistics above leave red free after their barrier).  Up to 16
+    // columns project_block already gives every thread its own row stream.
+    if (k > 16 && k <= 64) project_block_grouped<T, 4>(B, ldb, k, r0, nr, ys, red, Q);
+    else if (k > 64 && k <= 128) project_block_grouped<T, 8>(B, ldb, k, r0, nr, ys, red, Q);
+    else project_block<T>(B, ldb, k, r0, nr, ys, red, Q);
 }
 
 // y -= B c with c = sum_b Qin[b] (gated: second pass only when the DGKS

@@ -161,34 +161,6 @@ __device__ __forceinline__ bool needs_second_pass(const double* sb, const double
 // b = t / kp, t / kp + S, ... (S = 256 / kp) in order, then a shared-memory
 // tree over the S threads of each coefficient.  Loads are issued sixteen at
 // a time.  All threads must call it; red holds kBlock doubles.
-// Sum over the S = kBlock / kp threads t = o + kp * sidx that share output o
-// (kp a power of two): within a wavefront by a butterfly over the sidx bits,
-// then the kWaves wavefront sums in order (kp < kWave); with kp >= kWave the
-// S <= 4 partial sums of o lie in different wavefronts and are added in
-// order.  One barrier either way (two with the reuse guard); red holds kBlock
-// doubles.  Returns the sum in threads t < kp.
-__device__ __forceinline__ double reduce_shared_outputs(double acc, int kp, double* red) {
-    const int t = threadIdx.x, lane = t % kWave, w = t / kWave;
-    double v = 0.0;
-    if (kp < kWave) {
-        for (int off = kp; off < kWave; off <<= 1) acc += __shfl_xor(acc, off);
-        if (lane < kp) red[w * kp + lane] = acc;
-        __syncthreads();
-        if (t < kp) {
-#pragma unroll
-            for (int w2 = 0; w2 < dev::kWaves; ++w2) v += red[w2 * kp + t];
-        }
-    } else {
-        red[t] = acc;
-        __syncthreads();
-        if (t < kp) {
-            for (int q = 0; q < kBlock / kp; ++q) v += red[t + q * kp];
-        }
-    }
-    __syncthreads();
-    return v;
-}
-
 __device__ void sum_partials(const double* Q, int nbq, int k, double* cs, double* red) {
     const int t = threadIdx.x;
     for (int o0 = 0; o0 < k; o0 += kBlock) {
@@ -209,7 +181,7 @@ __device__ void sum_partials(const double* Q, int nbq, int k, double* cs, double
             }
             for (; b < nbq; b += S) acc += Q[static_cast<int64_t>(b) * k + o0 + o];
         }
-        const double v = reduce_shared_outputs(acc, kp, red);
+        const double v = dev::reduce_shared_outputs(acc, kp, red);
         if (t < kc) cs[o0 + t] = v;
         __syncthreads();
     }
@@ -298,7 +270,7 @@ __device__ void project_block(const T* __restrict__ B, int64_t ldb, int k, int64
             }
             for (; i < nr; i += S) acc = fma(static_cast<double>(B[(r0 + i) * ldb + oc]), ys[i], acc);
         }
-        const double v = reduce_shared_outputs(acc, kp, red);
+        const double v = dev::reduce_shared_outputs(acc, kp, red);
         if (t < kc) Q[static_cast<int64_t>(blockIdx.x) * k + o0 + t] = v;
     }
 }

@@ -63,13 +63,8 @@ __global__ __launch_bounds__(kBlock) void project_kernel(int64_t N, int k, const
             }
             for (; i < r1; i += S) acc = fma(static_cast<double>(B[i * ldb + o]), static_cast<double>(w[i * ldw]), acc);
         }
-        red[t] = acc;
-        __syncthreads();
-        for (int stride = S / 2; stride > 0; stride >>= 1) {
-            if (sidx < stride) red[t] += red[t + stride * kp];
-            __syncthreads();
-        }
-        if (t < k) partial[static_cast<int64_t>(blockIdx.x) * k + t] = red[t];
+        const double v = dev::reduce_shared_outputs(acc, kp, red);
+        if (t < k) partial[static_cast<int64_t>(blockIdx.x) * k + t] = v;
     }
     __threadfence();
     __syncthreads();
@@ -96,14 +91,8 @@ __global__ __launch_bounds__(kBlock) void project_kernel(int64_t N, int k, const
             }
             for (; b < nb; b += S) acc += partial[static_cast<int64_t>(b) * k + o];
         }
-        red[t] = acc;
-        __syncthreads();
-        for (int stride = S / 2; stride > 0; stride >>= 1) {
-            if (sidx < stride) red[t] += red[t + stride * kp];
-            __syncthreads();
-        }
+        const double v = dev::reduce_shared_outputs(acc, kp, red);
         if (t < k) {
-            const double v = red[t];
             c[t] = v;
             coeff[t] = accumulate ? coeff[t] + v : v;
         }

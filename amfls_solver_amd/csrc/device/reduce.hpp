@@ -27,6 +27,34 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// Sum over the S = kBlock / kp threads t = o + kp * sidx that share output o
+// (kp a power of two): within a wavefront by a butterfly over the sidx bits,
+// then the kWaves wavefront sums in order (kp < kWave); with kp >= kWave the
+// S <= 4 partial sums of o lie in different wavefronts and are added in
+// order.  One barrier either way (two with the reuse guard); red holds kBlock
+// doubles.  Returns the sum in threads t < kp.
+__device__ __forceinline__ double reduce_shared_outputs(double acc, int kp, double* red) {
+    const int t = threadIdx.x, lane = t % kWave, w = t / kWave;
+    double v = 0.0;
+    if (kp < kWave) {
+        for (int off = kp; off < kWave; off <<= 1) acc += __shfl_xor(acc, off);
+        if (lane < kp) red[w * kp + lane] = acc;
+        __syncthreads();
+        if (t < kp) {
+#pragma unroll
+            for (int w2 = 0; w2 < kWaves; ++w2) v += red[w2 * kp + t];
+        }
+    } else {
+        red[t] = acc;
+        __syncthreads();
+        if (t < kp) {
+            for (int q = 0; q < kBlock / kp; ++q) v += red[t + q * kp];
+        }
+    }
+    __syncthreads();
+    return v;
+}
+
 // ||v|| from (max |v|, sum (|v| 2^-e)^2, nonfinite count), e = frexp
 // exponent of the max; NaN when a value was nonfinite.
 __device__ __forceinline__ double nominal_norm(const double* st) {

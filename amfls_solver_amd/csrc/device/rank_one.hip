@@ -300,10 +300,9 @@ void rank_one_update(hipStream_t s, int64_t N, int k, const T* B, int64_t ldb, c
                      int64_t ldw, double* out, Scratch& scratch, unsigned* counter, const double* gate_a,
                      const double* gate_b, const double* before, double factor, double* gate_out) {
     if (N <= 0) return;
-    // wide bases (a wavefront per row): about 1024 blocks of at least 16 rows,
-    // so enough rows' loads are in flight; narrow ones: about 256 blocks
-    const int rpb = k > 16 ? static_cast<int>(std::max<int64_t>(16, std::min<int64_t>(1024, ceil_div(N, 1024))))
-                           : static_cast<int>(std::max<int64_t>(64, std::min<int64_t>(1024, ceil_div(N, 256))));
+    // about 256 blocks of at least 64 rows (smaller blocks, more of them, made
+    // the wide-basis update slower: every block takes a ticket on one counter)
+    const int rpb = static_cast<int>(std::max<int64_t>(64, std::min<int64_t>(1024, ceil_div(N, 256))));
     const int64_t nb = ceil_div(N, rpb);
     scratch.ensure(static_cast<size_t>(3 * nb) * sizeof(double));
     const size_t lds = static_cast<size_t>(rpb) * sizeof(double);

@@ -573,7 +573,7 @@ def run_amfls_sweep(matrix, b, lambdas, *, tolerance: float = 1e-8, failure_prob
     return out
 
 
-def run_amfls_batch(matrices, rhs, lambdas, *, workers: int = 4, tolerance: float = 1e-8,
+def run_amfls_batch(matrices, rhs, lambdas, *, workers: int = 6, tolerance: float = 1e-8,
                     failure_probability: float = 1e-6, maximum_epochs: int = 0, maximum_depth: int = 0,
                     maximum_auxiliary_width: int = 0, maximum_basis_size: int = 0, seed: int = 0, stream: int = 0,
                     precision: str = "fp64", clock: str = "wall", width_policy: int = 0,

@@ -270,6 +270,37 @@ def test_torch_ops_entry_points(torch_mod, fx):
     assert o2[2][9] == 6  # depth counter
 
 
+def test_torch_ops_every_dispatch_key(torch_mod, fx):
+    """run_amfls through each registered key on one ridge problem: host CSR
+    (the reference's SparseCsrCPU path), device CSR, host and device dense,
+    and a non-contiguous dense view (converted like the reference's
+    .contiguous()).  Every call certifies, and its true relative energy error
+    against a dense FP64 ridge solve is within the tolerance."""
+    torch = torch_mod
+    lam, tol = 1e-3, 1e-8
+    A_host = csr_tensor(torch, fx["csr_rp"], fx["csr_ci"], fx["csr_val"], (3000, 700))
+    b_host = torch.from_numpy(fx["csr_b"])
+    D = A_host.to_dense().to(torch.float64)
+    Dn = D.numpy()
+    H = Dn.T @ Dn + lam * np.eye(Dn.shape[1])
+    xs = np.linalg.solve(H, Dn.T @ fx["csr_b"])
+    denom = np.sqrt(xs @ H @ xs)
+    cases = {
+        "csr_cpu": (A_host, b_host),
+        "csr_cuda": (A_host.to("cuda"), b_host.cuda()),
+        "dense_cpu": (D, b_host),
+        "dense_cuda": (D.cuda(), b_host.cuda()),
+        "dense_view_cuda": (D.T.contiguous().cuda().T, b_host.cuda()),
+    }
+    assert not cases["dense_view_cuda"][0].is_contiguous()
+    for name, (A, b) in cases.items():
+        out = torch.ops.amfls.run_amfls(A, b, lam, tol, 1e-6, seed=5)
+        assert out[6] == 0, name
+        assert out[0].device.type == "cpu" and out[0].shape == (700,), name
+        d = out[0].numpy() - xs
+        assert np.sqrt(d @ H @ d) / denom <= tol, name
+
+
 @pytest.mark.parametrize("vt", ["bfloat16", "float32", "float64"])
 def test_dense_matrix_core_products(torch_mod, vt):
     """Row-major dense products: widths up to 8 on both the vector-ALU

@@ -504,6 +504,9 @@ void csr_spmm(hipStream_t s, const CsrView& A, int ncols, const T* X, int64_t ld
         // keep a caller's event pair well defined for an empty product
         if (ev_start) AMFLS_HIP_CHECK(hipEventRecord(ev_start, s));
         if (ev_stop) AMFLS_HIP_CHECK(hipEventRecord(ev_stop, s));
+        // and a zero start slot, so the timer charges an empty product nothing
+        // (the slot may still hold an earlier product's stamps)
+        if (stamp) AMFLS_HIP_CHECK(hipMemsetAsync(stamp, 0, sizeof(unsigned long long), s));
         return;
     }
     // Blocks wider than 256 columns are processed in 256-column slabs.

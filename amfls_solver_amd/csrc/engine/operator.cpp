@@ -46,12 +46,14 @@ ValidationModel norm_model_from_sums(const std::vector<double>& row_sum, const s
 
 // ------------------------------------------------------------------ profiling
 void DeviceOperator::set_profiling(bool on) {
+    std::lock_guard<std::mutex> g(prof_mu_);
     profiling_ = on;
     if (!on) resolve_pending();
 }
 
 void DeviceOperator::prof_begin(hipStream_t s) const {
     if (!profiling_) return;
+    std::lock_guard<std::mutex> g(prof_mu_);
     hipEvent_t e;
     if (pool_.empty()) AMFLS_HIP_CHECK(hipEventCreate(&e));
     else { e = pool_.back(); pool_.pop_back(); }
@@ -60,7 +62,9 @@ void DeviceOperator::prof_begin(hipStream_t s) const {
 }
 
 void DeviceOperator::prof_end(hipStream_t s, int ncols, bool transpose, int eb) const {
-    if (!profiling_ || open_start_ == nullptr) return;
+    if (!profiling_) return;
+    std::lock_guard<std::mutex> g(prof_mu_);
+    if (open_start_ == nullptr) return;
     hipEvent_t e;
     if (pool_.empty()) AMFLS_HIP_CHECK(hipEventCreate(&e));
     else { e = pool_.back(); pool_.pop_back(); }
@@ -72,6 +76,7 @@ void DeviceOperator::prof_end(hipStream_t s, int ncols, bool transpose, int eb) 
 void DeviceOperator::prof_events(hipEvent_t* start, hipEvent_t* stop) const {
     *start = *stop = nullptr;
     if (!profiling_) return;
+    std::lock_guard<std::mutex> g(prof_mu_);
     for (hipEvent_t* e : {start, stop}) {
         if (pool_.empty()) AMFLS_HIP_CHECK(hipEventCreate(e));
         else { *e = pool_.back(); pool_.pop_back(); }
@@ -80,6 +85,7 @@ void DeviceOperator::prof_events(hipEvent_t* start, hipEvent_t* stop) const {
 
 void DeviceOperator::prof_commit(hipEvent_t start, hipEvent_t stop, int ncols, bool transpose, int eb) const {
     if (start == nullptr || stop == nullptr) return;
+    std::lock_guard<std::mutex> g(prof_mu_);
     pending_.push_back({start, stop, product_bytes(ncols, transpose, eb)});
 }
 
@@ -98,6 +104,7 @@ void DeviceOperator::resolve_pending() {
 }
 
 void DeviceOperator::kernel_stats(long long* launches, double* seconds, double* bytes) {
+    std::lock_guard<std::mutex> g(prof_mu_);
     resolve_pending();
     *launches = launches_;
     *seconds = seconds_;

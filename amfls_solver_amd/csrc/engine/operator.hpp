@@ -58,13 +58,16 @@ public:
 
     // Operators whose product kernels time themselves: stamp_next() hands the
     // next apply / apply_t the device values its kernels stamp their start
-    // and end into (ProductStamp); the product consumes them.
+    // and end into (ProductStamp); the product consumes them.  The hand-off
+    // is per host thread (every caller sets it and runs the product on the
+    // same thread), so workers that share one operator cannot swap stamps.
     virtual bool times_own_kernels() const { return false; }
     void stamp_next(unsigned long long* slots) const { stamp_ = slots; }
 
     // Kernel timing with HIP events recorded on the launching stream around
     // every product (off by default).  Stats: launches, seconds, algorithmic
-    // bytes (product_bytes of each launch).
+    // bytes (product_bytes of each launch).  The event queues are locked; the
+    // begin/end pairing assumes one thread drives the operator while it is on.
     void set_profiling(bool on);
     void kernel_stats(long long* launches, double* seconds, double* bytes);
 
@@ -83,8 +86,9 @@ protected:
     int64_t rows_ = 0, cols_ = 0;
 
 private:
-    mutable unsigned long long* stamp_ = nullptr;
+    static inline thread_local unsigned long long* stamp_ = nullptr;
     struct ProfPair { hipEvent_t start, stop; double bytes; };
+    mutable std::mutex prof_mu_;  // products of one operator may run on several threads
     bool profiling_ = false;
     mutable std::vector<ProfPair> pending_;
     mutable std::vector<hipEvent_t> pool_;

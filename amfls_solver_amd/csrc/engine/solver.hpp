@@ -44,7 +44,8 @@ std::vector<LeastSquaresResult> solve_amfls_sweep(const DeviceOperator& op, cons
 // Independent problems solved concurrently: `workers` host threads, each
 // with its own context and stream on the caller's device, take problems in
 // order and run their regularization sweeps.  Operators must not hold
-// per-context scratch shared across threads (CSR operators qualify).
+// per-context scratch shared across threads (CSR operators qualify, also
+// when one operator serves several problems).
 struct BatchProblem {
     const DeviceOperator* op = nullptr;
     const double* b_dev = nullptr;

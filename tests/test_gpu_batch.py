@@ -112,6 +112,31 @@ def test_batch_matches_serial_sweeps(torch_mod):
         am.run_amfls_batch(mats[:1], bs[:1], lams[:1], workers=1, precision="mixed")
 
 
+def test_batch_shares_one_operator_across_workers(torch_mod):
+    """One CSR matrix serving four problems (different right-hand sides) on
+    four concurrent workers with the wall clock, so every product carries a
+    timing stamp: each result certifies, its true relative energy error
+    against a dense FP64 ridge solve is within the tolerance, and its own
+    product times were charged."""
+    import amfls_solver_amd as am
+    torch = torch_mod
+    m, n, tol = 6000, 400, 1e-8
+    M, (rp, ci, val) = _csr(torch, m, n, (4, 32), 77, value_dtype=torch.float32)
+    rng = np.random.default_rng(8)
+    bh = [rng.standard_normal(m) for _ in range(4)]
+    lams = np.tile(np.logspace(-4, -1, 3), (4, 1))
+    batch = am.run_amfls_batch([M] * 4, [torch.from_numpy(b).cuda() for b in bh], lams, workers=4, tolerance=tol)
+    rp, ci, val = rp.cpu().numpy(), ci.cpu().numpy(), val.cpu().numpy().astype(np.float64)
+    for i in range(4):
+        for lam, r in zip(lams[i], batch[i]):
+            assert r.status == 0, (i, lam)
+            xs, A = _ridge(rp, ci, val, (m, n), bh[i], lam)
+            d = r.solution.numpy() - xs
+            K = A.T @ A + lam * np.eye(n)
+            assert np.sqrt(d @ K @ d) / np.sqrt(xs @ K @ xs) <= tol, (i, lam)
+        assert batch[i][0].a_seconds > 0.0 and batch[i][0].at_seconds > 0.0
+
+
 def test_host_operator_matches_device_csr(torch_mod):
     """A matrix-free host operator (custom MatrixOperator counterpart) with
     the device operator's validation model certifies the same problem to

@@ -46,6 +46,8 @@ class Options(ctypes.Structure):
         ("maximum_refinements", c_int32),
         ("inner_tolerance", c_double),
         ("width_policy", c_int32),
+        ("forward_tolerance", c_double),
+        ("maximum_refinement_iterations", c_int32),
     ]
 
 
@@ -91,6 +93,7 @@ SIGNATURES = {
                                   POINTER(c_int64), POINTER(c_int64)]),
     "amfls_result_trace_extension": (c_int, [c_void_p, c_void_p]),
     "amfls_result_refinement_steps": (c_int, [c_void_p, POINTER(c_int32)]),
+    "amfls_result_forward_error_estimate": (c_int, [c_void_p, POINTER(c_double)]),
     "amfls_result_destroy": (c_int, [c_void_p]),
     "amfls_gaussian_fill": (c_int, [c_void_p, c_void_p, c_int64, c_int32, c_int64, c_int, c_uint64, c_uint64,
                                     c_uint64]),

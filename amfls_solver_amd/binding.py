@@ -103,6 +103,8 @@ class SolveResult:
     # engine extensions
     trace_extension: torch.Tensor | None = None
     refinement_steps: int = 0
+    # mixed precision: forward-error estimate ||P P^T g|| / ||x|| (NaN if none)
+    forward_error_estimate: float = float("nan")
 
     @property
     def success(self) -> bool:
@@ -443,19 +445,22 @@ def _result_tuple(ctx: Context, res_handle: ctypes.c_void_p):
         ext = np.empty((rows.value, 3), dtype=np.float64)
         st, sr = ctypes.c_int64(), ctypes.c_int64()
         steps = ctypes.c_int32()
+        fwd = ctypes.c_double()
         ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
         _lib.check(lib.amfls_result_copy(res_handle, ptr(sol), ptr(met), ptr(cnt), ptr(tim), ptr(tf), ptr(ti),
                                          ctypes.byref(st), ctypes.byref(sr)))
         _lib.check(lib.amfls_result_trace_extension(res_handle, ptr(ext)))
         _lib.check(lib.amfls_result_refinement_steps(res_handle, ctypes.byref(steps)))
+        _lib.check(lib.amfls_result_forward_error_estimate(res_handle, ctypes.byref(fwd)))
     finally:
         lib.amfls_result_destroy(res_handle)
     raw = (torch.from_numpy(sol), torch.from_numpy(met), torch.from_numpy(cnt), torch.from_numpy(tim),
            torch.from_numpy(tf), torch.from_numpy(ti), int(st.value), int(sr.value))
-    return raw, torch.from_numpy(ext), int(steps.value)
+    extra = {"refinement_steps": int(steps.value), "forward_error_estimate": float(fwd.value)}
+    return raw, torch.from_numpy(ext), extra
 
 
-def _decode(raw, ext=None, steps=0) -> SolveResult:
+def _decode(raw, ext=None, extra=None) -> SolveResult:
     """reference binding.py:159-255."""
     if not isinstance(raw, tuple) or len(raw) != 8:
         raise RuntimeError("AMFLS binding result must be an eight-item tuple")
@@ -474,7 +479,8 @@ def _decode(raw, ext=None, steps=0) -> SolveResult:
     for i, name in enumerate(_TIMING_NAMES):
         kw[name] = float(timings[i])
     kw.update(trace=trace_float[:, :18], trace_float=trace_float, trace_int=trace_int, status=int(status),
-              stop_reason=int(reason), trace_extension=ext, refinement_steps=steps)
+              stop_reason=int(reason), trace_extension=ext)
+    kw.update(extra or {})
     return SolveResult(**kw)
 
 
@@ -487,8 +493,9 @@ def _operator(matrix, value_dtype, context):
 def run_amfls_raw(matrix, b, regularization=0.0, tolerance=1e-8, failure_probability=1e-6, maximum_epochs=0,
                   maximum_depth=0, maximum_auxiliary_width=0, maximum_basis_size=0, seed=0, stream=0, *,
                   precision="fp64", clock="wall", value_dtype=None, maximum_refinements=8,
-                  inner_tolerance=0.0, width_policy=0, context=None):
-    """The engine call behind torch.ops.amfls.run_amfls; returns (8-tuple, ext, steps)."""
+                  inner_tolerance=0.0, width_policy=0, context=None, forward_tolerance=0.0,
+                  maximum_refinement_iterations=0):
+    """The engine call behind torch.ops.amfls.run_amfls; returns (8-tuple, ext, extra)."""
     op = _operator(matrix, value_dtype, context)
     ctx = op.context
     dev = op._val.device
@@ -501,7 +508,9 @@ def run_amfls_raw(matrix, b, regularization=0.0, tolerance=1e-8, failure_probabi
                      maximum_basis_size=int(maximum_basis_size), seed=int(seed) % _UINT64_MODULUS,
                      stream=int(stream) % _UINT64_MODULUS, precision=_lib.PRECISION[precision],
                      clock_mode=_lib.CLOCK[clock], maximum_refinements=int(maximum_refinements),
-                     inner_tolerance=float(inner_tolerance), width_policy=int(width_policy))
+                     inner_tolerance=float(inner_tolerance), width_policy=int(width_policy),
+                     forward_tolerance=float(forward_tolerance),
+                     maximum_refinement_iterations=int(maximum_refinement_iterations))
     res = ctypes.c_void_p()
     torch.cuda.synchronize()
     _lib.check(ctx._lib.amfls_solve(ctx.handle, op.handle, bd.data_ptr(), 1, ctypes.byref(o), ctypes.byref(res)))
@@ -512,18 +521,24 @@ def run_amfls(matrix, b, *, regularization: float = 0.0, tolerance: float = 1e-8
               failure_probability: float = 1e-6, maximum_epochs: int = 0, maximum_depth: int = 0,
               maximum_auxiliary_width: int = 0, maximum_basis_size: int = 0, seed: int = 0, stream: int = 0,
               precision: str = "fp64", clock: str = "wall", value_dtype=None, maximum_refinements: int = 8,
-              inner_tolerance: float = 0.0, width_policy: int = 0, context: Context | None = None) -> SolveResult:
-    """reference binding.py:259-290 on the HIP engine."""
+              inner_tolerance: float = 0.0, width_policy: int = 0, context: Context | None = None,
+              forward_tolerance: float = 0.0, maximum_refinement_iterations: int = 0) -> SolveResult:
+    """reference binding.py:259-290 on the HIP engine.
+
+    Engine extensions: precision "mixed" runs an FP32 search and an FP64
+    refinement preconditioned by its factorization; forward_tolerance > 0
+    keeps refining until the forward-error estimate is at most that value."""
     if not isinstance(matrix, DeviceMatrix):
         _check_inputs(matrix, b)
     seed_t = _canonical_uint64_to_int64_transport(seed, "seed")
     stream_t = _canonical_uint64_to_int64_transport(stream, "stream")
-    raw, ext, steps = run_amfls_raw(matrix, b, regularization, tolerance, failure_probability, maximum_epochs,
+    raw, ext, extra = run_amfls_raw(matrix, b, regularization, tolerance, failure_probability, maximum_epochs,
                                     maximum_depth, maximum_auxiliary_width, maximum_basis_size, seed_t, stream_t,
                                     precision=precision, clock=clock, value_dtype=value_dtype,
                                     maximum_refinements=maximum_refinements, inner_tolerance=inner_tolerance,
-                                    width_policy=width_policy, context=context)
-    return _decode(raw, ext, steps)
+                                    width_policy=width_policy, context=context, forward_tolerance=forward_tolerance,
+                                    maximum_refinement_iterations=maximum_refinement_iterations)
+    return _decode(raw, ext, extra)
 
 
 def _sweep_options(tolerance, failure_probability, maximum_epochs, maximum_depth, maximum_auxiliary_width,
@@ -637,8 +652,8 @@ def run_fixed_rbgk(matrix, b, *, auxiliary_width: int, maximum_depth: int, maxim
                                                int(maximum_depth), int(maximum_basis_size),
                                                int(seed) % _UINT64_MODULUS, int(stream) % _UINT64_MODULUS,
                                                _lib.CLOCK[clock], ctypes.byref(res)))
-    raw, ext, steps = _result_tuple(ctx, res)
-    return _decode(raw, ext, steps)
+    raw, ext, extra = _result_tuple(ctx, res)
+    return _decode(raw, ext, extra)
 
 
 def run_lsqr(matrix, b, *, regularization: float = 0.0, tolerance: float = 1e-8, maximum_iterations: int = 0,
@@ -654,5 +669,5 @@ def run_lsqr(matrix, b, *, regularization: float = 0.0, tolerance: float = 1e-8,
     _lib.check(ctx._lib.amfls_solve_lsqr(ctx.handle, op.handle, bd.data_ptr(), 1, float(regularization),
                                          float(tolerance), int(maximum_iterations), _lib.CLOCK[clock],
                                          ctypes.byref(res)))
-    raw, ext, steps = _result_tuple(ctx, res)
-    return _decode(raw, ext, steps)
+    raw, ext, extra = _result_tuple(ctx, res)
+    return _decode(raw, ext, extra)

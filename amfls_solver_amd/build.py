@@ -35,6 +35,7 @@ SOURCES = [
     CSRC / "engine" / "validation.cpp",
     CSRC / "engine" / "workspace.cpp",
     CSRC / "engine" / "solver.cpp",
+    CSRC / "engine" / "refine.cpp",
     CSRC / "engine" / "width_model.cpp",
     CSRC / "engine" / "capi.cpp",
 ]

@@ -149,6 +149,8 @@ void amfls_options_default(amfls_options* o) {
     o->maximum_refinements = 8;
     o->inner_tolerance = 0.0;
     o->width_policy = AMFLS_WIDTH_HEURISTIC;
+    o->forward_tolerance = 0.0;
+    o->maximum_refinement_iterations = 0;
 }
 
 int amfls_context_create(amfls_context** out) {
@@ -346,6 +348,12 @@ SolveOptions solve_options(const amfls_options* options) {
     o.maximum_refinements = options->maximum_refinements;
     o.inner_tolerance = options->inner_tolerance;
     o.width_policy = options->width_policy;
+    require(std::isfinite(options->forward_tolerance) && options->forward_tolerance >= 0.0 &&
+                options->forward_tolerance < 1.0,
+            "forward_tolerance must lie in [0, 1)");
+    require(options->maximum_refinement_iterations >= 0, "maximum_refinement_iterations must be nonnegative");
+    o.forward_tolerance = options->forward_tolerance;
+    o.maximum_refinement_iterations = options->maximum_refinement_iterations;
     require(o.width_policy >= AMFLS_WIDTH_HEURISTIC && o.width_policy <= AMFLS_WIDTH_RECORD_WIDEN,
             "unknown width policy");
     return o;
@@ -577,6 +585,13 @@ int amfls_result_refinement_steps(const amfls_result* res, int32_t* steps) {
     return guard([&] {
         require(res && steps, "null argument");
         *steps = res->r.refinement_steps;
+    });
+}
+
+int amfls_result_forward_error_estimate(const amfls_result* res, double* estimate) {
+    return guard([&] {
+        require(res && estimate, "null argument");
+        *estimate = res->r.forward_error_estimate;
     });
 }
 

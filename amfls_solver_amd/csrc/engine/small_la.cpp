@@ -287,6 +287,43 @@ bool dense_min_norm(const Staircase& T, const std::vector<double>& g, double rco
     return true;
 }
 
+// [T; sqrt(lambda) I] as a staircase: damping row j goes right after the
+// last T row of column j, which keeps the staircase shape; g is permuted
+// along (damping rows get 0).
+void damped_staircase(const Staircase& T, const std::vector<double>& g, double lambda, Staircase& A,
+                      std::vector<double>& ga) {
+    const int k = T.cols();
+    const double root = std::sqrt(lambda);
+    std::vector<int> hi(k);
+    for (int j = 0; j < k; ++j) hi[j] = T.lo[j] + static_cast<int>(T.val[j].size());
+    // new position of T row r: r + #{damping rows placed before it}; damping
+    // row j is placed after T row hi[j]-1, i.e. before T row hi[j]
+    std::vector<int> pos_t(T.rows + 1, 0);
+    {
+        int jd = 0;
+        for (int r = 0; r <= T.rows; ++r) {
+            while (jd < k && hi[jd] <= r) ++jd;  // damping rows with hi <= r come first
+            pos_t[r] = r + jd;
+        }
+    }
+    A.rows = T.rows + k;
+    A.lo.assign(k, 0);
+    A.val.assign(k, {});
+    ga.assign(A.rows, 0.0);
+    for (int r = 0; r < T.rows && r < static_cast<int>(g.size()); ++r) ga[pos_t[r]] = g[r];
+    for (int j = 0; j < k; ++j) {
+        // damping row j position: rows placed before it = T rows < hi[j]
+        // plus damping rows i < j (all have hi[i] <= hi[j]).
+        const int pd = hi[j] + j;
+        const int lo_new = T.val[j].empty() ? pd : pos_t[T.lo[j]];
+        A.lo[j] = lo_new;
+        A.val[j].assign(pd - lo_new + 1, 0.0);
+        for (size_t t = 0; t < T.val[j].size(); ++t)
+            A.val[j][pos_t[T.lo[j] + static_cast<int>(t)] - lo_new] = T.val[j][t];
+        A.val[j][pd - lo_new] = root;
+    }
+}
+
 }  // namespace
 
 bool projected_solve(const Staircase& T, const std::vector<double>& g, double lambda, double rcond,
@@ -301,38 +338,9 @@ bool projected_solve(const Staircase& T, const std::vector<double>& g, double la
     for (double v : g)
         if (!std::isfinite(v)) return false;
     if (lambda > 0.0) {
-        // Interleave damping row j right after the last T row of column j:
-        // the augmented [T; sqrt(lambda) I] keeps its staircase shape.
-        const double root = std::sqrt(lambda);
-        std::vector<int> hi(k);
-        for (int j = 0; j < k; ++j) hi[j] = T.lo[j] + static_cast<int>(T.val[j].size());
-        // new position of T row r: r + #{damping rows placed before it}
-        // damping row j is placed after T row hi[j]-1, i.e. before T row hi[j]
-        std::vector<int> pos_t(T.rows + 1, 0);
-        {
-            int jd = 0;
-            for (int r = 0; r <= T.rows; ++r) {
-                while (jd < k && hi[jd] <= r) ++jd;  // damping rows with hi <= r come first
-                pos_t[r] = r + jd;
-            }
-        }
         Staircase A;
-        A.rows = T.rows + k;
-        A.lo.resize(k);
-        A.val.resize(k);
-        std::vector<double> ga(A.rows, 0.0);
-        for (int r = 0; r < T.rows; ++r) ga[pos_t[r]] = g[r];
-        for (int j = 0; j < k; ++j) {
-            // damping row j position: rows placed before it = T rows < hi[j]
-            // plus damping rows i < j (all have hi[i] <= hi[j]).
-            const int pd = hi[j] + j;
-            const int lo_new = T.val[j].empty() ? pd : pos_t[T.lo[j]];
-            A.lo[j] = lo_new;
-            A.val[j].assign(pd - lo_new + 1, 0.0);
-            for (size_t t = 0; t < T.val[j].size(); ++t)
-                A.val[j][pos_t[T.lo[j] + static_cast<int>(t)] - lo_new] = T.val[j][t];
-            A.val[j][pd - lo_new] = root;
-        }
+        std::vector<double> ga;
+        damped_staircase(T, g, lambda, A, ga);
         StairQr q = staircase_qr(A, ga);
         for (double d : q.rdiag)
             if (d == 0.0) return dense_min_norm(T, g, 0.0, y);
@@ -354,6 +362,81 @@ bool projected_solve(const Staircase& T, const std::vector<double>& g, double la
     }
     if (used_dense_fallback) *used_dense_fallback = true;
     return dense_min_norm(T, g, rcond, y);
+}
+
+int staircase_r(const Staircase& T, double lambda, double floor_ratio, BandUpper& R) {
+    const int k = T.cols();
+    R.top.assign(k, 0);
+    R.col.assign(k, {});
+    if (k == 0) return 0;
+    std::vector<double> zero(static_cast<size_t>(std::max(T.rows, k) + k), 0.0);
+    StairQr q;
+    if (lambda > 0.0) {
+        Staircase A;
+        std::vector<double> ga;
+        damped_staircase(T, zero, lambda, A, ga);
+        q = staircase_qr(A, ga);
+    } else {
+        q = staircase_qr(T, zero);
+    }
+    double dmax = 0.0;
+    for (int j = 0; j < k; ++j) dmax = std::max(dmax, std::abs(q.rdiag[j]));
+    const double floor = floor_ratio * dmax;
+    int raised = 0;
+    for (int j = 0; j < k; ++j) {
+        R.top[j] = q.top[j];
+        R.col[j].assign(q.col[j].begin(), q.col[j].begin() + (j - q.top[j] + 1));
+        double& d = R.col[j].back();
+        if (!(std::abs(d) >= floor) || d == 0.0) {
+            d = d < 0.0 ? -std::max(floor, std::numeric_limits<double>::min())
+                        : std::max(floor, std::numeric_limits<double>::min());
+            ++raised;
+        }
+    }
+    return raised;
+}
+
+void band_solve(const BandUpper& R, std::vector<double>& x) {
+    const int k = R.size();
+    for (int j = k - 1; j >= 0; --j) {
+        const double xj = x[j] / R.diag(j);
+        x[j] = xj;
+        const std::vector<double>& c = R.col[j];
+        const int top = R.top[j];
+        for (int i = top; i < j; ++i) x[i] -= c[i - top] * xj;
+    }
+}
+
+void band_solve_t(const BandUpper& R, std::vector<double>& x) {
+    const int k = R.size();
+    for (int j = 0; j < k; ++j) {
+        const std::vector<double>& c = R.col[j];
+        const int top = R.top[j];
+        double s = x[j];
+        for (int i = top; i < j; ++i) s -= c[i - top] * x[i];
+        x[j] = s / R.diag(j);
+    }
+}
+
+double band_min_singular(const BandUpper& R, int iterations) {
+    const int k = R.size();
+    if (k == 0) return 0.0;
+    std::vector<double> x(k);
+    for (int i = 0; i < k; ++i) x[i] = 1.0 + 0.5 * std::sin(1.0 + 0.7 * i);  // fixed, not orthogonal to much
+    double lam = 0.0;
+    for (int it = 0; it < iterations; ++it) {
+        double nx = 0.0;
+        for (double v : x) nx += v * v;
+        nx = std::sqrt(nx);
+        if (!(nx > 0.0) || !std::isfinite(nx)) return 0.0;
+        for (double& v : x) v /= nx;
+        band_solve_t(R, x);
+        band_solve(R, x);  // x <- (R^T R)^{-1} x
+        double ny = 0.0;
+        for (double v : x) ny += v * v;
+        lam = std::sqrt(ny);  // Rayleigh-like growth factor ~ 1 / sigma_min^2
+    }
+    return lam > 0.0 ? 1.0 / std::sqrt(lam) : 0.0;
 }
 
 }  // namespace amflsx

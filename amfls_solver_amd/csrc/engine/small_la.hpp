@@ -54,4 +54,26 @@ struct Staircase {
 bool projected_solve(const Staircase& T, const std::vector<double>& g, double lambda, double rcond,
                      std::vector<double>& y, bool* used_dense_fallback);
 
+// Upper triangular k x k factor in column band form: column j holds
+// R(top[j] .. j, j) in col[j] (col[j].back() is the diagonal).
+struct BandUpper {
+    std::vector<int> top;
+    std::vector<std::vector<double>> col;
+    int size() const { return static_cast<int>(top.size()); }
+    double diag(int j) const { return col[j].back(); }
+};
+
+// R of the Householder QR of the staircase [T; sqrt(lambda) I] (the
+// triangular factor of the projected operator, T^T T + lambda I = R^T R).
+// Diagonal entries below `floor_ratio` times the largest are raised to that
+// floor (same sign), so R stays invertible on numerically rank-deficient T;
+// returns the number of entries raised.
+int staircase_r(const Staircase& T, double lambda, double floor_ratio, BandUpper& R);
+// x <- R^{-1} x and x <- R^{-T} x (band back / forward substitution).
+void band_solve(const BandUpper& R, std::vector<double>& x);
+void band_solve_t(const BandUpper& R, std::vector<double>& x);
+// Estimate of the smallest singular value of R: `iterations` steps of
+// inverse iteration on R^T R from a fixed start vector.
+double band_min_singular(const BandUpper& R, int iterations);
+
 }  // namespace amflsx

@@ -12,6 +12,7 @@
 #include <stdexcept>
 
 #include "controller.hpp"
+#include "refine.hpp"
 #include "validation.hpp"
 #include "workspace.hpp"
 
@@ -77,15 +78,22 @@ using LambdaEvaluator = std::function<LeastSquaresResult(double lambda)>;
 using CheckpointHook = std::function<void(const LambdaEvaluator& evaluate_at, int depth, bool final)>;
 
 // The adaptive driver (reference amfls.cpp:57-661) in working precision T.
+// Called with the workspace just before the driver returns (the basis and
+// the projected operator of the final state).
+template <typename T>
+using WorkspaceHook = std::function<void(Workspace<T>&)>;
+
 template <typename T>
 LeastSquaresResult amfls_driver(const DeviceOperator& op, const double* b_dev, const SolveOptions& o, Context& ctx,
-                                const WidthPolicy* policy, const CheckpointHook* hook = nullptr) {
+                                const WidthPolicy* policy, const CheckpointHook* hook = nullptr,
+                                const WorkspaceHook<T>* on_finish = nullptr) {
     const double lambda = o.regularization, tol = o.tolerance;
     Clock& clk = ctx.clock;
     const double total_start = clk.now();
     RunStatistics stats;
     ValidationEngine validator(op, b_dev, ctx, stats);
     Workspace<T> ws(op, b_dev, lambda, o.maximum_basis_size, stats, ctx, validator);
+    ws.set_fp32_rank_factor(o.fp32_rank_factor);
     // Operator seconds so far (waits for the timed products in flight).
     auto op_seconds = [&] {
         ctx.timer.flush();
@@ -159,6 +167,7 @@ LeastSquaresResult amfls_driver(const DeviceOperator& op, const double* b_dev, c
     };
     auto finish = [&](LeastSquaresResult f) {
         if (hook) (*hook)(evaluate_at, ws.depth(), true);
+        if (on_finish) (*on_finish)(ws);
         f.iterations = ws.depth();
         f.trace = trace;
         ctx.timer.flush();
@@ -404,103 +413,73 @@ LeastSquaresResult amfls_driver(const DeviceOperator& op, const double* b_dev, c
     }
 }
 
-void accumulate(RunStatistics& into, const RunStatistics& s) {
-#define ACC(f) into.f += s.f;
-    ACC(a_columns) ACC(at_columns) ACC(a_block_calls) ACC(at_block_calls) ACC(search_a_columns)
-    ACC(search_at_columns) ACC(validation_a_columns) ACC(validation_at_columns) ACC(gaussian_random_columns)
-    ACC(gaussian_random_values) ACC(a_seconds) ACC(at_seconds) ACC(orthogonalization_seconds)
-    ACC(projected_solve_seconds) ACC(validation_seconds) ACC(search_a_block_calls) ACC(search_at_block_calls)
-    ACC(validation_a_block_calls) ACC(validation_at_block_calls) ACC(base_validation_a_columns)
-    ACC(base_validation_at_columns) ACC(base_validation_a_block_calls) ACC(base_validation_at_block_calls)
-    ACC(ridge_correction_a_columns) ACC(ridge_correction_at_columns) ACC(ridge_correction_a_block_calls)
-    ACC(ridge_correction_at_block_calls) ACC(base_validation_seconds) ACC(ridge_correction_validation_seconds)
-    ACC(gaussian_random_block_requests)
-#undef ACC
-}
-
-// Mixed precision: FP32 block Golub-Kahan inner solves on the correction
-// problem, FP64 residual / candidate / certificate on the original problem.
+// Mixed precision: an FP32 block Golub-Kahan search (products and basis in
+// FP32; projected solves and certificates in FP64) on the original problem,
+// then FP64 LSQR refinement preconditioned by the search's factorization
+// (refine.hpp): every vector that carries the residual is FP64, so the
+// refinement reaches the FP64 solution instead of stalling at the FP32
+// search's floor.
 LeastSquaresResult mixed_driver(const DeviceOperator& op, const double* b_dev, const SolveOptions& o, Context& ctx,
                                 const WidthPolicy* policy) {
-    const double lambda = o.regularization, tol = o.tolerance;
-    const int64_t m = op.rows(), n = op.cols();
+    const int64_t n = op.cols();
     Clock& clk = ctx.clock;
     const double total_start = clk.now();
-    RunStatistics stats;
-    ValidationEngine outer(op, b_dev, ctx, stats);
-    std::unique_ptr<AugmentedOperator> aug;
-    if (lambda > 0.0) aug = std::make_unique<AugmentedOperator>(op, std::sqrt(lambda), ctx);
-    const DeviceOperator& inner_op = aug ? static_cast<const DeviceOperator&>(*aug) : op;
-    const int64_t mi = inner_op.rows();
-    DevBuf<double> x(static_cast<size_t>(n)), d(static_cast<size_t>(n)), rhs(static_cast<size_t>(mi));
-    fill<double>(ctx.stream, x.get(), n, 0.0);
-    std::vector<IterationRecord> trace;
-    LeastSquaresResult best;
-    double best_cert = kInfD;
-    bool have_best = false;
-    const double rho = o.inner_tolerance > 0.0 ? o.inner_tolerance : 1e-4;
-    int steps = 0;
-    LeastSquaresResult res;
-    for (int outer_it = 0; outer_it <= o.maximum_refinements; ++outer_it) {
-        res = outer.validate(x.get(), lambda, tol);
-        res.depth = outer_it;
-        res.basis_rank = 0;
-        res.statistics = stats;
-        trace.push_back(make_iteration_record(res, outer_it, outer_it, 0));
-        const double cert = active_certificate(res, lambda);
-        const double prev_best = best_cert;
-        if (!have_best || (std::isfinite(cert) && !std::isfinite(best_cert)) || cert < best_cert) {
-            best = res;
-            best_cert = cert;
-            have_best = true;
+    BasisCapture basis;
+    const WorkspaceHook<float> capture = [&](Workspace<float>& ws) {
+        basis.n = n;
+        basis.k = ws.basis_rank();
+        basis.T = ws.projected_operator();
+        if (basis.k > 0) {
+            basis.V.reset(static_cast<size_t>(n) * static_cast<size_t>(basis.k));
+            copy_block<float, float>(ctx.stream, n, basis.k, ws.basis(), ws.basis_ld(), basis.V.get(), basis.k, 1.0);
         }
-        if (res.status == SolverStatus::success || candidate_validation_failed_numerically(res)) break;
-        if (outer_it >= 2 && std::isfinite(prev_best) && !(cert < 0.5 * prev_best)) break;  // stagnated
-        if (outer_it == o.maximum_refinements) break;
-        // Correction problem with the FP64 residual / gradient of x:
-        //   lambda = 0: min ||A d - (b - A x)||, minimum norm;
-        //   lambda > 0: min ||[A; sqrt(lambda) I] d - [0; -g / sqrt(lambda)]||,
-        //   whose normal equations are (A^T A + lambda I) d = -g.
-        if (lambda > 0.0) {
-            fill<double>(ctx.stream, rhs.get(), m, 0.0);
-            copy_block<double, double>(ctx.stream, n, 1, outer.last_gradient(), 1, rhs.get() + m, 1,
-                                       -1.0 / std::sqrt(lambda));
-        } else {
-            copy_block<double, double>(ctx.stream, m, 1, outer.last_residual_negated(), 1, rhs.get(), 1, -1.0);
-        }
-        double c0 = 1.0;
-        {
-            ValidationEngine probe(inner_op, rhs.get(), ctx, stats);
-            fill<double>(ctx.stream, d.get(), n, 0.0);
-            LeastSquaresResult r0 = probe.validate(d.get(), 0.0, 0.5);
-            if (std::isfinite(r0.backward_error_upper_bound) && r0.backward_error_upper_bound > 0.0)
-                c0 = r0.backward_error_upper_bound;
-        }
-        const double need = std::isfinite(cert) && cert > tol ? tol / cert : 1.0;
-        SolveOptions io = o;
-        io.regularization = 0.0;
-        io.precision = Precision::fp32;
-        io.stop_on_stagnation = true;
-        io.tolerance = std::min(0.5, std::max({rho * c0, 0.25 * need * c0, 1e-15}));
-        LeastSquaresResult in = amfls_driver<float>(inner_op, rhs.get(), io, ctx, policy);
-        accumulate(stats, in.statistics);
-        ++steps;
-        if (in.solution.size() != static_cast<size_t>(n)) break;
-        AMFLS_HIP_CHECK(hipMemcpyAsync(d.get(), in.solution.data(), n * sizeof(double), hipMemcpyHostToDevice,
-                                       ctx.stream));
-        add_vec(ctx.stream, n, d.get(), x.get());
-        ctx.sync();
+        basis.valid = basis.k > 0 && basis.T.cols() == basis.k;
+    };
+    SolveOptions so = o;
+    so.precision = Precision::fp32;
+    // The search stops once its FP64-validated certificate stagnates (its
+    // candidates are FP32-accurate at best); the refinement takes it from
+    // there.  A forward-error target on an unregularized problem with at
+    // least as many rows as columns needs every direction of R^n resolved,
+    // so the search then runs on to a complete basis (or a rank truncation)
+    // and the preconditioner covers all of R^n.
+    const bool full_basis = o.regularization == 0.0 && o.forward_tolerance > 0.0 && op.rows() >= op.cols();
+    so.stop_on_stagnation = !full_basis;
+    // The search's rank rule only has to reject rounding noise: after CGS2
+    // an FP32 residual block carries about eps32 * scale of it.  Directions
+    // kept above that are real (a noisy one only weakens the preconditioner,
+    // which the FP64 refinement absorbs), so the cutoff is 8 eps32 * scale
+    // rather than the FP32 solver's 64 eps32 sqrt(k) * scale, which on the
+    // 1e6 x 2000 dense problem truncates the basis at rank 1730 of 2000.
+    if (so.fp32_rank_factor <= 0.0) so.fp32_rank_factor = 8.0 * std::numeric_limits<float>::epsilon();
+    if (o.inner_tolerance > 0.0) so.tolerance = o.inner_tolerance;
+    LeastSquaresResult search = amfls_driver<float>(op, b_dev, so, ctx, policy, nullptr, &capture);
+    const bool forward = o.forward_tolerance > 0.0;
+    const bool settled = search.status == SolverStatus::success && so.tolerance <= o.tolerance && !forward;
+    if (!basis.valid || o.maximum_refinements == 0 || settled || candidate_validation_failed_numerically(search) ||
+        search.solution.size() != static_cast<size_t>(n)) {
+        search.statistics.total_seconds = clk.now() - total_start;
+        return search;
     }
-    if (!have_best) {
-        res = outer.validate(x.get(), lambda, tol);
-        best = res;
-    }
-    best.trace = trace;
-    best.refinement_steps = steps;
-    best.iterations = steps;
+    RunStatistics stats = search.statistics;
+    DevBuf<double> x(static_cast<size_t>(n));
+    AMFLS_HIP_CHECK(hipMemcpyAsync(x.get(), search.solution.data(), static_cast<size_t>(n) * sizeof(double),
+                                   hipMemcpyHostToDevice, ctx.stream));
+    std::vector<IterationRecord> trace = search.trace;
+    RefineReport rep;
+    LeastSquaresResult r = refine_preconditioned(op, b_dev, o, ctx, stats, basis, x.get(), trace, rep);
+    if (search.status == SolverStatus::success && r.status != SolverStatus::success && !forward) r = search;
+    r.depth = search.depth;
+    r.auxiliary_width = search.auxiliary_width;
+    r.basis_rank = search.basis_rank;
+    r.refinement_steps = rep.iterations;
+    r.iterations = search.iterations + rep.iterations;
+    r.trace = std::move(trace);
+    ctx.timer.flush();
+    stats.validation_seconds = stats.base_validation_seconds + stats.ridge_correction_validation_seconds;
     stats.total_seconds = clk.now() - total_start;
-    best.statistics = stats;
-    return best;
+    r.statistics = stats;
+    return r;
 }
 
 }  // namespace

@@ -109,8 +109,12 @@ struct LeastSquaresResult {
     double ridge_base_energy_error_upper_bound = kNaN, ridge_corrected_energy_error_upper_bound = kNaN,
            ridge_correction_gamma = kNaN, ridge_correction_z_h_z = kNaN,
            ridge_correction_two_abs_z_t_q = kNaN, ridge_correction_q_norm_squared = kNaN;
-    // Engine extension: number of outer FP64 refinement passes (mixed mode).
+    // Engine extensions (mixed precision): LSQR iterations of the FP64
+    // refinement, and the forward-error estimate ||P P^T g|| / ||x|| of the
+    // returned solution (P the preconditioner from the FP32 search, g the
+    // FP64 gradient); NaN where no estimate was made.
     int refinement_steps = 0;
+    double forward_error_estimate = kNaN;
 };
 
 // Arithmetic of the search process.
@@ -139,6 +143,13 @@ struct SolveOptions {
     double inner_tolerance = 0.0;  // 0 = default for mixed mode
     int width_policy = 0;           // 0 = reference heuristic, 1 = learned predictor
     bool stop_on_stagnation = false;  // inner solves of the refinement loop
+    // mixed precision: the refinement also runs until the forward-error
+    // estimate is at most this (0 = certificate only), for at most this many
+    // LSQR iterations (0 = 2000)
+    double forward_tolerance = 0.0;
+    int maximum_refinement_iterations = 0;
+    // rank-cutoff factor of FP32 search blocks (0 = default rule)
+    double fp32_rank_factor = 0.0;
 };
 
 struct FixedRbgkOptions {

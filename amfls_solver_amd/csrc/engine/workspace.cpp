@@ -87,6 +87,7 @@ double Workspace<T>::rank_factor(int k, int w) const {
                                                                   static_cast<int64_t>(w)}));
         return 64.0 * std::numeric_limits<double>::epsilon() * dim;
     } else {
+        if (fp32_rank_factor_ > 0.0) return fp32_rank_factor_;
         const double dim = static_cast<double>(std::max({1, k, w}));
         return 64.0 * std::numeric_limits<float>::epsilon() * std::sqrt(dim);
     }

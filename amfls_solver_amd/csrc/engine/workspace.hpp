@@ -76,6 +76,9 @@ public:
     // Golub-Kahan process itself does not depend on it, so one basis can be
     // evaluated at several values.
     void set_regularization(double lambda) { lambda_ = lambda; }
+    // Replaces the rank-cutoff factor of FP32 blocks (cutoff = factor *
+    // scale); 0 restores the default rule.
+    void set_fp32_rank_factor(double f) { fp32_rank_factor_ = f; }
     double regularization() const { return lambda_; }
     // Candidate x = V y from the current projected solve (device, double).
     void candidate(double* x_dev);
@@ -88,6 +91,10 @@ public:
     double right_hand_side_norm() const { return bnorm_; }
     const StepAudit& last_step_audit() const { return audit_; }
     const Staircase& projected_operator() const { return T_; }
+    // The retained right basis V: n x basis_rank(), row-major with leading
+    // dimension basis_ld().
+    const T* basis() const { return V_.ptr(); }
+    int64_t basis_ld() const { return V_.ld(); }
 
 private:
     struct Factor {
@@ -156,6 +163,7 @@ private:
     size_t g_host_ = 0;      // leading entries of g_ already copied from gdev_
     int pending_requested_ = 0, pending_retained_ = 0;
     int depth_ = 0;
+    double fp32_rank_factor_ = 0.0;
     bool initialized_ = false, reliable_ = true;
     StepAudit audit_;
     std::vector<double> y_;  // last projected solution

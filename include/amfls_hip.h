@@ -98,6 +98,15 @@ typedef struct amfls_options {
     int32_t maximum_refinements;
     double inner_tolerance;
     int32_t width_policy;
+    /* Mixed precision (engine extension): after the FP32 search, FP64 LSQR
+     * preconditioned by the search's factorization refines the solution
+     * until the certificate reaches `tolerance` and, when
+     * forward_tolerance > 0, the forward-error estimate ||P P^T g|| / ||x||
+     * is at most forward_tolerance; at most maximum_refinement_iterations
+     * LSQR iterations (0 = 2000).  maximum_refinements = 0 skips the
+     * refinement; inner_tolerance > 0 is the FP32 search's tolerance. */
+    double forward_tolerance;
+    int32_t maximum_refinement_iterations;
 } amfls_options;
 
 const char* amfls_last_error(void);
@@ -209,6 +218,8 @@ int amfls_result_copy(const amfls_result* result, double* solution, double* metr
  * seconds, active width of the interval (trace_rows * 3). */
 int amfls_result_trace_extension(const amfls_result* result, double* out);
 int amfls_result_refinement_steps(const amfls_result* result, int32_t* steps);
+/* Forward-error estimate of a mixed-precision result (NaN where none). */
+int amfls_result_forward_error_estimate(const amfls_result* result, double* estimate);
 int amfls_result_destroy(amfls_result* result);
 
 /* Learned width controller (engine extension; it replaces the reference's

@@ -1,19 +1,23 @@
 #!/usr/bin/env python
-"""Time-to-tolerance benchmark of the AMFLS HIP engine on one MI355X.
+"""Time-to-relative-error benchmark of the AMFLS HIP engine on one MI355X.
 
-Headline workload: the 200000 x 20000 CSR least-squares problem (exactly 12
-distinct uniform columns per row, N(0,1) values, rows scaled by log-uniform
-factors in [1e-3, 1], values stored as float32, b ~ N(0,1), lambda = 1e-3,
-certified relative energy error 1e-8).  A step is one complete certified
-solve with the matrix and b already resident in HBM.  Reported: seconds per
-solve of the plain FP64 engine (headline), of mixed precision (FP32 search +
-FP64 refinement/certificate) and of the learned width policy; the CSR SpMM
-kernel's achieved algorithmic GB/s measured with HIP events on the engine
-stream over the timed solves; an SpMM bandwidth sweep on a CSR matrix larger
-than the 256 MB last-level cache; the 1,000,000 x 2000 dense BF16 problem
-(G D, D log-spaced over six decades, lambda = 0) with its matrix-core product
-rates; and the CPU restatement (oracle) timed on this host with one BLAS
-thread.
+Headline workload: the 1,000,000 x 2000 dense least-squares problem A = G D
+(G i.i.d. N(0,1) from the counter-addressed Gaussian stream, D log-spaced
+from 1 to 1e-6, A stored as bfloat16, b = A x* + 1e-8-relative noise,
+lambda = 0).  A step is one complete solve with A and b already resident in
+HBM.  `value` is the seconds per solve of mixed-precision AMFLS (FP32 block
+Golub-Kahan search on the split-bf16 matrix cores, FP64 LSQR refinement
+preconditioned by the search's factorization, FP64 certificate) run until its
+forward-error estimate is at most 1e-9; every timed solution's forward
+relative error against the FP64 oracle solution (tests/golden/c2_solution.npz,
+TSQR of the stored matrix) must be at most 1e-8, or the line reports
+"valid": false.  Beside it: plain FP64 AMFLS on the same workload (its time
+to the same forward error) and the ratio; the learned width policy; the
+product kernels' achieved HBM bandwidth in the timed solves (HIP events on
+the engine stream); the 200000 x 20000 CSR solve (lambda 1e-3) with its SpMM
+bandwidth; an SpMM sweep beyond the 256 MB last-level cache; the batched CSR
+configuration; and the FP64 CPU restatement (oracle) timed on a sample of the
+dense workload on this host's cores.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -92,50 +96,155 @@ def spmm_sweep(torch, np, am, P, widths=(1, 4, 16, 64), reps=20):
     return out
 
 
-def dense_c2(torch, am, P, rows=1_000_000, cols=2000):
-    """The 1e6 x 2000 BF16 dense problem: FP64 certified solve (A upconverted
-    exactly on the matrix cores) and the product kernels' rates."""
+C2_FIXTURE = os.path.join(ROOT, "tests", "golden", "c2_solution.npz")
+
+
+def bf16_checksums(torch, np, a):
+    """Bit checksums of a row-major bf16 matrix (as tests/golden/make_c2_fixture.py computes them)."""
+    rows, cols = a.shape
+    tot = np.zeros(3, dtype=np.int64)
+    wcol = torch.arange(1, cols + 1, device=a.device, dtype=torch.int64)[None, :]
+    for i0 in range(0, rows, 50_000):
+        i1 = min(rows, i0 + 50_000)
+        bits = a[i0:i1].view(torch.int16).to(torch.int64) & 0xFFFF
+        wrow = (torch.arange(i0, i1, device=a.device, dtype=torch.int64) % 1009 + 1)[:, None]
+        tot += np.array([int(bits.sum()), int((bits * wcol).sum()), int((bits * wrow).sum())], dtype=np.int64)
+    return tot
+
+
+def dense_products(torch, M, rows, cols, widths=(1, 16, 32, 64), reps=5):
+    """Per-launch time and achieved bandwidth of the dense products (FP32 and FP64 blocks)."""
     import ctypes
-    a, b, xs = P.config2_device(rows, cols)
-    M = am.DeviceMatrix(a, value_dtype=torch.bfloat16)
-    out = {"matrix": f"dense_{rows}x{cols}_bf16_GD_6decades_lambda0"}
     lib, h = M.context._lib, M.context.handle
     prod = {}
-    for s in (1, 8, 16, 64):
+    for s_ in widths:
         for dt, code, name in ((torch.float64, 0, "f64"), (torch.float32, 1, "f32")):
-            X = torch.randn(cols, s, device="cuda", dtype=dt)
-            Y = torch.empty(rows, s, device="cuda", dtype=dt)
-            Z = torch.empty(cols, s, device="cuda", dtype=dt)
+            X = torch.randn(cols, s_, device="cuda", dtype=dt)
+            Y = torch.empty(rows, s_, device="cuda", dtype=dt)
+            Z = torch.empty(cols, s_, device="cuda", dtype=dt)
             for tr, (src, dst) in ((0, (X, Y)), (1, (Y, Z))):
-                # widths <= 8: the default kernel and the other family (vector
-                # ALUs by default for one column, matrix cores above)
-                alt = ((1, "_mfma") if s == 1 else (0, "_valu"))
-                for var, tag in (((-1, ""), alt) if s <= 8 else ((-1, ""),)):
-                    M.set_variant(s, bool(tr), var)
-                    lib.amfls_operator_apply_rows(h, M.handle, tr, ctypes.c_void_p(src.data_ptr()), s, s,
-                                                  ctypes.c_void_p(dst.data_ptr()), s, code)
-                    M.context.synchronize()
-                    M.profile(True)
-                    for _ in range(5):
-                        lib.amfls_operator_apply_rows(h, M.handle, tr, ctypes.c_void_p(src.data_ptr()), s, s,
-                                                      ctypes.c_void_p(dst.data_ptr()), s, code)
-                    M.context.synchronize()
-                    n, sec, by = M.kernel_stats()
-                    M.profile(False)
-                    prod[f"{'AT' if tr else 'A'}_s{s}_{name}{tag}"] = {
-                        "ms": 1e3 * sec / n, "GBs": by / sec / 1e9, "TFs": 2.0 * rows * cols * s / (sec / n) / 1e12}
-                M.set_variant(s, bool(tr), -1)
-    out["products"] = prod
-    for mode in ("fp64", "mixed"):
+                lib.amfls_operator_apply_rows(h, M.handle, tr, ctypes.c_void_p(src.data_ptr()), s_, s_,
+                                              ctypes.c_void_p(dst.data_ptr()), s_, code)
+                M.context.synchronize()
+                M.profile(True)
+                for _ in range(reps):
+                    lib.amfls_operator_apply_rows(h, M.handle, tr, ctypes.c_void_p(src.data_ptr()), s_, s_,
+                                                  ctypes.c_void_p(dst.data_ptr()), s_, code)
+                M.context.synchronize()
+                n, sec, by = M.kernel_stats()
+                M.profile(False)
+                prod[f"{'AT' if tr else 'A'}_s{s_}_{name}"] = {
+                    "ms": 1e3 * sec / n, "GBs": by / sec / 1e9, "TFs": 2.0 * rows * cols * s_ / (sec / n) / 1e12}
+    return prod
+
+
+def c2_headline(torch, np, am, P, steps, warmup, forward_target=1e-8):
+    """The 1e6 x 2000 bf16 dense workload: K timed solves per engine, forward
+    errors against the FP64 oracle, kernel bandwidth of the mixed solves."""
+    a, b, _ = P.config2_device()
+    rows, cols = a.shape
+    fx = np.load(C2_FIXTURE) if os.path.exists(C2_FIXTURE) else None
+    same = fx is not None and bool(np.array_equal(bf16_checksums(torch, np, a), fx["bits_checksums"]))
+    x_ref = fx["x_ls"] if same else None
+    M = am.DeviceMatrix(a, value_dtype=torch.bfloat16)
+    engines = {
+        "mixed": dict(precision="mixed", forward_tolerance=0.1 * forward_target),
+        "fp64": dict(precision="fp64"),
+    }
+    out = {"matrix": f"dense_{rows}x{cols}_bf16_GD_6decades_lambda0", "matrix_matches_fixture": same}
+    for name, kw in engines.items():
+        for _ in range(warmup):
+            am.run_amfls(M, b, tolerance=1e-8, **kw)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        r = am.run_amfls(M, b, tolerance=1e-8, precision=mode)
+        res = []
+        for _ in range(steps):
+            res.append(am.run_amfls(M, b, tolerance=1e-8, **kw))
         torch.cuda.synchronize()
-        out[mode] = {"seconds": time.perf_counter() - t0, "status": int(r.status), "depth": r.depth,
-                     "auxiliary_width": r.auxiliary_width, "basis_rank": r.basis_rank,
-                     "backward_error": r.backward_error_upper_bound, "refinement_steps": r.refinement_steps}
-        log(f"C2 {mode}: {out[mode]}")
-    del M, a
+        dt = (time.perf_counter() - t0) / steps
+        fe = [float(np.linalg.norm(r.solution.numpy() - x_ref) / np.linalg.norm(x_ref)) if x_ref is not None
+              else float("nan") for r in res]
+        r = res[-1]
+        out[name] = {"seconds": dt, "status": [int(q.status) for q in res], "forward_error": fe,
+                     "forward_error_max": max(fe), "forward_error_estimate": r.forward_error_estimate,
+                     "depth": r.depth, "auxiliary_width": r.auxiliary_width, "basis_rank": r.basis_rank,
+                     "refinement_iterations": r.refinement_steps,
+                     "backward_error": r.backward_error_upper_bound,
+                     "block_products": r.a_block_calls + r.at_block_calls,
+                     "block_products_per_s": (r.a_block_calls + r.at_block_calls) / dt,
+                     "operator_seconds": r.a_seconds + r.at_seconds}
+        log(f"C2 {name}: {dt:.4f} s/solve status {out[name]['status']} forward error max {max(fe):.3e} "
+            f"depth {r.depth} width {r.auxiliary_width + 1} refine {r.refinement_steps}")
+    # the same mixed solves again with every product's kernel timed by HIP
+    # events on the engine stream (kept out of the headline timing)
+    M.profile(True)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        am.run_amfls(M, b, tolerance=1e-8, **engines["mixed"])
+    torch.cuda.synchronize()
+    out["profiled_seconds_per_solve"] = (time.perf_counter() - t1) / steps
+    n, sec, by = M.kernel_stats()
+    M.profile(False)
+    out["kernel"] = {"launches": n, "seconds": sec, "bytes": by}
+    # learned width policy (shipped model) on the mixed engine
+    from amfls_solver_amd.width_model import WidthModel
+    wm = WidthModel.load(context=M.context)
+    wm.attach()
+    am.run_amfls(M, b, tolerance=1e-8, width_policy=1, context=M.context, **engines["mixed"])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rl = am.run_amfls(M, b, tolerance=1e-8, width_policy=1, context=M.context, **engines["mixed"])
+    torch.cuda.synchronize()
+    fel = float(np.linalg.norm(rl.solution.numpy() - x_ref) / np.linalg.norm(x_ref)) if x_ref is not None else None
+    out["learned_mixed"] = {"seconds": time.perf_counter() - t0, "status": int(rl.status), "forward_error": fel,
+                            "depth": rl.depth, "auxiliary_width": rl.auxiliary_width}
+    out["products"] = dense_products(torch, M, rows, cols)
+    del M, a, b
+    torch.cuda.empty_cache()
+    return out
+
+
+def c1_section(torch, np, am, P, steps, warmup):
+    """The 200000 x 20000 CSR workload (lambda 1e-3): FP64 and mixed solves
+    and the SpMM kernels' bandwidth in the timed FP64 solves."""
+    p = P.config1()
+    M = am.DeviceMatrix(csr_tensor(torch, np, p), value_dtype=torch.float32)
+    b = torch.from_numpy(p.b).cuda()
+    lib = M.context._lib
+    mean_a, mean_at = M.nnz / M.rows, M.nnz / M.cols
+    variants = {"A": lib.amfls_spmm_variant_name(lib.amfls_spmm_default_variant(1, mean_a)).decode(),
+                "AT": lib.amfls_spmm_variant_name(lib.amfls_spmm_default_variant(1, mean_at)).decode()}
+    out = {"matrix": "csr_200000x20000_nnz12_f32values_lambda1e-3", "spmm_variants_width1": variants}
+    sols = {}
+    for mode in ("fp64", "mixed"):
+        for _ in range(warmup):
+            am.run_amfls(M, b, regularization=p.regularization, tolerance=1e-8, precision=mode)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            r = am.run_amfls(M, b, regularization=p.regularization, tolerance=1e-8, precision=mode)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        sols[mode] = r.solution.numpy()
+        out[mode] = {"seconds": dt, "status": int(r.status), "certificate": r.relative_energy_error_upper_bound,
+                     "depth": r.depth, "auxiliary_width": r.auxiliary_width, "basis_rank": r.basis_rank,
+                     "refinement_iterations": r.refinement_steps,
+                     "block_products_per_s": (r.a_block_calls + r.at_block_calls) / dt}
+        log(f"C1 {mode}: {dt:.4f} s/solve status {r.status} cert {r.relative_energy_error_upper_bound:.3e}")
+    M.profile(True)
+    torch.cuda.synchronize()
+    for _ in range(steps):
+        am.run_amfls(M, b, regularization=p.regularization, tolerance=1e-8, precision="fp64")
+    torch.cuda.synchronize()
+    n, sec, by = M.kernel_stats()
+    M.profile(False)
+    achieved = by / sec / 1e9 if sec > 0 else 0.0
+    out["spmm"] = {"launches": n, "avg_us": 1e6 * sec / max(n, 1), "achieved_GBs": achieved,
+                   "frac": achieved / HBM_PEAK_GBS, "traffic_bytes_per_launch_committed": measured_traffic(),
+                   "note": "working set fits the 256 MB last-level cache; spmm_sweep is HBM-resident"}
+    out["mixed_rel_diff_vs_fp64"] = float(np.linalg.norm(sols["mixed"] - sols["fp64"]) /
+                                          np.linalg.norm(sols["fp64"]))
     return out
 
 
@@ -206,36 +315,57 @@ def batch_c3(torch, np, am, problems=256, lambdas=8, workers=6, seed=3, toleranc
     return out
 
 
-def cpu_baseline(np, P, seconds_budget=25.0):
-    """Oracle (CPU restatement, numpy/scipy FP64) on the same workload, 1 BLAS thread."""
+def host_cpu():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count() or 1
+
+
+def cpu_baseline(torch, np, P, rows=100_000, cols=2000):
+    """The FP64 CPU restatement (oracle AMFLS with numpy BLAS) on a sample of
+    the dense workload: its first `rows` rows, same G D construction, same
+    tolerance; time scaled by 1e6 / rows to the full workload (products and
+    left-block work are linear in the rows)."""
     from threadpoolctl import threadpool_limits
     from oracle import operators as O
     from oracle import solvers as S
-    p = P.config1()
-    times = []
-    res = None
-    with threadpool_limits(limits=1):
-        op = O.CsrOperator(p.rows, p.cols, p.row_ptr, p.col_idx, p.values.astype(np.float64))
-        t_start = time.time()
-        while len(times) < 3 and (time.time() - t_start) < seconds_budget:
-            t0 = time.perf_counter()
-            res = S.solve_amfls(op, p.b, regularization=p.regularization, tolerance=1e-8)
-            times.append(time.perf_counter() - t0)
-    return float(np.median(times)), len(times), res
+    a, b, _ = P.config2_device(rows, cols)
+    ah = a.double().cpu().numpy()
+    bh = b.cpu().numpy()
+    del a, b
+    model, ncpu = host_cpu()
+    cores = min(16, ncpu)
+    with threadpool_limits(limits=cores):
+        op = O.RowMajorDenseOperator(ah)
+        t0 = time.perf_counter()
+        res = S.solve_amfls(op, bh, tolerance=1e-8)
+        dt = time.perf_counter() - t0
+    scale = 1_000_000 / rows
+    return {"value": dt * scale, "unit": "s", "cores": cores, "kind": "port",
+            "sample": f"oracle FP64 AMFLS (numpy/scipy restatement, BLAS on {cores} threads) on the first {rows} "
+                      f"rows of the dense G D workload (x{scale:g} to 1e6 rows); measured {dt:.2f} s, "
+                      f"status {int(res.status)}, depth {int(res.depth)}",
+            "host_cpu": model, "host_nproc": ncpu}
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--headline", choices=["mixed", "fp64"], default="fp64")
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-c1", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
-    ap.add_argument("--no-dense", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--batch-problems", type=int, default=64,
-                    help="problems of the batched configuration to run (the full configuration has 256)")
+    ap.add_argument("--batch-problems", type=int, default=256,
+                    help="problems of the batched configuration (256 is the full configuration)")
     ap.add_argument("--batch-workers", type=int, default=6,
                     help="host threads of the batch, a stream each (six fill the GPU best on the batched workload)")
     ap.add_argument("--pmc-calibrate", action="store_true",
@@ -260,126 +390,60 @@ def main() -> int:
         log("pmc calibration copy: 1 GiB read + 1 GiB written")
         del src, dst
 
-    log("generating 200000x20000 CSR (12 nnz/row, lambda 1e-3)")
-    p = P.config1()
-    M = am.DeviceMatrix(csr_tensor(torch, np, p), value_dtype=torch.float32)
-    b = torch.from_numpy(p.b).cuda()
-    # width-one kernel variants: the measured default table (an autotuned
-    # choice varies from run to run between variants within noise of each
-    # other, which would make the PMC traffic runs incomparable)
-    lib = M.context._lib
-    mean_a, mean_at = M.nnz / M.rows, M.nnz / M.cols
-    variants = {"A": lib.amfls_spmm_variant_name(lib.amfls_spmm_default_variant(1, mean_a)).decode(),
-                "AT": lib.amfls_spmm_variant_name(lib.amfls_spmm_default_variant(1, mean_at)).decode()}
-    modes = ["fp64", "mixed"]
-    results = {}
-    for mode in modes:
-        for _ in range(a.warmup):
-            am.run_amfls(M, b, regularization=p.regularization, tolerance=1e-8, precision=mode)
-        log(f"warm {mode}")
-    for mode in modes:
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        last = None
-        for _ in range(a.steps):
-            last = am.run_amfls(M, b, regularization=p.regularization, tolerance=1e-8, precision=mode)
-        torch.cuda.synchronize()
-        dt = (time.perf_counter() - t0) / a.steps
-        if mode == a.headline:
-            # the same K solves again with every product's kernel timed by its
-            # dispatch timestamps (kept out of the headline timing: recording
-            # and resolving the event pairs costs host time)
-            M.profile(True)
-            torch.cuda.synchronize()
-            t1 = time.perf_counter()
-            for _ in range(a.steps):
-                am.run_amfls(M, b, regularization=p.regularization, tolerance=1e-8, precision=mode)
-            torch.cuda.synchronize()
-            dtp = (time.perf_counter() - t1) / a.steps
-            n, sec, by = M.kernel_stats()
-            M.profile(False)
-            results["kernel"] = (n, sec, by, dtp)
-        results[mode] = (dt, last)
-        log(f"{mode}: {dt:.4f} s/solve status {last.status} cert {last.relative_energy_error_upper_bound:.3e} "
-            f"depth {last.depth} width {last.auxiliary_width} rank {last.basis_rank} refine {last.refinement_steps}")
+    log("dense 1000000x2000 bf16 workload")
+    c2 = c2_headline(torch, np, am, P, a.steps, a.warmup)
+    mixed, fp64 = c2["mixed"], c2["fp64"]
+    valid = (c2["matrix_matches_fixture"] and mixed["forward_error_max"] <= 1e-8 and
+             fp64["forward_error_max"] <= 1e-8 and all(st == 0 for st in mixed["status"]))
+    kn, ksec, kby = c2["kernel"]["launches"], c2["kernel"]["seconds"], c2["kernel"]["bytes"]
+    achieved = kby / ksec / 1e9 if ksec > 0 else 0.0
 
-    # learned width policy (shipped model) on the same workload, FP64
-    from amfls_solver_amd.width_model import WidthModel
-    wm = WidthModel.load(context=M.context)
-    wm.attach()
-    am.run_amfls(M, b, regularization=p.regularization, tolerance=1e-8, width_policy=1, context=M.context)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        rl = am.run_amfls(M, b, regularization=p.regularization, tolerance=1e-8, width_policy=1, context=M.context)
-    torch.cuda.synchronize()
-    learned_s = (time.perf_counter() - t0) / a.steps
-    log(f"learned width policy: {learned_s:.4f} s/solve status {rl.status} width {rl.auxiliary_width}")
-
-    dt, r = results[a.headline]
-    other = "mixed" if a.headline == "fp64" else "fp64"
-    n, sec, by, dt_profiled = results["kernel"]
-    achieved = by / sec / 1e9 if sec > 0 else 0.0
-    sol = {m: results[m][1].solution.numpy() for m in modes}
-
-    cpu = None
-    if not a.no_cpu_baseline:
-        log("cpu baseline (oracle, 1 thread)")
-        cpu_s, nrun, ro = cpu_baseline(np, P)
-        rel = {m: float(np.linalg.norm(sol[m] - ro.solution) / np.linalg.norm(ro.solution)) for m in modes}
-        cpu = {"value": cpu_s, "unit": "s", "cores": 1, "kind": "port",
-               "sample": f"{nrun} full certified solves of the same 200000x20000 CSR workload "
-                         f"(numpy/scipy FP64 restatement, BLAS limited to 1 thread); median",
-               "status": int(ro.status), "depth": int(ro.depth)}
-    else:
-        rel = {}
-
-    dense = None
-    if not a.no_dense:
-        log("dense 1000000x2000 bf16 (matrix cores)")
-        dense = dense_c2(torch, am, P)
-
+    c1 = None if a.no_c1 else c1_section(torch, np, am, P, a.steps, a.warmup)
     sweep = None
     if not a.no_sweep:
         log("SpMM sweep on 2000000x200000 CSR (beyond the 256 MB cache)")
         sweep = spmm_sweep(torch, np, am, P)
-
     batch = None
     if not a.no_batch and a.batch_problems > 0:
         log(f"batched CSR problems: {a.batch_problems} x 8 lambdas")
         batch = batch_c3(torch, np, am, problems=a.batch_problems, workers=a.batch_workers)
         log(f"batch: {batch}")
+    cpu = None
+    if not a.no_cpu_baseline:
+        log("cpu baseline (oracle on a sample of the dense workload)")
+        cpu = cpu_baseline(torch, np, P)
+        log(f"cpu baseline: {cpu}")
 
     line = {
         "metric": "time-to-rel-err 1e-8 (s): mixed/learned vs FP64 AMFLS, 1 GPU; SpMM GB/s",
-        "value": dt,
+        "value": mixed["seconds"],
         "unit": "s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": 1e3 * dt,
+        "ms_per_step": 1e3 * mixed["seconds"],
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if a.headline == "fp64" else "f32+f64",
+        "dtype": "f32+f64",
         "data": "synthetic",
+        "valid": valid,
         "config": {
-            "workload": "csr_200000x20000_nnz12_f32values_lambda1e-3_relenergy1e-8",
-            "mode": a.headline,
-            "certified_status": int(r.status),
-            "certificate": r.relative_energy_error_upper_bound,
-            "depth": r.depth, "auxiliary_width": r.auxiliary_width, "basis_rank": r.basis_rank,
-            "block_products": r.a_block_calls + r.at_block_calls,
-            "block_products_per_s": (r.a_block_calls + r.at_block_calls) / dt,
-            f"{other}_seconds": results[other][0],
-            f"{other}_status": int(results[other][1].status),
-            f"{other}_certificate": results[other][1].relative_energy_error_upper_bound,
-            "mixed_refinement_steps": results["mixed"][1].refinement_steps,
-            "learned_width_seconds": learned_s,
-            "learned_width_status": int(rl.status),
-            "learned_model_heldout_speedup": wm.meta.get("heldout_speedup"),
-            "rel_diff_vs_oracle": rel,
-            "spmm_variants_width1": variants,
+            "workload": "dense_1000000x2000_bf16_GD_6decades_lambda0_forward_rel_err_1e-8",
+            "engine": "mixed: FP32 block Golub-Kahan search (split-bf16 matrix cores) + FP64 LSQR refinement "
+                      "preconditioned by its factorization, FP64 certificate",
+            "forward_error_max": mixed["forward_error_max"],
+            "forward_error_estimate": mixed["forward_error_estimate"],
+            "status": mixed["status"],
+            "depth": mixed["depth"], "auxiliary_width": mixed["auxiliary_width"],
+            "refinement_iterations": mixed["refinement_iterations"],
+            "block_products_per_s": mixed["block_products_per_s"],
+            "fp64_seconds": fp64["seconds"],
+            "fp64_forward_error_max": fp64["forward_error_max"],
+            "fp64_status": fp64["status"],
+            "speedup_mixed_vs_fp64": fp64["seconds"] / mixed["seconds"],
+            "learned_mixed": c2["learned_mixed"],
+            "matrix_matches_oracle_fixture": c2["matrix_matches_fixture"],
             "parallelism": "single_gpu",
         },
         "roofline": {
@@ -388,17 +452,18 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(),
-            "kernel": "csr_spmm (A and A^T products of the timed solves)",
-            "launches": n,
-            "avg_us": 1e6 * sec / max(n, 1),
-            "timed_pass": f"{a.steps} further solves of the workload with per-product dispatch-timestamp events "
-                          f"({1e3 * dt_profiled:.2f} ms per solve with the events)",
-            "note": "working set fits the 256 MB last-level cache; see spmm_sweep for an HBM-resident matrix",
+            "traffic": None,
+            "kernel": "dense product kernels (A X and A^T Y on the bf16 matrix, every width and precision) of "
+                      "the timed mixed solves",
+            "launches": kn,
+            "avg_us": 1e6 * ksec / max(kn, 1),
+            "timed_pass": f"{a.steps} further mixed solves with HIP events around every product on the engine "
+                          f"stream ({1e3 * c2['profiled_seconds_per_solve']:.1f} ms per solve with the events)",
         },
         "cpu_baseline": cpu,
+        "dense_1e6x2000_bf16": c2,
+        "csr_200000x20000": c1,
         "spmm_sweep": sweep,
-        "dense_1e6x2000_bf16": dense,
         "batched_csr": batch,
     }
     print(json.dumps(line), flush=True)

@@ -73,6 +73,8 @@ SIGNATURES = {
     "amfls_operator_apply": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p, c_int]),
     "amfls_operator_apply_rows": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_int64, c_void_p,
                                           c_int64, c_int]),
+    "amfls_operator_apply_rows_noalloc": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_int64, c_void_p,
+                                                  c_int64, c_int]),
     "amfls_operator_product_bytes": (c_double, [c_void_p, c_int, c_int, c_int]),
     "amfls_operator_product_flops": (c_double, [c_void_p, c_int]),
     "amfls_operator_set_variant": (c_int, [c_void_p, c_int, c_int, c_int]),

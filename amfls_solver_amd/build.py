@@ -30,6 +30,7 @@ SOURCES = [
     CSRC / "device" / "mlp.hip",
     CSRC / "device" / "generate.hip",
     CSRC / "device" / "dense_narrow.hip",
+    CSRC / "device" / "noalloc.hip",
     CSRC / "engine" / "operator.cpp",
     CSRC / "engine" / "small_la.cpp",
     CSRC / "engine" / "validation.cpp",

@@ -8,6 +8,8 @@
 // accumulates reductions in FP64 with a deterministic two-stage split.
 #include "kernels.hpp"
 
+#include <string>
+
 #include <algorithm>
 #include <cmath>
 #include <type_traits>
@@ -16,6 +18,9 @@ namespace amflsx {
 
 void Scratch::ensure(size_t need) {
     if (need <= bytes) return;
+    if (fixed)
+        throw std::length_error("the product needs " + std::to_string(need) + " bytes of scratch; the fixed scratch holds " +
+                                std::to_string(bytes));
     if (ptr) AMFLS_HIP_CHECK(hipFree(ptr));
     ptr = nullptr;
     size_t grow = std::max(need, bytes * 2);
@@ -23,7 +28,17 @@ void Scratch::ensure(size_t need) {
     bytes = grow;
 }
 
+int64_t fit_chunks(const Scratch& scratch, size_t per_chunk_bytes, int64_t want) {
+    if (!scratch.fixed) return want;
+    const int64_t room = static_cast<int64_t>(scratch.bytes / std::max<size_t>(per_chunk_bytes, 1));
+    if (room < 1)
+        throw std::length_error("one chunk of partial sums needs " + std::to_string(per_chunk_bytes) +
+                                " bytes; the fixed scratch holds " + std::to_string(scratch.bytes));
+    return std::min(want, room);
+}
+
 void Scratch::release() {
+    if (fixed) return;
     if (ptr) (void)hipFree(ptr);
     ptr = nullptr;
     bytes = 0;
@@ -616,6 +631,7 @@ void gemm_tn_dispatch(hipStream_t s, int64_t M, int64_t K, int N, const V* A, in
     if (KN <= 4096 && K <= 4096) {
         // ~512 blocks, each a contiguous row range
         int64_t nb = std::max<int64_t>(1, std::min<int64_t>(512, ceil_div(std::max<int64_t>(M, 1), 32)));
+        nb = fit_chunks(scratch, static_cast<size_t>(KN) * sizeof(double), nb);
         const int64_t rows_per_block = ceil_div(std::max<int64_t>(M, 1), nb);
         nb = ceil_div(std::max<int64_t>(M, 1), rows_per_block);
         int BR = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(64, 6144 / (K + N))));
@@ -652,6 +668,7 @@ void gemm_tn_dispatch(hipStream_t s, int64_t M, int64_t K, int N, const V* A, in
     const int64_t col_blocks = ceil_div(K, 64) * ceil_div(N, 64);
     int64_t chunks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(M, 256), ceil_div(1024, col_blocks)));
     chunks = std::min<int64_t>(chunks, 65535);
+    chunks = fit_chunks(scratch, static_cast<size_t>(KN) * sizeof(double), chunks);
     const int64_t rows_per_chunk = std::max<int64_t>(1, ceil_div(std::max<int64_t>(M, 1), chunks));
     chunks = std::max<int64_t>(1, ceil_div(std::max<int64_t>(M, 1), rows_per_chunk));
     scratch.ensure(static_cast<size_t>(chunks * KN) * sizeof(double));

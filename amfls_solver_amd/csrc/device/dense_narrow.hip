@@ -228,6 +228,7 @@ void tn_typed(hipStream_t st, int64_t M, int64_t N, int s, const TA* A, int64_t 
               int64_t ldz, Scratch& scratch) {
     const int64_t strips = ceil_div(N, 8 * kThreads);
     int64_t chunks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(1024, strips), ceil_div(M, 256)));
+    chunks = fit_chunks(scratch, static_cast<size_t>(N * s) * sizeof(double), chunks);
     const int64_t rows_per_chunk = ceil_div(ceil_div(M, chunks), 64) * 64;
     chunks = ceil_div(M, rows_per_chunk);
     scratch.ensure(static_cast<size_t>(chunks * N * s) * sizeof(double));

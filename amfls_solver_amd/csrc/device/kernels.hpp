@@ -28,9 +28,24 @@ struct Strided {
 struct Scratch {
     void* ptr = nullptr;
     size_t bytes = 0;
+    // A fixed scratch never grows: ensure() throws std::length_error when
+    // `bytes` is too small, and launchers that split work into chunks of
+    // partial sums use no more chunks than fit (fit_chunks).
+    bool fixed = false;
     void ensure(size_t need);
     void release();
 };
+
+// Number of chunks, each needing `per_chunk_bytes` of scratch, to launch:
+// `want`, or fewer when a fixed scratch holds fewer (at least one fits, or
+// std::length_error).
+int64_t fit_chunks(const Scratch& scratch, size_t per_chunk_bytes, int64_t want);
+
+// This device's allocation-free scratch (noalloc.hip): a view of a
+// file-scope device array of kNoAllocScratchBytes, resolved with
+// hipGetSymbolAddress once per device on first use; fixed, never freed.
+constexpr size_t kNoAllocScratchBytes = size_t(16) << 20;
+Scratch noalloc_scratch();
 
 // ---------------------------------------------------------------- GEMMs
 // Y(M x N, T row-major) = alpha * A(M x K) * B(K x N, TB row-major) + beta * Y.

@@ -157,6 +157,10 @@ int amfls_context_create(amfls_context** out) {
     return guard([&] {
         require(out != nullptr, "output pointer is null");
         *out = new amfls_context();
+        // Set-up of the allocation-free product on this device: resolving
+        // the address of its static array loads the array (once per device)
+        // here, so the entry point itself never allocates.
+        (void)noalloc_scratch();
     });
 }
 
@@ -268,6 +272,19 @@ int amfls_operator_apply_rows(amfls_context* ctx, const amfls_operator* op, int 
         } else {
             throw std::invalid_argument("block dtype must be float64 or float32");
         }
+    });
+}
+
+int amfls_operator_apply_rows_noalloc(amfls_context* ctx, const amfls_operator* op, int transpose, const void* in,
+                                      int block_cols, int64_t ld_in, void* out, int64_t ld_out, int dtype) {
+    return guard([&] {
+        require(ctx && op && in && out, "null argument");
+        require(block_cols > 0 && ld_in >= block_cols && ld_out >= block_cols, "invalid block shape");
+        require(dtype == AMFLS_F64 || dtype == AMFLS_F32, "block dtype must be float64 or float32");
+        Scratch fixed = noalloc_scratch();
+        if (!op->op->product_noalloc(ctx->ctx.stream, transpose != 0, in, block_cols, ld_in, out, ld_out,
+                                     dtype == AMFLS_F64, fixed))
+            throw std::invalid_argument("the allocation-free product supports device CSR and dense operators");
     });
 }
 

@@ -277,6 +277,19 @@ void CsrOperator::apply_t(hipStream_t s, const float* Y, int n, int64_t ldy, flo
     prof_commit(e0, e1, n, true, 4);
 }
 
+bool CsrOperator::product_noalloc(hipStream_t s, bool transpose, const void* in, int n, int64_t ldi, void* out,
+                                  int64_t ldo, bool fp64, Scratch& /*scratch: the SpMM needs none*/) const {
+    const CsrView& v = transpose ? at_ : a_;
+    const int var = variant_for(n, transpose);
+    if (fp64)
+        csr_spmm<double>(s, v, n, static_cast<const double*>(in), ldi, static_cast<double*>(out), ldo, nullptr, 0, 0.0,
+                         var, nullptr, nullptr, nullptr);
+    else
+        csr_spmm<float>(s, v, n, static_cast<const float*>(in), ldi, static_cast<float*>(out), ldo, nullptr, 0, 0.0,
+                        var, nullptr, nullptr, nullptr);
+    return true;
+}
+
 template <typename T>
 void CsrOperator::apply_t_add(hipStream_t s, const T* Y, int n, int64_t ldy, T* X, int64_t ldx, const T* add,
                               int64_t ld_add, double a) const {
@@ -387,43 +400,61 @@ bool DenseOperator::use_narrow(int ncols, bool transpose) const {
 }
 
 template <typename T>
-void DenseOperator::do_apply(hipStream_t s, const T* X, int n, int64_t ldx, T* Y, int64_t ldy) const {
-    prof_begin(s);
+void DenseOperator::do_apply(hipStream_t s, const T* X, int n, int64_t ldx, T* Y, int64_t ldy, Scratch& scratch,
+                             bool profile) const {
+    if (profile) prof_begin(s);
     if (use_narrow(n, false))
         dense_narrow_nn<T>(s, a_, rows_, cols_, n, X, ldx, Y, ldy);
     else if (use_mfma(n))
         dense_mfma_nn<T>(s, dense_mfma_mode(a_.type, sizeof(T) == 8, reduced_), a_, rows_, cols_, n, X, ldx, Y, ldy,
-                         ctx_.scratch);
+                         scratch);
     else
         gemm_nn<T, T>(s, rows_, n, cols_, a_, X, ldx, Y, ldy, 1.0, 0.0);
-    prof_end(s, n, false, static_cast<int>(sizeof(T)));
+    if (profile) prof_end(s, n, false, static_cast<int>(sizeof(T)));
 }
 
 template <typename T>
-void DenseOperator::do_apply_t(hipStream_t s, const T* Y, int n, int64_t ldy, T* X, int64_t ldx) const {
+void DenseOperator::do_apply_t(hipStream_t s, const T* Y, int n, int64_t ldy, T* X, int64_t ldx, Scratch& scratch,
+                               bool profile) const {
     const ValueType ct = sizeof(T) == 8 ? ValueType::f64 : ValueType::f32;
-    prof_begin(s);
+    if (profile) prof_begin(s);
     if (use_narrow(n, true))
-        dense_narrow_tn<T>(s, a_, rows_, cols_, n, Y, ldy, X, ldx, ctx_.scratch);
+        dense_narrow_tn<T>(s, a_, rows_, cols_, n, Y, ldy, X, ldx, scratch);
     else if (use_mfma(n))
         dense_mfma_tn<T>(s, dense_mfma_mode(a_.type, sizeof(T) == 8, reduced_), a_, rows_, cols_, n, Y, ldy, X, ldx,
-                         ctx_.scratch);
+                         scratch);
     else
-        gemm_tn<T>(s, rows_, cols_, n, a_, Y, ldy, X, ct, ldx, 1.0, ctx_.scratch);
-    prof_end(s, n, true, static_cast<int>(sizeof(T)));
+        gemm_tn<T>(s, rows_, cols_, n, a_, Y, ldy, X, ct, ldx, 1.0, scratch);
+    if (profile) prof_end(s, n, true, static_cast<int>(sizeof(T)));
 }
 
 void DenseOperator::apply(hipStream_t s, const double* X, int n, int64_t ldx, double* Y, int64_t ldy) const {
-    do_apply<double>(s, X, n, ldx, Y, ldy);
+    do_apply<double>(s, X, n, ldx, Y, ldy, ctx_.scratch, true);
 }
 void DenseOperator::apply_t(hipStream_t s, const double* Y, int n, int64_t ldy, double* X, int64_t ldx) const {
-    do_apply_t<double>(s, Y, n, ldy, X, ldx);
+    do_apply_t<double>(s, Y, n, ldy, X, ldx, ctx_.scratch, true);
 }
 void DenseOperator::apply(hipStream_t s, const float* X, int n, int64_t ldx, float* Y, int64_t ldy) const {
-    do_apply<float>(s, X, n, ldx, Y, ldy);
+    do_apply<float>(s, X, n, ldx, Y, ldy, ctx_.scratch, true);
 }
 void DenseOperator::apply_t(hipStream_t s, const float* Y, int n, int64_t ldy, float* X, int64_t ldx) const {
-    do_apply_t<float>(s, Y, n, ldy, X, ldx);
+    do_apply_t<float>(s, Y, n, ldy, X, ldx, ctx_.scratch, true);
+}
+
+bool DenseOperator::product_noalloc(hipStream_t s, bool transpose, const void* in, int n, int64_t ldi, void* out,
+                                    int64_t ldo, bool fp64, Scratch& scratch) const {
+    if (fp64) {
+        if (transpose)
+            do_apply_t<double>(s, static_cast<const double*>(in), n, ldi, static_cast<double*>(out), ldo, scratch, false);
+        else
+            do_apply<double>(s, static_cast<const double*>(in), n, ldi, static_cast<double*>(out), ldo, scratch, false);
+    } else {
+        if (transpose)
+            do_apply_t<float>(s, static_cast<const float*>(in), n, ldi, static_cast<float*>(out), ldo, scratch, false);
+        else
+            do_apply<float>(s, static_cast<const float*>(in), n, ldi, static_cast<float*>(out), ldo, scratch, false);
+    }
+    return true;
 }
 
 double DenseOperator::product_bytes(int s, bool transpose, int eb) const {

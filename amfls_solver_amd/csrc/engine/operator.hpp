@@ -56,6 +56,16 @@ public:
     // Selectable kernel variants (CSR SpMM); -1 = default.
     virtual void set_variant(int, bool, int) {}
 
+    // The same product as apply / apply_t on FP64 (fp64 = true) or FP32
+    // blocks, enqueued on `s` with `scratch` (a fixed, non-growing buffer)
+    // as the only device memory beyond the blocks: no allocation, no timing
+    // events or stamps.  Returns false for operators without such a path.
+    virtual bool product_noalloc(hipStream_t, bool /*transpose*/, const void* /*in*/, int /*ncols*/,
+                                 int64_t /*ld_in*/, void* /*out*/, int64_t /*ld_out*/, bool /*fp64*/,
+                                 Scratch& /*scratch*/) const {
+        return false;
+    }
+
     // Operators whose product kernels time themselves: stamp_next() hands the
     // next apply / apply_t the device values its kernels stamp their start
     // and end into (ProductStamp); the product consumes them.  The hand-off
@@ -116,6 +126,8 @@ public:
     double product_bytes(int s, bool transpose, int eb) const override;
     double product_flops(int s) const override { return 2.0 * static_cast<double>(nnz_) * s; }
     void set_variant(int s, bool transpose, int variant) override;
+    bool product_noalloc(hipStream_t s, bool transpose, const void* in, int ncols, int64_t ld_in, void* out,
+                         int64_t ld_out, bool fp64, Scratch& scratch) const override;
     bool times_own_kernels() const override { return true; }
     // Times the SpMM variants whose column lanes fit each block width (the
     // next power of two and twice it) on this matrix, with FP64 or FP32
@@ -183,15 +195,19 @@ public:
     // matrix cores, -1 the default.
     void set_variant(int s, bool transpose, int variant) override;
     bool use_narrow(int ncols, bool transpose) const;
+    bool product_noalloc(hipStream_t s, bool transpose, const void* in, int ncols, int64_t ld_in, void* out,
+                         int64_t ld_out, bool fp64, Scratch& scratch) const override;
 
 private:
     bool mfma_ = true;
     bool reduced_ = false;
     std::map<std::pair<int, bool>, int> variants_;
     template <typename T>
-    void do_apply(hipStream_t s, const T* X, int n, int64_t ldx, T* Y, int64_t ldy) const;
+    void do_apply(hipStream_t s, const T* X, int n, int64_t ldx, T* Y, int64_t ldy, Scratch& scratch,
+                  bool profile) const;
     template <typename T>
-    void do_apply_t(hipStream_t s, const T* Y, int n, int64_t ldy, T* X, int64_t ldx) const;
+    void do_apply_t(hipStream_t s, const T* Y, int n, int64_t ldy, T* X, int64_t ldx, Scratch& scratch,
+                    bool profile) const;
     void ensure_host_copy() const;
 
     Strided a_;

@@ -148,6 +148,19 @@ int amfls_operator_apply(amfls_context* ctx, const amfls_operator* op, int trans
 /* Same product on the engine's row-major blocks with leading dimensions. */
 int amfls_operator_apply_rows(amfls_context* ctx, const amfls_operator* op, int transpose, const void* in,
                               int block_cols, int64_t ld_in, void* out, int64_t ld_out, int dtype);
+/* Allocation-free variant of amfls_operator_apply_rows; it has no
+ * counterpart in the reference.  It is for callers that cannot allocate
+ * device memory (code under stream capture, serving loops, harnesses that
+ * forbid hipMalloc).  The same product on the same blocks, but its only
+ * device memory beyond in/out is a file-scope array of 16 MiB per GPU:
+ * packed block planes and per-chunk partial sums, fewer chunks when the
+ * default would not fit.  No timing events are recorded.  Device CSR and
+ * dense operators only; a product that cannot fit the array fails with
+ * AMFLS_ERR_LENGTH.  Calls that overlap on one GPU share the array and
+ * race; results are independent of earlier calls (everything read is
+ * written by the same call). */
+int amfls_operator_apply_rows_noalloc(amfls_context* ctx, const amfls_operator* op, int transpose, const void* in,
+                                      int block_cols, int64_t ld_in, void* out, int64_t ld_out, int dtype);
 /* Algorithmic HBM bytes and flops of one product of width block_cols. */
 double amfls_operator_product_bytes(const amfls_operator* op, int block_cols, int transpose, int dtype);
 double amfls_operator_product_flops(const amfls_operator* op, int block_cols);

@@ -106,6 +106,12 @@ SIGNATURES = {
                                     c_void_p, c_int64, c_double]),
     "amfls_block_rowmul": (c_int, [c_void_p, c_int64, c_int32, c_int32, c_void_p, c_int64, c_void_p, c_void_p,
                                    c_int64]),
+    "amfls_block_gemm_nn_f32": (c_int, [c_void_p, c_int64, c_int32, c_int32, c_void_p, c_int64, c_void_p, c_int64,
+                                        c_void_p, c_int64, c_double, c_double]),
+    "amfls_block_gemm_tn_f32": (c_int, [c_void_p, c_int64, c_int32, c_int32, c_void_p, c_int64, c_void_p, c_int64,
+                                        c_void_p, c_int64, c_double]),
+    "amfls_block_rowmul_f32": (c_int, [c_void_p, c_int64, c_int32, c_int32, c_void_p, c_int64, c_void_p, c_void_p,
+                                       c_int64]),
     "amfls_block_stats": (c_int, [c_void_p, c_int64, c_int32, c_void_p, c_int64, c_void_p]),
     "amfls_width_model_create": (c_int, [c_void_p, c_void_p, c_void_p, POINTER(c_void_p)]),
     "amfls_width_model_destroy": (c_int, [c_void_p]),

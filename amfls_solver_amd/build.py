@@ -31,6 +31,7 @@ SOURCES = [
     CSRC / "device" / "generate.hip",
     CSRC / "device" / "dense_narrow.hip",
     CSRC / "device" / "noalloc.hip",
+    CSRC / "device" / "tall.hip",
     CSRC / "engine" / "operator.cpp",
     CSRC / "engine" / "small_la.cpp",
     CSRC / "engine" / "validation.cpp",

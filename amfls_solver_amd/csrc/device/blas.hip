@@ -417,6 +417,13 @@ void gemm_nn_dispatch(hipStream_t s, int64_t M, int N, int64_t K, const V* A, in
     using Acc = T;
     if (M <= 0 || N <= 0) return;
     const bool row_major = (cs == 1);
+    if constexpr (std::is_same_v<V, float> && std::is_same_v<T, float> && std::is_same_v<TB, double>) {
+        // FP32 blocks times FP64 coefficients: the matrix-core tall kernel
+        if (row_major && tall_wide_supported(static_cast<int>(std::min<int64_t>(K, 65)), N)) {
+            tall_nn_wide<float>(s, M, N, static_cast<int>(K), A, rs, B, ldb, Y, ldy, alpha, beta, gate);
+            return;
+        }
+    }
     if constexpr (std::is_same_v<V, double> && std::is_same_v<TB, double> && std::is_same_v<T, double>) {
         if (row_major && N <= kTallMax && K <= kTallMax) {
             tall_nn_f64(s, M, N, static_cast<int>(K), A, rs, B, ldb, Y, ldy, alpha, beta, gate);
@@ -610,6 +617,12 @@ void gemm_tn_dispatch(hipStream_t s, int64_t M, int64_t K, int N, const V* A, in
     if (K <= 0 || N <= 0) return;
     const bool row_major = (cs == 1);
     const int64_t KN = K * N;
+    if constexpr (std::is_same_v<V, float> && std::is_same_v<TB, float>) {
+        if (row_major && K <= 64 && tall_wide_supported(static_cast<int>(K), N)) {
+            tall_tn_wide<float>(s, M, static_cast<int>(K), N, A, rs, B, ldb, C, ctype, ldc, alpha, scratch, gate);
+            return;
+        }
+    }
     if constexpr (std::is_same_v<V, double> && std::is_same_v<TB, double>) {
         if (row_major && K <= kTallMax && N <= kTallMax) {
             int64_t nb = 0, rows_per_block = 0;
@@ -1127,6 +1140,12 @@ void rowmul(hipStream_t s, int64_t rows, int w, int r, const T* X, int64_t ldx, 
             T* Y, int64_t ldy) {
     if (rows <= 0 || r <= 0) return;
     if (w > 128 || r > 128) throw std::invalid_argument("rowmul supports widths up to 128");
+    if constexpr (std::is_same_v<T, float>) {
+        if (tall_wide_supported(w, r)) {
+            tall_nn_wide<float>(s, rows, r, w, X, ldx, M, r, Y, ldy, 1.0, 0.0, nullptr);
+            return;
+        }
+    }
     if constexpr (std::is_same_v<T, double>) {
         // Y = X M with X read whole per tile before Y is written (in place allowed)
         if (w <= kTallMax && r <= kTallMax) {

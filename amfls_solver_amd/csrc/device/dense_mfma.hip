@@ -29,6 +29,7 @@
 #include "kernels.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace amflsx {
 namespace {
@@ -245,6 +246,74 @@ __global__ __launch_bounds__(kThreads) void dense_nn_kernel(int64_t M, int64_t N
     }
 }
 
+// Split-bf16 Y = A X for bf16 A: as dense_nn_kernel, but the A fragments
+// are bf16 already (reinterpreted, not converted) and each X fragment
+// (three bf16 planes) is loaded once per 32-wide K step and column tile and
+// fed to all RT row tiles of the wavefront.
+template <typename TX, int NT, int RT>
+__global__ __launch_bounds__(kThreads) void nn_split3_kernel(int64_t M, int64_t N, int s,
+                                                             const __hip_bfloat16* __restrict__ A, int64_t lda,
+                                                             const __bf16* __restrict__ XT, int64_t nk, int64_t plane,
+                                                             TX* __restrict__ Y, int64_t ldy) {
+    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    const int c = lane % 16, q = lane / 16;
+    const int64_t row0 = (static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + wave) * (16 * RT);
+    if (row0 >= M) return;
+    floatx4 acc[RT][NT];
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+        for (int j = 0; j < NT; ++j) acc[r][j] = floatx4{0, 0, 0, 0};
+    Frag8<__hip_bfloat16> cur[RT][2], nxt[RT][2];
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) cur[r][h].load(A, lda, row0 + 16 * r + c, M, 16 * q + 8 * h, N);
+    for (int64_t k0 = 0; k0 < N; k0 += 64) {
+        if (k0 + 64 < N) {
+#pragma unroll
+            for (int r = 0; r < RT; ++r)
+#pragma unroll
+                for (int h = 0; h < 2; ++h) nxt[r][h].load(A, lda, row0 + 16 * r + c, M, k0 + 64 + 16 * q + 8 * h, N);
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            bf16x8 av[RT];
+#pragma unroll
+            for (int r = 0; r < RT; ++r) av[r] = __builtin_bit_cast(bf16x8, cur[r][h].v);
+#pragma unroll
+            for (int j = 0; j < NT; ++j) {
+                const int64_t off = static_cast<int64_t>(16 * j + c) * nk + k0 + 16 * q + 8 * h;
+                const bf16x8 xh = *reinterpret_cast<const bf16x8*>(XT + off);
+                const bf16x8 xm = *reinterpret_cast<const bf16x8*>(XT + plane + off);
+                const bf16x8 xl = *reinterpret_cast<const bf16x8*>(XT + 2 * plane + off);
+#pragma unroll
+                for (int r = 0; r < RT; ++r) {
+                    acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[r], xl, acc[r][j], 0, 0, 0);
+                    acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[r], xm, acc[r][j], 0, 0, 0);
+                    acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[r], xh, acc[r][j], 0, 0, 0);
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RT; ++r)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) cur[r][h] = nxt[r][h];
+    }
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+        const int col = 16 * j + c;
+        if (col >= s) continue;
+#pragma unroll
+        for (int r = 0; r < RT; ++r)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int64_t row = row0 + 16 * r + 4 * q + e;
+                if (row < M) Y[row * ldy + col] = static_cast<TX>(acc[r][j][e]);
+            }
+    }
+}
+
 // FP64 blocks, s <= 32: as dense_nn_kernel, but the X^T slice of each
 // 64-wide K step (16 NT columns x 64 doubles) is staged in LDS once per
 // block and shared by its four wavefronts instead of every wavefront
@@ -422,6 +491,176 @@ __global__ __launch_bounds__(kThreads) void dense_tn_kernel(int64_t M, int64_t N
     }
 }
 
+// Split-bf16 partial Z = A^T Y (bf16 A, FP32 or FP64 blocks Y of at most
+// 64 columns) over one chunk of rows for one strip of 256 columns of A:
+// P[chunk][j][col].  The reduction runs over rows, so both matrix-core
+// operands need eight consecutive rows per lane: each 64-row trip stages
+// the A tile transposed in LDS (a thread loads an 8 x 8 sub-tile with
+// 16-byte row loads, transposes it in registers, and stores eight 16-byte
+// column pieces) and the Y tile as three transposed bf16 planes (split
+// once per trip, shared by the four wavefronts).  Wavefront w owns column
+// tiles w, w + 4, w + 8, w + 12 of the strip and all NT column tiles of Y;
+// the next trip's A and Y are in registers while the current trip runs on
+// the matrix cores.  Chunks of one strip set are mapped to one XCD (blocks
+// are dispatched to XCDs round robin), so the strips of a chunk read its Y
+// rows from the same L2.
+constexpr int kTnRows = 64, kTnCols = 256, kTnPad = 8;
+
+__device__ __forceinline__ uint32_t pack_lo(uint32_t a, uint32_t b) { return (a & 0xFFFFu) | (b << 16); }
+__device__ __forceinline__ uint32_t pack_hi(uint32_t a, uint32_t b) { return (a >> 16) | (b & 0xFFFF0000u); }
+
+template <typename TX, int NT>
+__global__ __launch_bounds__(kThreads, 2) void tn_split3_kernel(int64_t M, int64_t N, int s,
+                                                                  const __hip_bfloat16* __restrict__ A, int64_t lda,
+                                                                  const TX* __restrict__ Yb, int64_t ldy,
+                                                                  int64_t rows_per_chunk, int strips, int chunks,
+                                                                  double* __restrict__ P) {
+    constexpr int KP = kTnRows + kTnPad;  // LDS row length (bf16)
+    constexpr int SW = 16 * NT;
+    constexpr int YPAIRS = (SW * (kTnRows / 8) + kThreads - 1) / kThreads;  // (column, 8-row group) per thread
+    __shared__ __align__(16) __bf16 At[kTnCols][KP];
+    __shared__ __align__(16) __bf16 Yp[3][SW][KP];
+    // block -> (strip, chunk); with chunks a multiple of 8 the strips of one
+    // chunk share an XCD
+    const int b = blockIdx.x;
+    int strip, chunk;
+    if (chunks % 8 == 0) {
+        const int x = b % 8, q = b / 8;
+        strip = q % strips;
+        chunk = (q / strips) * 8 + x;
+    } else {
+        strip = b % strips;
+        chunk = b / strips;
+    }
+    const int64_t j0 = static_cast<int64_t>(strip) * kTnCols;
+    const int64_t r0 = static_cast<int64_t>(chunk) * rows_per_chunk;
+    const int64_t r1 = std::min<int64_t>(M, r0 + rows_per_chunk);
+    const int t = threadIdx.x, lane = t % kWave, wave = t / kWave;
+    const int c16 = lane % 16, q = lane / 16;
+    // this thread's 8 x 8 sub-tile of the A tile: eight consecutive threads
+    // take the eight row groups of one column group, so the transposed
+    // 16-byte stores of a column are contiguous in LDS (a column-group-major
+    // assignment strides them 8 rows apart: 1152 bytes, one bank in two)
+    const int bi = t % 8, bj = t / 8;
+    const int64_t acol = j0 + 8 * bj;
+    floatx4 acc[4][NT];
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+#pragma unroll
+        for (int j = 0; j < NT; ++j) acc[e][j] = floatx4{0, 0, 0, 0};
+    uint4 ar[8];
+    float yr[YPAIRS][8];
+    auto fetch = [&](int64_t i0) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int64_t row = i0 + 8 * bi + u;
+            if (row < r1 && acol + 8 <= N) {
+                ar[u] = *reinterpret_cast<const uint4*>(A + row * lda + acol);
+            } else {
+                uint32_t w[4] = {0, 0, 0, 0};
+                if (row < r1)
+                    for (int e = 0; e < 8; ++e)
+                        if (acol + e < N) {
+                            const uint16_t bits = __builtin_bit_cast(uint16_t, A[row * lda + acol + e]);
+                            w[e / 2] |= static_cast<uint32_t>(bits) << (16 * (e % 2));
+                        }
+                ar[u] = uint4{w[0], w[1], w[2], w[3]};
+            }
+        }
+#pragma unroll
+        for (int pp = 0; pp < YPAIRS; ++pp) {
+            const int pair = t + pp * kThreads;
+            const int col = pair % SW, kg = pair / SW;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int64_t row = i0 + 8 * kg + u;
+                yr[pp][u] = (kg < kTnRows / 8 && col < s && row < r1) ? static_cast<float>(Yb[row * ldy + col]) : 0.0f;
+            }
+        }
+    };
+    auto store = [&]() {
+        // 8 x 8 transpose: column c of the sub-tile, rows 8 bi .. 8 bi + 8
+        const uint32_t* w = reinterpret_cast<const uint32_t*>(ar);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            uint4 lo, hi;
+            lo.x = pack_lo(w[0 * 4 + k], w[1 * 4 + k]);
+            lo.y = pack_lo(w[2 * 4 + k], w[3 * 4 + k]);
+            lo.z = pack_lo(w[4 * 4 + k], w[5 * 4 + k]);
+            lo.w = pack_lo(w[6 * 4 + k], w[7 * 4 + k]);
+            hi.x = pack_hi(w[0 * 4 + k], w[1 * 4 + k]);
+            hi.y = pack_hi(w[2 * 4 + k], w[3 * 4 + k]);
+            hi.z = pack_hi(w[4 * 4 + k], w[5 * 4 + k]);
+            hi.w = pack_hi(w[6 * 4 + k], w[7 * 4 + k]);
+            *reinterpret_cast<uint4*>(&At[8 * bj + 2 * k][8 * bi]) = lo;
+            *reinterpret_cast<uint4*>(&At[8 * bj + 2 * k + 1][8 * bi]) = hi;
+        }
+#pragma unroll
+        for (int pp = 0; pp < YPAIRS; ++pp) {
+            const int pair = t + pp * kThreads;
+            const int col = pair % SW, kg = pair / SW;
+            if (kg >= kTnRows / 8) continue;
+            bf16x8 h, m, l;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                __bf16 x0, x1, x2;
+                split3(static_cast<double>(yr[pp][u]), x0, x1, x2);
+                h[u] = x0;
+                m[u] = x1;
+                l[u] = x2;
+            }
+            *reinterpret_cast<bf16x8*>(&Yp[0][col][8 * kg]) = h;
+            *reinterpret_cast<bf16x8*>(&Yp[1][col][8 * kg]) = m;
+            *reinterpret_cast<bf16x8*>(&Yp[2][col][8 * kg]) = l;
+        }
+    };
+    if (r0 < r1) {
+        fetch(r0);
+        store();
+    }
+    __syncthreads();
+    for (int64_t i0 = r0; i0 < r1; i0 += kTnRows) {
+        const bool more = i0 + kTnRows < r1;
+        if (more) fetch(i0 + kTnRows);
+#pragma unroll
+        for (int h = 0; h < kTnRows / 32; ++h) {
+            const int k = 32 * h + 8 * q;
+            bf16x8 a[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) a[e] = *reinterpret_cast<const bf16x8*>(&At[16 * (wave + 4 * e) + c16][k]);
+#pragma unroll
+            for (int j = 0; j < NT; ++j) {
+                const bf16x8 yh = *reinterpret_cast<const bf16x8*>(&Yp[0][16 * j + c16][k]);
+                const bf16x8 ym = *reinterpret_cast<const bf16x8*>(&Yp[1][16 * j + c16][k]);
+                const bf16x8 yl = *reinterpret_cast<const bf16x8*>(&Yp[2][16 * j + c16][k]);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    acc[e][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[e], yl, acc[e][j], 0, 0, 0);
+                    acc[e][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[e], ym, acc[e][j], 0, 0, 0);
+                    acc[e][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[e], yh, acc[e][j], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+        if (more) store();
+        __syncthreads();
+    }
+    // result element r of lane (c16, q): column 16 j + c16 of Y, column
+    // 16 (wave + 4 e) + 4 q + r of the strip
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+        const int col = 16 * j + c16;
+        if (col >= s) continue;
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t jj = j0 + 16 * (wave + 4 * e) + 4 * q + r;
+                if (jj < N) P[(static_cast<int64_t>(chunk) * N + jj) * s + col] = static_cast<double>(acc[e][j][r]);
+            }
+    }
+}
+
 // Z[j][col] = sum over chunks of P[chunk][j][col]: a wavefront per output,
 // lanes over chunks in order, then the butterfly (a fixed order).
 template <typename TX>
@@ -469,6 +708,18 @@ void nn_dispatch(hipStream_t st, int64_t M, int64_t N, int s, const TA* A, int64
             return;
         }
     }
+    if constexpr (MODE == kSplit3 && std::is_same_v<TA, __hip_bfloat16>) {
+#define AMFLS_NN3(NT)                                                                                          \
+        hipLaunchKernelGGL((nn_split3_kernel<TX, NT, RT>), grid, dim3(kThreads), 0, st, M, N, s, A, lda,           \
+                           static_cast<const __bf16*>(XT), nk, plane, Y, ldy)
+        if (NTt == 1) { AMFLS_NN3(1); }
+        else if (NTt == 2) { AMFLS_NN3(2); }
+        else if (NTt == 3) { AMFLS_NN3(3); }
+        else { AMFLS_NN3(4); }
+#undef AMFLS_NN3
+        AMFLS_LAUNCH_CHECK();
+        return;
+    }
 #define AMFLS_NN(NT)                                                                                           \
     hipLaunchKernelGGL((dense_nn_kernel<TA, TX, MODE, NT, RT>), grid, dim3(kThreads), 0, st, M, N, s, A, lda, XT, \
                        nk, plane, Y, ldy)
@@ -480,9 +731,44 @@ void nn_dispatch(hipStream_t st, int64_t M, int64_t N, int s, const TA* A, int64
     AMFLS_LAUNCH_CHECK();
 }
 
+template <typename TX>
+void tn_split3_dispatch(hipStream_t st, int64_t M, int64_t N, int s, const __hip_bfloat16* A, int64_t lda,
+                        const TX* Y, int64_t ldy, TX* Z, int64_t ldz, Scratch& scratch) {
+    const int strips = static_cast<int>(ceil_div(N, kTnCols));
+    // about four blocks per CU over 256 CUs, whole 64-row trips per chunk,
+    // a multiple of 8 chunks for the XCD mapping
+    int64_t chunks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(1024, strips), ceil_div(M, 4 * kTnRows)));
+    chunks = fit_chunks(scratch, static_cast<size_t>(N * s) * sizeof(double), chunks);
+    if (chunks >= 8) chunks -= chunks % 8;
+    const int64_t rows_per_chunk = ceil_div(ceil_div(M, chunks), kTnRows) * kTnRows;
+    chunks = ceil_div(M, rows_per_chunk);
+    scratch.ensure(static_cast<size_t>(chunks * N * s) * sizeof(double));
+    double* P = static_cast<double*>(scratch.ptr);
+    const dim3 grid(static_cast<unsigned>(strips * chunks));
+    const int ch = static_cast<int>(chunks);
+#define AMFLS_TN3(NT)                                                                                             \
+    hipLaunchKernelGGL((tn_split3_kernel<TX, NT>), grid, dim3(kThreads), 0, st, M, N, s, A, lda, Y, ldy,          \
+                       rows_per_chunk, strips, ch, P)
+    if (s <= 16) { AMFLS_TN3(1); }
+    else if (s <= 32) { AMFLS_TN3(2); }
+    else if (s <= 48) { AMFLS_TN3(3); }
+    else { AMFLS_TN3(4); }
+#undef AMFLS_TN3
+    AMFLS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(dense_tn_reduce_kernel<TX>, dim3(static_cast<unsigned>(ceil_div(N * s, kThreads / kWave))),
+                       dim3(kThreads), 0, st, N, s, chunks, P, Z, ldz);
+    AMFLS_LAUNCH_CHECK();
+}
+
 template <typename TA, typename TX, int MODE>
 void tn_dispatch(hipStream_t st, int64_t M, int64_t N, int s, const TA* A, int64_t lda, const TX* Y, int64_t ldy,
                  TX* Z, int64_t ldz, Scratch& scratch) {
+    if constexpr (MODE == kSplit3 && std::is_same_v<TA, __hip_bfloat16>) {
+        if ((reinterpret_cast<uintptr_t>(A) & 15) == 0 && lda % 8 == 0) {
+            tn_split3_dispatch<TX>(st, M, N, s, A, lda, Y, ldy, Z, ldz, scratch);
+            return;
+        }
+    }
     const int64_t strips = ceil_div(N, 64);
     int64_t chunks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(2048, strips), ceil_div(M, 256)));
     chunks = fit_chunks(scratch, static_cast<size_t>(N * s) * sizeof(double), chunks);

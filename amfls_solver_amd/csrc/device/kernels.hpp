@@ -62,6 +62,26 @@ void gemm_tn(hipStream_t s, int64_t M, int64_t K, int N, const Strided& A, const
              void* C, ValueType ctype, int64_t ldc, double alpha, Scratch& scratch,
              const double* gate = nullptr);
 
+// Tall-skinny products on the matrix cores (tall.hip) for row-major blocks
+// with K, N <= 64: C = alpha A^T B (FP64 accumulation, partials through
+// `scratch`) and Y = alpha A C + beta Y (C FP64 on the device, K x N with
+// leading dimension ldc; Y may alias A).
+bool tall_wide_supported(int K, int N);
+template <typename T>
+void tall_tn_wide(hipStream_t s, int64_t M, int K, int N, const T* A, int64_t rs, const T* B, int64_t ldb, void* C,
+                  ValueType ctype, int64_t ldc, double alpha, Scratch& scratch, const double* gate = nullptr);
+template <typename T>
+void tall_nn_wide(hipStream_t s, int64_t M, int N, int K, const T* A, int64_t rs, const double* C, int64_t ldc, T* Y,
+                  int64_t ldy, double alpha, double beta, const double* gate = nullptr);
+// y = alpha V t + beta y and c = V^T g for a row-major basis V (n x k) and
+// FP64 vectors (tall.hip).
+template <typename TV>
+void basis_apply(hipStream_t s, int64_t n, int k, const TV* V, int64_t ldv, const double* t, double* y, double alpha,
+                 double beta);
+template <typename TV>
+void basis_project(hipStream_t s, int64_t n, int k, const TV* V, int64_t ldv, const double* g, double* c,
+                   Scratch& scratch);
+
 // In-place safe row-local product: Y[i, 0:r] = X[i, 0:w] * M (w x r, double
 // row-major, device).  w, r <= 128.
 template <typename T>

@@ -752,6 +752,38 @@ int amfls_block_gemm_tn(amfls_context* ctx, int64_t m, int32_t k, int32_t n, con
     });
 }
 
+int amfls_block_gemm_nn_f32(amfls_context* ctx, int64_t m, int32_t n, int32_t k, const float* a, int64_t lda,
+                            const double* c, int64_t ldc, float* y, int64_t ldy, double alpha, double beta) {
+    return guard([&] {
+        require(ctx && (m == 0 || (a && y)) && (k == 0 || c), "null argument");
+        require(m >= 0 && n >= 0 && k >= 0 && lda >= k && ldc >= n && ldy >= n, "invalid block shape");
+        const Strided A{a, ValueType::f32, lda, 1};
+        gemm_nn<float, double>(ctx->ctx.stream, m, n, k, A, c, ldc, y, ldy, alpha, beta);
+        ctx->ctx.sync();
+    });
+}
+
+int amfls_block_gemm_tn_f32(amfls_context* ctx, int64_t m, int32_t k, int32_t n, const float* a, int64_t lda,
+                            const float* b, int64_t ldb, double* c, int64_t ldc, double alpha) {
+    return guard([&] {
+        require(ctx && c && (m == 0 || (a && b)), "null argument");
+        require(m >= 0 && n >= 0 && k >= 0 && lda >= k && ldb >= n && ldc >= n, "invalid block shape");
+        const Strided A{a, ValueType::f32, lda, 1};
+        gemm_tn<float>(ctx->ctx.stream, m, k, n, A, b, ldb, c, ValueType::f64, ldc, alpha, ctx->ctx.scratch);
+        ctx->ctx.sync();
+    });
+}
+
+int amfls_block_rowmul_f32(amfls_context* ctx, int64_t rows, int32_t w, int32_t r, const float* x, int64_t ldx,
+                           const double* m, float* y, int64_t ldy) {
+    return guard([&] {
+        require(ctx && m && (rows == 0 || (x && y)), "null argument");
+        require(rows >= 0 && w > 0 && r > 0 && ldx >= w && ldy >= r, "invalid block shape");
+        rowmul<float>(ctx->ctx.stream, rows, w, r, x, ldx, m, y, ldy);
+        ctx->ctx.sync();
+    });
+}
+
 int amfls_block_rowmul(amfls_context* ctx, int64_t rows, int32_t w, int32_t r, const double* x, int64_t ldx,
                        const double* m, double* y, int64_t ldy) {
     return guard([&] {

@@ -47,25 +47,25 @@ BasisPreconditioner::BasisPreconditioner(Context& ctx, const BasisCapture& basis
 
 void BasisPreconditioner::apply(const std::vector<double>& a, const double* c, double* y) {
     hipStream_t s = ctx_.stream;
-    const Strided V{basis_.V.get(), ValueType::f32, k_, 1};
+    const float* V = basis_.V.get();
     t_ = a;
     band_solve(R_, t_);
     const double* td = ctx_.upload(t_.data(), t_.size());
-    gemm_nn<double, double>(s, n_, 1, k_, V, td, 1, y, 1, 1.0, 0.0);
+    basis_apply<float>(s, n_, k_, V, k_, td, y, 1.0, 0.0);
     if (complement_ && c) {
-        gemm_tn<double>(s, n_, k_, 1, V, c, 1, proj_.get(), ValueType::f64, 1, 1.0, ctx_.scratch);
+        basis_project<float>(s, n_, k_, V, k_, c, proj_.get(), ctx_.scratch);
         axpy_block<double>(s, n_, 1, tau_, c, 1, y, 1);
-        gemm_nn<double, double>(s, n_, 1, k_, V, proj_.get(), 1, y, 1, -tau_, 1.0);
+        basis_apply<float>(s, n_, k_, V, k_, proj_.get(), y, -tau_, 1.0);
     }
 }
 
 void BasisPreconditioner::apply_t(const double* g, std::vector<double>& a, double* c) {
     hipStream_t s = ctx_.stream;
-    const Strided V{basis_.V.get(), ValueType::f32, k_, 1};
-    gemm_tn<double>(s, n_, k_, 1, V, g, 1, proj_.get(), ValueType::f64, 1, 1.0, ctx_.scratch);
+    const float* V = basis_.V.get();
+    basis_project<float>(s, n_, k_, V, k_, g, proj_.get(), ctx_.scratch);
     if (complement_ && c) {
         copy_block<double, double>(s, n_, 1, g, 1, c, 1, tau_);
-        gemm_nn<double, double>(s, n_, 1, k_, V, proj_.get(), 1, c, 1, -tau_, 1.0);
+        basis_apply<float>(s, n_, k_, V, k_, proj_.get(), c, -tau_, 1.0);
     }
     a.resize(static_cast<size_t>(k_));
     ctx_.download(a.data(), proj_.get(), static_cast<size_t>(k_));

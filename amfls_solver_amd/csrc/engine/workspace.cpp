@@ -277,6 +277,26 @@ typename Workspace<T>::Factor Workspace<T>::orthogonalize_and_factor(T* W, int64
         small_multiply(W, ldw, N, w, w, Rinv);
         Rtot = matmul(R, Rtot);
     }
+    // A block whose triangular factor provably keeps every column needs no
+    // SVD: sigma_min(R) >= 1 / ||R^{-1}||_F, so when that bound clears the
+    // cutoff the numerical rank is w, nothing is discarded, and W = Q R is
+    // already a rank-revealing factorization (Q spans the same columns as
+    // the singular vectors; the block recurrence and the projected problem
+    // are invariant under that rotation).  Near-deficient blocks, and blocks
+    // capped below their width, take the SVD.
+    if (w <= max_rank) {
+        const HostMat Rinv = upper_inverse(Rtot);
+        double fro = 0.0;
+        for (double v : Rinv.a) fro += v * v;
+        if (std::isfinite(fro) && fro > 0.0 && 1.0 / std::sqrt(fro) > f.cutoff) {
+            f.factor = Rtot;
+            f.rank = w;
+            f.discarded = 0.0;
+            f.discarded_tracked = 0.0;
+            done();
+            return f;
+        }
+    }
     Svd sv = jacobi_svd(Rtot);
     int rank = 0;
     while (rank < static_cast<int>(sv.s.size()) && rank < max_rank && std::isfinite(sv.s[rank]) &&

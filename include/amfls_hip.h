@@ -306,6 +306,15 @@ int amfls_block_gemm_tn(amfls_context* ctx, int64_t m, int32_t k, int32_t n, con
 int amfls_block_rowmul(amfls_context* ctx, int64_t rows, int32_t w, int32_t r, const double* x, int64_t ldx,
                        const double* m, double* y, int64_t ldy);
 int amfls_block_stats(amfls_context* ctx, int64_t rows, int32_t cols, const double* x, int64_t ldx, double* out);
+/* The same products on FP32 blocks (the mixed-precision search's local
+ * work): A, B, X, Y FP32; C and M FP64; block_gemm_tn_f32 accumulates in
+ * FP64 and writes FP64 C. */
+int amfls_block_gemm_nn_f32(amfls_context* ctx, int64_t m, int32_t n, int32_t k, const float* a, int64_t lda,
+                            const double* c, int64_t ldc, float* y, int64_t ldy, double alpha, double beta);
+int amfls_block_gemm_tn_f32(amfls_context* ctx, int64_t m, int32_t k, int32_t n, const float* a, int64_t lda,
+                            const float* b, int64_t ldb, double* c, int64_t ldc, double alpha);
+int amfls_block_rowmul_f32(amfls_context* ctx, int64_t rows, int32_t w, int32_t r, const float* x, int64_t ldx,
+                           const double* m, float* y, int64_t ldy);
 
 #ifdef __cplusplus
 }

@@ -1,4 +1,4 @@
-"""The engine's FP64 block kernels (tall-skinny products, the row-local
+"""The engine's FP64 and FP32 block kernels (tall-skinny products, the row-local
 product and the norm statistics) through their C entry points, against
 float64 references computed on the host: every width class the dispatch
 distinguishes (the matrix-core kernels for K, N <= 16 and the tiled kernels
@@ -118,3 +118,69 @@ def test_block_stats(env, rows, cols, ld):
         mx, ss, bad = _stats_ref(v)
         assert out[0] == mx and out[2] == bad, case
         assert abs(out[1] - ss) <= 1e-13 * max(ss, 1e-300) * np.log2(v.size + 2), case
+
+
+# FP32 blocks (the mixed-precision search's local products): the matrix-core
+# tall kernels for K, N <= 64, against float64 host products of the same
+# FP32 inputs.  Products with FP32 outputs are checked to FP32 rounding of
+# the absolute-value product; A^T B accumulates exact products in FP64.
+SHAPES_F32 = [(1, 1, 1), (37, 5, 3), (1000, 16, 16), (1000, 17, 20), (4099, 32, 32), (777, 33, 8), (513, 8, 40),
+              (20001, 64, 64), (300, 1, 64), (300, 64, 1), (100003, 48, 64)]
+
+
+def _padded32(torch, rng, m, k, ld):
+    a = torch.zeros(m, ld, dtype=torch.float32)
+    a[:, :k] = torch.from_numpy(rng.standard_normal((m, k)).astype(np.float32))
+    return a.cuda()
+
+
+@pytest.mark.parametrize("m,k,n", SHAPES_F32)
+def test_block_gemm_nn_f32(env, m, k, n):
+    torch, ctx, lib = env
+    from amfls_solver_amd import _lib
+    rng = np.random.default_rng(m + 5 * k + 11 * n)
+    for lda, ldy, alpha, beta in ((k, n, 1.0, 0.0), (k + 3, n + 5, -0.5, 2.0)):
+        a = _padded32(torch, rng, m, k, lda)
+        c = torch.from_numpy(rng.standard_normal((k, n))).cuda()
+        y = _padded32(torch, rng, m, n, ldy)
+        a64, y64 = a[:, :k].double().cpu(), y[:, :n].double().cpu()
+        want = alpha * (a64 @ c.cpu()) + beta * y64
+        _lib.check(lib.amfls_block_gemm_nn_f32(ctx.handle, m, n, k, _p(a), lda, _p(c), n, _p(y), ldy, alpha, beta))
+        got = y[:, :n].double().cpu()
+        scale = abs(alpha) * (a64.abs() @ c.abs().cpu()) + abs(beta) * y64.abs()
+        assert ((got - want).abs() <= 4e-7 * (k + 2) * scale + 1e-30).all(), (lda, ldy)
+        if ldy > n:
+            assert torch.isfinite(y[:, n:]).all()
+
+
+@pytest.mark.parametrize("m,k,n", SHAPES_F32)
+def test_block_gemm_tn_f32(env, m, k, n):
+    torch, ctx, lib = env
+    from amfls_solver_amd import _lib
+    rng = np.random.default_rng(7 * m + k + 3 * n)
+    for lda, ldb, alpha in ((k, n, 1.0), (k + 2, n + 1, 0.25)):
+        a = _padded32(torch, rng, m, k, lda)
+        b = _padded32(torch, rng, m, n, ldb)
+        c = torch.full((k, n + 2), 7.0, dtype=torch.float64, device="cuda")
+        _lib.check(lib.amfls_block_gemm_tn_f32(ctx.handle, m, k, n, _p(a), lda, _p(b), ldb, _p(c), n + 2, alpha))
+        a64, b64 = a[:, :k].double().cpu(), b[:, :n].double().cpu()
+        want = alpha * (a64.T @ b64)
+        scale = abs(alpha) * (a64.abs().T @ b64.abs()).max().item()
+        assert (c[:, :n].cpu() - want).abs().max().item() <= 1e-14 * scale * np.sqrt(m) * 4, (lda, ldb)
+        assert (c[:, n:] == 7.0).all()
+
+
+@pytest.mark.parametrize("rows,w,r", [(1, 1, 1), (1000, 16, 16), (4097, 32, 32), (999, 20, 7), (500, 40, 33),
+                                      (100001, 64, 64)])
+def test_block_rowmul_f32_in_place(env, rows, w, r):
+    torch, ctx, lib = env
+    from amfls_solver_amd import _lib
+    rng = np.random.default_rng(rows + 3 * w + r)
+    ld = max(w, r) + 1
+    x = _padded32(torch, rng, rows, w, ld)
+    mat = torch.from_numpy(rng.standard_normal((w, r))).cuda()
+    x64 = x[:, :w].double().cpu()
+    want = x64 @ mat.cpu()
+    _lib.check(lib.amfls_block_rowmul_f32(ctx.handle, rows, w, r, _p(x), ld, _p(mat), _p(x), ld))
+    scale = x64.abs() @ mat.abs().cpu()
+    assert ((x[:, :r].double().cpu() - want).abs() <= 4e-7 * (w + 1) * scale + 1e-30).all()

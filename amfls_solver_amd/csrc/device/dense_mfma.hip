@@ -22,8 +22,10 @@
 //
 // Y = A X: one wavefront per 16 rows, K stepped by 32 (eight consecutive
 // entries of an A row per lane, one 16-byte load for bf16), X read through
-// the cache (it is small).  Z = A^T Y: a block owns a strip of 64 columns of
-// A and a chunk of rows; 32 x 64 tiles of A and 32 x s tiles of Y are staged
+// the cache (it is small); the split-bf16 product on aligned bf16 rows
+// stages both the A tile and the X slice of each K step in LDS instead
+// (nn_split3_kernel).  Z = A^T Y: a block owns a strip of 64 columns of A
+// and a chunk of rows; 32 x 64 tiles of A and 32 x s tiles of Y are staged
 // in LDS with coalesced loads; per-chunk partial sums are combined by a
 // second kernel in a fixed order (deterministic, no atomics).
 #include "kernels.hpp"
@@ -246,71 +248,114 @@ __global__ __launch_bounds__(kThreads) void dense_nn_kernel(int64_t M, int64_t N
     }
 }
 
-// Split-bf16 Y = A X for bf16 A: as dense_nn_kernel, but the A fragments
-// are bf16 already (reinterpreted, not converted) and each X fragment
-// (three bf16 planes) is loaded once per 32-wide K step and column tile and
-// fed to all RT row tiles of the wavefront.
-template <typename TX, int NT, int RT>
-__global__ __launch_bounds__(kThreads) void nn_split3_kernel(int64_t M, int64_t N, int s,
-                                                             const __hip_bfloat16* __restrict__ A, int64_t lda,
-                                                             const __bf16* __restrict__ XT, int64_t nk, int64_t plane,
-                                                             TX* __restrict__ Y, int64_t ldy) {
-    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+// Split-bf16 Y = A X for bf16 A (16-byte aligned rows): a block of eight
+// wavefronts owns 128 rows, a wavefront one 16-row tile.  Each 64-wide K
+// trip stages the block's A tile (128 rows x 128 bytes) and the trip's X
+// slice (three bf16 planes x 16 NT columns x 64 entries) in LDS with
+// coalesced 16-byte loads (eight lanes read the 128 contiguous bytes of a
+// row); the global loads of trip t + 3 are in registers while trip t runs
+// on the matrix cores, and the two LDS buffers alternate with one barrier
+// per trip.  LDS rows are swizzled (slot s of row r holds 16-byte piece
+// s ^ (r mod 8)) so the MFMA operand reads of eight consecutive lanes hit
+// distinct banks.  Lane (c, q) supplies k = 32 h + 8 q + t of step h for
+// both operands.
+constexpr int kPipeWaves = 8, kPipeStages = 4;
+
+template <typename TX, int NT>
+__global__ __launch_bounds__(64 * kPipeWaves) void nn_split3_kernel(int64_t M, int64_t N, int s,
+                                                                    const __hip_bfloat16* __restrict__ A, int64_t lda,
+                                                                    const __bf16* __restrict__ XT, int64_t nk,
+                                                                    int64_t plane, TX* __restrict__ Y, int64_t ldy) {
+    constexpr int NTH = 64 * kPipeWaves, RB = 16 * kPipeWaves, SW = 16 * NT, S = kPipeStages;
+    constexpr int APIECES = RB * 8, XPIECES = 3 * SW * 8, PIECES = APIECES + XPIECES;
+    constexpr int PPT = (PIECES + NTH - 1) / NTH;
+    __shared__ __align__(16) uint4 As[2][APIECES];
+    __shared__ __align__(16) uint4 Xs[2][XPIECES];
+    const int t = threadIdx.x, lane = t % kWave, wave = t / kWave;
     const int c = lane % 16, q = lane / 16;
-    const int64_t row0 = (static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + wave) * (16 * RT);
-    if (row0 >= M) return;
-    floatx4 acc[RT][NT];
+    const int64_t rb0 = static_cast<int64_t>(blockIdx.x) * RB;
+    const int64_t trips = (N + 63) / 64;
+    floatx4 acc[NT];
 #pragma unroll
-    for (int r = 0; r < RT; ++r)
+    for (int j = 0; j < NT; ++j) acc[j] = floatx4{0, 0, 0, 0};
+    uint4 stg[S - 1][PPT];
+    auto fetch = [&](uint4 (&dst)[PPT], int64_t k0) {
 #pragma unroll
-        for (int j = 0; j < NT; ++j) acc[r][j] = floatx4{0, 0, 0, 0};
-    Frag8<__hip_bfloat16> cur[RT][2], nxt[RT][2];
-#pragma unroll
-    for (int r = 0; r < RT; ++r)
-#pragma unroll
-        for (int h = 0; h < 2; ++h) cur[r][h].load(A, lda, row0 + 16 * r + c, M, 16 * q + 8 * h, N);
-    for (int64_t k0 = 0; k0 < N; k0 += 64) {
-        if (k0 + 64 < N) {
-#pragma unroll
-            for (int r = 0; r < RT; ++r)
-#pragma unroll
-                for (int h = 0; h < 2; ++h) nxt[r][h].load(A, lda, row0 + 16 * r + c, M, k0 + 64 + 16 * q + 8 * h, N);
-        }
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            bf16x8 av[RT];
-#pragma unroll
-            for (int r = 0; r < RT; ++r) av[r] = __builtin_bit_cast(bf16x8, cur[r][h].v);
-#pragma unroll
-            for (int j = 0; j < NT; ++j) {
-                const int64_t off = static_cast<int64_t>(16 * j + c) * nk + k0 + 16 * q + 8 * h;
-                const bf16x8 xh = *reinterpret_cast<const bf16x8*>(XT + off);
-                const bf16x8 xm = *reinterpret_cast<const bf16x8*>(XT + plane + off);
-                const bf16x8 xl = *reinterpret_cast<const bf16x8*>(XT + 2 * plane + off);
-#pragma unroll
-                for (int r = 0; r < RT; ++r) {
-                    acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[r], xl, acc[r][j], 0, 0, 0);
-                    acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[r], xm, acc[r][j], 0, 0, 0);
-                    acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[r], xh, acc[r][j], 0, 0, 0);
+        for (int p = 0; p < PPT; ++p) {
+            const int e = t + p * NTH;
+            if (e < APIECES) {
+                const int row = e / 8, piece = (e % 8) ^ (row & 7);
+                const int64_t grow = rb0 + row, kk = k0 + 8 * piece;
+                uint4 v = uint4{0, 0, 0, 0};
+                if (grow < M && kk + 8 <= N) {
+                    v = *reinterpret_cast<const uint4*>(A + grow * lda + kk);
+                } else if (grow < M && kk < N) {
+                    uint32_t w[4] = {0, 0, 0, 0};
+                    for (int u = 0; u < 8; ++u)
+                        if (kk + u < N)
+                            w[u / 2] |= static_cast<uint32_t>(__builtin_bit_cast(uint16_t, A[grow * lda + kk + u]))
+                                        << (16 * (u % 2));
+                    v = uint4{w[0], w[1], w[2], w[3]};
                 }
+                dst[p] = v;
+            } else if (PIECES % NTH == 0 || e < PIECES) {
+                // X planes are zero padded to nk, a multiple of 64: the slice is always whole
+                const int x = e - APIECES, plc = x / 8, col = plc % SW, pl = plc / SW;
+                const int piece = (x % 8) ^ (col & 7);
+                dst[p] = *reinterpret_cast<const uint4*>(XT + pl * plane + static_cast<int64_t>(col) * nk + k0 +
+                                                         8 * piece);
             }
         }
+    };
+    auto store = [&](const uint4 (&src)[PPT], int buf) {
 #pragma unroll
-        for (int r = 0; r < RT; ++r)
+        for (int p = 0; p < PPT; ++p) {
+            const int e = t + p * NTH;
+            if (e < APIECES) As[buf][e] = src[p];
+            else if (PIECES % NTH == 0 || e < PIECES) Xs[buf][e - APIECES] = src[p];
+        }
+    };
+    auto compute = [&](int buf) {
 #pragma unroll
-            for (int h = 0; h < 2; ++h) cur[r][h] = nxt[r][h];
+        for (int h = 0; h < 2; ++h) {
+            const int sl = (4 * h + q) ^ (c & 7);
+            const bf16x8 av = __builtin_bit_cast(bf16x8, As[buf][(16 * wave + c) * 8 + sl]);
+#pragma unroll
+            for (int j = 0; j < NT; ++j) {
+                const int col = 16 * j + c;
+                const bf16x8 xh = __builtin_bit_cast(bf16x8, Xs[buf][(0 * SW + col) * 8 + sl]);
+                const bf16x8 xm = __builtin_bit_cast(bf16x8, Xs[buf][(1 * SW + col) * 8 + sl]);
+                const bf16x8 xl = __builtin_bit_cast(bf16x8, Xs[buf][(2 * SW + col) * 8 + sl]);
+                acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, xl, acc[j], 0, 0, 0);
+                acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, xm, acc[j], 0, 0, 0);
+                acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, xh, acc[j], 0, 0, 0);
+            }
+        }
+    };
+#pragma unroll
+    for (int i = 0; i < S - 1; ++i)
+        if (i < trips) fetch(stg[i], 64 * i);
+    for (int64_t t0 = 0; t0 < trips; t0 += S - 1) {
+#pragma unroll
+        for (int u = 0; u < S - 1; ++u) {
+            const int64_t tt = t0 + u;
+            if (tt < trips) {
+                store(stg[u], static_cast<int>(tt & 1));
+                __syncthreads();
+                if (tt + S - 1 < trips) fetch(stg[u], 64 * (tt + S - 1));
+                compute(static_cast<int>(tt & 1));
+            }
+        }
     }
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
         const int col = 16 * j + c;
         if (col >= s) continue;
 #pragma unroll
-        for (int r = 0; r < RT; ++r)
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int64_t row = row0 + 16 * r + 4 * q + e;
-                if (row < M) Y[row * ldy + col] = static_cast<TX>(acc[r][j][e]);
-            }
+        for (int e = 0; e < 4; ++e) {
+            const int64_t row = rb0 + 16 * wave + 4 * q + e;
+            if (row < M) Y[row * ldy + col] = static_cast<TX>(acc[j][e]);
+        }
     }
 }
 
@@ -709,16 +754,19 @@ void nn_dispatch(hipStream_t st, int64_t M, int64_t N, int s, const TA* A, int64
         }
     }
     if constexpr (MODE == kSplit3 && std::is_same_v<TA, __hip_bfloat16>) {
+        if ((reinterpret_cast<uintptr_t>(A) & 15) == 0 && lda % 8 == 0) {
+            const dim3 g3(static_cast<unsigned>(ceil_div(M, 16 * kPipeWaves)));
 #define AMFLS_NN3(NT)                                                                                          \
-        hipLaunchKernelGGL((nn_split3_kernel<TX, NT, RT>), grid, dim3(kThreads), 0, st, M, N, s, A, lda,           \
-                           static_cast<const __bf16*>(XT), nk, plane, Y, ldy)
-        if (NTt == 1) { AMFLS_NN3(1); }
-        else if (NTt == 2) { AMFLS_NN3(2); }
-        else if (NTt == 3) { AMFLS_NN3(3); }
-        else { AMFLS_NN3(4); }
+            hipLaunchKernelGGL((nn_split3_kernel<TX, NT>), g3, dim3(64 * kPipeWaves), 0, st, M, N, s, A, lda,      \
+                               static_cast<const __bf16*>(XT), nk, plane, Y, ldy)
+            if (NTt == 1) { AMFLS_NN3(1); }
+            else if (NTt == 2) { AMFLS_NN3(2); }
+            else if (NTt == 3) { AMFLS_NN3(3); }
+            else { AMFLS_NN3(4); }
 #undef AMFLS_NN3
-        AMFLS_LAUNCH_CHECK();
-        return;
+            AMFLS_LAUNCH_CHECK();
+            return;
+        }
     }
 #define AMFLS_NN(NT)                                                                                           \
     hipLaunchKernelGGL((dense_nn_kernel<TA, TX, MODE, NT, RT>), grid, dim3(kThreads), 0, st, M, N, s, A, lda, XT, \
@@ -735,9 +783,10 @@ template <typename TX>
 void tn_split3_dispatch(hipStream_t st, int64_t M, int64_t N, int s, const __hip_bfloat16* A, int64_t lda,
                         const TX* Y, int64_t ldy, TX* Z, int64_t ldz, Scratch& scratch) {
     const int strips = static_cast<int>(ceil_div(N, kTnCols));
-    // about four blocks per CU over 256 CUs, whole 64-row trips per chunk,
-    // a multiple of 8 chunks for the XCD mapping
-    int64_t chunks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(1024, strips), ceil_div(M, 4 * kTnRows)));
+    // one round of three resident blocks per CU over 256 CUs (768 blocks:
+    // a second, partial round measured 10% slower), whole 64-row trips per
+    // chunk, a multiple of 8 chunks for the XCD mapping
+    int64_t chunks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(768, strips), ceil_div(M, 4 * kTnRows)));
     chunks = fit_chunks(scratch, static_cast<size_t>(N * s) * sizeof(double), chunks);
     if (chunks >= 8) chunks -= chunks % 8;
     const int64_t rows_per_chunk = ceil_div(ceil_div(M, chunks), kTnRows) * kTnRows;

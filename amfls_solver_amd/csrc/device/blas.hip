@@ -419,7 +419,7 @@ void gemm_nn_dispatch(hipStream_t s, int64_t M, int N, int64_t K, const V* A, in
     const bool row_major = (cs == 1);
     if constexpr (std::is_same_v<V, float> && std::is_same_v<T, float> && std::is_same_v<TB, double>) {
         // FP32 blocks times FP64 coefficients: the matrix-core tall kernel
-        if (row_major && tall_wide_supported(static_cast<int>(std::min<int64_t>(K, 65)), N)) {
+        if (row_major && K <= (int64_t{1} << 30) && tall_wide_supported(static_cast<int>(K), N)) {
             tall_nn_wide<float>(s, M, N, static_cast<int>(K), A, rs, B, ldb, Y, ldy, alpha, beta, gate);
             return;
         }
@@ -618,7 +618,7 @@ void gemm_tn_dispatch(hipStream_t s, int64_t M, int64_t K, int N, const V* A, in
     const bool row_major = (cs == 1);
     const int64_t KN = K * N;
     if constexpr (std::is_same_v<V, float> && std::is_same_v<TB, float>) {
-        if (row_major && K <= 64 && tall_wide_supported(static_cast<int>(K), N)) {
+        if (row_major && K <= (int64_t{1} << 30) && tall_wide_supported(static_cast<int>(K), N)) {
             tall_tn_wide<float>(s, M, static_cast<int>(K), N, A, rs, B, ldb, C, ctype, ldc, alpha, scratch, gate);
             return;
         }

@@ -71,6 +71,11 @@ __global__ __launch_bounds__(kThreads) void tall_tn_rows_kernel(int64_t M, int K
     if (gate_off(gate)) return;
     constexpr int W = kThreads / kWave;
     __shared__ double red[W][256];
+    // slab blockIdx.y of 16 KT columns of A (rows k0 .. of C)
+    const int k0 = static_cast<int>(blockIdx.y) * 16 * KT;
+    double* out = P + static_cast<int64_t>(blockIdx.x) * K * N + static_cast<int64_t>(k0) * N;
+    A += k0;
+    K = std::min(K - k0, 16 * KT);
     const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
     const int c = lane % 16, q = lane / 16;
     const int64_t r_begin = static_cast<int64_t>(blockIdx.x) * rows_per_block;
@@ -97,7 +102,6 @@ __global__ __launch_bounds__(kThreads) void tall_tn_rows_kernel(int64_t M, int K
                     acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(static_cast<double>(a[u][m]),
                                                                      static_cast<double>(b[u][n]), acc[m][n], 0, 0, 0);
     }
-    double* out = P + static_cast<int64_t>(blockIdx.x) * K * N;
 #pragma unroll
     for (int m = 0; m < KT; ++m)
 #pragma unroll
@@ -173,6 +177,54 @@ __global__ __launch_bounds__(kThreads) void tall_nn_wide_kernel(int64_t M, int N
     }
 }
 
+// As tall_nn_wide_kernel for any K: a wavefront owns one 16-row tile and
+// the K sum runs over slabs of 64 (C slab in LDS, one barrier pair per
+// slab), for the short products against a whole basis (n rows, k columns).
+template <typename T, int NT>
+__global__ __launch_bounds__(kThreads) void tall_nn_deep_kernel(int64_t M, int N, int K, const T* A, int64_t rs,
+                                                                const double* __restrict__ C, int64_t ldc, T* Y,
+                                                                int64_t ldy, double alpha, double beta,
+                                                                const double* gate) {
+    if (gate_off(gate)) return;
+    constexpr int KL = 16, KM = 64, NC = 16 * NT;
+    __shared__ double Cs[KM][NC + 1];
+    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    const int i = lane % 16, q = lane / 16;
+    const int64_t tile = static_cast<int64_t>(blockIdx.x) * (kThreads / kWave) + wave;
+    f64x4 acc[NT];
+#pragma unroll
+    for (int n = 0; n < NT; ++n) acc[n] = f64x4{0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < K; k0 += KM) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < KM * NC; e += kThreads) {
+            const int k = e / NC, n = e % NC;
+            Cs[k][n] = (k0 + k < K && n < N) ? C[static_cast<int64_t>(k0 + k) * ldc + n] : 0.0;
+        }
+        __syncthreads();
+        T a[KL];
+        load_row_piece<T, KL>(A, rs, tile * 16 + i, M, k0 + KL * q, K, a);
+#pragma unroll
+        for (int n = 0; n < NT; ++n)
+#pragma unroll
+            for (int t = 0; t < KL; ++t)
+                acc[n] = __builtin_amdgcn_mfma_f64_16x16x4f64(static_cast<double>(a[t]), Cs[KL * q + t][16 * n + i],
+                                                              acc[n], 0, 0, 0);
+    }
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+        const int col = 16 * n + i;
+        if (col >= N) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int64_t row = tile * 16 + 4 * r + q;
+            if (row >= M) continue;
+            double v = alpha * acc[n][r];
+            if (beta != 0.0) v = fma(beta, static_cast<double>(Y[row * ldy + col]), v);
+            Y[row * ldy + col] = static_cast<T>(v);
+        }
+    }
+}
+
 // y = alpha V t + beta y for a row-major V (n x k, FP32 or FP64) and FP64
 // vectors: a wavefront per row, lanes striding over the row (coalesced),
 // t read through the cache.
@@ -229,23 +281,24 @@ __global__ __launch_bounds__(kThreads) void chunk_sum_kernel(int64_t chunks, int
 
 }  // namespace
 
-bool tall_wide_supported(int K, int N) { return K >= 1 && N >= 1 && K <= 64 && N <= 64; }
+bool tall_wide_supported(int K, int N) { return K >= 1 && N >= 1 && N <= 64; }
 
 template <typename T>
 void tall_tn_wide(hipStream_t s, int64_t M, int K, int N, const T* A, int64_t rs, const T* B, int64_t ldb, void* C,
                   ValueType ctype, int64_t ldc, double alpha, Scratch& scratch, const double* gate) {
     if (K <= 0 || N <= 0) return;
-    const int kt = static_cast<int>(ceil_div(K, 16)), nt = static_cast<int>(ceil_div(N, 16));
+    const int kt = static_cast<int>(std::min<int64_t>(4, ceil_div(K, 16))), nt = static_cast<int>(ceil_div(N, 16));
+    const int slabs = static_cast<int>(ceil_div(K, 64));  // K > 64: slabs of 64 rows of C on grid.y
     constexpr int U = 2;
     constexpr int64_t trip = 4 * 4 * U;  // four wavefronts of U row quads
     const int64_t KN = static_cast<int64_t>(K) * N;
-    int64_t nb = std::max<int64_t>(1, std::min<int64_t>(1024, ceil_div(std::max<int64_t>(M, 1), 256)));
+    int64_t nb = std::max<int64_t>(1, std::min<int64_t>(ceil_div(1024, slabs), ceil_div(std::max<int64_t>(M, 1), 256)));
     nb = fit_chunks(scratch, static_cast<size_t>(KN) * sizeof(double), nb);
     const int64_t rows_per_block = ceil_div(ceil_div(std::max<int64_t>(M, 1), nb), trip) * trip;
     nb = ceil_div(std::max<int64_t>(M, 1), rows_per_block);
     scratch.ensure(static_cast<size_t>(nb * KN) * sizeof(double));
     double* P = static_cast<double*>(scratch.ptr);
-    const dim3 grid(static_cast<unsigned>(nb));
+    const dim3 grid(static_cast<unsigned>(nb), static_cast<unsigned>(slabs));
 #define AMFLS_TTN(KT, NT)                                                                                              \
     hipLaunchKernelGGL((tall_tn_rows_kernel<T, KT, NT, U>), grid, dim3(kThreads), 0, s, M, K, N, A, rs, B, ldb, P, \
                        rows_per_block, gate)
@@ -310,13 +363,22 @@ template <typename T>
 void tall_nn_wide(hipStream_t s, int64_t M, int N, int K, const T* A, int64_t rs, const double* C, int64_t ldc, T* Y,
                   int64_t ldy, double alpha, double beta, const double* gate) {
     if (M <= 0 || N <= 0) return;
+    const int nt = static_cast<int>(ceil_div(N, 16));
+    if (K > 64) {
+        const dim3 gd(static_cast<unsigned>(ceil_div(ceil_div(M, 16), kThreads / kWave)));
+#define AMFLS_DNN(NT) \
+        hipLaunchKernelGGL((tall_nn_deep_kernel<T, NT>), gd, dim3(kThreads), 0, s, M, N, K, A, rs, C, ldc, Y, ldy, alpha, beta, gate)
+        if (nt <= 1) { AMFLS_DNN(1); } else if (nt <= 2) { AMFLS_DNN(2); } else if (nt <= 3) { AMFLS_DNN(3); } else { AMFLS_DNN(4); }
+#undef AMFLS_DNN
+        AMFLS_LAUNCH_CHECK();
+        return;
+    }
     constexpr int TPW = 2;
     const int64_t waves = ceil_div(ceil_div(M, 16), TPW);
     const dim3 grid(static_cast<unsigned>(ceil_div(waves, kThreads / kWave)));
 #define AMFLS_TNN(KL, NT)                                                                                           \
     hipLaunchKernelGGL((tall_nn_wide_kernel<T, KL, NT, TPW>), grid, dim3(kThreads), 0, s, M, N, K, A, rs, C, ldc, Y, \
                        ldy, alpha, beta, gate)
-    const int nt = static_cast<int>(ceil_div(N, 16));
     if (K <= 16) {
         if (nt <= 1) { AMFLS_TNN(4, 1); } else if (nt <= 2) { AMFLS_TNN(4, 2); } else if (nt <= 3) { AMFLS_TNN(4, 3); } else { AMFLS_TNN(4, 4); }
     } else if (K <= 32) {

@@ -125,7 +125,8 @@ def test_block_stats(env, rows, cols, ld):
 # FP32 inputs.  Products with FP32 outputs are checked to FP32 rounding of
 # the absolute-value product; A^T B accumulates exact products in FP64.
 SHAPES_F32 = [(1, 1, 1), (37, 5, 3), (1000, 16, 16), (1000, 17, 20), (4099, 32, 32), (777, 33, 8), (513, 8, 40),
-              (20001, 64, 64), (300, 1, 64), (300, 64, 1), (100003, 48, 64)]
+              (20001, 64, 64), (300, 1, 64), (300, 64, 1), (100003, 48, 64), (2000, 300, 32), (777, 130, 17),
+              (3001, 2000, 64)]
 
 
 def _padded32(torch, rng, m, k, ld):
@@ -171,7 +172,7 @@ def test_block_gemm_tn_f32(env, m, k, n):
 
 
 @pytest.mark.parametrize("rows,w,r", [(1, 1, 1), (1000, 16, 16), (4097, 32, 32), (999, 20, 7), (500, 40, 33),
-                                      (100001, 64, 64)])
+                                      (100001, 64, 64), (700, 100, 50), (300, 128, 64)])
 def test_block_rowmul_f32_in_place(env, rows, w, r):
     torch, ctx, lib = env
     from amfls_solver_amd import _lib

@@ -1,8 +1,9 @@
 """Study-runner parity for the dense comparative suites.
 
-Restates the reference's synthetic dense problem generator
-(experiments/runners/run_comparative_study.py:193-384), its external
-diagnostics (:577-655) and its CSV schema (:38-107), so that the reference's
+Draws the reference's synthetic dense problems
+(experiments/runners/run_comparative_study.py:193-384), measures a solution
+with the same external error definitions (:577-655) and writes the CSV
+schema (:38-107), so that the reference's
 study configurations run on the HIP engine and produce rows in the same
 layout.  The generator draws from a torch CPU generator in the reference's
 order (left basis, right basis, coefficients, then signs or noise), so an
@@ -43,7 +44,7 @@ def _orthonormal(rows: int, cols: int, gen: torch.Generator, coherence: str) -> 
         out[idx, idx] = 1.0
         return out
     if coherence != "haar":
-        raise ValueError(f"unknown coherence: {coherence}")
+        raise ValueError(f"coherence {coherence!r} is neither 'haar' nor 'coherent'")
     g = torch.randn((rows, cols), generator=gen, dtype=torch.float64)
     q, r = torch.linalg.qr(g, mode="reduced")
     d = torch.sign(torch.diagonal(r))
@@ -85,7 +86,7 @@ def _spectrum(case: dict, rank: int) -> torch.Tensor:
         v = one.clone()
         v[-1] = float(case["two_mode_small_singular_value"])
         return v
-    raise ValueError(f"unknown spectrum: {kind}")
+    raise ValueError(f"no singular-value profile is named {kind!r}")
 
 
 def make_problem(case: dict, seed: int) -> dict:
@@ -97,7 +98,7 @@ def make_problem(case: dict, seed: int) -> dict:
     row_basis, operator_norm, regularization, reference_objective."""
     m, n, rank = int(case["m"]), int(case["n"]), int(case["rank"])
     if rank <= 0 or rank > min(m, n):
-        raise ValueError(f"invalid rank {rank} for shape ({m},{n})")
+        raise ValueError(f"a {m} x {n} case cannot have rank {rank}")
     gen = torch.Generator(device="cpu").manual_seed(seed)
     coherence = str(case.get("coherence", "haar"))
     U = _orthonormal(m, rank, gen, coherence)
@@ -129,12 +130,12 @@ def make_problem(case: dict, seed: int) -> dict:
         sg[sg == 0.0] = 1.0
         c = c * sg
     elif rhs not in ("random", "consistent", "inconsistent"):
-        raise ValueError(f"unknown right-hand side: {rhs}")
+        raise ValueError(f"no right-hand-side recipe is named {rhs!r}")
     signal = U @ (sigma * c)
     b = signal.clone()
     if rhs == "inconsistent":
         if rank >= m:
-            raise ValueError("an inconsistent case requires rank < m")
+            raise ValueError(f"rank {rank} leaves no room in R^{m} for a residual component")
         if exact_inconsistent:
             noise = torch.zeros(m, dtype=torch.float64)
             noise[rank:] = torch.randn(m - rank, generator=gen, dtype=torch.float64)
@@ -143,13 +144,13 @@ def make_problem(case: dict, seed: int) -> dict:
             noise -= U @ (U.T @ noise)
         nn_ = float(torch.linalg.vector_norm(noise))
         if nn_ == 0.0:
-            raise RuntimeError("failed to construct an inconsistent component")
+            raise RuntimeError("the residual component drawn for this seed vanished")
         b += float(case.get("noise_ratio", 0.05)) * float(torch.linalg.vector_norm(signal)) * noise / nn_
     lam = ratio * float(sigma[0]) ** 2
     x = V @ ((sigma.square() / (sigma.square() + lam)) * c)
     bn = float(torch.linalg.vector_norm(b))
     if bn == 0.0:
-        raise ValueError("right-hand side cannot be zero")
+        raise ValueError("this case and seed give b = 0, which has only the trivial solution")
     if exact_inconsistent:  # exact power-of-two scaling keeps the coordinates exact
         _, e = math.frexp(bn)
         scale = math.ldexp(1.0, -e)
@@ -164,55 +165,78 @@ def make_problem(case: dict, seed: int) -> dict:
             "reference_objective": float(torch.dot(r, r) + lam * torch.dot(x, x))}
 
 
-def _ratio(num: float, den: float) -> float:
+def _quotient(num: float, den: float) -> float:
+    """num / den for error ratios: 0 for a zero numerator, +inf for a zero
+    denominator."""
     if num == 0.0:
         return 0.0
     return math.inf if den == 0.0 else num / den
 
 
+# A row passes externally when its primary error is within the tolerance up
+# to this allowance for the rounding of the check itself (the reference's
+# study uses the same allowance, so success columns compare row for row).
+def _allowance(tolerance: float) -> float:
+    return 256.0 * np.finfo(np.float64).eps + 1.0e-7 * tolerance
+
+
 def diagnostics(problem: dict, solution: torch.Tensor, tolerance: float) -> dict:
-    """External accuracy of a solution against the problem's exact answer:
-    backward errors (compatible, least squares, their minimum), relative
-    energy and solution errors, the null-space component of minimum-norm
-    problems, and external_success (the primary error, energy for lambda > 0
-    else backward, within tolerance plus a small slack)."""
-    A, b, x_ref, V = problem["matrix"], problem["b"], problem["reference"], problem["row_basis"]
-    lam, anorm = float(problem["regularization"]), float(problem["operator_norm"])
-    x = solution.to(dtype=torch.float64, device="cpu")
+    """Externally computed accuracy of `solution` on `problem`.
+
+    Everything is measured on the augmented system K = [A; sqrt(lam) I],
+    whose residual is [A x - b; sqrt(lam) x] and whose normal residual is
+    g = A^T (A x - b) + lam x:
+
+    * backward errors: compatible ||r_K|| / (||K|| ||x|| + ||b||), least
+      squares ||g|| / (||K|| ||r_K||), and the smaller of the two, with
+      ||K|| = sqrt(sigma_1^2 + lam);
+    * the relative error in the energy norm ||K e|| / ||K x_ref||, and the
+      relative 2-norm error of x;
+    * for rank-deficient unregularized cases, the share of x outside the
+      row space (minimum-norm check).
+
+    external_success uses the energy error when lam > 0 and the backward
+    error otherwise, plus the minimum-norm check where it applies."""
+    A = problem["matrix"].numpy()
+    b = problem["b"].numpy()
+    x_ref = problem["reference"].numpy()
+    V = problem["row_basis"].numpy()
+    lam = float(problem["regularization"])
+    x = solution.to(dtype=torch.float64, device="cpu").numpy()
+    root = math.sqrt(lam)
+    norm = np.linalg.norm
+
+    def augmented(v):  # K v
+        return np.concatenate([A @ v, root * v])
+
     r = A @ x - b
+    r_aug = np.concatenate([r, root * x])
     g = A.T @ r + lam * x
-    rn, gn, xn, bn = (float(torch.linalg.vector_norm(v)) for v in (r, g, x, b))
-    aug_r = math.sqrt(rn * rn + lam * xn * xn)
-    aug_a = math.sqrt(anorm * anorm + lam)
-    compatible = _ratio(aug_r, aug_a * xn + bn)
-    least_squares = _ratio(gn, aug_a * aug_r)
+    k_norm = math.hypot(float(problem["operator_norm"]), root)
+    x_norm, b_norm, r_aug_norm = norm(x), norm(b), norm(r_aug)
+    compatible = _quotient(r_aug_norm, k_norm * x_norm + b_norm)
+    least_squares = _quotient(norm(g), k_norm * r_aug_norm)
+    err = x - x_ref
+    energy = _quotient(norm(augmented(err)), norm(augmented(x_ref)))
     backward = min(compatible, least_squares)
-    e = x - x_ref
-    en, refn = float(torch.linalg.vector_norm(e)), float(torch.linalg.vector_norm(x_ref))
-    ea, refa = float(torch.linalg.vector_norm(A @ e)), float(torch.linalg.vector_norm(A @ x_ref))
-    energy = _ratio(math.sqrt(ea * ea + lam * en * en), math.sqrt(refa * refa + lam * refn * refn))
+    tol = tolerance + _allowance(tolerance)
+    full_row_space = lam > 0.0 or V.shape[1] >= A.shape[1]
+    outside = 0.0 if full_row_space else _quotient(norm(x - V @ (V.T @ x)), norm(x_ref))
+    objective = float(r_aug @ r_aug)
+    best = float(problem["reference_objective"])
     primary = energy if lam > 0.0 else backward
-    eps = float(torch.finfo(torch.float64).eps)
-    slack = 256.0 * eps + 1.0e-7 * tolerance
-    obj = float(torch.dot(r, r) + lam * torch.dot(x, x))
-    ref_obj = float(problem["reference_objective"])
-    if lam > 0.0 or V.shape[1] >= A.shape[1]:
-        null_rel, min_norm_ok = 0.0, True
-    else:
-        null_rel = _ratio(float(torch.linalg.vector_norm(x - V @ (V.T @ x))), refn)
-        min_norm_ok = null_rel <= tolerance + slack
     return {
-        "external_success": int(primary <= tolerance + slack and min_norm_ok),
-        "objective": obj,
-        "relative_objective_gap": (obj - ref_obj) / max(ref_obj, float(torch.finfo(torch.float64).tiny)),
-        "residual_norm": rn,
-        "normal_residual_norm": gn,
+        "external_success": int(primary <= tol and outside <= tol),
+        "objective": objective,
+        "relative_objective_gap": (objective - best) / max(best, np.finfo(np.float64).tiny),
+        "residual_norm": float(norm(r)),
+        "normal_residual_norm": float(norm(g)),
         "compatible_backward_error_external": compatible,
         "least_squares_backward_error_external": least_squares,
         "backward_error_external": backward,
         "relative_energy_error_external": energy,
-        "relative_solution_error": _ratio(en, refn),
-        "relative_null_component": null_rel,
+        "relative_solution_error": _quotient(norm(err), norm(x_ref)),
+        "relative_null_component": outside,
     }
 
 

@@ -186,10 +186,10 @@ def test_matrix_market_operator_solves(torch_mod, tmp_path):
     scipy.io.mmwrite(str(tmp_path / "A.mtx"), A)
     scipy.io.mmwrite(str(tmp_path / "b.mtx"), rng.standard_normal((3000, 1)))
     M = MM.load_csr(tmp_path / "A.mtx")
-    b = MM.read_rhs(tmp_path / "b.mtx", 3000)
+    b = MM.load_vector(tmp_path / "b.mtx", 3000)
     r = am.run_amfls(M, torch.from_numpy(b).cuda(), regularization=1e-2, tolerance=1e-8)
     assert r.status == 0
-    Ad = MM.read_matrix_market(tmp_path / "A.mtx").toarray()
+    Ad = MM.load_sparse(tmp_path / "A.mtx").toarray()
     K = Ad.T @ Ad + 1e-2 * np.eye(200)
     x_ref = np.linalg.solve(K, Ad.T @ b)
     d = r.solution.numpy() - x_ref

@@ -6,7 +6,8 @@ Python mirror of the reference's binding (run_amfls, run_fixed_rbgk,
 run_lsqr, SolveResult) plus the torch.ops.amfls registration, and the
 engine's extensions: regularization sweeps (run_amfls_sweep), concurrent
 batches of CSR problems (run_amfls_batch) and the SpMM autotuner
-(DeviceMatrix.autotune).
+(DeviceMatrix.autotune) and the certified validation of a given
+candidate (validate_candidate).
 """
 from .binding import (  # noqa: F401
     Context,
@@ -18,6 +19,7 @@ from .binding import (  # noqa: F401
     run_amfls_sweep,
     run_fixed_rbgk,
     run_lsqr,
+    validate_candidate,
 )
 
 from . import torch_ops as _torch_ops
@@ -25,4 +27,4 @@ from . import torch_ops as _torch_ops
 _torch_ops.register()
 
 __all__ = ["Context", "DeviceMatrix", "HostOperator", "SolveResult", "run_amfls", "run_amfls_batch", "run_amfls_sweep",
-           "run_fixed_rbgk", "run_lsqr"]
+           "run_fixed_rbgk", "run_lsqr", "validate_candidate"]

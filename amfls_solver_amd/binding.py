@@ -656,6 +656,29 @@ def run_fixed_rbgk(matrix, b, *, auxiliary_width: int, maximum_depth: int, maxim
     return _decode(raw, ext, extra)
 
 
+def validate_candidate(matrix, b, x, *, regularization: float = 0.0, tolerance: float = 1e-8, value_dtype=None,
+                       context: Context | None = None) -> SolveResult:
+    """Certify a given solution x of min ||Ax - b||^2 + lambda ||x||^2 with
+    the engine's FP64 validator (reference candidate_validation.cpp:1176-1378
+    validate_original_candidate): the result carries x, its residual and
+    gradient norms, the certified backward-error / energy bounds and the
+    status the certificate implies at `tolerance`."""
+    if not isinstance(matrix, DeviceMatrix):
+        _check_inputs(matrix, b)
+    op = _operator(matrix, value_dtype, context)
+    ctx = op.context
+    bd = _rhs(b, op._val.device)
+    xd = _rhs(x, op._val.device)
+    if xd.numel() != op.cols:
+        raise ValueError(f"the candidate has {xd.numel()} entries, the operator {op.cols} columns")
+    res = ctypes.c_void_p()
+    torch.cuda.synchronize()
+    _lib.check(ctx._lib.amfls_validate_candidate(ctx.handle, op.handle, bd.data_ptr(), 1, xd.data_ptr(), 1,
+                                                 float(regularization), float(tolerance), ctypes.byref(res)))
+    raw, ext, extra = _result_tuple(ctx, res)
+    return _decode(raw, ext, extra)
+
+
 def run_lsqr(matrix, b, *, regularization: float = 0.0, tolerance: float = 1e-8, maximum_iterations: int = 0,
              clock: str = "wall", value_dtype=None, context: Context | None = None) -> SolveResult:
     """reference binding.py run_lsqr / lsqr.cpp:84-279 on the HIP engine."""

@@ -10,6 +10,7 @@
 
 #include "../../../include/amfls_hip.h"
 #include "solver.hpp"
+#include "validation.hpp"
 #include "width_model.hpp"
 
 using namespace amflsx;
@@ -503,6 +504,30 @@ int amfls_solve_lsqr(amfls_context* ctx, const amfls_operator* op, const double*
         RhsHolder rhs(ctx->ctx, b, op->op->rows(), b_on_device != 0);
         auto res = std::make_unique<amfls_result>();
         res->r = solve_lsqr(*op->op, rhs.ptr, o, ctx->ctx);
+        *out = res.release();
+    });
+}
+
+int amfls_validate_candidate(amfls_context* ctx, const amfls_operator* op, const double* b, int b_on_device,
+                             const double* x, int x_on_device, double regularization, double tolerance,
+                             amfls_result** out) {
+    return guard([&] {
+        require(ctx && op && out, "null argument");
+        require(std::isfinite(regularization) && regularization >= 0.0,
+                "regularization must be finite and nonnegative");
+        require(std::isfinite(tolerance) && tolerance > 0.0, "tolerance must be finite and positive");
+        const int64_t n = op->op->cols();
+        RhsHolder rhs(ctx->ctx, b, op->op->rows(), b_on_device != 0);
+        RhsHolder cand(ctx->ctx, x, n, x_on_device != 0);
+        auto res = std::make_unique<amfls_result>();
+        RunStatistics stats;
+        ValidationEngine ve(*op->op, rhs.ptr, ctx->ctx, stats);
+        res->r = ve.validate(cand.ptr, regularization, tolerance);
+        res->r.solution.resize(static_cast<size_t>(n));
+        AMFLS_HIP_CHECK(hipMemcpyAsync(res->r.solution.data(), cand.ptr, n * sizeof(double), hipMemcpyDeviceToHost,
+                                       ctx->ctx.stream));
+        ctx->ctx.sync();
+        res->r.statistics = stats;
         *out = res.release();
     });
 }

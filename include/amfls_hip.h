@@ -221,6 +221,18 @@ int amfls_solve_lsqr(amfls_context* ctx, const amfls_operator* op, const double*
                      double regularization, double tolerance, int32_t maximum_iterations, int32_t clock_mode,
                      amfls_result** out);
 
+/* Certified validation of a given candidate x (length cols) for
+ * min ||Ax - b||^2 + lambda ||x||^2: fresh FP64 products, outward-rounded
+ * norm intervals and rounding radii, the backward-error / ridge-energy
+ * bounds, the long-double refinement branch and the one-step ridge
+ * correction; the result carries x, the certificate metrics and the
+ * validation counters (no trace rows).  Replaces the reference's
+ * validate_original_candidate
+ * (src/algorithms/mathematics/krylov/candidate_validation.cpp:1176-1378),
+ * which the reference runs on every candidate of a solve. */
+int amfls_validate_candidate(amfls_context* ctx, const amfls_operator* op, const double* b, int b_on_device,
+                             const double* x, int x_on_device, double regularization, double tolerance,
+                             amfls_result** out);
 int amfls_result_sizes(const amfls_result* result, int64_t* cols, int64_t* trace_rows);
 /* Host buffers: solution[cols], metrics[21], counters[36], timings[9],
  * trace_float[trace_rows * 29], trace_int[trace_rows * 4] (row-major). */

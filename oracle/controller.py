@@ -9,7 +9,10 @@ MatchedHorizonPlan helpers).  long double arithmetic uses numpy.longdouble.
 Pinned by tests/test_oracle_controller.py: brute-force enumeration of the
 dyadic increment and feasibility helpers over small integer grids against
 their defining properties, hand-computed cost-gate decisions, and the
-history shift/pool invariants.
+history shift/pool invariants; and by tests/test_oracle_pins.py: hand-worked
+tables for record_deepen_progress, make_initial_width_candidate (scalar
+slowdown, costlier target, block stall, guards) and the MatchedHorizonPlan
+life cycle.
 """
 from __future__ import annotations
 

@@ -17,7 +17,12 @@ error computed independently (numpy 2-norm, closed-form ridge solution) on
 seeded problems, and an exact solution of a diagonal system certifies the
 compatible backward error below 1e-14 with a zero normal residual
 (test_validation_bounds_enclose_true_backward_error,
-test_ridge_energy_bound_encloses_truth).
+test_ridge_energy_bound_encloses_truth); by tests/test_oracle_pins.py: the
+callback and one-operation radii against exact rationals, the dense and CSR
+operator models on hand-computed matrices, a case that forces the
+long-double refinement (exact-rational enclosure of the normal residual)
+and one that forces the one-step ridge correction (single-mode error, the
+corrected bound equal to the exact energy error up to the radii).
 """
 from __future__ import annotations
 

@@ -92,6 +92,8 @@ SIGNATURES = {
                                  POINTER(c_void_p)]),
     "amfls_validate_candidate": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int, c_double, c_double,
                                          POINTER(c_void_p)]),
+    "amfls_block_rank_reveal": (c_int, [c_void_p, c_int64, c_int32, c_void_p, c_int64, c_int, c_int32, c_double,
+                                        POINTER(c_int32), POINTER(c_double), POINTER(c_double), c_void_p]),
     "amfls_result_sizes": (c_int, [c_void_p, POINTER(c_int64), POINTER(c_int64)]),
     "amfls_result_copy": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                   POINTER(c_int64), POINTER(c_int64)]),

@@ -1,6 +1,7 @@
 // extern "C" boundary of the AMFLS HIP engine (declared in include/amfls_hip.h).
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 #include <exception>
 #include <memory>
@@ -11,6 +12,7 @@
 #include "../../../include/amfls_hip.h"
 #include "solver.hpp"
 #include "validation.hpp"
+#include "workspace.hpp"
 #include "width_model.hpp"
 
 using namespace amflsx;
@@ -816,6 +818,36 @@ int amfls_block_rowmul(amfls_context* ctx, int64_t rows, int32_t w, int32_t r, c
         require(rows >= 0 && w >= 1 && r >= 1 && ldx >= w && ldy >= r, "invalid block shape");
         rowmul<double>(ctx->ctx.stream, rows, w, r, x, ldx, m, y, ldy);
         ctx->ctx.sync();
+    });
+}
+
+int amfls_block_rank_reveal(amfls_context* ctx, int64_t rows, int32_t w, void* x, int64_t ldx, int dtype,
+                            int32_t max_rank, double fp32_rank_factor, int32_t* rank, double* cutoff,
+                            double* discarded, double* singular_values) {
+    return guard([&] {
+        require(ctx && x && rank && cutoff && discarded && singular_values, "null argument");
+        require(rows >= 1 && w >= 1 && w <= 256 && ldx >= w && max_rank >= 0, "invalid block shape");
+        require(dtype == AMFLS_F64 || dtype == AMFLS_F32, "rank reveal takes FP64 or FP32 blocks");
+        // the operator only supplies the block's shape to the workspace
+        DenseOperator op(rows, w, x, to_vt(dtype), true, ldx, ctx->ctx);
+        DevBuf<double> b(static_cast<size_t>(rows));
+        fill<double>(ctx->ctx.stream, b.get(), rows, 0.0);
+        RunStatistics stats;
+        ValidationEngine ve(op, b.get(), ctx->ctx, stats);
+        std::vector<double> sv;
+        auto run = [&](auto* block) {
+            using T = std::remove_pointer_t<decltype(block)>;
+            Workspace<T> ws(op, b.get(), 0.0, 0, stats, ctx->ctx, ve);
+            if constexpr (sizeof(T) == 4) ws.set_fp32_rank_factor(fp32_rank_factor);
+            const auto rr = ws.rank_reveal_block(block, ldx, w, max_rank);
+            *rank = rr.rank;
+            *cutoff = rr.cutoff;
+            *discarded = rr.discarded;
+            sv = rr.singular_values;
+        };
+        if (dtype == AMFLS_F64) run(static_cast<double*>(x));
+        else run(static_cast<float*>(x));
+        for (int i = 0; i < w; ++i) singular_values[i] = i < static_cast<int>(sv.size()) ? sv[i] : 0.0;
     });
 }
 

@@ -240,7 +240,13 @@ typename Workspace<T>::Factor Workspace<T>::orthogonalize_and_factor(T* W, int64
     // the identity to working precision (a diagonal shift keeps the first
     // Cholesky defined for numerically rank-deficient blocks), then a host
     // SVD of the small R gives the singular values of W.
-    const double u = unit_roundoff<T>();
+    // u: the storage precision of W, which sets the orthogonality target.
+    // The Gram matrix is accumulated in FP64 from exact products of the
+    // stored entries and the Cholesky runs in FP64 for both block types, so
+    // when the Cholesky needs a shift, and how large it must be, is an FP64
+    // question (ug); an FP32-sized shift would keep FP32 blocks of moderate
+    // condition from converging in the allotted passes.
+    const double u = unit_roundoff<T>(), ug = unit_roundoff<double>();
     HostMat Rtot(w, w);
     for (int i = 0; i < w; ++i) Rtot(i, i) = 1.0;
     std::vector<double> G(static_cast<size_t>(w) * w);
@@ -264,10 +270,10 @@ typename Workspace<T>::Factor Workspace<T>::orthogonalize_and_factor(T* W, int64
         HostMat R;
         double ratio = 0.0;
         bool ok = cholesky_upper(Gm, R, &ratio);
-        if (!ok || ratio < 64.0 * std::sqrt(u * static_cast<double>(w))) {
-            const double shift = 11.0 * (static_cast<double>(w) * (w + 1) * u +
-                                         (static_cast<double>(N) * w + static_cast<double>(w) * (w + 1)) *
-                                             unit_roundoff<double>()) * trace;
+        if (!ok || ratio < 64.0 * std::sqrt(ug * static_cast<double>(w))) {
+            const double shift = 11.0 * (static_cast<double>(w) * (w + 1) * ug +
+                                         (static_cast<double>(N) * w + static_cast<double>(w) * (w + 1)) * ug) *
+                                 trace;
             HostMat Gs = Gm;
             for (int i = 0; i < w; ++i) Gs(i, i) += std::max(shift, std::numeric_limits<double>::min());
             ok = cholesky_upper(Gs, R, &ratio);
@@ -325,6 +331,19 @@ typename Workspace<T>::Factor Workspace<T>::orthogonalize_and_factor(T* W, int64
     }
     done();
     return f;
+}
+
+template <typename T>
+typename Workspace<T>::RankReveal Workspace<T>::rank_reveal_block(T* W, int64_t ldw, int w, int max_rank) {
+    if (w <= 0 || ldw < w) throw std::invalid_argument("rank reveal needs a block of at least one column");
+    ctx_.block_stats_async<T>(m_, w, W, ldw, slots_.get() + kScaleSlot);
+    Factor f = orthogonalize_and_factor(W, ldw, m_, w, nullptr, 0, 0, max_rank, 0, false);
+    RankReveal r;
+    r.rank = f.rank;
+    r.cutoff = f.cutoff;
+    r.discarded = f.discarded;
+    if (f.rank > 0) r.singular_values = jacobi_svd(f.factor).s;
+    return r;
 }
 
 template <typename T>

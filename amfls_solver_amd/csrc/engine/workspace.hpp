@@ -96,6 +96,17 @@ public:
     const T* basis() const { return V_.ptr(); }
     int64_t basis_ld() const { return V_.ld(); }
 
+    // Rank reveal of one block W (m x w, row-major, leading dimension ldw)
+    // with no basis to orthogonalize against: the factorization every new
+    // search block goes through (the cutoff scale is W's Frobenius norm).
+    // W is overwritten with its orthonormal factor.
+    struct RankReveal {
+        int rank = 0;
+        double cutoff = 0.0, discarded = 0.0;
+        std::vector<double> singular_values;  // of the retained factor, decreasing
+    };
+    RankReveal rank_reveal_block(T* W, int64_t ldw, int w, int max_rank);
+
 private:
     struct Factor {
         std::vector<double> coefficients;  // basis_cols x w (row-major), when retained

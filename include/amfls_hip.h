@@ -318,6 +318,20 @@ int amfls_block_gemm_tn(amfls_context* ctx, int64_t m, int32_t k, int32_t n, con
 int amfls_block_rowmul(amfls_context* ctx, int64_t rows, int32_t w, int32_t r, const double* x, int64_t ldx,
                        const double* m, double* y, int64_t ldy);
 int amfls_block_stats(amfls_context* ctx, int64_t rows, int32_t cols, const double* x, int64_t ldx, double* out);
+/* Rank reveal of one device block W (rows x w, row-major, FP64 or FP32) as
+ * the search factors every new block: CholeskyQR passes, singular values
+ * of the triangular factor, numerical rank at the cutoff (the reference's
+ * rank_revealing_cutoff scaled by ||W||_F; fp32_rank_factor > 0 replaces
+ * the FP32 rule's factor).  W is overwritten with its orthonormal factor.
+ * Outputs: rank, cutoff, the norm of the discarded singular values, and the
+ * w retained singular values in decreasing order (zeros past the rank).
+ * Replaces the SVD step of the reference's orthogonalize_and_factor
+ * (src/algorithms/mathematics/krylov/krylov_workspace.cpp:263-309); for
+ * tests and diagnostics. */
+int amfls_block_rank_reveal(amfls_context* ctx, int64_t rows, int32_t w, void* x, int64_t ldx, int dtype,
+                            int32_t max_rank, double fp32_rank_factor, int32_t* rank, double* cutoff,
+                            double* discarded, double* singular_values);
+
 /* The same products on FP32 blocks (the mixed-precision search's local
  * work): A, B, X, Y FP32; C and M FP64; block_gemm_tn_f32 accumulates in
  * FP64 and writes FP64 C. */

@@ -9,6 +9,10 @@
   1e-8 against the FP64 oracle solution in tests/golden/c2_solution.npz
   (TSQR of the stored matrix, computed by tests/golden/make_c2_fixture.py);
   the matrix built on the GPU is first shown to be the fixture's bit for bit.
+* 200000 x 20000 CSR (lambda 1e-3): trajectory parity with the oracle and
+  mutually consistent certificates (FP64 and mixed).
+* 256 CSR problems x 8 lambdas: every solve certified or explained, true
+  energy errors within the certificates on a sample.
 """
 import os
 
@@ -120,3 +124,126 @@ def test_config2_forward_error(torch_mod, config2, precision):
         assert r.forward_error_estimate <= 1e-9
         assert fe <= 10 * r.forward_error_estimate + 1e-12
         assert 0 < r.refinement_steps <= 64
+
+
+def test_config1_csr_200000x20000(torch_mod):
+    """C1 at full size (200000 x 20000, 12 distinct columns per row, float32
+    values, lambda 1e-3, tolerance 1e-8): on the synthetic clock the FP64
+    engine follows the oracle's trajectory (depth, basis rank, width,
+    status) and both certificates hold against each other in the ridge
+    energy norm; the mixed engine certifies the same tolerance."""
+    import scipy.sparse as sp
+    import amfls_solver_amd as am
+    from amfls_solver_amd import problems as P
+    from oracle import clock as CL, operators as O, solvers as S
+    torch = torch_mod
+    p = P.config1()
+    lam, tol = p.regularization, 1e-8
+    A = torch.sparse_csr_tensor(torch.from_numpy(p.row_ptr), torch.from_numpy(p.col_idx.astype(np.int64)),
+                                torch.from_numpy(p.values), size=(p.rows, p.cols))
+    M = am.DeviceMatrix(A, value_dtype=torch.float32)
+    b = torch.from_numpy(p.b)
+    op = O.CsrOperator(p.rows, p.cols, p.row_ptr, p.col_idx, p.values.astype(np.float64))
+    ro = S.solve_amfls(op, p.b, regularization=lam, tolerance=tol, clock=CL.SyntheticClock())
+    r = am.run_amfls(M, b, regularization=lam, tolerance=tol, clock="synthetic")
+    assert r.status == ro.status == 0
+    assert (r.depth, r.basis_rank, r.auxiliary_width) == (ro.depth, ro.basis_rank, ro.auxiliary_width)
+    As = sp.csr_matrix((p.values.astype(np.float64), p.col_idx, p.row_ptr), shape=(p.rows, p.cols))
+
+    def k_norm(v):
+        return float(np.hypot(np.linalg.norm(As @ v), np.sqrt(lam) * np.linalg.norm(v)))
+
+    xo = ro.solution
+    for res in (r, am.run_amfls(M, b, regularization=lam, tolerance=tol, precision="mixed")):
+        assert res.status == 0 and res.relative_energy_error_upper_bound <= tol
+        gap = k_norm(res.solution.numpy() - xo) / k_norm(xo)
+        assert gap <= 1.0001 * (res.relative_energy_error_upper_bound + ro.relative_energy_error_upper_bound), gap
+
+
+def test_config3_batch_256x8(torch_mod):
+    """C3 at full size: 256 CSR problems (m ~ U[5000, 200000],
+    n ~ U[500, min(10000, m)], 4..32 entries per row, float32 values) with
+    eight lambdas logspace(-6, -1) each, solved concurrently.
+
+    * Every solve ends certified (status 0) or with the basis exhausted
+      (status 2, basis rank min(m, n)); the exhausted ones are, for every
+      problem, its smallest lambdas.  There the certificate has a floor:
+      the validator's rounding radii (gamma_k ||A||_upper ||r|| with k the
+      column count, or the dot length in the long-double branch) divided by
+      sqrt(lambda) exceed the tolerance even for the converged candidate,
+      so the search runs to the basis limit as the reference's controller
+      does.  For the worst one, the oracle's validator on the engine's
+      solution takes the same branches (long-double refinement, one-step
+      ridge correction) and returns the same certificate, above the
+      tolerance, while the candidate's normal residual is at rounding level.
+    * For problems with n <= 1500 the certified results' true relative
+      energy errors, against a dense FP64 ridge solve, are within their
+      certificates."""
+    import scipy.sparse as sp
+    import amfls_solver_amd as am
+    from amfls_solver_amd.binding import gaussian_block, generate_csr
+    from oracle import clock as CL, operators as O, solvers as S
+    torch = torch_mod
+    tol = 1e-8
+    rng = np.random.default_rng(3)
+    dims = []
+    for _ in range(256):
+        m = int(rng.integers(5000, 200_001))
+        dims.append((m, int(rng.integers(500, min(10_000, m) + 1))))
+    mats, bs, arrays = [], [], []
+    for i, (m, n) in enumerate(dims):
+        rp, ci, val = generate_csr(m, n, (4, 32), seed=1000 + i, value_dtype=torch.float32)
+        mats.append(am.DeviceMatrix.from_csr_arrays(m, n, rp, ci, val))
+        bs.append(gaussian_block(m, 1, 2000 + i, 0).reshape(-1))
+        arrays.append((rp, ci, val))
+    lams = np.tile(np.logspace(-6.0, -1.0, 8), (256, 1))
+    res = am.run_amfls_batch(mats, bs, lams, workers=6, tolerance=tol)
+    exhausted = []
+    for i, row in enumerate(res):
+        m, n = dims[i]
+        for j, r in enumerate(row):
+            assert r.status in (0, 2), (i, j, r.status)
+            if r.status == 0:
+                assert r.relative_energy_error_upper_bound <= tol
+            else:
+                assert r.basis_rank == min(m, n), (i, j, r.basis_rank)
+                exhausted.append((i, j))
+        # the floor falls like 1/sqrt(lambda): a problem's exhausted lambdas are its smallest ones
+        statuses = [r.status for r in row]
+        k = next((jj for jj, st in enumerate(statuses) if st == 0), 8)
+        assert all(st == 2 for st in statuses[:k]) and all(st == 0 for st in statuses[k:]), (i, statuses)
+    certified = sum(1 for row in res for r in row if r.status == 0)
+    by_lambda = np.bincount([j for _, j in exhausted], minlength=8)
+    print(f"C3: {certified} of 2048 certified; exhausted per lambda index {by_lambda.tolist()}")
+    assert certified >= 0.9 * 2048
+    checked = 0
+    for i, (m, n) in enumerate(dims):
+        if n > 1500 or checked >= 4:
+            continue
+        rp, ci, val = (t.cpu().numpy() for t in arrays[i])
+        A = sp.csr_matrix((val.astype(np.float64), ci, rp), shape=(m, n)).toarray()
+        bh = bs[i].cpu().numpy()
+        AtA, Atb = A.T @ A, A.T @ bh
+        for lam, r in zip(lams[i], res[i]):
+            if r.status != 0:
+                continue
+            K = AtA + lam * np.eye(n)
+            xs = np.linalg.solve(K, Atb)
+            d = r.solution.numpy() - xs
+            assert np.sqrt(d @ K @ d) / np.sqrt(xs @ K @ xs) <= r.relative_energy_error_upper_bound * 1.0001
+        checked += 1
+    assert checked >= 1
+    if exhausted:
+        from oracle import results as R, validation as V
+        i, j = max(exhausted, key=lambda e: res[e[0]][e[1]].relative_energy_error_upper_bound)
+        m, n = dims[i]
+        r = res[i][j]
+        rp, ci, val = (t.cpu().numpy() for t in arrays[i])
+        op = O.CsrOperator(m, n, rp.astype(np.int64), ci.astype(np.int64), val.astype(np.float64))
+        c = V.Counted(op, R.RunStatistics(), CL.SyntheticClock())
+        o = V.validate_original_candidate(c, bs[i].cpu().numpy(), r.solution.numpy(), float(lams[i][j]), tol)
+        assert o.status != 0 and c.stats.base_validation_a_columns == 2        # refinement branch taken
+        assert o.ridge_correction_disposition == r.ridge_correction_disposition
+        assert abs(o.relative_energy_error_upper_bound / r.relative_energy_error_upper_bound - 1) < 1e-6
+        assert r.relative_energy_error_upper_bound > tol
+        assert o.least_squares_backward_error_upper_bound < 1e-10             # converged to rounding level

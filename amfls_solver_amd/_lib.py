@@ -65,6 +65,8 @@ SIGNATURES = {
                                    POINTER(c_void_p)]),
     "amfls_host_operator_create": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_double,
                                            c_double, c_int64, c_int64, POINTER(c_void_p)]),
+    "amfls_host_operator_create_ex": (c_int, [c_void_p, c_int64, c_int64, c_void_p, c_double, c_double, c_int64,
+                                              c_int64, POINTER(c_void_p)]),
     "amfls_operator_destroy": (c_int, [c_void_p]),
     "amfls_operator_shape": (c_int, [c_void_p, POINTER(c_int64), POINTER(c_int64)]),
     "amfls_operator_validation_model": (c_int, [c_void_p, POINTER(c_double), POINTER(c_double),

@@ -215,6 +215,26 @@ int amfls_host_operator_create(amfls_context* ctx, int64_t rows, int64_t cols, a
     });
 }
 
+int amfls_host_operator_create_ex(amfls_context* ctx, int64_t rows, int64_t cols, const amfls_host_callbacks* cb,
+                                  double norm_upper, double norm_lower, int64_t steps_apply, int64_t steps_transpose,
+                                  amfls_operator** out) {
+    return guard([&] {
+        require(ctx != nullptr && out != nullptr && cb != nullptr, "null context, callbacks or output");
+        ValidationModel m;
+        m.upper = norm_upper;
+        m.lower = norm_lower;
+        m.steps_apply = steps_apply;
+        m.steps_transpose = steps_transpose;
+        HostHooks hooks;
+        hooks.cost = cb->relative_block_product_cost;
+        hooks.refine = cb->apply_refinement;
+        hooks.refine_t = cb->apply_transpose_refinement;
+        auto h = std::make_unique<amfls_operator>();
+        h->op = std::make_unique<HostOperator>(rows, cols, cb->apply, cb->apply_transpose, cb->user, m, hooks);
+        *out = h.release();
+    });
+}
+
 int amfls_operator_destroy(amfls_operator* op) {
     return guard([&] { delete op; });
 }

@@ -585,8 +585,8 @@ void AugmentedOperator::apply_t(hipStream_t s, const float* Y, int n, int64_t ld
 namespace amflsx {
 
 HostOperator::HostOperator(int64_t rows, int64_t cols, HostApplyFn apply, HostApplyFn apply_transpose, void* user,
-                           ValidationModel model)
-    : apply_(apply), apply_t_(apply_transpose), user_(user), model_(model) {
+                           ValidationModel model, HostHooks hooks)
+    : apply_(apply), apply_t_(apply_transpose), user_(user), model_(model), hooks_(hooks) {
     if (rows <= 0 || cols <= 0) throw std::invalid_argument("operator dimensions must be positive");
     if (apply == nullptr || apply_transpose == nullptr) throw std::invalid_argument("operator callbacks are missing");
     if (std::isnan(model.upper) || std::isnan(model.lower) || model.lower < 0.0 || model.steps_apply < 0 ||

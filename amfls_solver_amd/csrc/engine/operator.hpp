@@ -226,11 +226,31 @@ private:
 // nonzero return is an error.  FP32 blocks are widened to FP64 for the
 // callback and rounded back.
 using HostApplyFn = int (*)(void* user, const double* in, int block_cols, double* out);
+// Optional hooks of a caller's operator (reference matrix_operator.hpp:50-88):
+// the scheduler's relative block-product cost (absent: s) and the
+// long-double validation refinement products y = A x, x = A^T y (absent or
+// either missing: no refinement), each returning 0 on success.
+using HostCostFn = double (*)(void* user, int block_cols);
+using HostRefineFn = int (*)(void* user, const long double* in, long double* out);
+struct HostHooks {
+    HostCostFn cost = nullptr;
+    HostRefineFn refine = nullptr, refine_t = nullptr;
+};
 
 class HostOperator final : public DeviceOperator {
 public:
     HostOperator(int64_t rows, int64_t cols, HostApplyFn apply, HostApplyFn apply_transpose, void* user,
-                 ValidationModel model);
+                 ValidationModel model, HostHooks hooks = {});
+    double relative_block_product_cost(int s) const override {
+        return hooks_.cost ? hooks_.cost(user_, s) : DeviceOperator::relative_block_product_cost(s);
+    }
+    bool supports_refinement() const override { return hooks_.refine != nullptr && hooks_.refine_t != nullptr; }
+    bool refine_apply(const long double* x, long double* y) const override {
+        return supports_refinement() && hooks_.refine(user_, x, y) == 0;
+    }
+    bool refine_apply_t(const long double* y, long double* x) const override {
+        return supports_refinement() && hooks_.refine_t(user_, y, x) == 0;
+    }
     void apply(hipStream_t s, const double* X, int n, int64_t ldx, double* Y, int64_t ldy) const override;
     void apply_t(hipStream_t s, const double* Y, int n, int64_t ldy, double* X, int64_t ldx) const override;
     void apply(hipStream_t s, const float* X, int n, int64_t ldx, float* Y, int64_t ldy) const override;
@@ -250,6 +270,7 @@ private:
     HostApplyFn apply_, apply_t_;
     void* user_;
     ValidationModel model_;
+    HostHooks hooks_;
     mutable PinnedBuf<char> hin_, hout_;
     mutable std::vector<double> cin_, cout_;
 };

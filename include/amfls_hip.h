@@ -135,6 +135,25 @@ int amfls_host_operator_create(amfls_context* ctx, int64_t rows, int64_t cols, a
                                amfls_host_apply_fn apply_transpose, void* user, double norm_upper,
                                double norm_lower, int64_t steps_apply, int64_t steps_transpose,
                                amfls_operator** out);
+/* The full adapter for amfls::MatrixOperator subclasses
+ * (matrix_operator.hpp:29-89): the two products as above, plus
+ * relative_block_product_cost(user, s) for the width scheduler (NULL: s,
+ * the reference default, matrix_operator.hpp:50-55) and the long-double
+ * validation refinement products (matrix_operator.hpp:70-88; both NULL: the
+ * operator does not support refinement), which return 0 on success. */
+typedef double (*amfls_host_cost_fn)(void* user, int block_cols);
+typedef int (*amfls_host_refine_fn)(void* user, const long double* in, long double* out);
+typedef struct amfls_host_callbacks {
+    amfls_host_apply_fn apply;
+    amfls_host_apply_fn apply_transpose;
+    amfls_host_cost_fn relative_block_product_cost;
+    amfls_host_refine_fn apply_refinement;
+    amfls_host_refine_fn apply_transpose_refinement;
+    void* user;
+} amfls_host_callbacks;
+int amfls_host_operator_create_ex(amfls_context* ctx, int64_t rows, int64_t cols, const amfls_host_callbacks* cb,
+                                  double norm_upper, double norm_lower, int64_t steps_apply, int64_t steps_transpose,
+                                  amfls_operator** out);
 int amfls_operator_destroy(amfls_operator* op);
 int amfls_operator_shape(const amfls_operator* op, int64_t* rows, int64_t* cols);
 int amfls_operator_validation_model(const amfls_operator* op, double* norm_upper, double* norm_lower,

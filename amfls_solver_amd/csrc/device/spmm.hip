@@ -23,18 +23,18 @@ namespace {
 constexpr int kThreads = 256;
 
 // In-kernel product timing (ProductStamp): block 0 of the first launch
-// stamps the start, each of the last eight blocks of the final launch its
-// end once all its threads are done.
+// stamps the start; every block of the final launch, once all its threads
+// are done, raises the end slot to its own end time with an atomic maximum,
+// so the end is the last block to finish whatever the row lengths (the
+// clock only grows, so a value left by an earlier product is smaller).
 __device__ __forceinline__ void stamp_begin(ProductStamp st) {
     if (st.slots != nullptr && st.first && blockIdx.x == 0 && threadIdx.x == 0) st.slots[0] = wall_clock64();
 }
 
 __device__ __forceinline__ void stamp_end(ProductStamp st) {
-    if (st.slots == nullptr || !st.last) return;
-    const unsigned g = gridDim.x, b = blockIdx.x;
-    if (b + 8 < g) return;  // uniform per block: the barrier below sees every thread or none
+    if (st.slots == nullptr || !st.last) return;  // uniform: the barrier sees every thread or none
     __syncthreads();
-    if (threadIdx.x == 0) st.slots[1 + (g - 1 - b)] = wall_clock64();
+    if (threadIdx.x == 0) atomicMax(st.slots + 1, static_cast<unsigned long long>(wall_clock64()));
 }
 
 template <typename V, typename T, int P, int LC, int CPL>

@@ -12,6 +12,7 @@
 
 #include <map>
 #include <memory>
+#include <atomic>
 #include <mutex>
 #include <vector>
 
@@ -73,6 +74,7 @@ public:
     // same thread), so workers that share one operator cannot swap stamps.
     virtual bool times_own_kernels() const { return false; }
     void stamp_next(unsigned long long* slots) const { stamp_ = slots; }
+    void clear_stamp() const { stamp_ = nullptr; }
 
     // Kernel timing with HIP events recorded on the launching stream around
     // every product (off by default).  Stats: launches, seconds, algorithmic
@@ -99,13 +101,22 @@ private:
     static inline thread_local unsigned long long* stamp_ = nullptr;
     struct ProfPair { hipEvent_t start, stop; double bytes; };
     mutable std::mutex prof_mu_;  // products of one operator may run on several threads
-    bool profiling_ = false;
+    std::atomic<bool> profiling_{false};
     mutable std::vector<ProfPair> pending_;
     mutable std::vector<hipEvent_t> pool_;
     mutable hipEvent_t open_start_ = nullptr;
     long long launches_ = 0;
     double seconds_ = 0.0, bytes_ = 0.0;
     void resolve_pending();
+};
+
+// Ends a stamp hand-off with the product that was to consume it: the
+// destructor clears the thread's pending stamp whether the product returned
+// or threw, so no stamp outlives its product (its slots may belong to a
+// timer that is gone by the next product on this thread).
+struct StampScope {
+    const DeviceOperator& op;
+    ~StampScope() { op.clear_stamp(); }
 };
 
 // Non-owning CSR (int64 row pointers, int32 column indices, fp32/fp64

@@ -309,6 +309,7 @@ void ValidationEngine::apply_counted(const double* in, double* out, bool transpo
         // the product's kernels stamp their own time, added to the operator
         // and validation seconds when the timer is flushed: no waits here
         RunStatistics& st = stats_;
+        const StampScope stamp_scope{op_};
         op_.stamp_next(ctx_.timer.kernel_stamp(ctx_.stream, transpose ? &st.at_seconds : &st.a_seconds,
                                                base ? &st.base_validation_seconds
                                                     : &st.ridge_correction_validation_seconds));

@@ -111,6 +111,7 @@ void Workspace<T>::apply_search(const T* X, int s, int64_t ldx, T* Y, int64_t ld
     clk.charge_operator(op_.relative_block_product_cost(s));
     if (!wall) *seconds += clk.now() - charged_from;
     const bool events = wall && !time_with_stamp(seconds);
+    const StampScope stamp_scope{op_};
     if (events) ctx_.timer.begin(ctx_.stream);
     if (transpose) op_.apply_t(ctx_.stream, X, s, ldx, Y, ldy);
     else op_.apply(ctx_.stream, X, s, ldx, Y, ldy);
@@ -550,6 +551,7 @@ void Workspace<T>::enqueue_product(const T* X, int64_t ldx, T* Y, int64_t ldy, b
     // Counted and (on the synthetic clock) charged when the level commits.
     double* seconds = transpose ? &stats_.at_seconds : &stats_.a_seconds;
     const bool events = !ctx_.clock.synthetic() && !time_with_stamp(seconds);
+    const StampScope stamp_scope{op_};
     if (events) ctx_.timer.begin(ctx_.stream);
     if (transpose) op_.apply_t(ctx_.stream, X, 1, ldx, Y, ldy);
     else op_.apply(ctx_.stream, X, 1, ldx, Y, ldy);

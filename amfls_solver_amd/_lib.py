@@ -48,6 +48,8 @@ class Options(ctypes.Structure):
         ("width_policy", c_int32),
         ("forward_tolerance", c_double),
         ("maximum_refinement_iterations", c_int32),
+        ("width_script", c_void_p),
+        ("width_script_length", c_int32),
     ]
 
 

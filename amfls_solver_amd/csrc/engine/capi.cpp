@@ -154,6 +154,8 @@ void amfls_options_default(amfls_options* o) {
     o->width_policy = AMFLS_WIDTH_HEURISTIC;
     o->forward_tolerance = 0.0;
     o->maximum_refinement_iterations = 0;
+    o->width_script = nullptr;
+    o->width_script_length = 0;
 }
 
 int amfls_context_create(amfls_context** out) {
@@ -402,6 +404,23 @@ SolveOptions solve_options(const amfls_options* options) {
 // Widening predicate of a width policy: the reference heuristic (empty),
 // the learned model, constant choices, or the heuristic / always-widen with
 // every decision's model inputs recorded into `records`.
+// The policy of `base` with its first decisions replaced by a script (0
+// hold, 1 widen, other: the base policy).  The base policy still sees every
+// decision, so a recording base records the scripted ones too.
+WidthPolicy scripted_policy(WidthPolicy base, const int8_t* script, int32_t length) {
+    require(length >= 0 && (length == 0 || script != nullptr), "invalid width script");
+    if (length == 0) return base;
+    auto steps = std::make_shared<std::vector<int8_t>>(script, script + length);
+    auto next = std::make_shared<size_t>(0);
+    return [base, steps, next](const WidthFeatures& f) {
+        const size_t i = (*next)++;
+        const bool chosen = base ? base(f) : f.heuristic;
+        if (i < steps->size() && ((*steps)[i] == 0 || (*steps)[i] == 1))
+            return (*steps)[i] == 1 && f.active_width + f.proposed_increment <= 64;
+        return chosen;
+    };
+}
+
 WidthPolicy width_policy(const amfls_context* ctx, int code, std::vector<float>* records) {
     if (code == AMFLS_WIDTH_LEARNED) {
         require(ctx->width_model != nullptr, "the learned width policy needs a model set on the context");
@@ -436,7 +455,8 @@ int amfls_solve(amfls_context* ctx, const amfls_operator* op, const double* b, i
         const SolveOptions o = solve_options(options);
         RhsHolder rhs(ctx->ctx, b, op->op->rows(), b_on_device != 0);
         auto res = std::make_unique<amfls_result>();
-        const WidthPolicy policy = width_policy(ctx, o.width_policy, &res->width_records);
+        const WidthPolicy policy = scripted_policy(width_policy(ctx, o.width_policy, &res->width_records),
+                                                   options->width_script, options->width_script_length);
         res->r = solve_amfls(*op->op, rhs.ptr, o, ctx->ctx, policy ? &policy : nullptr);
         *out = res.release();
     });

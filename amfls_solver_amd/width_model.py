@@ -7,20 +7,28 @@ are what AMFLS records about itself at each decision: the active width, the
 last four log10 certificate decreases, the last four per-level interval
 times, and log10 of the operator's rows, columns and nonzeros.
 
-Labels.  Each training problem is solved to the same certified tolerance
-with a constant predicate, "always widen" and "never widen" (wall-clock time
-to certification on the GPU), and twice more while recording the model
-inputs of every decision met, once following the heuristic and once always
-widening.  Those decisions are labelled
-"widen" when always-widen certified faster than never-widen; problems that
-certify under neither are left out.  The model therefore learns, from what
-the solver observes early on, whether widening pays for the problem at
-hand.  It is trained on the GPU with the engine's own forward/backward
-kernels and Adam; the FP64 certificate still decides termination, so a poor
-model costs time, never correctness.
+Labels.  Each decision is labelled by a counterfactual: the solve is
+replayed from the start with every earlier decision scripted to what the
+recorded run chose (the engine's width_script option; products and
+reductions are deterministic, so the replay reaches the same state), the
+decision itself forced to widen and then to hold, and the heuristic taking
+every later decision; the decision is labelled "widen" when the widened
+replay certified sooner (wall clock on the GPU, best of two runs for short
+solves) and dropped when the two are within 3 percent.  Decisions are taken
+along the heuristic's path and along the always-widen path, so the states
+the model meets after its own widenings are represented too.  The model is
+trained on the GPU with the engine's own forward/backward kernels and Adam;
+the FP64 certificate still decides termination, so a poor model costs time,
+never correctness.
 
-CLI (GPU):  python -m amfls_solver_amd.width_model train --problems 300
-            python -m amfls_solver_amd.width_model evaluate
+Problems are drawn like the benchmark configurations (CSR like the 200000 x
+20000 and batched ones, dense G D up to 40000 rows on the host, and dense
+G D of 2e5..1e6 rows stored as bf16 and solved by the mixed-precision engine
+like the 1e6 x 2000 one); see training_problem.
+
+CLI (GPU):  python -m amfls_solver_amd.width_model train --first 0 --problems 1000 --collect-only --data part0.npz
+            python -m amfls_solver_amd.width_model train --from-data part0.npz part1.npz --wide --c2
+            python -m amfls_solver_amd.width_model evaluate --wide --c2
 """
 from __future__ import annotations
 
@@ -149,9 +157,18 @@ def training_problem(index: int, seed: int = 11):
     factors in [1e-3, 1]) or dense G D as the tall dense configuration
     (m log-uniform in [1000, 40000], n in [100, min(1000, m)], D log-spaced
     over 1-6 decades, b = A x* + 1e-8-relative noise); lambda = 0 or one of
-    logspace(-6, -1, 8).  Returns (matrix tensor, b, lambda, description)."""
+    logspace(-6, -1, 8); one problem in fifty is a tall dense G D of 2e5..1e6
+    rows and 500..2000 columns built on the GPU in bf16 (2..6 decades,
+    lambda 0), solved by the mixed-precision engine with a 1e-9 forward-error
+    target.  Returns (matrix tensor, b, lambda, description)."""
     from . import problems as P
     r = np.random.default_rng([seed, index])
+    if index % 50 == 7:  # tall dense bf16 like the 1e6 x 2000 configuration, mixed precision
+        m = int(round(10.0 ** r.uniform(np.log10(200_000), 6.0)))
+        n = int(r.integers(500, 2001))
+        dec = float(r.uniform(2.0, 6.0))
+        a, b, _ = P.config2_device(m, n, decades=dec, seed=int(r.integers(0, 2**31)))
+        return a, b, 0.0, f"dense-bf16 {m}x{n} dec {dec:.1f} lam 0 (mixed)"
     lam = 0.0 if r.random() < 0.4 else float(10.0 ** (-6.0 + 5.0 * int(r.integers(0, 8)) / 7.0))
     if r.random() < 0.65:
         m = int(round(10.0 ** r.uniform(np.log10(5000), np.log10(200_000))))
@@ -182,27 +199,90 @@ def wide_problem(rows: int = 50_000, cols: int = 500_000, per_row: int = 8, seed
 
 
 def _matrix(A, context=None):
+    if A.is_cuda and A.dtype == torch.bfloat16:
+        return DeviceMatrix(A, value_dtype=torch.bfloat16, context=context)
     return DeviceMatrix(A, value_dtype=torch.float32 if A.layout == torch.sparse_csr else torch.float64,
                         context=context)
 
 
-def _run(M, b, lam, tolerance, policy, record_decisions=False):
-    """(seconds to certification or inf, result summary, decision records)."""
+def _mode(desc: str) -> dict:
+    """Engine options of a problem: the mixed-precision engine with a
+    forward-error target for the dense bf16 ones, FP64 otherwise."""
+    return {"precision": "mixed", "forward_tolerance": 1e-9} if "(mixed)" in desc else {"precision": "fp64"}
+
+
+def _solve(M, b, lam, tolerance, mode, policy="heuristic", script=None):
+    """(wall seconds to certification or inf, status, auxiliary width,
+    decision records rows x 13 or None) of one solve."""
+    from .binding import _decode, _result_tuple
+    ctx = M.context
+    o = _lib.options(regularization=float(lam), tolerance=float(tolerance), width_policy=_lib.WIDTH_POLICY[policy],
+                     precision=_lib.PRECISION[mode["precision"]],
+                     forward_tolerance=float(mode.get("forward_tolerance", 0.0)))
+    arr = None
+    if script is not None and len(script) > 0:
+        arr = np.ascontiguousarray(script, dtype=np.int8)
+        o.width_script = arr.ctypes.data
+        o.width_script_length = arr.size
+    res = ctypes.c_void_p()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    if record_decisions:
-        rec, status, aux = _record_raw(M, b, lam, tolerance, policy)
-    else:
-        r = run_amfls(M, b, regularization=lam, tolerance=tolerance, width_policy=_lib.WIDTH_POLICY[policy],
-                      context=M.context)
-        rec, status, aux = None, int(r.status), int(r.auxiliary_width)
+    _lib.check(ctx._lib.amfls_solve(ctx.handle, M.handle, b.data_ptr(), 1, ctypes.byref(o), ctypes.byref(res)))
+    torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    return (dt if status == 0 else math.inf), aux, rec
+    rec = None
+    try:
+        if policy.startswith("record"):
+            rows = ctypes.c_int64()
+            _lib.check(ctx._lib.amfls_result_width_records(res, None, ctypes.byref(rows)))
+            rec = np.empty((rows.value, N_IN + 1), dtype=np.float32)
+            _lib.check(ctx._lib.amfls_result_width_records(res, rec.ctypes.data_as(ctypes.c_void_p),
+                                                           ctypes.byref(rows)))
+    except Exception:
+        ctx._lib.amfls_result_destroy(res)
+        raise
+    r = _decode(*_result_tuple(ctx, res))  # releases the result
+    return (dt if r.status == 0 else math.inf), int(r.status), int(r.auxiliary_width), rec
+
+
+def counterfactual_labels(M, b, lam, tolerance, mode, *, heuristic_decisions=3, widen_decisions=1,
+                          margin=0.03):
+    """Decision records with counterfactual labels for one problem.
+
+    Records the decisions of the heuristic's run and of the always-widen
+    run; for the first few decisions d of each, replays the solve with the
+    recorded choices before d, d forced to widen or to hold, and the
+    heuristic after d.  Returns (X rows x 12, y, summary)."""
+    _solve(M, b, lam, tolerance, mode)  # warm
+    X, y, out = [], [], {"heuristic_path": [], "widen_path": []}
+    for path, policy, limit in (("heuristic_path", "record", heuristic_decisions),
+                                ("widen_path", "record_widen", widen_decisions)):
+        t_path, _, _, rec = _solve(M, b, lam, tolerance, mode, policy)
+        if rec is None or rec.shape[0] == 0:
+            continue
+        chosen = rec[:, N_IN].astype(np.int8) if policy == "record" else np.ones(rec.shape[0], np.int8)
+        for d in range(min(limit, rec.shape[0])):
+            times = []
+            for c in (1, 0):
+                script = np.concatenate([chosen[:d], [c]]).astype(np.int8)
+                t = _solve(M, b, lam, tolerance, mode, "heuristic", script)[0]
+                if t < 0.05 or not math.isfinite(t):
+                    t = min(t, _solve(M, b, lam, tolerance, mode, "heuristic", script)[0])
+                times.append(t)
+            t_widen, t_hold = times
+            entry = {"decision": d, "widen_seconds": t_widen, "hold_seconds": t_hold, "heuristic": int(rec[d, N_IN])}
+            if math.isfinite(t_widen) or math.isfinite(t_hold):
+                if abs(t_widen - t_hold) > margin * min(t_widen, t_hold):
+                    X.append(rec[d, :N_IN])
+                    y.append(1.0 if t_widen < t_hold else 0.0)
+                    entry["label"] = int(y[-1])
+            out[path].append(entry)
+    return np.array(X, dtype=np.float32).reshape(-1, N_IN), np.array(y, dtype=np.float32), out
 
 
 def collect(indices, *, seed: int = 11, tolerance: float = 1e-8, budget_seconds: float = math.inf, log=print):
-    """Decision records and labels for the given problem indices (stops
-    starting new problems once `budget_seconds` have passed)."""
+    """Counterfactually labelled decision records for the given problem
+    indices (stops starting new problems once `budget_seconds` have passed)."""
     X, y, info = [], [], []
     t_start = time.perf_counter()
     for i in indices:
@@ -212,27 +292,12 @@ def collect(indices, *, seed: int = 11, tolerance: float = 1e-8, budget_seconds:
         A, b, lam, desc = training_problem(i, seed)
         M = _matrix(A)
         bd = b.cuda()
-        _run(M, bd, lam, tolerance, "heuristic")  # warm
-        # decisions met along the heuristic's and the always-widen path, and
-        # the time to certification of those runs and of never widening
-        t_heur, _, rec_h = _run(M, bd, lam, tolerance, "record", True)
-        t_widen, aux_w, rec_w = _run(M, bd, lam, tolerance, "record_widen", True)
-        t_hold, _, _ = _run(M, bd, lam, tolerance, "hold")
-        if min(t_widen, t_hold) < 0.05:  # short solves: best of two
-            t_widen = min(t_widen, _run(M, bd, lam, tolerance, "widen")[0])
-            t_hold = min(t_hold, _run(M, bd, lam, tolerance, "hold")[0])
-        rec = np.concatenate([rec_h, rec_w])
-        usable = math.isfinite(t_widen) or math.isfinite(t_hold)
-        label = 1.0 if t_widen < t_hold else 0.0
-        if usable:
-            for row in rec:
-                X.append(row[:N_IN])
-                y.append(label)
-        info.append({"index": int(i), "problem": desc, "widen_seconds": t_widen, "hold_seconds": t_hold,
-                     "heuristic_seconds": t_heur, "label": label, "decisions": len(rec), "used": usable})
-        log(f"  [{i}] {desc}: widen {t_widen:.4f}s (aux {aux_w}), hold {t_hold:.4f}s, "
-            f"heuristic {t_heur:.4f}s -> label {int(label)} ({len(rec)} decisions{'' if usable else ', unused'})")
-        del M
+        Xi, yi, summary = counterfactual_labels(M, bd, lam, tolerance, _mode(desc))
+        X.extend(Xi)
+        y.extend(yi)
+        info.append({"index": int(i), "problem": desc, "labelled": int(len(yi)), **summary})
+        log(f"  [{i}] {desc}: {len(yi)} labelled decisions ({int(np.sum(yi))} widen)")
+        del M, A, bd
     return np.array(X, dtype=np.float32).reshape(-1, N_IN), np.array(y, dtype=np.float32), info
 
 
@@ -264,10 +329,17 @@ def _record_raw(M, b, lam, tolerance, policy):
 
 
 def feature_domain(X: np.ndarray, margin: float = 0.05):
-    """Per-input box of the training inputs, widened by `margin` of each range."""
+    """Box of the training inputs, widened by `margin` of each range, for the
+    solver-state inputs (width, certificate decreases, level times); the
+    operator-size inputs (log rows, columns, nonzeros) are left open, so the
+    model also decides for operators larger or wider than it was trained
+    on, from the state they put the solver in."""
     lo, hi = X.min(axis=0), X.max(axis=0)
     pad = margin * np.maximum(hi - lo, 1e-3)
-    return (lo - pad).astype(np.float32), (hi + pad).astype(np.float32)
+    lo, hi = (lo - pad).astype(np.float32), (hi + pad).astype(np.float32)
+    lo[9:] = -np.inf
+    hi[9:] = np.inf
+    return lo, hi
 
 
 def train(X: np.ndarray, y: np.ndarray, *, epochs: int = 400, lr: float = 3e-3, batch: int = 256, seed: int = 0,
@@ -299,26 +371,22 @@ def train(X: np.ndarray, y: np.ndarray, *, epochs: int = 400, lr: float = 3e-3, 
 def evaluate(model: WidthModel, problems, *, seed: int = 11, tolerance: float = 1e-8, repeats: int = 2, log=print):
     """Time-to-tolerance of the heuristic and the learned policy per problem
     (median of `repeats` runs each, after a warm run).  `problems` holds
-    distribution indices or (matrix, b, lambda, description) tuples."""
+    distribution indices or (matrix, b, lambda, description) tuples; dense
+    bf16 problems run on the mixed-precision engine."""
     model.attach()
     rows = []
     for item in problems:
         A, b, lam, desc = training_problem(item, seed) if isinstance(item, (int, np.integer)) else item
         M = _matrix(A, model.context)
         bd = b.cuda()
+        mode = _mode(desc)
         out = {}
-        for pol, code in (("heuristic", 0), ("learned", 1)):
-            run_amfls(M, bd, regularization=lam, tolerance=tolerance, width_policy=code, context=model.context)
+        for pol in ("heuristic", "learned"):
+            _solve(M, bd, lam, tolerance, mode, pol)
             d0 = model.decisions()
-            ts = []
-            for _ in range(repeats):
-                torch.cuda.synchronize()
-                t0 = time.perf_counter()
-                r = run_amfls(M, bd, regularization=lam, tolerance=tolerance, width_policy=code,
-                              context=model.context)
-                ts.append(time.perf_counter() - t0 if r.status == 0 else math.inf)
+            runs = [_solve(M, bd, lam, tolerance, mode, pol) for _ in range(repeats)]
             d1 = model.decisions()
-            out[pol] = (float(np.median(ts)), int(r.status), int(r.auxiliary_width),
+            out[pol] = (float(np.median([r[0] for r in runs])), runs[-1][1], runs[-1][2],
                         (d1[0] - d0[0]) // repeats, (d1[1] - d0[1]) // repeats)
         rows.append({"index": int(item) if isinstance(item, (int, np.integer)) else -1, "problem": desc,
                      "heuristic": out["heuristic"][:3], "learned": out["learned"][:3],
@@ -331,6 +399,13 @@ def evaluate(model: WidthModel, problems, *, seed: int = 11, tolerance: float = 
               if math.isfinite(r["heuristic"][0]) and math.isfinite(r["learned"][0])]
     gm = float(np.exp(np.mean(np.log(ratios)))) if ratios else float("nan")
     return gm, rows
+
+
+def c2_problem():
+    """The 1e6 x 2000 bf16 dense configuration (mixed precision, forward-error target)."""
+    from . import problems as P
+    a, b, _ = P.config2_device()
+    return a, b, 0.0, "dense-bf16 1000000x2000 dec 6.0 lam 0 (mixed)"
 
 
 def gain_cases(rows, factor: float = 1.2, limit: int = 4):
@@ -356,19 +431,37 @@ def main(argv=None) -> int:
     t.add_argument("--out", type=Path, default=DEFAULT_PATH)
     t.add_argument("--data", type=Path, default=None, help="also save the decision records (npz) here")
     t.add_argument("--wide", action="store_true", help="also evaluate the wide 50000 x 500000 minimum-norm CSR case")
+    t.add_argument("--c2", action="store_true", help="also evaluate the 1e6 x 2000 bf16 dense case (mixed)")
+    t.add_argument("--first", type=int, default=0, help="first training problem index")
+    t.add_argument("--collect-only", action="store_true", help="only collect decision records (to --data)")
+    t.add_argument("--from-data", type=Path, nargs="*", default=None,
+                   help="train on these collected record files instead of collecting")
     e = sub.add_parser("evaluate")
     e.add_argument("--model", type=Path, default=DEFAULT_PATH)
     e.add_argument("--heldout", type=int, default=40)
     e.add_argument("--update", type=Path, default=None, help="write the model with the held-out result here")
     e.add_argument("--wide", action="store_true", help="also the wide 50000 x 500000 minimum-norm CSR case")
+    e.add_argument("--c2", action="store_true", help="also the 1e6 x 2000 bf16 dense case (mixed)")
     a = ap.parse_args(argv)
     if a.cmd == "train":
-        print(f"collecting {a.problems} training problems", flush=True)
-        X, y, info = collect(range(a.problems), budget_seconds=a.budget_seconds, log=lambda s: print(s, flush=True))
-        print(f"{X.shape[0]} decisions from {len(info)} problems, {y.mean():.3f} labelled widen", flush=True)
+        if a.from_data:
+            Xs, ys, info = [], [], []
+            for f in a.from_data:
+                with np.load(f, allow_pickle=False) as z:
+                    Xs.append(z["X"])
+                    ys.append(z["y"])
+                    info.extend(json.loads(str(z["info"])))
+            X, y = np.concatenate(Xs), np.concatenate(ys)
+        else:
+            print(f"collecting problems {a.first}..{a.first + a.problems - 1}", flush=True)
+            X, y, info = collect(range(a.first, a.first + a.problems), budget_seconds=a.budget_seconds,
+                                 log=lambda s: print(s, flush=True))
+        print(f"{X.shape[0]} labelled decisions from {len(info)} problems, {y.mean():.3f} labelled widen", flush=True)
         if a.data:
             a.data.parent.mkdir(parents=True, exist_ok=True)
             np.savez(a.data, X=X, y=y, info=np.array(json.dumps(info)))
+        if a.collect_only:
+            return 0
         model = train(X, y, epochs=a.epochs, log=lambda s: print(s, flush=True))
         z = model.logits(X)
         acc = float(np.mean((z > 0) == (y > 0.5)))
@@ -376,12 +469,13 @@ def main(argv=None) -> int:
         gm, rows = evaluate(model, range(100000, 100000 + a.heldout), log=lambda s: print(s, flush=True))
         print(f"held-out geometric-mean speedup of learned over heuristic: {gm:.3f}", flush=True)
         meta = dict(problems=len(info), decisions=int(X.shape[0]), widen_fraction=float(y.mean()),
-                    train_accuracy=acc, heldout=a.heldout, heldout_speedup=gm, epochs=a.epochs,
-                    gain_cases=gain_cases(rows))
-        if a.wide:
-            _, wr = evaluate(model, [wide_problem()], repeats=1, log=lambda s: print(s, flush=True))
-            meta.update(wide_case={k: wr[0][k] for k in ("problem", "heuristic", "learned", "model_decisions",
-                                                         "heuristic_fallbacks")})
+                    labels="per-decision counterfactual replays", train_accuracy=acc, heldout=a.heldout,
+                    heldout_speedup=gm, epochs=a.epochs, gain_cases=gain_cases(rows))
+        for flag, name, make in ((a.wide, "wide_case", wide_problem), (a.c2, "c2_case", c2_problem)):
+            if flag:
+                _, wr = evaluate(model, [make()], repeats=2, log=lambda s: print(s, flush=True))
+                meta[name] = {k: wr[0][k] for k in ("problem", "heuristic", "learned", "model_decisions",
+                                                     "heuristic_fallbacks")}
         model.save(a.out, **meta)
         print(f"saved {a.out}", flush=True)
     else:
@@ -390,10 +484,11 @@ def main(argv=None) -> int:
         print(f"held-out geometric-mean speedup of learned over heuristic: {gm:.3f}", flush=True)
         meta = dict(model.meta)
         meta.update(heldout=a.heldout, heldout_speedup=gm, gain_cases=gain_cases(rows))
-        if a.wide:
-            _, wr = evaluate(model, [wide_problem()], repeats=1, log=lambda s: print(s, flush=True))
-            meta.update(wide_case={k: wr[0][k] for k in ("problem", "heuristic", "learned", "model_decisions",
-                                                         "heuristic_fallbacks")})
+        for flag, name, make in ((a.wide, "wide_case", wide_problem), (a.c2, "c2_case", c2_problem)):
+            if flag:
+                _, wr = evaluate(model, [make()], repeats=2, log=lambda s: print(s, flush=True))
+                meta[name] = {k: wr[0][k] for k in ("problem", "heuristic", "learned", "model_decisions",
+                                                     "heuristic_fallbacks")}
         if a.update:
             model.save(a.update, **meta)
             print(f"wrote {a.update}", flush=True)

@@ -107,6 +107,14 @@ typedef struct amfls_options {
      * refinement; inner_tolerance > 0 is the FP32 search's tolerance. */
     double forward_tolerance;
     int32_t maximum_refinement_iterations;
+    /* Scripted width decisions (engine extension, single solves): the i-th
+     * widening decision of the solve takes width_script[i] (0 hold, 1 widen,
+     * any other value: the width policy) for i < width_script_length; later
+     * decisions follow the width policy.  Replaying a recorded run's choices
+     * and forcing one decision either way gives the counterfactual labels of
+     * the learned width controller.  NULL / 0: no script. */
+    const int8_t* width_script;
+    int32_t width_script_length;
 } amfls_options;
 
 const char* amfls_last_error(void);

@@ -431,17 +431,21 @@ def test_learned_width_policy_gains_over_heuristic(torch_mod):
 
 
 def test_learned_width_policy_falls_back_outside_its_domain(torch_mod):
-    """Inputs outside the box the model was trained on go to the heuristic:
-    on a problem smaller than any training problem, with the synthetic clock,
-    the learned policy takes no decision itself and reproduces the heuristic
-    trajectory exactly."""
+    """Solver states outside the box the model was trained on go to the
+    heuristic: with a domain no decision can fall in (active widths of
+    2^10..2^11) and the synthetic clock, the learned policy takes no decision
+    itself and reproduces the heuristic trajectory exactly."""
     import amfls_solver_amd as am
     from amfls_solver_amd import problems as P
     from amfls_solver_amd import width_model as W
     torch = torch_mod
     model = W.WidthModel.load()
     assert model.domain is not None
-    a, b, _ = P.dense_gd(600, 120, 4.0, seed=3, noise=1e-6)  # rows below the training range
+    lo, hi = model.domain
+    lo, hi = lo.copy(), hi.copy()
+    lo[0], hi[0] = 10.0, 11.0
+    model.set_domain(lo, hi)
+    a, b, _ = P.dense_gd(3000, 300, 4.0, seed=3, noise=1e-6)
     M = am.DeviceMatrix(torch.from_numpy(np.ascontiguousarray(a)), context=model.context)
     model.attach()
     bd = torch.from_numpy(b).cuda()
@@ -454,6 +458,53 @@ def test_learned_width_policy_falls_back_outside_its_domain(torch_mod):
     assert r_l.status == r_h.status and r_l.stop_reason == r_h.stop_reason
     assert (r_l.depth, r_l.auxiliary_width, r_l.basis_rank) == (r_h.depth, r_h.auxiliary_width, r_h.basis_rank)
     assert torch.equal(r_l.solution, r_h.solution)
+
+
+def test_width_script_replays_and_forks_decisions(torch_mod):
+    """The counterfactual replays behind the learned controller's labels:
+    scripting a recorded run's own choices reproduces its trajectory and
+    solution exactly (synthetic clock), and flipping the first decision
+    changes the widths the solve takes."""
+    import ctypes
+    import amfls_solver_amd as am
+    from amfls_solver_amd import _lib
+    from amfls_solver_amd import width_model as W
+    from amfls_solver_amd.binding import _decode, _result_tuple
+    torch = torch_mod
+    A, b, lam, _ = W.training_problem(3)
+    M = am.DeviceMatrix(A, value_dtype=torch.float32 if A.layout == torch.sparse_csr else torch.float64)
+    bd = b.cuda()
+
+    def solve(policy, script=None):
+        o = _lib.options(regularization=float(lam), tolerance=1e-8, width_policy=_lib.WIDTH_POLICY[policy],
+                         clock_mode=_lib.CLOCK["synthetic"])
+        arr = None
+        if script is not None:
+            arr = np.ascontiguousarray(script, dtype=np.int8)
+            o.width_script, o.width_script_length = arr.ctypes.data, arr.size
+        res = ctypes.c_void_p()
+        torch.cuda.synchronize()
+        _lib.check(M.context._lib.amfls_solve(M.context.handle, M.handle, bd.data_ptr(), 1, ctypes.byref(o),
+                                              ctypes.byref(res)))
+        rec = None
+        if policy == "record":
+            rows = ctypes.c_int64()
+            _lib.check(M.context._lib.amfls_result_width_records(res, None, ctypes.byref(rows)))
+            rec = np.empty((rows.value, 13), dtype=np.float32)
+            _lib.check(M.context._lib.amfls_result_width_records(res, rec.ctypes.data_as(ctypes.c_void_p),
+                                                                 ctypes.byref(rows)))
+        return _decode(*_result_tuple(M.context, res)), rec
+
+    base, rec = solve("record")
+    assert rec.shape[0] >= 1
+    chosen = rec[:, 12].astype(np.int8)
+    again, _ = solve("heuristic", chosen)
+    assert (again.depth, again.auxiliary_width, again.basis_rank) == (base.depth, base.auxiliary_width,
+                                                                        base.basis_rank)
+    assert torch.equal(again.solution, base.solution)
+    flipped, _ = solve("heuristic", [1 - chosen[0]])
+    assert flipped.status == 0
+    assert flipped.auxiliary_width != base.auxiliary_width or flipped.depth != base.depth
 
 
 def _edge_problems():

@@ -84,6 +84,8 @@ SIGNATURES = {
     "amfls_operator_set_variant": (c_int, [c_void_p, c_int, c_int, c_int]),
     "amfls_operator_profile": (c_int, [c_void_p, c_int]),
     "amfls_operator_kernel_stats": (c_int, [c_void_p, POINTER(c_int64), POINTER(c_double), POINTER(c_double)]),
+    "amfls_operator_kernel_stats_class": (c_int, [c_void_p, c_int, c_int32, c_int32, c_int, POINTER(c_int64),
+                                                  POINTER(c_double), POINTER(c_double)]),
     "amfls_spmm_variant_count": (c_int, []),
     "amfls_spmm_variant_name": (c_char_p, [c_int]),
     "amfls_spmm_variant_supports": (c_int, [c_int, c_int]),

@@ -309,6 +309,17 @@ class DeviceMatrix:
                                                                 ctypes.byref(by)))
         return l.value, sec.value, by.value
 
+    def kernel_stats_class(self, transpose: bool, min_cols: int, max_cols: int,
+                           dtype: torch.dtype = torch.float32):
+        """(launches, seconds, algorithmic bytes) of the timed products of one
+        class (direction, widths in [min_cols, max_cols], block dtype) since
+        the last kernel_stats; does not reset."""
+        l, sec, by = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
+        _lib.check(self.context._lib.amfls_operator_kernel_stats_class(
+            self.handle, int(transpose), min_cols, max_cols, _DTYPE_CODE[dtype], ctypes.byref(l),
+            ctypes.byref(sec), ctypes.byref(by)))
+        return l.value, sec.value, by.value
+
     def product_bytes(self, s: int, transpose: bool = False, dtype: torch.dtype = torch.float32) -> float:
         return self.context._lib.amfls_operator_product_bytes(self.handle, s, int(transpose), _DTYPE_CODE[dtype])
 

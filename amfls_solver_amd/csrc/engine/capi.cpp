@@ -347,6 +347,19 @@ int amfls_operator_kernel_stats(amfls_operator* op, int64_t* launches, double* s
     });
 }
 
+int amfls_operator_kernel_stats_class(amfls_operator* op, int transpose, int32_t min_cols, int32_t max_cols,
+                                      int dtype, int64_t* launches, double* seconds, double* bytes) {
+    return guard([&] {
+        require(op && launches && seconds && bytes, "null argument");
+        require(dtype == AMFLS_F64 || dtype == AMFLS_F32, "block products are float64 or float32");
+        require(min_cols >= 1 && min_cols <= max_cols, "width range is empty");
+        long long l = 0;
+        op->op->kernel_stats_class(transpose != 0, min_cols, max_cols, dtype == AMFLS_F64 ? 8 : 4, &l, seconds,
+                                   bytes);
+        *launches = l;
+    });
+}
+
 int amfls_operator_autotune(amfls_context* ctx, amfls_operator* op, const int32_t* widths, int32_t count,
                             int dtype, int32_t reps, int32_t* chosen) {
     return guard([&] {

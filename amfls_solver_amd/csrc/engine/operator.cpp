@@ -69,7 +69,7 @@ void DeviceOperator::prof_end(hipStream_t s, int ncols, bool transpose, int eb) 
     if (pool_.empty()) AMFLS_HIP_CHECK(hipEventCreate(&e));
     else { e = pool_.back(); pool_.pop_back(); }
     AMFLS_HIP_CHECK(hipEventRecord(e, s));
-    pending_.push_back({open_start_, e, product_bytes(ncols, transpose, eb)});
+    pending_.push_back({open_start_, e, product_bytes(ncols, transpose, eb), ncols, eb, transpose});
     open_start_ = nullptr;
 }
 
@@ -86,7 +86,7 @@ void DeviceOperator::prof_events(hipEvent_t* start, hipEvent_t* stop) const {
 void DeviceOperator::prof_commit(hipEvent_t start, hipEvent_t stop, int ncols, bool transpose, int eb) const {
     if (start == nullptr || stop == nullptr) return;
     std::lock_guard<std::mutex> g(prof_mu_);
-    pending_.push_back({start, stop, product_bytes(ncols, transpose, eb)});
+    pending_.push_back({start, stop, product_bytes(ncols, transpose, eb), ncols, eb, transpose});
 }
 
 void DeviceOperator::resolve_pending() {
@@ -97,6 +97,7 @@ void DeviceOperator::resolve_pending() {
         ++launches_;
         seconds_ += 1e-3 * ms;
         bytes_ += p.bytes;
+        records_.push_back({1e-3 * ms, p.bytes, p.ncols, p.elem_bytes, p.transpose});
         pool_.push_back(p.start);
         pool_.push_back(p.stop);
     }
@@ -111,6 +112,22 @@ void DeviceOperator::kernel_stats(long long* launches, double* seconds, double* 
     *bytes = bytes_;
     launches_ = 0;
     seconds_ = bytes_ = 0.0;
+    records_.clear();
+}
+
+void DeviceOperator::kernel_stats_class(bool transpose, int min_cols, int max_cols, int elem_bytes,
+                                        long long* launches, double* seconds, double* bytes) {
+    std::lock_guard<std::mutex> g(prof_mu_);
+    resolve_pending();
+    *launches = 0;
+    *seconds = *bytes = 0.0;
+    for (const ProfRecord& r : records_) {
+        if (r.transpose != transpose || r.ncols < min_cols || r.ncols > max_cols || r.elem_bytes != elem_bytes)
+            continue;
+        ++*launches;
+        *seconds += r.seconds;
+        *bytes += r.bytes;
+    }
 }
 
 // ------------------------------------------------------------------ CSR

@@ -82,6 +82,11 @@ public:
     // begin/end pairing assumes one thread drives the operator while it is on.
     void set_profiling(bool on);
     void kernel_stats(long long* launches, double* seconds, double* bytes);
+    // The same totals over the products of one class (direction, block
+    // widths in [min_cols, max_cols], block element bytes) since the last
+    // kernel_stats; does not reset.
+    void kernel_stats_class(bool transpose, int min_cols, int max_cols, int elem_bytes, long long* launches,
+                            double* seconds, double* bytes);
 
 protected:
     void prof_begin(hipStream_t s) const;
@@ -99,10 +104,12 @@ protected:
 
 private:
     static inline thread_local unsigned long long* stamp_ = nullptr;
-    struct ProfPair { hipEvent_t start, stop; double bytes; };
+    struct ProfPair { hipEvent_t start, stop; double bytes; int ncols, elem_bytes; bool transpose; };
+    struct ProfRecord { double seconds, bytes; int ncols, elem_bytes; bool transpose; };
     mutable std::mutex prof_mu_;  // products of one operator may run on several threads
     std::atomic<bool> profiling_{false};
     mutable std::vector<ProfPair> pending_;
+    std::vector<ProfRecord> records_;
     mutable std::vector<hipEvent_t> pool_;
     mutable hipEvent_t open_start_ = nullptr;
     long long launches_ = 0;

@@ -138,6 +138,18 @@ def dense_products(torch, M, rows, cols, widths=(1, 16, 32, 64), reps=5):
     return prod
 
 
+def dense_kernel_name(c):
+    """Product kernel of one class of bf16 dense products (the events around
+    the product also cover its small pack or reduce launch)."""
+    lo, hi = c["widths"]
+    if hi == 1:
+        return f"narrow_{'tn' if c['transpose'] else 'nn'}_kernel<bf16, {'float' if c['block'] == 'f32' else 'double'}, 1>"
+    nt = (hi + 15) // 16
+    if c["block"] == "f32":
+        return f"{'tn' if c['transpose'] else 'nn'}_split3_kernel<float, {nt}>"
+    return f"dense_{'tn' if c['transpose'] else 'nn_f64'}_kernel<bf16, double, ..., {nt}>"
+
+
 def c2_headline(torch, np, am, P, steps, warmup, forward_target=1e-8):
     """The 1e6 x 2000 bf16 dense workload: K timed solves per engine, forward
     errors against the FP64 oracle, kernel bandwidth of the mixed solves."""
@@ -184,9 +196,21 @@ def c2_headline(torch, np, am, P, steps, warmup, forward_target=1e-8):
         am.run_amfls(M, b, tolerance=1e-8, **engines["mixed"])
     torch.cuda.synchronize()
     out["profiled_seconds_per_solve"] = (time.perf_counter() - t1) / steps
+    # per-class view: each (direction, width range, block type) class runs one
+    # product kernel; the roofline reports the class with the most time
+    classes = []
+    for tr in (False, True):
+        for lo, hi in ((1, 1), (2, 16), (17, 32), (33, 48), (49, 64)):
+            for dt in (torch.float32, torch.float64):
+                cn, cs, cb = M.kernel_stats_class(tr, lo, hi, dt)
+                if cn:
+                    classes.append({"transpose": tr, "widths": [lo, hi], "block": "f32" if dt == torch.float32
+                                    else "f64", "launches": cn, "seconds": cs, "bytes": cb,
+                                    "avg_us": 1e6 * cs / cn, "GBs": cb / cs / 1e9})
     n, sec, by = M.kernel_stats()
     M.profile(False)
-    out["kernel"] = {"launches": n, "seconds": sec, "bytes": by}
+    classes.sort(key=lambda c: -c["seconds"])
+    out["kernel"] = {"launches": n, "seconds": sec, "bytes": by, "classes": classes}
     # learned width policy (shipped model) on the mixed engine
     from amfls_solver_amd.width_model import WidthModel
     wm = WidthModel.load(context=M.context)
@@ -328,30 +352,42 @@ def host_cpu():
     return model, os.cpu_count() or 1
 
 
-def cpu_baseline(torch, np, P, rows=100_000, cols=2000):
-    """The FP64 CPU restatement (oracle AMFLS with numpy BLAS) on a sample of
-    the dense workload: its first `rows` rows, same G D construction, same
-    tolerance; time scaled by 1e6 / rows to the full workload (products and
-    left-block work are linear in the rows)."""
+def cpu_baseline(torch, np, P, rows=100_000, cols=2000, single_rows=20_000):
+    """The FP64 CPU restatement (oracle AMFLS with numpy BLAS) on samples of
+    the dense workload: its first `rows` rows with BLAS on every host core
+    the GPU box grants (at most 16), and its first `single_rows` rows on one
+    thread; same G D construction, same tolerance; times scaled by
+    1e6 / rows to the full workload (products and left-block work are linear
+    in the rows)."""
     from threadpoolctl import threadpool_limits
     from oracle import operators as O
     from oracle import solvers as S
-    a, b, _ = P.config2_device(rows, cols)
-    ah = a.double().cpu().numpy()
-    bh = b.cpu().numpy()
-    del a, b
     model, ncpu = host_cpu()
     cores = min(16, ncpu)
-    with threadpool_limits(limits=cores):
-        op = O.RowMajorDenseOperator(ah)
-        t0 = time.perf_counter()
-        res = S.solve_amfls(op, bh, tolerance=1e-8)
-        dt = time.perf_counter() - t0
-    scale = 1_000_000 / rows
+    runs = {}
+    for label, nrows, threads in (("all", rows, cores), ("single", single_rows, 1)):
+        a, b, _ = P.config2_device(nrows, cols)
+        ah = a.double().cpu().numpy()
+        bh = b.cpu().numpy()
+        del a, b
+        with threadpool_limits(limits=threads):
+            op = O.RowMajorDenseOperator(ah)
+            t0 = time.perf_counter()
+            res = S.solve_amfls(op, bh, tolerance=1e-8)
+            dt = time.perf_counter() - t0
+        del ah, op
+        runs[label] = (nrows, threads, dt, res)
+    nrows, threads, dt, res = runs["all"]
+    scale = 1_000_000 / nrows
+    srows, _, sdt, sres = runs["single"]
+    sscale = 1_000_000 / srows
     return {"value": dt * scale, "unit": "s", "cores": cores, "kind": "port",
-            "sample": f"oracle FP64 AMFLS (numpy/scipy restatement, BLAS on {cores} threads) on the first {rows} "
+            "sample": f"oracle FP64 AMFLS (numpy/scipy restatement, BLAS on {cores} threads) on the first {nrows} "
                       f"rows of the dense G D workload (x{scale:g} to 1e6 rows); measured {dt:.2f} s, "
                       f"status {int(res.status)}, depth {int(res.depth)}",
+            "single_thread": {"value": sdt * sscale, "unit": "s", "cores": 1,
+                              "sample": f"same restatement on one thread, first {srows} rows (x{sscale:g}); "
+                                        f"measured {sdt:.2f} s, status {int(sres.status)}, depth {int(sres.depth)}"},
             "host_cpu": model, "host_nproc": ncpu}
 
 
@@ -395,7 +431,8 @@ def main() -> int:
     mixed, fp64 = c2["mixed"], c2["fp64"]
     valid = (c2["matrix_matches_fixture"] and mixed["forward_error_max"] <= 1e-8 and
              fp64["forward_error_max"] <= 1e-8 and all(st == 0 for st in mixed["status"]))
-    kn, ksec, kby = c2["kernel"]["launches"], c2["kernel"]["seconds"], c2["kernel"]["bytes"]
+    top = c2["kernel"]["classes"][0]
+    kn, ksec, kby = top["launches"], top["seconds"], top["bytes"]
     achieved = kby / ksec / 1e9 if ksec > 0 else 0.0
 
     c1 = None if a.no_c1 else c1_section(torch, np, am, P, a.steps, a.warmup)
@@ -453,10 +490,13 @@ def main() -> int:
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": "dense product kernels (A X and A^T Y on the bf16 matrix, every width and precision) of "
-                      "the timed mixed solves",
+            "kernel": dense_kernel_name(top),
+            "product_class": {k: top[k] for k in ("transpose", "widths", "block")},
+            "bytes_per_launch": kby / max(kn, 1),
             "launches": kn,
             "avg_us": 1e6 * ksec / max(kn, 1),
+            "all_products_GBs": c2["kernel"]["bytes"] / c2["kernel"]["seconds"] / 1e9
+            if c2["kernel"]["seconds"] > 0 else None,
             "timed_pass": f"{a.steps} further mixed solves with HIP events around every product on the engine "
                           f"stream ({1e3 * c2['profiled_seconds_per_solve']:.1f} ms per solve with the events)",
         },

@@ -210,6 +210,14 @@ int amfls_operator_autotune(amfls_context* ctx, amfls_operator* op, const int32_
  * resets: launches, total seconds and total algorithmic bytes. */
 int amfls_operator_profile(amfls_operator* op, int enable);
 int amfls_operator_kernel_stats(amfls_operator* op, int64_t* launches, double* seconds, double* bytes);
+/* The same totals over one class of the products timed since the last
+ * kernel_stats (it does not reset): direction (transpose 0: A X, 1: A^T Y),
+ * block widths in [min_cols, max_cols] and block type (AMFLS_F64 or
+ * AMFLS_F32).  A per-kernel view for rooflines: each class runs one product
+ * kernel.  No counterpart in the reference (its operator seconds are host
+ * wall time, matrix_operator.hpp). */
+int amfls_operator_kernel_stats_class(amfls_operator* op, int transpose, int32_t min_cols, int32_t max_cols,
+                                      int dtype, int64_t* launches, double* seconds, double* bytes);
 int amfls_spmm_variant_count(void);
 const char* amfls_spmm_variant_name(int variant);
 int amfls_spmm_variant_supports(int variant, int block_cols);

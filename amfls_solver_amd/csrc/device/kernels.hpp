@@ -234,17 +234,23 @@ int csr_stream_budget();
 int csr_spmm_stream_variant();
 
 // Wall-clock stamps of one product taken inside its kernels (device memory,
-// zeroed when allocated): slot 0 = the device wall clock when block 0 of the
-// first launch begins; slots 1..8 = when each of the last eight blocks of the
-// final launch (one per XCD, dispatched last there) ends (a grid of fewer
-// blocks leaves older, smaller values in the rest).  The product's time is
-// max(slots 1..8) - slot 0, in ticks of hipDeviceAttributeWallClockRate.  No
-// atomics: the stamps are plain stores read after the kernel.
-constexpr int kStampSlots = 9;
+// zeroed when allocated).  The stamps of product i live at slots[i + j *
+// kStampStride]: j = 0 is the device wall clock when block 0 of the first
+// launch begins; j = 1..kStampEnds are end slots, and every block of the
+// final launch raises end slot 1 + (block % kStampEnds) to its own end time
+// with an atomic maximum, so the largest end slot is the last block to
+// finish whatever the row lengths.  The end slots of one product lie
+// kStampStride values apart (distinct cache lines and memory channels), so
+// the atomics of a 3000-block grid spread over kStampEnds addresses instead
+// of serialising on one.  The product's time is max(end slots) - slot 0, in
+// ticks of hipDeviceAttributeWallClockRate.
+constexpr int kStampEnds = 64;
+constexpr int kStampSlots = 1 + kStampEnds;
+constexpr int64_t kStampStride = 4096;  // products per stamp buffer
 struct ProductStamp {
-    unsigned long long* slots = nullptr;  // kStampSlots values, or null: untimed
+    unsigned long long* slots = nullptr;  // slot 0 of one product, or null: untimed
     bool first = true;                    // this launch writes slot 0
-    bool last = true;                     // this launch writes slots 1..8
+    bool last = true;                     // this launch writes the end slots
 };
 
 // Y(rows x ncols) = A * X (+ add_scale * Add[0:rows]); X, Y, Add row-major.

@@ -24,9 +24,10 @@ constexpr int kThreads = 256;
 
 // In-kernel product timing (ProductStamp): block 0 of the first launch
 // stamps the start; every block of the final launch, once all its threads
-// are done, raises the end slot to its own end time with an atomic maximum,
-// so the end is the last block to finish whatever the row lengths (the
-// clock only grows, so a value left by an earlier product is smaller).
+// are done, raises its end slot (one of kStampEnds, by block index) to its
+// own end time with an atomic maximum, so the largest end slot is the last
+// block to finish whatever the row lengths (the clock only grows, so a value
+// left by an earlier product is smaller).
 __device__ __forceinline__ void stamp_begin(ProductStamp st) {
     if (st.slots != nullptr && st.first && blockIdx.x == 0 && threadIdx.x == 0) st.slots[0] = wall_clock64();
 }
@@ -34,7 +35,9 @@ __device__ __forceinline__ void stamp_begin(ProductStamp st) {
 __device__ __forceinline__ void stamp_end(ProductStamp st) {
     if (st.slots == nullptr || !st.last) return;  // uniform: the barrier sees every thread or none
     __syncthreads();
-    if (threadIdx.x == 0) atomicMax(st.slots + 1, static_cast<unsigned long long>(wall_clock64()));
+    if (threadIdx.x == 0)
+        atomicMax(st.slots + kStampStride * (1 + static_cast<int64_t>(blockIdx.x % kStampEnds)),
+                  static_cast<unsigned long long>(wall_clock64()));
 }
 
 template <typename V, typename T, int P, int LC, int CPL>

@@ -133,15 +133,12 @@ public:
     unsigned long long* kernel_stamp(hipStream_t s, double* target, double* target2 = nullptr) {
         if (dstamp_ == nullptr) {
             tick_seconds_ = wall_tick_seconds();
-            grow(kInitialProducts, s);
+            allocate(s);
         }
         if (!stamp_targets_.empty() && s != stamp_stream_) flush();
-        if (stamp_targets_.size() == capacity_) {
-            flush();
-            grow(2 * capacity_, s);
-        }
+        if (stamp_targets_.size() == static_cast<size_t>(kStampStride)) flush();  // reuse from product 0
         stamp_stream_ = s;
-        unsigned long long* slots = dstamp_ + stamp_targets_.size() * kStampSlots;
+        unsigned long long* slots = dstamp_ + stamp_targets_.size();
         stamp_targets_.push_back({target, target2});
         return slots;
     }
@@ -168,15 +165,19 @@ public:
     }
     void flush() {
         if (!stamp_targets_.empty()) {
-            const size_t n = stamp_targets_.size() * kStampSlots;
+            const size_t n = stamp_targets_.size();
             // No clearing between uses: every launched product rewrites slot 0,
             // and end slots left from an earlier product hold earlier clock
             // values, below this product's own ends, so the maximum is unchanged.
-            AMFLS_HIP_CHECK(hipMemcpyAsync(hstamp_, dstamp_, n * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                                           stamp_stream_));
+            // Slot j of the n pending products: n contiguous values per row.
+            AMFLS_HIP_CHECK(hipMemcpy2DAsync(hstamp_, n * sizeof(unsigned long long), dstamp_,
+                                             kStampStride * sizeof(unsigned long long),
+                                             n * sizeof(unsigned long long), kStampSlots, hipMemcpyDeviceToHost,
+                                             stamp_stream_));
             AMFLS_HIP_CHECK(hipStreamSynchronize(stamp_stream_));
-            for (size_t i = 0; i < stamp_targets_.size(); ++i) {
-                const unsigned long long* v = hstamp_ + i * kStampSlots;
+            for (size_t i = 0; i < n; ++i) {
+                unsigned long long v[kStampSlots];
+                for (int j = 0; j < kStampSlots; ++j) v[j] = hstamp_[j * n + i];
                 unsigned long long end = 0;
                 for (int j = 1; j < kStampSlots; ++j) end = std::max(end, v[j]);
                 if (v[0] == 0 || end <= v[0]) continue;
@@ -213,7 +214,6 @@ private:
     std::vector<hipEvent_t> pool_;
     std::vector<Pending> pending_;
     hipEvent_t open_ = nullptr;
-    static constexpr size_t kInitialProducts = 1024;
     static double wall_tick_seconds() {
         int dev = 0, khz = 0;
         AMFLS_HIP_CHECK(hipGetDevice(&dev));
@@ -221,21 +221,15 @@ private:
         if (khz <= 0) throw std::runtime_error("device reports no wall clock rate");
         return 1.0 / (1e3 * static_cast<double>(khz));
     }
-    // Room for `products` stamped products (nothing pending).
-    void grow(size_t products, hipStream_t s) {
-        if (dstamp_) AMFLS_HIP_CHECK(hipFree(dstamp_));
-        if (hstamp_) AMFLS_HIP_CHECK(hipHostFree(hstamp_));
-        dstamp_ = nullptr;
-        hstamp_ = nullptr;
-        const size_t bytes = products * kStampSlots * sizeof(unsigned long long);
+    // Stamps of kStampStride products, slot-major (2.1 MB on the device).
+    void allocate(hipStream_t s) {
+        const size_t bytes = static_cast<size_t>(kStampStride) * kStampSlots * sizeof(unsigned long long);
         AMFLS_HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&dstamp_), bytes));
         AMFLS_HIP_CHECK(hipMemsetAsync(dstamp_, 0, bytes, s));
         AMFLS_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&hstamp_), bytes, hipHostMallocDefault));
-        capacity_ = products;
     }
     unsigned long long* dstamp_ = nullptr;
     unsigned long long* hstamp_ = nullptr;
-    size_t capacity_ = 0;
     hipStream_t stamp_stream_ = nullptr;
     std::vector<std::pair<double*, double*>> stamp_targets_;
     double tick_seconds_ = 0.0;

@@ -143,6 +143,7 @@ SIGNATURES = {
     "amfls_width_model_set_domain": (c_int, [c_void_p, c_void_p, c_void_p]),
     "amfls_width_model_decisions": (c_int, [c_void_p, POINTER(c_int64), POINTER(c_int64)]),
     "amfls_context_set_width_model": (c_int, [c_void_p, c_void_p]),
+    "amfls_context_set_level_graphs": (c_int, [c_void_p, c_int]),
 }
 
 _lib = None

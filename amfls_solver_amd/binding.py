@@ -144,6 +144,11 @@ class Context:
     def synchronize(self) -> None:
         _lib.check(self._lib.amfls_context_synchronize(self.handle))
 
+    def set_level_graphs(self, enabled: bool) -> None:
+        """Capture width-one level batches into HIP graphs (True) or enqueue
+        them kernel by kernel (False, the default); results are identical."""
+        _lib.check(self._lib.amfls_context_set_level_graphs(self.handle, int(bool(enabled))))
+
     def __del__(self):
         h = getattr(self, "handle", None)
         if h is not None and h.value:

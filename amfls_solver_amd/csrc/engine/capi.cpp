@@ -774,6 +774,13 @@ int amfls_context_set_width_model(amfls_context* ctx, amfls_width_model* model) 
     });
 }
 
+int amfls_context_set_level_graphs(amfls_context* ctx, int enabled) {
+    return guard([&] {
+        require(ctx != nullptr, "null argument");
+        ctx->ctx.level_graphs = enabled != 0;
+    });
+}
+
 int amfls_result_destroy(amfls_result* res) {
     return guard([&] { delete res; });
 }

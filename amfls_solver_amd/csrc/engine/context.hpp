@@ -142,6 +142,20 @@ public:
         stamp_targets_.push_back({target, target2});
         return slots;
     }
+    // Make room for the stamps of `products` further products on `s` now
+    // (allocating or flushing here), so that handing them out never
+    // synchronizes a stream: for products enqueued under stream capture.
+    void reserve(hipStream_t s, size_t products) {
+        if (products > static_cast<size_t>(kStampStride))
+            throw std::length_error("more stamped products than the stamp buffer holds");
+        if (dstamp_ == nullptr) {
+            tick_seconds_ = wall_tick_seconds();
+            allocate(s);
+        }
+        if (!stamp_targets_.empty() && (s != stamp_stream_ || stamp_targets_.size() + products >
+                                                                  static_cast<size_t>(kStampStride)))
+            flush();
+    }
     void begin(hipStream_t s) {
         open_ = take();
         AMFLS_HIP_CHECK(hipEventRecord(open_, s));
@@ -247,6 +261,9 @@ struct Context {
     PinnedBuf<char> landing;   // host side of small downloads
     ProductTimer timer;
     Clock clock;
+    // Width-one level batches are captured into a HIP graph and launched as
+    // one graph instead of being enqueued kernel by kernel.
+    bool level_graphs = false;
 
     static constexpr size_t kStageDoubles = 1 << 20;
     static constexpr size_t kLandingBytes = 1 << 20;

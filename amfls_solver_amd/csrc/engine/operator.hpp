@@ -81,6 +81,7 @@ public:
     // bytes (product_bytes of each launch).  The event queues are locked; the
     // begin/end pairing assumes one thread drives the operator while it is on.
     void set_profiling(bool on);
+    bool profiling() const { return profiling_.load(); }
     void kernel_stats(long long* launches, double* seconds, double* bytes);
     // The same totals over the products of one class (direction, block
     // widths in [min_cols, max_cols], block element bytes) since the last

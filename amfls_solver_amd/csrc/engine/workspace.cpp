@@ -582,6 +582,46 @@ int Workspace<T>::fast_batch_limit() const {
     return static_cast<int>(std::max<int64_t>(0, lim));
 }
 
+namespace {
+
+// Stream capture of one batch of levels into a graph that is launched once
+// (Context::level_graphs).  The graph is destroyed with this object, after
+// the batch's results have been downloaded (the download waits for them).
+class BatchGraph {
+public:
+    explicit BatchGraph(hipStream_t s) : s_(s) {
+        if (s_ == nullptr) return;
+        AMFLS_HIP_CHECK(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
+        capturing_ = true;
+    }
+    void launch() {
+        if (s_ == nullptr) return;
+        capturing_ = false;
+        AMFLS_HIP_CHECK(hipStreamEndCapture(s_, &graph_));
+        AMFLS_HIP_CHECK(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+        AMFLS_HIP_CHECK(hipGraphLaunch(exec_, s_));
+    }
+    ~BatchGraph() {
+        if (capturing_) {  // an exception left the capture open: close it, drop what was captured
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(s_, &g);
+            if (g) (void)hipGraphDestroy(g);
+        }
+        if (exec_) (void)hipGraphExecDestroy(exec_);
+        if (graph_) (void)hipGraphDestroy(graph_);
+    }
+    BatchGraph(const BatchGraph&) = delete;
+    BatchGraph& operator=(const BatchGraph&) = delete;
+
+private:
+    hipStream_t s_ = nullptr;
+    bool capturing_ = false;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t exec_ = nullptr;
+};
+
+}  // namespace
+
 template <typename T>
 int Workspace<T>::run_fast_batch(int count) {
     // Each level: r = A^T u - beta v_prev, DGKS against V, rank decision,
@@ -618,6 +658,13 @@ int Workspace<T>::run_fast_batch(int count) {
     double* Q1 = f2 + spart;            // projection partials, first pass
     double* Q2 = Q1 + qmax;             // second pass
     double* gpart = Q2 + qmax;          // g partials, nbN per level
+    // Graph capture needs every product to be enqueued without host work on
+    // the stream: no event pairs (operator profiling, event-timed products)
+    // and stamp room reserved beforehand.
+    const bool graph = ctx_.level_graphs && !op_.profiling() &&
+                       (ctx_.clock.synthetic() || op_.times_own_kernels());
+    if (graph && !ctx_.clock.synthetic()) ctx_.timer.reserve(s, 2 * static_cast<size_t>(count));
+    BatchGraph batch_graph(graph ? s : nullptr);
     fill<double>(s, fast_rec_.get(), 8 * static_cast<int64_t>(count), 0.0);
     fill<double>(s, go, 1, 1.0);
     fill<double>(s, beta, 1, coupling_(0, 0));
@@ -650,6 +697,7 @@ int Workspace<T>::run_fast_batch(int count) {
         chain_normalize<T>(s, m_, 1, l, ldl, l, ldl, sP, bP, f1, f2, dgks, fac_l, Q1, Q2, b_dev_,
                            gpart + static_cast<int64_t>(j) * nbN, rec + 4, beta + j + 1, go);
     }
+    batch_graph.launch();
     fast_gpart_.resize(static_cast<size_t>(count) * nbN);
     fast_host_.resize(static_cast<size_t>(8 * count));
     AMFLS_HIP_CHECK(hipMemcpyAsync(fast_gpart_.data(), gpart, fast_gpart_.size() * sizeof(double),

@@ -256,6 +256,22 @@ def c1_section(torch, np, am, P, steps, warmup):
                      "refinement_iterations": r.refinement_steps,
                      "block_products_per_s": (r.a_block_calls + r.at_block_calls) / dt}
         log(f"C1 {mode}: {dt:.4f} s/solve status {r.status} cert {r.relative_energy_error_upper_bound:.3e}")
+    # the same FP64 solve with its width-one level batches captured into HIP graphs
+    M.context.set_level_graphs(True)
+    try:
+        am.run_amfls(M, b, regularization=p.regularization, tolerance=1e-8, precision="fp64")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            r = am.run_amfls(M, b, regularization=p.regularization, tolerance=1e-8, precision="fp64")
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+    finally:
+        M.context.set_level_graphs(False)
+    out["fp64_level_graphs"] = {"seconds": dt, "status": int(r.status), "depth": r.depth,
+                                "rel_diff_vs_direct": float(np.linalg.norm(r.solution.numpy() - sols["fp64"]) /
+                                                            np.linalg.norm(sols["fp64"]))}
+    log(f"C1 fp64 with level graphs: {dt:.4f} s/solve status {r.status}")
     M.profile(True)
     torch.cuda.synchronize()
     for _ in range(steps):

@@ -313,6 +313,14 @@ int amfls_width_model_set_domain(amfls_width_model* model, const float* lo, cons
 int amfls_width_model_decisions(const amfls_width_model* model, int64_t* by_model, int64_t* by_heuristic);
 /* The model used by AMFLS_WIDTH_LEARNED solves on this context (NULL clears). */
 int amfls_context_set_width_model(amfls_context* ctx, amfls_width_model* model);
+/* Width-one level batches of later solves on this context are captured into
+ * a HIP graph (stream capture of the batch's products and level-chain
+ * kernels) and launched as one graph (enabled != 0), or enqueued kernel by
+ * kernel (0, the default).  No counterpart in the reference, which has no
+ * device queue; results are identical either way.  Operators timed by
+ * events (dense products on the wall clock) and operators being profiled
+ * keep the direct enqueue. */
+int amfls_context_set_level_graphs(amfls_context* ctx, int enabled);
 /* Recorded decisions of an AMFLS_WIDTH_RECORD solve: rows x 13 floats
  * (out may be NULL to query rows). */
 int amfls_result_width_records(const amfls_result* result, float* out, int64_t* rows);

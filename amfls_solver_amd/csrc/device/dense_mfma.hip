@@ -922,6 +922,22 @@ void dense_mfma_tn(hipStream_t s, DenseMfmaMode mode, const Strided& A, int64_t 
     }
 }
 
+size_t dense_product_scratch_bound(int64_t M, int64_t N, int max_cols) {
+    if (M <= 0 || N <= 0 || max_cols <= 0) return 0;
+    const int64_t s = std::min(max_cols, 64);
+    const int64_t row_chunks = std::max<int64_t>(1, ceil_div(M, 256));
+    // tn_split3_dispatch: at most ceil(768 / strips) chunks of N x s doubles
+    const int64_t c3 = std::max<int64_t>(1, std::min<int64_t>(ceil_div(768, ceil_div(N, kTnCols)), row_chunks));
+    // tn_dispatch (fp32 / fp64 matrix cores): at most ceil(2048 / strips) chunks
+    const int64_t c2 = std::max<int64_t>(1, std::min<int64_t>(ceil_div(2048, ceil_div(N, 64)), row_chunks));
+    // the width-one vector-ALU A^T product: at most ceil(1024 / strips) chunks of N doubles
+    const int64_t c1 = std::max<int64_t>(1, std::min<int64_t>(ceil_div(1024, ceil_div(N, 8 * kThreads)), row_chunks));
+    // nn_dispatch: the packed X^T planes, 64 columns by N rounded up to 64, at most 8 bytes an entry
+    const int64_t planes = 64 * ceil_div(N, 64) * 8;
+    const int64_t need = std::max({std::max(c3, c2) * N * s * 8, c1 * N * 8, planes});
+    return static_cast<size_t>(need);
+}
+
 template void dense_mfma_nn<double>(hipStream_t, DenseMfmaMode, const Strided&, int64_t, int64_t, int, const double*,
                                     int64_t, double*, int64_t, Scratch&);
 template void dense_mfma_nn<float>(hipStream_t, DenseMfmaMode, const Strided&, int64_t, int64_t, int, const float*,

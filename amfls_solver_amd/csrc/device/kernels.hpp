@@ -112,6 +112,12 @@ void dense_mfma_nn(hipStream_t s, DenseMfmaMode mode, const Strided& A, int64_t 
 template <typename TX>
 void dense_mfma_tn(hipStream_t s, DenseMfmaMode mode, const Strided& A, int64_t M, int64_t N, int ncols, const TX* Y,
                    int64_t ldy, TX* Z, int64_t ldz, Scratch& scratch);
+// Upper bound of the scratch bytes that a product of an M x N row-major dense
+// operator with a block of at most max_cols (<= 64) columns takes (partial
+// sums of the A^T products, split planes of X); the operator grows its
+// context's scratch to it once, so no product allocates (or frees, which
+// waits for the device) in the middle of a timed solve.
+size_t dense_product_scratch_bound(int64_t M, int64_t N, int max_cols);
 
 // ---------------------------------------------------------------- vectors
 template <typename Tin, typename Tout>

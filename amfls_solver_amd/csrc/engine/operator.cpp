@@ -396,6 +396,12 @@ DenseOperator::DenseOperator(int64_t rows, int64_t cols, const void* values, Val
         model_.upper = kInfD;
         model_.lower = 0.0;
     }
+    // Grow the context's scratch now to what the widest product needs: a
+    // product that grows it frees the old arena (which waits for the device)
+    // and allocates inside a timed interval, so the first wide products of a
+    // process used to look slow to the width controller and steer the first
+    // solve onto another trajectory than every later one.
+    if (a_.cs == 1) ctx.scratch.ensure(dense_product_scratch_bound(rows, cols, 64));
 }
 
 void DenseOperator::set_variant(int s, bool transpose, int variant) {

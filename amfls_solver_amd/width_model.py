@@ -188,6 +188,17 @@ def training_problem(index: int, seed: int = 11):
     return torch.from_numpy(np.ascontiguousarray(a)), torch.from_numpy(b), lam, f"dense {m}x{n} dec {dec:.1f} lam {lam:.1e}"
 
 
+def problem_kind(index: int, seed: int = 11) -> str:
+    """"dense-bf16", "csr" or "dense": the kind of training_problem(index,
+    seed), found from the same random draws without building the matrix."""
+    if index % 50 == 7:
+        return "dense-bf16"
+    r = np.random.default_rng([seed, index])
+    if r.random() >= 0.4:
+        r.integers(0, 8)
+    return "csr" if r.random() < 0.65 else "dense"
+
+
 def wide_problem(rows: int = 50_000, cols: int = 500_000, per_row: int = 8, seed: int = 5):
     """The wide minimum-norm evaluation case: CSR rows x cols with `per_row`
     distinct uniform columns per row, N(0,1) values, b ~ N(0,1), lambda = 0."""
@@ -246,13 +257,14 @@ def _solve(M, b, lam, tolerance, mode, policy="heuristic", script=None):
 
 
 def counterfactual_labels(M, b, lam, tolerance, mode, *, heuristic_decisions=3, widen_decisions=1,
-                          margin=0.03):
+                          first_decision=0, margin=0.03):
     """Decision records with counterfactual labels for one problem.
 
     Records the decisions of the heuristic's run and of the always-widen
-    run; for the first few decisions d of each, replays the solve with the
-    recorded choices before d, d forced to widen or to hold, and the
-    heuristic after d.  Returns (X rows x 12, y, summary)."""
+    run; for decisions first_decision .. first_decision + count - 1 of each,
+    replays the solve with the recorded choices before d, d forced to widen
+    or to hold, and the heuristic after d.  Returns (X rows x 12, y,
+    summary)."""
     _solve(M, b, lam, tolerance, mode)  # warm
     X, y, out = [], [], {"heuristic_path": [], "widen_path": []}
     for path, policy, limit in (("heuristic_path", "record", heuristic_decisions),
@@ -261,7 +273,7 @@ def counterfactual_labels(M, b, lam, tolerance, mode, *, heuristic_decisions=3, 
         if rec is None or rec.shape[0] == 0:
             continue
         chosen = rec[:, N_IN].astype(np.int8) if policy == "record" else np.ones(rec.shape[0], np.int8)
-        for d in range(min(limit, rec.shape[0])):
+        for d in range(first_decision, min(first_decision + limit, rec.shape[0])):
             times = []
             for c in (1, 0):
                 script = np.concatenate([chosen[:d], [c]]).astype(np.int8)
@@ -280,19 +292,27 @@ def counterfactual_labels(M, b, lam, tolerance, mode, *, heuristic_decisions=3, 
     return np.array(X, dtype=np.float32).reshape(-1, N_IN), np.array(y, dtype=np.float32), out
 
 
-def collect(indices, *, seed: int = 11, tolerance: float = 1e-8, budget_seconds: float = math.inf, log=print):
+def collect(indices, *, seed: int = 11, tolerance: float = 1e-8, budget_seconds: float = math.inf, log=print,
+            kinds=None, first_decision: int = 0, heuristic_decisions: int = 3, widen_decisions: int = 1):
     """Counterfactually labelled decision records for the given problem
-    indices (stops starting new problems once `budget_seconds` have passed)."""
+    indices (stops starting new problems once `budget_seconds` have passed);
+    `kinds` restricts them to problem kinds (problem_kind), and the decision
+    window is first_decision .. + heuristic_decisions on the heuristic's
+    path (+ widen_decisions on the always-widen path)."""
     X, y, info = [], [], []
     t_start = time.perf_counter()
     for i in indices:
+        if kinds is not None and problem_kind(i, seed) not in kinds:
+            continue
         if time.perf_counter() - t_start > budget_seconds:
             log(f"  collection budget of {budget_seconds:.0f}s reached after {len(info)} problems")
             break
         A, b, lam, desc = training_problem(i, seed)
         M = _matrix(A)
         bd = b.cuda()
-        Xi, yi, summary = counterfactual_labels(M, bd, lam, tolerance, _mode(desc))
+        Xi, yi, summary = counterfactual_labels(M, bd, lam, tolerance, _mode(desc), first_decision=first_decision,
+                                                heuristic_decisions=heuristic_decisions,
+                                                widen_decisions=widen_decisions)
         X.extend(Xi)
         y.extend(yi)
         info.append({"index": int(i), "problem": desc, "labelled": int(len(yi)), **summary})
@@ -434,6 +454,10 @@ def main(argv=None) -> int:
     t.add_argument("--c2", action="store_true", help="also evaluate the 1e6 x 2000 bf16 dense case (mixed)")
     t.add_argument("--first", type=int, default=0, help="first training problem index")
     t.add_argument("--collect-only", action="store_true", help="only collect decision records (to --data)")
+    t.add_argument("--kinds", default=None, help="comma-separated problem kinds to collect (csr, dense, dense-bf16)")
+    t.add_argument("--first-decision", type=int, default=0, help="first labelled decision of each path")
+    t.add_argument("--decisions", type=int, default=3, help="labelled decisions on the heuristic's path")
+    t.add_argument("--widen-decisions", type=int, default=1, help="labelled decisions on the always-widen path")
     t.add_argument("--from-data", type=Path, nargs="*", default=None,
                    help="train on these collected record files instead of collecting")
     e = sub.add_parser("evaluate")
@@ -455,7 +479,10 @@ def main(argv=None) -> int:
         else:
             print(f"collecting problems {a.first}..{a.first + a.problems - 1}", flush=True)
             X, y, info = collect(range(a.first, a.first + a.problems), budget_seconds=a.budget_seconds,
-                                 log=lambda s: print(s, flush=True))
+                                 log=lambda s: print(s, flush=True),
+                                 kinds=None if a.kinds is None else set(a.kinds.split(",")),
+                                 first_decision=a.first_decision, heuristic_decisions=a.decisions,
+                                 widen_decisions=a.widen_decisions)
         print(f"{X.shape[0]} labelled decisions from {len(info)} problems, {y.mean():.3f} labelled widen", flush=True)
         if a.data:
             a.data.parent.mkdir(parents=True, exist_ok=True)

@@ -106,6 +106,7 @@ SIGNATURES = {
     "amfls_result_trace_extension": (c_int, [c_void_p, c_void_p]),
     "amfls_result_refinement_steps": (c_int, [c_void_p, POINTER(c_int32)]),
     "amfls_result_forward_error_estimate": (c_int, [c_void_p, POINTER(c_double)]),
+    "amfls_result_fp64_fallback": (c_int, [c_void_p, POINTER(c_int32)]),
     "amfls_result_destroy": (c_int, [c_void_p]),
     "amfls_gaussian_fill": (c_int, [c_void_p, c_void_p, c_int64, c_int32, c_int64, c_int, c_uint64, c_uint64,
                                     c_uint64]),

@@ -105,6 +105,7 @@ class SolveResult:
     refinement_steps: int = 0
     # mixed precision: forward-error estimate ||P P^T g|| / ||x|| (NaN if none)
     forward_error_estimate: float = float("nan")
+    fp64_fallback: bool = False
 
     @property
     def success(self) -> bool:
@@ -462,17 +463,20 @@ def _result_tuple(ctx: Context, res_handle: ctypes.c_void_p):
         st, sr = ctypes.c_int64(), ctypes.c_int64()
         steps = ctypes.c_int32()
         fwd = ctypes.c_double()
+        fell = ctypes.c_int32()
         ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
         _lib.check(lib.amfls_result_copy(res_handle, ptr(sol), ptr(met), ptr(cnt), ptr(tim), ptr(tf), ptr(ti),
                                          ctypes.byref(st), ctypes.byref(sr)))
         _lib.check(lib.amfls_result_trace_extension(res_handle, ptr(ext)))
         _lib.check(lib.amfls_result_refinement_steps(res_handle, ctypes.byref(steps)))
         _lib.check(lib.amfls_result_forward_error_estimate(res_handle, ctypes.byref(fwd)))
+        _lib.check(lib.amfls_result_fp64_fallback(res_handle, ctypes.byref(fell)))
     finally:
         lib.amfls_result_destroy(res_handle)
     raw = (torch.from_numpy(sol), torch.from_numpy(met), torch.from_numpy(cnt), torch.from_numpy(tim),
            torch.from_numpy(tf), torch.from_numpy(ti), int(st.value), int(sr.value))
-    extra = {"refinement_steps": int(steps.value), "forward_error_estimate": float(fwd.value)}
+    extra = {"refinement_steps": int(steps.value), "forward_error_estimate": float(fwd.value),
+             "fp64_fallback": bool(fell.value)}
     return raw, torch.from_numpy(ext), extra
 
 

@@ -685,6 +685,13 @@ int amfls_result_refinement_steps(const amfls_result* res, int32_t* steps) {
     });
 }
 
+int amfls_result_fp64_fallback(const amfls_result* res, int32_t* fell_back) {
+    return guard([&] {
+        require(res && fell_back, "null argument");
+        *fell_back = res->r.fp64_fallback ? 1 : 0;
+    });
+}
+
 int amfls_result_forward_error_estimate(const amfls_result* res, double* estimate) {
     return guard([&] {
         require(res && estimate, "null argument");

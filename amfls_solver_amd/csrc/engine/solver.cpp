@@ -468,6 +468,35 @@ LeastSquaresResult mixed_driver(const DeviceOperator& op, const double* b_dev, c
     std::vector<IterationRecord> trace = search.trace;
     RefineReport rep;
     LeastSquaresResult r = refine_preconditioned(op, b_dev, o, ctx, stats, basis, x.get(), trace, rep);
+    if (forward && r.status != SolverStatus::success) {
+        // The search's factorization did not precondition the problem well
+        // enough for the refinement to reach the target within its budget (a
+        // search stopped early by a rank truncation leaves part of R^n to
+        // LSQR): the FP64 engine solves it instead.  Its result is returned,
+        // with the trace and the product counters of both runs.
+        SolveOptions fo = o;
+        fo.precision = Precision::fp64;
+        LeastSquaresResult f = amfls_driver<double>(op, b_dev, fo, ctx, policy);
+        RunStatistics fs = f.statistics;
+        fs.a_columns += stats.a_columns;
+        fs.at_columns += stats.at_columns;
+        fs.a_block_calls += stats.a_block_calls;
+        fs.at_block_calls += stats.at_block_calls;
+        fs.a_seconds += stats.a_seconds;
+        fs.at_seconds += stats.at_seconds;
+        fs.iterative_a_columns += stats.iterative_a_columns;
+        fs.iterative_at_columns += stats.iterative_at_columns;
+        fs.iterative_a_block_calls += stats.iterative_a_block_calls;
+        fs.iterative_at_block_calls += stats.iterative_at_block_calls;
+        trace.insert(trace.end(), f.trace.begin(), f.trace.end());
+        f.trace = std::move(trace);
+        f.refinement_steps = rep.iterations;
+        f.iterations += search.iterations + rep.iterations;
+        fs.total_seconds = clk.now() - total_start;
+        f.statistics = fs;
+        f.fp64_fallback = true;
+        return f;
+    }
     if (search.status == SolverStatus::success && r.status != SolverStatus::success && !forward) r = search;
     r.depth = search.depth;
     r.auxiliary_width = search.auxiliary_width;

@@ -114,6 +114,9 @@ struct LeastSquaresResult {
     // returned solution (P the preconditioner from the FP32 search, g the
     // FP64 gradient); NaN where no estimate was made.
     int refinement_steps = 0;
+    // mixed precision with a forward-error target: the refinement missed the
+    // target and the FP64 engine produced this result
+    bool fp64_fallback = false;
     double forward_error_estimate = kNaN;
 };
 

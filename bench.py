@@ -5,16 +5,19 @@ Headline workload: the 1,000,000 x 2000 dense least-squares problem A = G D
 (G i.i.d. N(0,1) from the counter-addressed Gaussian stream, D log-spaced
 from 1 to 1e-6, A stored as bfloat16, b = A x* + 1e-8-relative noise,
 lambda = 0).  A step is one complete solve with A and b already resident in
-HBM.  `value` is the seconds per solve of mixed-precision AMFLS (FP32 block
-Golub-Kahan search on the split-bf16 matrix cores, FP64 LSQR refinement
-preconditioned by the search's factorization, FP64 certificate) run until its
-forward-error estimate is at most 1e-9; every timed solution's forward
-relative error against the FP64 oracle solution (tests/golden/c2_solution.npz,
-TSQR of the stored matrix) must be at most 1e-8, or the line reports
-"valid": false.  Beside it: plain FP64 AMFLS on the same workload (its time
-to the same forward error) and the ratio; the learned width policy; the
-product kernels' achieved HBM bandwidth in the timed solves (HIP events on
-the engine stream); the 200000 x 20000 CSR solve (lambda 1e-3) with its SpMM
+HBM.  `value` is the seconds per solve of mixed-precision AMFLS with the
+learned width controller (FP32 block Golub-Kahan search on the split-bf16
+matrix cores whose widening decisions the shipped MLP takes, FP64 LSQR
+refinement preconditioned by the search's factorization, FP64 certificate)
+run until its forward-error estimate is at most 1e-9; every timed solution's
+forward relative error against the FP64 oracle solution
+(tests/golden/c2_solution.npz, TSQR of the stored matrix) must be at most
+1e-8, with no fallback to the FP64 engine, or the line reports "valid":
+false.  Beside it: the same mixed engine with the reference's heuristic
+width controller, plain FP64 AMFLS on the same workload (its time to the
+same forward error) and the ratios; the product kernels' achieved HBM
+bandwidth in the timed solves (HIP events on the engine stream); the 200000
+x 20000 CSR solve (lambda 1e-3) with its SpMM
 bandwidth; an SpMM sweep beyond the 256 MB last-level cache; the batched CSR
 configuration; and the FP64 CPU restatement (oracle) timed on a sample of the
 dense workload on this host's cores.
@@ -36,19 +39,24 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def measured_traffic():
-    """HBM bytes per launch of the headline SpMM kernels from the committed
-    PMC summary (FETCH_SIZE and WRITE_SIZE passes scaled by a calibration
-    copy), or None when the summary is absent."""
-    path = os.path.join(ROOT, "profiles", "csr_spmm_c1_pmc_traffic.txt")
+def measured_traffic(name="csr_spmm_c1_pmc_traffic.txt", kernel=None):
+    """HBM bytes per launch from a committed PMC summary under profiles/
+    (FETCH_SIZE and WRITE_SIZE passes scaled by a calibration copy): the
+    "traffic_bytes_per_launch:" line, or with `kernel` the
+    "traffic_bytes_per_launch[<kernel>]:" line; None when absent."""
+    path = os.path.join(ROOT, "profiles", name)
+    key = "traffic_bytes_per_launch:" if kernel is None else f"traffic_bytes_per_launch[{kernel}]:"
     try:
         with open(path) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
+                if line.startswith(key):
                     return float(line.split(":", 1)[1])
     except OSError:
         return None
     return None
+
+
+DENSE_PMC_PROFILE = "dense_c2_split3_pmc_traffic.txt"
 
 
 def log(*a):
@@ -159,7 +167,13 @@ def c2_headline(torch, np, am, P, steps, warmup, forward_target=1e-8):
     same = fx is not None and bool(np.array_equal(bf16_checksums(torch, np, a), fx["bits_checksums"]))
     x_ref = fx["x_ls"] if same else None
     M = am.DeviceMatrix(a, value_dtype=torch.bfloat16)
+    # the shipped learned width controller, attached to the operator's context
+    from amfls_solver_amd.width_model import WidthModel
+    wm = WidthModel.load(context=M.context)
+    wm.attach()
     engines = {
+        "mixed_learned": dict(precision="mixed", forward_tolerance=0.1 * forward_target, width_policy=1,
+                              context=M.context),
         "mixed": dict(precision="mixed", forward_tolerance=0.1 * forward_target),
         "fp64": dict(precision="fp64"),
     }
@@ -178,6 +192,7 @@ def c2_headline(torch, np, am, P, steps, warmup, forward_target=1e-8):
               else float("nan") for r in res]
         r = res[-1]
         out[name] = {"seconds": dt, "status": [int(q.status) for q in res], "forward_error": fe,
+                     "fp64_fallback": [bool(q.fp64_fallback) for q in res],
                      "forward_error_max": max(fe), "forward_error_estimate": r.forward_error_estimate,
                      "depth": r.depth, "auxiliary_width": r.auxiliary_width, "basis_rank": r.basis_rank,
                      "refinement_iterations": r.refinement_steps,
@@ -187,13 +202,18 @@ def c2_headline(torch, np, am, P, steps, warmup, forward_target=1e-8):
                      "operator_seconds": r.a_seconds + r.at_seconds}
         log(f"C2 {name}: {dt:.4f} s/solve status {out[name]['status']} forward error max {max(fe):.3e} "
             f"depth {r.depth} width {r.auxiliary_width + 1} refine {r.refinement_steps}")
-    # the same mixed solves again with every product's kernel timed by HIP
+    d0 = wm.decisions()
+    am.run_amfls(M, b, tolerance=1e-8, **engines["mixed_learned"])
+    d1 = wm.decisions()
+    out["mixed_learned"]["model_decisions"] = d1[0] - d0[0]
+    out["mixed_learned"]["heuristic_fallbacks"] = d1[1] - d0[1]
+    # the same headline solves again with every product's kernel timed by HIP
     # events on the engine stream (kept out of the headline timing)
     M.profile(True)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     for _ in range(steps):
-        am.run_amfls(M, b, tolerance=1e-8, **engines["mixed"])
+        am.run_amfls(M, b, tolerance=1e-8, **engines["mixed_learned"])
     torch.cuda.synchronize()
     out["profiled_seconds_per_solve"] = (time.perf_counter() - t1) / steps
     # per-class view: each (direction, width range, block type) class runs one
@@ -211,18 +231,6 @@ def c2_headline(torch, np, am, P, steps, warmup, forward_target=1e-8):
     M.profile(False)
     classes.sort(key=lambda c: -c["seconds"])
     out["kernel"] = {"launches": n, "seconds": sec, "bytes": by, "classes": classes}
-    # learned width policy (shipped model) on the mixed engine
-    from amfls_solver_amd.width_model import WidthModel
-    wm = WidthModel.load(context=M.context)
-    wm.attach()
-    am.run_amfls(M, b, tolerance=1e-8, width_policy=1, context=M.context, **engines["mixed"])
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    rl = am.run_amfls(M, b, tolerance=1e-8, width_policy=1, context=M.context, **engines["mixed"])
-    torch.cuda.synchronize()
-    fel = float(np.linalg.norm(rl.solution.numpy() - x_ref) / np.linalg.norm(x_ref)) if x_ref is not None else None
-    out["learned_mixed"] = {"seconds": time.perf_counter() - t0, "status": int(rl.status), "forward_error": fel,
-                            "depth": rl.depth, "auxiliary_width": rl.auxiliary_width}
     out["products"] = dense_products(torch, M, rows, cols)
     del M, a, b
     torch.cuda.empty_cache()
@@ -444,9 +452,11 @@ def main() -> int:
 
     log("dense 1000000x2000 bf16 workload")
     c2 = c2_headline(torch, np, am, P, a.steps, a.warmup)
-    mixed, fp64 = c2["mixed"], c2["fp64"]
+    # the headline engine is mixed precision with the learned width controller
+    mixed, heur, fp64 = c2["mixed_learned"], c2["mixed"], c2["fp64"]
     valid = (c2["matrix_matches_fixture"] and mixed["forward_error_max"] <= 1e-8 and
-             fp64["forward_error_max"] <= 1e-8 and all(st == 0 for st in mixed["status"]))
+             fp64["forward_error_max"] <= 1e-8 and all(st == 0 for st in mixed["status"]) and
+             not any(mixed["fp64_fallback"]))
     top = c2["kernel"]["classes"][0]
     kn, ksec, kby = top["launches"], top["seconds"], top["bytes"]
     achieved = kby / ksec / 1e9 if ksec > 0 else 0.0
@@ -483,8 +493,8 @@ def main() -> int:
         "valid": valid,
         "config": {
             "workload": "dense_1000000x2000_bf16_GD_6decades_lambda0_forward_rel_err_1e-8",
-            "engine": "mixed: FP32 block Golub-Kahan search (split-bf16 matrix cores) + FP64 LSQR refinement "
-                      "preconditioned by its factorization, FP64 certificate",
+            "engine": "mixed/learned: FP32 block Golub-Kahan search (split-bf16 matrix cores) with the learned "
+                      "width controller, FP64 LSQR refinement preconditioned by its factorization, FP64 certificate",
             "forward_error_max": mixed["forward_error_max"],
             "forward_error_estimate": mixed["forward_error_estimate"],
             "status": mixed["status"],
@@ -494,8 +504,12 @@ def main() -> int:
             "fp64_seconds": fp64["seconds"],
             "fp64_forward_error_max": fp64["forward_error_max"],
             "fp64_status": fp64["status"],
-            "speedup_mixed_vs_fp64": fp64["seconds"] / mixed["seconds"],
-            "learned_mixed": c2["learned_mixed"],
+            "speedup_mixed_learned_vs_fp64": fp64["seconds"] / mixed["seconds"],
+            "model_decisions_per_solve": mixed["model_decisions"],
+            "mixed_heuristic_seconds": heur["seconds"],
+            "mixed_heuristic_forward_error_max": heur["forward_error_max"],
+            "mixed_heuristic_status": heur["status"],
+            "speedup_mixed_heuristic_vs_fp64": fp64["seconds"] / heur["seconds"],
             "matrix_matches_oracle_fixture": c2["matrix_matches_fixture"],
             "parallelism": "single_gpu",
         },
@@ -505,7 +519,10 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(DENSE_PMC_PROFILE, dense_kernel_name(top)),
+            "traffic_source": f"profiles/{DENSE_PMC_PROFILE} (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE of the same "
+                              f"kernel at its widest width in the class, scaled by a 1 GiB calibration copy; "
+                              f"committed, not measured in this run)",
             "kernel": dense_kernel_name(top),
             "product_class": {k: top[k] for k in ("transpose", "widths", "block")},
             "bytes_per_launch": kby / max(kn, 1),

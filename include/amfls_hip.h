@@ -280,6 +280,11 @@ int amfls_result_trace_extension(const amfls_result* result, double* out);
 int amfls_result_refinement_steps(const amfls_result* result, int32_t* steps);
 /* Forward-error estimate of a mixed-precision result (NaN where none). */
 int amfls_result_forward_error_estimate(const amfls_result* result, double* estimate);
+/* 1 when a mixed-precision solve with a forward-error target missed the target
+ * in its FP64 refinement and the returned result is the FP64 engine's (the
+ * trace and product counters then cover both runs), else 0.  No counterpart
+ * in the reference, which has no mixed-precision path. */
+int amfls_result_fp64_fallback(const amfls_result* result, int32_t* fell_back);
 int amfls_result_destroy(amfls_result* result);
 
 /* Learned width controller (engine extension; it replaces the reference's

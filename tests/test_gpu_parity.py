@@ -214,6 +214,33 @@ def test_mixed_precision_reaches_fp64_tolerance(torch_mod, lam):
     assert rel(r.solution.numpy(), ro.solution) < 1e-6
 
 
+def test_mixed_forward_target_falls_back_to_fp64(torch_mod):
+    """A mixed solve whose FP64 refinement cannot reach its forward-error
+    target within its iteration budget hands the problem to the FP64 engine:
+    the result is certified, flagged, and agrees with a dense least-squares
+    solve; with room to refine, the same problem reaches the target on the
+    mixed path itself (no fallback)."""
+    import amfls_solver_amd as am
+    from amfls_solver_amd import problems as P
+    torch = torch_mod
+    a, b, _ = P.dense_gd(4000, 300, 4.0, seed=12, noise=1e-8, relative_noise=True)
+    A = torch.from_numpy(np.ascontiguousarray(a))
+    x_ls = np.linalg.lstsq(a, b, rcond=None)[0]
+    kw = dict(tolerance=1e-8, precision="mixed")
+    ok = am.run_amfls(A, torch.from_numpy(b), forward_tolerance=1e-9, **kw)
+    assert ok.success and not ok.fp64_fallback
+    assert ok.forward_error_estimate <= 1e-9
+    assert rel(ok.solution.numpy(), x_ls) < 1e-8
+    # a forward target no refinement reaches in two LSQR iterations
+    fb = am.run_amfls(A, torch.from_numpy(b), forward_tolerance=1e-30, maximum_refinement_iterations=2, **kw)
+    assert fb.fp64_fallback and fb.success
+    assert fb.backward_error_upper_bound <= 1e-8
+    assert fb.refinement_steps == 2
+    # certified backward error 1e-8 on a matrix of condition 1e4
+    assert rel(fb.solution.numpy(), x_ls) < 1e-6
+    assert fb.trace_float.shape[0] > ok.trace_float.shape[0]  # both runs' checkpoints
+
+
 def test_edge_cases(torch_mod):
     import amfls_solver_amd as am
     from amfls_solver_amd import problems as P

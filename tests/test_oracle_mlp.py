@@ -46,3 +46,27 @@ def test_loss_and_forward_against_per_sample_loop():
         tot += -(y[i] * math.log(s) + (1 - y[i]) * math.log(1 - s))
     assert abs(tot / X.shape[0] - M.loss(p, norm, X, y)) <= 1e-12
     assert M.N_PARAMS == 5057 and M.init_params(0).shape == (5057,)
+
+
+def test_shipped_width_model_reproduces_its_training_accuracy():
+    """The shipped width controller (trained on the GPU by the engine's own
+    kernels) evaluated by the float64 oracle forward pass on its committed
+    training decisions: the accuracy recorded at training time, the logit
+    sign per decision agreeing with the labels as often as the model claimed,
+    and a training box that contains every training input."""
+    import json
+    from pathlib import Path
+    models = Path(__file__).resolve().parents[1] / "amfls_solver_amd" / "models"
+    with np.load(models / "width_mlp.npz", allow_pickle=False) as z:
+        p, norm = z["params"].astype(np.float64), z["norm"].astype(np.float64)
+        meta = json.loads(str(z["meta"]))
+        lo, hi = z["domain_lo"], z["domain_hi"]
+    with np.load(models / "width_training_decisions.npz", allow_pickle=False) as z:
+        X, y = z["X"].astype(np.float64), z["y"].astype(np.float64)
+    assert X.shape == (meta["decisions"], M.N_IN)
+    assert abs(float(y.mean()) - meta["widen_fraction"]) < 1e-6
+    logits, _ = M.forward(p, norm, X)
+    acc = float(np.mean((logits > 0) == (y > 0.5)))
+    # the GPU forward pass is FP32: a handful of logits within rounding of 0 may flip
+    assert abs(acc - meta["train_accuracy"]) <= 2.0 / len(y), (acc, meta["train_accuracy"])
+    assert np.all(X >= lo) and np.all(X <= hi)

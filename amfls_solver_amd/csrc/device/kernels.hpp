@@ -228,12 +228,16 @@ struct CsrView {
     // boundaries, device), or null.
     const int64_t* stream_rows = nullptr;
     int64_t stream_blocks = 0;
+    // the same for the wide-budget stream kernel (8192 entries a block)
+    const int64_t* stream_rows_wide = nullptr;
+    int64_t stream_blocks_wide = 0;
 };
 
 // Row boundaries (host) of the stream kernel's partition: consecutive rows
-// whose entries fit one block's budget (csr_stream_budget()), at most 1024
-// rows per block; a longer row gets a block of its own.
-std::vector<int64_t> csr_stream_partition(const std::vector<int64_t>& row_ptr);
+// whose entries fit one block's budget (csr_stream_budget(), or 8192 for the
+// wide kernel), at most 1024 (4096) rows per block; a longer row gets a
+// block of its own.
+std::vector<int64_t> csr_stream_partition(const std::vector<int64_t>& row_ptr, bool wide = false);
 int csr_stream_budget();
 // Index of the width-one stream variant (CSR-Stream: entries streamed into
 // LDS with coalesced loads, then summed per row).

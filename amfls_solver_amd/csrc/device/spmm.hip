@@ -135,15 +135,21 @@ __global__ __launch_bounds__(kThreads) void csr_spmm_kernel(
 
 // Width-one products by CSR-Stream (J. L. Greathouse and M. Daga, "Efficient
 // sparse matrix-vector multiplication on GPUs using the CSR storage format",
-// SC14): a block owns consecutive rows holding at most kStreamBudget entries;
-// its threads stream those entries with coalesced loads, multiply by the
+// SC14): a block owns consecutive rows holding at most B entries; its
+// threads stream those entries with coalesced loads, multiply by the
 // gathered x and park the products in LDS; then a thread per row sums its
 // products in entry order.  A row longer than the budget gets a block that
-// reduces it with all threads.
+// reduces it with all threads.  Two budgets: 2048 entries (8 per thread) and
+// 8192 (32 per thread, 4096 rows at most).  A block's life is a fixed chain
+// of dependent memory trips (partition, row pointers, entries, gathers,
+// sums), so on a matrix of millions of rows the wide budget does four times
+// the work per chain with a quarter of the blocks.
 constexpr int kStreamBudget = 2048;
 constexpr int kStreamMaxRows = 1024;
+constexpr int kStreamWideBudget = 8192;
+constexpr int kStreamWideMaxRows = 4096;
 
-template <typename V, typename T>
+template <typename V, typename T, int B>
 __device__ __forceinline__ void stream_block(const int64_t* __restrict__ rp, const int32_t* __restrict__ ci,
                                              const V* __restrict__ val, const int64_t* __restrict__ br,
                                              const T* __restrict__ X, int64_t ldx, T* __restrict__ Y, int64_t ldy,
@@ -152,10 +158,10 @@ __device__ __forceinline__ void stream_block(const int64_t* __restrict__ rp, con
     const int t = threadIdx.x;
     const int64_t r0 = br[blockIdx.x], r1 = br[blockIdx.x + 1];
     const int64_t e0 = rp[r0], e1 = rp[r1];
-    if (e1 - e0 <= kStreamBudget) {
+    if (e1 - e0 <= B) {
         // all of a thread's entries in two waves of independent loads: the
         // column indices and values, then the gathered x
-        constexpr int U = kStreamBudget / kThreads;
+        constexpr int U = B / kThreads;
         int32_t c[U];
         T v[U], xv[U];
 #pragma unroll
@@ -202,15 +208,15 @@ __device__ __forceinline__ void stream_block(const int64_t* __restrict__ rp, con
     }
 }
 
-template <typename V, typename T>
+template <typename V, typename T, int B>
 __global__ __launch_bounds__(kThreads) void csr_stream_kernel(
     const int64_t* __restrict__ rp, const int32_t* __restrict__ ci, const V* __restrict__ val,
     const int64_t* __restrict__ br, const T* __restrict__ X, int64_t ldx, T* __restrict__ Y, int64_t ldy,
     const T* __restrict__ Add, int64_t ld_add, T add_scale, ProductStamp stamp) {
-    __shared__ T prod[kStreamBudget];
+    __shared__ T prod[B];
     __shared__ T red[kThreads / kWave];
     stamp_begin(stamp);
-    stream_block<V, T>(rp, ci, val, br, X, ldx, Y, ldy, Add, ld_add, add_scale, prod, red);
+    stream_block<V, T, B>(rp, ci, val, br, X, ldx, Y, ldy, Add, ld_add, add_scale, prod, red);
     stamp_end(stamp);
 }
 
@@ -272,29 +278,31 @@ void launch_variant(hipStream_t s, const CsrView& A, int ncols, const T* X, int6
     AMFLS_LAUNCH_CHECK();
 }
 
-template <typename V, typename T>
+template <typename V, typename T, int B>
 void launch_stream_typed(hipStream_t s, const CsrView& A, const T* X, int64_t ldx, T* Y, int64_t ldy,
                          const T* Add, int64_t ld_add, double add_scale, hipEvent_t e0, hipEvent_t e1,
                          ProductStamp st) {
-    const dim3 grid(static_cast<unsigned>(A.stream_blocks));
+    const bool wide = B == kStreamWideBudget;
+    const int64_t* br = wide ? A.stream_rows_wide : A.stream_rows;
+    const dim3 grid(static_cast<unsigned>(wide ? A.stream_blocks_wide : A.stream_blocks));
     if (e0 || e1)
-        hipExtLaunchKernelGGL((csr_stream_kernel<V, T>), grid, dim3(kThreads), 0, s, e0, e1, 0, A.row_ptr, A.col_idx,
-                              static_cast<const V*>(A.values), A.stream_rows, X, ldx, Y, ldy, Add, ld_add,
+        hipExtLaunchKernelGGL((csr_stream_kernel<V, T, B>), grid, dim3(kThreads), 0, s, e0, e1, 0, A.row_ptr,
+                              A.col_idx, static_cast<const V*>(A.values), br, X, ldx, Y, ldy, Add, ld_add,
                               static_cast<T>(add_scale), st);
     else
-        hipLaunchKernelGGL((csr_stream_kernel<V, T>), grid, dim3(kThreads), 0, s, A.row_ptr, A.col_idx,
-                           static_cast<const V*>(A.values), A.stream_rows, X, ldx, Y, ldy, Add, ld_add,
+        hipLaunchKernelGGL((csr_stream_kernel<V, T, B>), grid, dim3(kThreads), 0, s, A.row_ptr, A.col_idx,
+                           static_cast<const V*>(A.values), br, X, ldx, Y, ldy, Add, ld_add,
                            static_cast<T>(add_scale), st);
     AMFLS_LAUNCH_CHECK();
 }
 
-template <typename T>
+template <typename T, int B>
 void launch_stream(hipStream_t s, const CsrView& A, const T* X, int64_t ldx, T* Y, int64_t ldy, const T* Add,
                    int64_t ld_add, double add_scale, hipEvent_t e0, hipEvent_t e1, ProductStamp st) {
     if (A.type == ValueType::f32)
-        launch_stream_typed<float, T>(s, A, X, ldx, Y, ldy, Add, ld_add, add_scale, e0, e1, st);
+        launch_stream_typed<float, T, B>(s, A, X, ldx, Y, ldy, Add, ld_add, add_scale, e0, e1, st);
     else if (A.type == ValueType::f64)
-        launch_stream_typed<double, T>(s, A, X, ldx, Y, ldy, Add, ld_add, add_scale, e0, e1, st);
+        launch_stream_typed<double, T, B>(s, A, X, ldx, Y, ldy, Add, ld_add, add_scale, e0, e1, st);
     else
         throw std::invalid_argument("CSR values must be float32 or float64");
 }
@@ -436,19 +444,22 @@ __global__ void csr_validate_kernel(int64_t rows, int64_t cols, int64_t nnz, con
 
 // The stream kernel follows the table's variants.
 constexpr int kStreamVariant = kVariantCount;
+constexpr int kStreamWideVariant = kVariantCount + 1;
 
-int csr_spmm_variant_count() { return kVariantCount + 1; }
+int csr_spmm_variant_count() { return kVariantCount + 2; }
 int csr_spmm_stream_variant() { return kStreamVariant; }
 int csr_stream_budget() { return kStreamBudget; }
 
-std::vector<int64_t> csr_stream_partition(const std::vector<int64_t>& rp) {
+std::vector<int64_t> csr_stream_partition(const std::vector<int64_t>& rp, bool wide) {
     const int64_t rows = static_cast<int64_t>(rp.size()) - 1;
+    const int64_t budget = wide ? kStreamWideBudget : kStreamBudget;
+    const int64_t max_rows = wide ? kStreamWideMaxRows : kStreamMaxRows;
     std::vector<int64_t> br{0};
     int64_t r = 0;
     while (r < rows) {
         const int64_t e0 = rp[r];
         int64_t end = r + 1;
-        while (end < rows && rp[end + 1] - e0 <= kStreamBudget && end - r < kStreamMaxRows) ++end;
+        while (end < rows && rp[end + 1] - e0 <= budget && end - r < max_rows) ++end;
         br.push_back(end);
         r = end;
     }
@@ -456,18 +467,19 @@ std::vector<int64_t> csr_stream_partition(const std::vector<int64_t>& rp) {
 }
 
 bool csr_spmm_variant_supports(int variant, int ncols) {
-    if (variant == kStreamVariant) return ncols == 1;
+    if (variant == kStreamVariant || variant == kStreamWideVariant) return ncols == 1;
     if (variant < 0 || variant >= kVariantCount) return false;
     return ncols <= kVariants[variant].lc * kVariants[variant].cpl;
 }
 
 const char* csr_spmm_variant_name(int variant) {
     if (variant == kStreamVariant) return "stream";
+    if (variant == kStreamWideVariant) return "stream8k";
     return (variant >= 0 && variant < kVariantCount) ? kVariants[variant].name : "invalid";
 }
 
 bool csr_spmm_variant_shape(int variant, int* p, int* lc, int* cpl) {
-    if (variant == kStreamVariant) {
+    if (variant == kStreamVariant || variant == kStreamWideVariant) {
         *p = 0;
         *lc = 1;
         *cpl = 1;
@@ -516,7 +528,9 @@ void csr_spmm(hipStream_t s, const CsrView& A, int ncols, const T* X, int64_t ld
     for (int c0 = 0; c0 < ncols; c0 += 256) {
         const int nc = std::min(256, ncols - c0);
         int v = variant;
-        if (v == kStreamVariant && A.stream_rows == nullptr) v = -1;
+        if ((v == kStreamVariant && A.stream_rows == nullptr) ||
+            (v == kStreamWideVariant && A.stream_rows_wide == nullptr))
+            v = -1;
         if (v < 0 || !csr_spmm_variant_supports(v, nc))
             v = csr_spmm_default_variant(nc, A.rows > 0 ? static_cast<double>(A.nnz) / A.rows : 0.0);
         const T* add = Add ? Add + c0 : nullptr;
@@ -524,7 +538,11 @@ void csr_spmm(hipStream_t s, const CsrView& A, int ncols, const T* X, int64_t ld
         hipEvent_t e1 = c0 + nc == ncols ? ev_stop : nullptr;
         const ProductStamp st{stamp, c0 == 0, c0 + nc == ncols};
         if (v == kStreamVariant) {
-            launch_stream<T>(s, A, X, ldx, Y, ldy, add, ld_add, add_scale, e0, e1, st);
+            launch_stream<T, kStreamBudget>(s, A, X, ldx, Y, ldy, add, ld_add, add_scale, e0, e1, st);
+            continue;
+        }
+        if (v == kStreamWideVariant) {
+            launch_stream<T, kStreamWideBudget>(s, A, X, ldx, Y, ldy, add, ld_add, add_scale, e0, e1, st);
             continue;
         }
         switch (A.type) {

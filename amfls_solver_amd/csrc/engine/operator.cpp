@@ -173,17 +173,27 @@ CsrOperator::CsrOperator(int64_t rows, int64_t cols, int64_t nnz, const int64_t*
     ctx.download(hrp.data(), row_ptr, rows + 1);
     ctx.download(htrp.data(), t_rp_.get(), cols + 1);
     if (hm[1] > 0.0) throw std::invalid_argument("CSR values must be finite");
-    // row partitions of the width-one stream kernel, for A and A^T
+    // row partitions of the width-one stream kernels (both budgets), for A and A^T
+    std::vector<int64_t> parts[4];
     for (int side = 0; side < 2; ++side) {
-        const std::vector<int64_t> br = csr_stream_partition(side == 0 ? hrp : htrp);
-        DevBuf<int64_t>& dst = side == 0 ? stream_a_ : stream_at_;
-        dst.reset(br.size());
-        AMFLS_HIP_CHECK(hipMemcpyAsync(dst.get(), br.data(), br.size() * sizeof(int64_t), hipMemcpyHostToDevice, s));
-        CsrView& v = side == 0 ? a_ : at_;
-        v.stream_rows = dst.get();
-        v.stream_blocks = static_cast<int64_t>(br.size()) - 1;
+        for (int wide = 0; wide < 2; ++wide) {
+            std::vector<int64_t>& br = parts[2 * side + wide];
+            br = csr_stream_partition(side == 0 ? hrp : htrp, wide != 0);
+            DevBuf<int64_t>& dst = side == 0 ? (wide ? stream_a_wide_ : stream_a_) : (wide ? stream_at_wide_ : stream_at_);
+            dst.reset(br.size());
+            AMFLS_HIP_CHECK(hipMemcpyAsync(dst.get(), br.data(), br.size() * sizeof(int64_t), hipMemcpyHostToDevice,
+                                           s));
+            CsrView& v = side == 0 ? a_ : at_;
+            if (wide) {
+                v.stream_rows_wide = dst.get();
+                v.stream_blocks_wide = static_cast<int64_t>(br.size()) - 1;
+            } else {
+                v.stream_rows = dst.get();
+                v.stream_blocks = static_cast<int64_t>(br.size()) - 1;
+            }
+        }
     }
-    ctx.sync();
+    ctx.sync();  // the host partitions stay alive until their copies are done
     std::vector<long long> rops(rows), cops(cols);
     long long max_row = 0, max_col = 0;
     for (int64_t i = 0; i < rows; ++i) {

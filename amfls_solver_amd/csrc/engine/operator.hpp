@@ -170,7 +170,7 @@ private:
 
     CsrView a_, at_;
     int64_t nnz_ = 0;
-    DevBuf<int64_t> t_rp_, stream_a_, stream_at_;
+    DevBuf<int64_t> t_rp_, stream_a_, stream_at_, stream_a_wide_, stream_at_wide_;
     DevBuf<int32_t> t_ci_;
     DevBuf<char> t_val_;
     ValidationModel model_;

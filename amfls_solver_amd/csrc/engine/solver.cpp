@@ -468,12 +468,15 @@ LeastSquaresResult mixed_driver(const DeviceOperator& op, const double* b_dev, c
     std::vector<IterationRecord> trace = search.trace;
     RefineReport rep;
     LeastSquaresResult r = refine_preconditioned(op, b_dev, o, ctx, stats, basis, x.get(), trace, rep);
-    if (forward && r.status != SolverStatus::success) {
+    const bool target_met = std::isfinite(r.forward_error_estimate) && r.forward_error_estimate <= o.forward_tolerance;
+    if (forward && !target_met) {
         // The search's factorization did not precondition the problem well
-        // enough for the refinement to reach the target within its budget (a
-        // search stopped early by a rank truncation leaves part of R^n to
-        // LSQR): the FP64 engine solves it instead.  Its result is returned,
-        // with the trace and the product counters of both runs.
+        // enough for the refinement to reach the forward-error target within
+        // its budget (a search stopped early by a rank truncation leaves part
+        // of R^n to LSQR): the FP64 engine solves it instead.  Its result is
+        // returned, with the trace and the product counters of both runs.  (A
+        // met target with a certificate stuck at its rounding floor is kept:
+        // the FP64 engine stops at the same floor.)
         SolveOptions fo = o;
         fo.precision = Precision::fp64;
         LeastSquaresResult f = amfls_driver<double>(op, b_dev, fo, ctx, policy);

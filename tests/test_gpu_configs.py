@@ -13,6 +13,8 @@
   mutually consistent certificates (FP64 and mixed).
 * 256 CSR problems x 8 lambdas: every solve certified or explained, true
   energy errors within the certificates on a sample.
+* The learned width controller on the 1e6 x 2000 workload (forward error as
+  above) and on the wide 50000 x 500000 minimum-norm CSR case.
 """
 import os
 
@@ -105,22 +107,34 @@ def test_config2_matrix_is_the_fixture(torch_mod, config2):
     assert np.isclose(float(torch_mod.linalg.vector_norm(b)), float(fx["b_norm"]), rtol=1e-12)
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed-learned"])
 def test_config2_forward_error(torch_mod, config2, precision):
     """C2: the FP64 engine at tolerance 1e-8 and the mixed engine with a
-    forward-error target both land within 1e-8 of the FP64 oracle solution."""
+    forward-error target (with the reference's heuristic width controller or
+    the shipped learned one) all land within 1e-8 of the FP64 oracle
+    solution."""
     import amfls_solver_amd as am
+    from amfls_solver_amd.width_model import WidthModel
     torch = torch_mod
     a, b, fx = config2
     M = am.DeviceMatrix(a, value_dtype=torch.bfloat16)
-    kw = {"forward_tolerance": 1e-9} if precision == "mixed" else {}
-    r = am.run_amfls(M, b, tolerance=1e-8, precision=precision, **kw)
+    kw = {"forward_tolerance": 1e-9} if precision.startswith("mixed") else {}
+    model = None
+    if precision == "mixed-learned":
+        model = WidthModel.load(context=M.context)
+        model.attach()
+        kw.update(width_policy=1, context=M.context)
+    d0 = model.decisions() if model else None
+    r = am.run_amfls(M, b, tolerance=1e-8, precision=precision.split("-")[0], **kw)
     assert r.success, (r.status, r.stop_reason)
+    assert not r.fp64_fallback
+    if model is not None:
+        assert model.decisions()[0] > d0[0]  # the network took decisions on this problem
     fe = rel(r.solution.numpy(), fx["x_ls"])
     print(f"{precision}: forward error {fe:.3e} estimate {r.forward_error_estimate:.3e} depth {r.depth} "
           f"rank {r.basis_rank} refinement {r.refinement_steps}")
     assert fe <= 1e-8, fe
-    if precision == "mixed":
+    if precision.startswith("mixed"):
         assert r.forward_error_estimate <= 1e-9
         assert fe <= 10 * r.forward_error_estimate + 1e-12
         assert 0 < r.refinement_steps <= 64
@@ -247,3 +261,37 @@ def test_config3_batch_256x8(torch_mod):
         assert abs(o.relative_energy_error_upper_bound / r.relative_energy_error_upper_bound - 1) < 1e-6
         assert r.relative_energy_error_upper_bound > tol
         assert o.least_squares_backward_error_upper_bound < 1e-10             # converged to rounding level
+
+
+def test_config4_wide_case_learned(torch_mod):
+    """C4's evaluation case: the wide 50000 x 500000 minimum-norm CSR problem
+    (8 distinct uniform columns per row, N(0,1) values, lambda 0) through the
+    shipped learned width controller.  The network takes decisions itself,
+    and both controllers certify the same minimum-norm solution: A x = b to
+    the tolerance, x in the row space of A (x = A^T y), and the two solutions
+    equal to 1e-6."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    import amfls_solver_amd as am
+    from amfls_solver_amd import width_model as W
+    A, b, lam, desc = W.wide_problem()
+    model = W.WidthModel.load()
+    model.attach()
+    M = am.DeviceMatrix(A, value_dtype=torch_mod.float32, context=model.context)
+    bd = b.cuda()
+    rh = am.run_amfls(M, bd, tolerance=1e-8, width_policy=0, context=model.context)
+    d0 = model.decisions()
+    rl = am.run_amfls(M, bd, tolerance=1e-8, width_policy=1, context=model.context)
+    d1 = model.decisions()
+    assert d1[0] > d0[0], (d0, d1)
+    assert rh.success and rl.success, ((rh.status, rh.stop_reason), (rl.status, rl.stop_reason))
+    crow, col, val = A.crow_indices().numpy(), A.col_indices().numpy(), A.values().numpy()
+    As = sp.csr_matrix((val.astype(np.float64), col, crow), shape=tuple(A.shape))
+    bn = b.numpy()
+    for r in (rh, rl):
+        x = r.solution.numpy()
+        assert np.linalg.norm(As @ x - bn) <= 1e-7 * np.linalg.norm(bn)
+        # row space: x = A^T y for the y that least-squares fits it
+        y = spla.lsqr(As.T, x, atol=1e-14, btol=1e-14, iter_lim=5000)[0]
+        assert np.linalg.norm(As.T @ y - x) <= 1e-5 * np.linalg.norm(x)
+    assert rel(rl.solution.numpy(), rh.solution.numpy()) <= 1e-6

@@ -607,6 +607,9 @@ public:
             (void)hipStreamEndCapture(s_, &g);
             if (g) (void)hipGraphDestroy(g);
         }
+        // a launched graph has finished before it is destroyed (the batch's
+        // download waits for it; after an exception this wait does)
+        if (exec_) (void)hipStreamSynchronize(s_);
         if (exec_) (void)hipGraphExecDestroy(exec_);
         if (graph_) (void)hipGraphDestroy(graph_);
     }

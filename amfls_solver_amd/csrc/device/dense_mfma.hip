@@ -595,31 +595,67 @@ __global__ __launch_bounds__(kThreads, 2) void tn_split3_kernel(int64_t M, int64
         for (int j = 0; j < NT; ++j) acc[e][j] = floatx4{0, 0, 0, 0};
     uint4 ar[8];
     float yr[YPAIRS][8];
+    // Tiles of at most 32 Y columns load without per-row branches: a thread
+    // whose column group lies inside the matrix loads its eight rows from
+    // in-bounds addresses (rows past the matrix read its last row) and zeroes
+    // the rows outside the chunk with selects; a group past the edge holds
+    // zeros, and one that straddles it (N not a multiple of 8) gathers
+    // element by element.  That branch is per thread and fixed for the
+    // kernel.  (Per-row branches cost 8-11% of the A^T Y time at widths up
+    // to 32; wider Y tiles keep them, since the selects hold more registers
+    // there and took the four-tile kernel to two waves per SIMD.)
+    const bool col_full = acol + 8 <= N;
+    const bool col_part = !col_full && acol < N;
+    auto gather_row = [&](int64_t row) {
+        uint32_t w[4] = {0, 0, 0, 0};
+        if (row < r1)
+            for (int e = 0; e < 8; ++e)
+                if (acol + e < N) {
+                    const uint16_t bits = __builtin_bit_cast(uint16_t, A[row * lda + acol + e]);
+                    w[e / 2] |= static_cast<uint32_t>(bits) << (16 * (e % 2));
+                }
+        return uint4{w[0], w[1], w[2], w[3]};
+    };
     auto fetch = [&](int64_t i0) {
+        if constexpr (NT > 2) {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int64_t row = i0 + 8 * bi + u;
-            if (row < r1 && acol + 8 <= N) {
-                ar[u] = *reinterpret_cast<const uint4*>(A + row * lda + acol);
-            } else {
-                uint32_t w[4] = {0, 0, 0, 0};
-                if (row < r1)
-                    for (int e = 0; e < 8; ++e)
-                        if (acol + e < N) {
-                            const uint16_t bits = __builtin_bit_cast(uint16_t, A[row * lda + acol + e]);
-                            w[e / 2] |= static_cast<uint32_t>(bits) << (16 * (e % 2));
-                        }
-                ar[u] = uint4{w[0], w[1], w[2], w[3]};
+            for (int u = 0; u < 8; ++u) {
+                const int64_t row = i0 + 8 * bi + u;
+                ar[u] = (row < r1 && col_full) ? *reinterpret_cast<const uint4*>(A + row * lda + acol)
+                                               : gather_row(row);
             }
+        } else if (col_full) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int64_t row = i0 + 8 * bi + u;
+                const int64_t rl = row < M ? row : M - 1;
+                const uint4 v = *reinterpret_cast<const uint4*>(A + rl * lda + acol);
+                ar[u] = row < r1 ? v : uint4{0, 0, 0, 0};
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) ar[u] = col_part ? gather_row(i0 + 8 * bi + u) : uint4{0, 0, 0, 0};
         }
 #pragma unroll
         for (int pp = 0; pp < YPAIRS; ++pp) {
             const int pair = t + pp * kThreads;
             const int col = pair % SW, kg = pair / SW;
+            const bool in = kg < kTnRows / 8 && col < s;
+            if constexpr (NT > 2) {
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int64_t row = i0 + 8 * kg + u;
-                yr[pp][u] = (kg < kTnRows / 8 && col < s && row < r1) ? static_cast<float>(Yb[row * ldy + col]) : 0.0f;
+                for (int u = 0; u < 8; ++u) {
+                    const int64_t row = i0 + 8 * kg + u;
+                    yr[pp][u] = (in && row < r1) ? static_cast<float>(Yb[row * ldy + col]) : 0.0f;
+                }
+            } else {
+                const int cl = in ? col : 0;
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int64_t row = i0 + 8 * kg + u;
+                    const int64_t rl = row < M ? row : M - 1;
+                    const float v = static_cast<float>(Yb[rl * ldy + cl]);
+                    yr[pp][u] = (in && row < r1) ? v : 0.0f;
+                }
             }
         }
     };

@@ -345,10 +345,14 @@ def test_dense_matrix_core_products(torch_mod, vt):
     import amfls_solver_amd as am
     torch = torch_mod
     rng = np.random.default_rng(7)
-    for m, n in ((1000, 203), (600, 517), (37, 64), (4099, 1)):
-        a = torch.from_numpy(rng.standard_normal((m, n))).to(getattr(torch, vt))
-        stored = a.double().numpy()
-        M = am.DeviceMatrix(a.contiguous(), value_dtype=getattr(torch, vt))
+    # the last case is a view with a padded leading dimension (208), so the
+    # 16-byte-aligned A^T kernels run on a column count that is not a
+    # multiple of 8 and a last row chunk that is partial
+    for m, n, pad in ((1000, 203, 0), (600, 517, 0), (37, 64, 0), (4099, 1, 0), (1531, 203, 5)):
+        full = torch.from_numpy(rng.standard_normal((m, n + pad))).to(getattr(torch, vt))
+        a = full.cuda()[:, :n] if pad else full
+        stored = a.double().cpu().numpy()
+        M = am.DeviceMatrix(a if pad else a.contiguous(), value_dtype=getattr(torch, vt))
         for s in (1, 5, 16, 17, 40, 64, 70):
             X = rng.standard_normal((n, s))
             Y = stored @ X

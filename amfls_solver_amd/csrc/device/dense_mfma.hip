@@ -248,9 +248,10 @@ __global__ __launch_bounds__(kThreads) void dense_nn_kernel(int64_t M, int64_t N
     }
 }
 
-// Split-bf16 Y = A X for bf16 A (16-byte aligned rows): a block of eight
-// wavefronts owns 128 rows, a wavefront one 16-row tile.  Each 64-wide K
-// trip stages the block's A tile (128 rows x 128 bytes) and the trip's X
+// Split-bf16 Y = A X for bf16 A (16-byte aligned rows): a block of W
+// wavefronts owns 16 RT W rows, a wavefront RT 16-row tiles that share each
+// X fragment it reads from LDS.  Each 64-wide K
+// trip stages the block's A tile (16 RT W rows x 128 bytes) and the trip's X
 // slice (three bf16 planes x 16 NT columns x 64 entries) in LDS with
 // coalesced 16-byte loads (eight lanes read the 128 contiguous bytes of a
 // row); the global loads of trip t + 3 are in registers while trip t runs
@@ -261,12 +262,12 @@ __global__ __launch_bounds__(kThreads) void dense_nn_kernel(int64_t M, int64_t N
 // both operands.
 constexpr int kPipeWaves = 8, kPipeStages = 4;
 
-template <typename TX, int NT>
-__global__ __launch_bounds__(64 * kPipeWaves) void nn_split3_kernel(int64_t M, int64_t N, int s,
+template <typename TX, int NT, int W = kPipeWaves, int RT = 1>
+__global__ __launch_bounds__(64 * W) void nn_split3_kernel(int64_t M, int64_t N, int s,
                                                                     const __hip_bfloat16* __restrict__ A, int64_t lda,
                                                                     const __bf16* __restrict__ XT, int64_t nk,
                                                                     int64_t plane, TX* __restrict__ Y, int64_t ldy) {
-    constexpr int NTH = 64 * kPipeWaves, RB = 16 * kPipeWaves, SW = 16 * NT, S = kPipeStages;
+    constexpr int NTH = 64 * W, RB = 16 * RT * W, SW = 16 * NT, S = kPipeStages;
     constexpr int APIECES = RB * 8, XPIECES = 3 * SW * 8, PIECES = APIECES + XPIECES;
     constexpr int PPT = (PIECES + NTH - 1) / NTH;
     __shared__ __align__(16) uint4 As[2][APIECES];
@@ -275,9 +276,11 @@ __global__ __launch_bounds__(64 * kPipeWaves) void nn_split3_kernel(int64_t M, i
     const int c = lane % 16, q = lane / 16;
     const int64_t rb0 = static_cast<int64_t>(blockIdx.x) * RB;
     const int64_t trips = (N + 63) / 64;
-    floatx4 acc[NT];
+    floatx4 acc[RT][NT];
 #pragma unroll
-    for (int j = 0; j < NT; ++j) acc[j] = floatx4{0, 0, 0, 0};
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+        for (int j = 0; j < NT; ++j) acc[r][j] = floatx4{0, 0, 0, 0};
     uint4 stg[S - 1][PPT];
     auto fetch = [&](uint4 (&dst)[PPT], int64_t k0) {
 #pragma unroll
@@ -319,16 +322,22 @@ __global__ __launch_bounds__(64 * kPipeWaves) void nn_split3_kernel(int64_t M, i
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int sl = (4 * h + q) ^ (c & 7);
-            const bf16x8 av = __builtin_bit_cast(bf16x8, As[buf][(16 * wave + c) * 8 + sl]);
+            bf16x8 av[RT];
+#pragma unroll
+            for (int r = 0; r < RT; ++r)
+                av[r] = __builtin_bit_cast(bf16x8, As[buf][(16 * (RT * wave + r) + c) * 8 + sl]);
 #pragma unroll
             for (int j = 0; j < NT; ++j) {
                 const int col = 16 * j + c;
                 const bf16x8 xh = __builtin_bit_cast(bf16x8, Xs[buf][(0 * SW + col) * 8 + sl]);
                 const bf16x8 xm = __builtin_bit_cast(bf16x8, Xs[buf][(1 * SW + col) * 8 + sl]);
                 const bf16x8 xl = __builtin_bit_cast(bf16x8, Xs[buf][(2 * SW + col) * 8 + sl]);
-                acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, xl, acc[j], 0, 0, 0);
-                acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, xm, acc[j], 0, 0, 0);
-                acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, xh, acc[j], 0, 0, 0);
+#pragma unroll
+                for (int r = 0; r < RT; ++r) {
+                    acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[r], xl, acc[r][j], 0, 0, 0);
+                    acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[r], xm, acc[r][j], 0, 0, 0);
+                    acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[r], xh, acc[r][j], 0, 0, 0);
+                }
             }
         }
     };
@@ -352,10 +361,12 @@ __global__ __launch_bounds__(64 * kPipeWaves) void nn_split3_kernel(int64_t M, i
         const int col = 16 * j + c;
         if (col >= s) continue;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int64_t row = rb0 + 16 * wave + 4 * q + e;
-            if (row < M) Y[row * ldy + col] = static_cast<TX>(acc[j][e]);
-        }
+        for (int r = 0; r < RT; ++r)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int64_t row = rb0 + 16 * (RT * wave + r) + 4 * q + e;
+                if (row < M) Y[row * ldy + col] = static_cast<TX>(acc[r][j][e]);
+            }
     }
 }
 
@@ -791,9 +802,13 @@ void nn_dispatch(hipStream_t st, int64_t M, int64_t N, int s, const TA* A, int64
     }
     if constexpr (MODE == kSplit3 && std::is_same_v<TA, __hip_bfloat16>) {
         if ((reinterpret_cast<uintptr_t>(A) & 15) == 0 && lda % 8 == 0) {
-            const dim3 g3(static_cast<unsigned>(ceil_div(M, 16 * kPipeWaves)));
+            // two row tiles per wavefront where the block still fits the CU as
+            // often (16 and 64 columns: A X 7-9% and 20% faster), one where the
+            // larger A tile halves the resident blocks (32 columns: 16% slower)
 #define AMFLS_NN3(NT)                                                                                          \
-            hipLaunchKernelGGL((nn_split3_kernel<TX, NT>), g3, dim3(64 * kPipeWaves), 0, st, M, N, s, A, lda,      \
+            constexpr int RT = (NT == 1 || NT == 4) ? 2 : 1;                                                    \
+            const dim3 g3(static_cast<unsigned>(ceil_div(M, 16 * RT * kPipeWaves)));                           \
+            hipLaunchKernelGGL((nn_split3_kernel<TX, NT, kPipeWaves, RT>), g3, dim3(64 * kPipeWaves), 0, st, M, N, s, A, lda, \
                                static_cast<const __bf16*>(XT), nk, plane, Y, ldy)
             if (NTt == 1) { AMFLS_NN3(1); }
             else if (NTt == 2) { AMFLS_NN3(2); }

@@ -231,6 +231,19 @@ struct CsrView {
     // the same for the wide-budget stream kernel (8192 entries a block)
     const int64_t* stream_rows_wide = nullptr;
     int64_t stream_blocks_wide = 0;
+    // Slab-tiled copy of the entries for the width-one slab kernel, or null.
+    // Rows are cut into panels (slab_panel_count + 1 row boundaries) and
+    // columns into slab_count slabs of csr_slab_cols() columns; tile
+    // (panel p, slab k) holds entries slab_tiles[p * slab_count + k] ..
+    // slab_tiles[p * slab_count + k + 1] - 1, in (row, column) order, each
+    // index packed as (row - first row of the panel) << 16 | (column - first
+    // column of the slab), values in A's type.
+    const int64_t* slab_panels = nullptr;
+    const int64_t* slab_tiles = nullptr;
+    const uint32_t* slab_idx = nullptr;
+    const void* slab_val = nullptr;
+    int64_t slab_panel_count = 0;
+    int slab_count = 0;
 };
 
 // Row boundaries (host) of the stream kernel's partition: consecutive rows
@@ -242,6 +255,23 @@ int csr_stream_budget();
 // Index of the width-one stream variant (CSR-Stream: entries streamed into
 // LDS with coalesced loads, then summed per row).
 int csr_spmm_stream_variant();
+
+// Width-one slab kernel (x staged in LDS one column slab at a time).
+int csr_spmm_slab_variant();
+int csr_slab_cols();  // columns per slab
+// Slabs of a matrix with `cols` columns, or -1 beyond the format's limit.
+int csr_slab_count(int64_t cols);
+// Panel row boundaries (host) for the slab copy: consecutive rows with at
+// most `panel_entries` entries (a longer row gets a panel of its own) and at
+// most the kernel's row capacity.
+std::vector<int64_t> csr_slab_partition(const std::vector<int64_t>& row_ptr, int64_t panel_entries);
+// Entries per panel for a matrix of nnz entries.
+int64_t csr_slab_panel_entries(int64_t nnz);
+// Builds the slab copy of A on `s` from device panel boundaries: tiles
+// (npanels * slab_count + 1 offsets), packed indices and values (nnz each,
+// values of A's type).  Stable, so every tile keeps A's (row, column) order.
+void csr_slab_build(hipStream_t s, const CsrView& A, const int64_t* panels, int64_t npanels, int slab_count,
+                    int64_t* tiles, uint32_t* idx, void* val, Scratch& scratch);
 
 // Wall-clock stamps of one product taken inside its kernels (device memory,
 // zeroed when allocated).  The stamps of product i live at slots[i + j *

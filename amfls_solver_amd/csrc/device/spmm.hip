@@ -220,6 +220,240 @@ __global__ __launch_bounds__(kThreads) void csr_stream_kernel(
     stamp_end(stamp);
 }
 
+// Width-one products with x staged in LDS, on the slab-tiled copy (a
+// panel's entries grouped by column slab of kSlabCols columns, then by row
+// and column).  A block owns a panel.  For each slab that holds some of the
+// panel's entries, it stages the slab's x in LDS with coalesced loads; each
+// entry then reads its x from LDS (a random 8-byte LDS read instead of a
+// cache line through the vector L1, which bounds the row kernels' gathers at
+// about one per clock per CU), and a run of one row's products is summed in
+// entry order and added to the row's sum in LDS.  The next slab's x and its
+// first kSlabAhead chunks of entries are loaded into registers while the
+// current slab is processed.  A row's sum is ((run in its first slab) + run
+// in the next slab) + ..., each run summed in column order (a run split by a
+// chunk boundary adds its two parts in order): deterministic, but rounded
+// differently from the row kernels' single fma chain.
+constexpr int kSlabCols = 8192;       // columns per slab
+constexpr int kSlabRows = 8192;       // rows per panel at most (16-bit row offsets)
+constexpr int kSlabMaxCount = 1024;   // slabs per matrix at most
+constexpr int kSlabThreads = 1024;
+constexpr int kSlabU = 2;             // entries per thread per chunk
+constexpr int kSlabChunk = kSlabThreads * kSlabU;
+constexpr int kSlabAhead = 2;         // chunks of the next slab loaded ahead
+constexpr int kSlabStage = kSlabCols / kSlabThreads;
+constexpr int kSlabPanels = 256;      // panels per matrix, about one per CU
+static_assert(kSlabCols <= 65536 && kSlabRows <= 65536, "packed 16-bit offsets");
+
+// Barrier for LDS traffic only: waits for this wave's LDS operations, not
+// for its global loads, so the next slab's loads stay in flight across it
+// (__syncthreads() also waits for every outstanding global load).
+__device__ __forceinline__ void slab_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+template <typename T>
+struct SlabEntries {
+    uint32_t i[kSlabAhead][kSlabU];
+    T v[kSlabAhead][kSlabU];
+};
+
+// Loads of slab k: its x (kSlabStage values per thread, zero past the last
+// column) and the first kSlabAhead chunks of the tile's n entries at `base`.
+template <typename V, typename T>
+__device__ __forceinline__ void slab_fetch(int k, int64_t cols, const T* __restrict__ X, int64_t ldx,
+                                           const uint32_t* __restrict__ idx, const V* __restrict__ val,
+                                           int64_t base, int n, T (&x)[kSlabStage], SlabEntries<T>& e) {
+    const int t = threadIdx.x;
+    const int64_t c0 = static_cast<int64_t>(k) * kSlabCols;
+#pragma unroll
+    for (int u = 0; u < kSlabStage; ++u) {
+        const int64_t c = c0 + t + u * kSlabThreads;
+        x[u] = c < cols ? X[c * ldx] : T(0);
+    }
+#pragma unroll
+    for (int q = 0; q < kSlabAhead; ++q) {
+#pragma unroll
+        for (int u = 0; u < kSlabU; ++u) {
+            const int pos = q * kSlabChunk + t + u * kSlabThreads;
+            const bool in = pos < n;
+            e.i[q][u] = in ? idx[base + pos] : 0u;
+            e.v[q][u] = in ? load_as<T, V>(val[base + pos]) : T(0);
+        }
+    }
+}
+
+// Cross-lane moves (DPP) of one 32-bit value; lanes whose source lies
+// outside the pattern, or outside the rows that ROW_MASK selects, get `old`.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t old, uint32_t x) {
+    return static_cast<uint32_t>(
+        __builtin_amdgcn_update_dpp(static_cast<int>(old), static_cast<int>(x), CTRL, ROW_MASK, 0xF, false));
+}
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_zero(float x) {
+    return __uint_as_float(dpp_u32<CTRL, ROW_MASK>(0u, __float_as_uint(x)));
+}
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_zero(double x) {
+    const uint64_t b = static_cast<uint64_t>(__double_as_longlong(x));
+    const uint32_t lo = dpp_u32<CTRL, ROW_MASK>(0u, static_cast<uint32_t>(b));
+    const uint32_t hi = dpp_u32<CTRL, ROW_MASK>(0u, static_cast<uint32_t>(b >> 32));
+    return __longlong_as_double(static_cast<long long>((static_cast<uint64_t>(hi) << 32) | lo));
+}
+
+// One step of the segmented scan: add the partial sum of the lane the
+// pattern names when that lane holds the same key (keys are sorted, so every
+// lane between the two holds it too).
+template <int CTRL, int ROW_MASK, typename T>
+__device__ __forceinline__ void seg_step(T& v, uint32_t key) {
+    const uint32_t k = dpp_u32<CTRL, ROW_MASK>(0xFFFFFFFFu, key);
+    const T u = dpp_zero<CTRL, ROW_MASK>(v);
+    if (k == key) v += u;
+}
+
+// Segmented inclusive sum over a wavefront of sorted keys: afterwards lane
+// i holds the sum of its key's values from the key's first lane in the
+// wavefront to lane i.  Shifts by 1, 2, 4 and 8 lanes inside each row of 16
+// lanes, then lane 15 of each row into the next row (rows 1 and 3) and lane
+// 31 into rows 2 and 3 (gfx9 row broadcasts): a fixed tree, so the sum's
+// rounding is the same on every run.
+template <typename T>
+__device__ __forceinline__ void wave_segmented_scan(T& v, uint32_t key) {
+    seg_step<0x111, 0xF>(v, key);  // row_shr:1
+    seg_step<0x112, 0xF>(v, key);  // row_shr:2
+    seg_step<0x114, 0xF>(v, key);  // row_shr:4
+    seg_step<0x118, 0xF>(v, key);  // row_shr:8
+    seg_step<0x142, 0xA>(v, key);  // row_bcast:15
+    seg_step<0x143, 0xC>(v, key);  // row_bcast:31
+}
+
+// One chunk of n <= kSlabChunk entries of the current tile.  Entry `pos`
+// sits in group g = pos / 64 (one wavefront's slot: g = u * waves + wave)
+// at lane pos % 64.  Each group sums its rows' runs with a segmented scan;
+// the lane ending a run owns it unless the run began in an earlier group,
+// adds the head runs of the following groups that continue it (published in
+// LDS), and adds the total to the row's sum: one owner per row and chunk,
+// and a row's parts are added in entry order.  Ends with a barrier.
+constexpr int kSlabGroups = kSlabChunk / kWave;
+constexpr uint32_t kSlabNoKey = 0xFFFFFFFEu;  // keys of positions past the chunk
+
+template <typename T>
+__device__ __forceinline__ void slab_chunk(int n, const uint32_t (&ii)[kSlabU], const T (&vv)[kSlabU],
+                                           const T* xs, T* acc, T* hsum, uint32_t* hkey, int* hend,
+                                           uint32_t* tkey) {
+    const int lane = threadIdx.x % kWave;
+    const int w = threadIdx.x / kWave;
+    T v[kSlabU];
+    uint32_t key[kSlabU];
+    bool end[kSlabU];
+    int e0[kSlabU];
+#pragma unroll
+    for (int u = 0; u < kSlabU; ++u) {
+        const int pos = (u * (kSlabThreads / kWave) + w) * kWave + lane;
+        const bool in = pos < n;
+        v[u] = in ? vv[u] * xs[ii[u] & 0xFFFFu] : T(0);
+        key[u] = in ? (ii[u] >> 16) : kSlabNoKey;
+    }
+#pragma unroll
+    for (int u = 0; u < kSlabU; ++u) wave_segmented_scan(v[u], key[u]);
+#pragma unroll
+    for (int u = 0; u < kSlabU; ++u) {
+        const int g = u * (kSlabThreads / kWave) + w;
+        const uint32_t next = dpp_u32<0x130, 0xF>(0xFFFFFFFFu, key[u]);  // wave_shl:1, lane i + 1's key
+        end[u] = key[u] != kSlabNoKey && next != key[u];
+        const unsigned long long ends = __ballot(end[u]);
+        e0[u] = ends != 0ull ? __ffsll(ends) - 1 : kWave;
+        if (end[u] && lane == e0[u]) {
+            hsum[g] = v[u];
+            hkey[g] = key[u];
+            hend[g] = e0[u];
+        }
+        if (key[u] != kSlabNoKey && (lane == kWave - 1 || next == kSlabNoKey)) tkey[g] = key[u];
+    }
+    slab_barrier();
+#pragma unroll
+    for (int u = 0; u < kSlabU; ++u) {
+        if (!end[u]) continue;
+        const int g = u * (kSlabThreads / kWave) + w;
+        // a head run that continues the previous group's last run is added by that run's owner
+        if (lane == e0[u] && g > 0 && tkey[g - 1] == key[u]) continue;
+        T total = v[u];
+        if (lane == kWave - 1) {
+            for (int g2 = g + 1; g2 < kSlabGroups && g2 * kWave < n && hkey[g2] == key[u]; ++g2) {
+                total += hsum[g2];
+                if (hend[g2] != kWave - 1) break;
+            }
+        }
+        acc[key[u]] += total;
+    }
+    slab_barrier();
+}
+
+template <typename V, typename T>
+__global__ __launch_bounds__(kSlabThreads) void csr_slab_kernel(
+    int64_t cols, int nslab, const int64_t* __restrict__ panels, const int64_t* __restrict__ tiles,
+    const uint32_t* __restrict__ idx, const V* __restrict__ val, const T* __restrict__ X, int64_t ldx,
+    T* __restrict__ Y, int64_t ldy, const T* __restrict__ Add, int64_t ld_add, T add_scale, ProductStamp stamp) {
+    __shared__ T xs[kSlabCols];
+    __shared__ T acc[kSlabRows];
+    __shared__ T hsum[kSlabGroups];
+    __shared__ uint32_t hkey[kSlabGroups];
+    __shared__ int hend[kSlabGroups];
+    __shared__ uint32_t tkey[kSlabGroups];
+    __shared__ int32_t toff[kSlabMaxCount + 1];
+    stamp_begin(stamp);
+    const int t = threadIdx.x;
+    const int64_t p = blockIdx.x;
+    const int64_t r0 = panels[p];
+    const int nrows = static_cast<int>(panels[p + 1] - r0);
+    const int64_t tb = p * nslab;
+    const int64_t pe0 = tiles[tb];
+    for (int k = t; k <= nslab; k += kSlabThreads) toff[k] = static_cast<int32_t>(tiles[tb + k] - pe0);
+    for (int i = t; i < nrows; i += kSlabThreads) acc[i] = T(0);
+    slab_barrier();
+    int k = 0;
+    while (k < nslab && toff[k + 1] == toff[k]) ++k;
+    T xn[kSlabStage];
+    SlabEntries<T> en;
+    if (k < nslab) slab_fetch<V, T>(k, cols, X, ldx, idx, val, pe0 + toff[k], toff[k + 1] - toff[k], xn, en);
+    while (k < nslab) {
+        // the previous tile's last chunk ended with a barrier: xs is free
+#pragma unroll
+        for (int u = 0; u < kSlabStage; ++u) xs[t + u * kSlabThreads] = xn[u];
+        const SlabEntries<T> cur = en;
+        const int64_t base = pe0 + toff[k];
+        const int n = toff[k + 1] - toff[k];
+        int kn = k + 1;
+        while (kn < nslab && toff[kn + 1] == toff[kn]) ++kn;
+        if (kn < nslab)
+            slab_fetch<V, T>(kn, cols, X, ldx, idx, val, pe0 + toff[kn], toff[kn + 1] - toff[kn], xn, en);
+        slab_barrier();
+#pragma unroll
+        for (int q = 0; q < kSlabAhead; ++q)
+            if (q * kSlabChunk < n)
+                slab_chunk<T>(min(kSlabChunk, n - q * kSlabChunk), cur.i[q], cur.v[q], xs, acc, hsum, hkey, hend, tkey);
+        for (int c = kSlabAhead * kSlabChunk; c < n; c += kSlabChunk) {
+            uint32_t ii[kSlabU];
+            T vv[kSlabU];
+#pragma unroll
+            for (int u = 0; u < kSlabU; ++u) {
+                const int pos = c + t + u * kSlabThreads;
+                const bool in = pos < n;
+                ii[u] = in ? idx[base + pos] : 0u;
+                vv[u] = in ? load_as<T, V>(val[base + pos]) : T(0);
+            }
+            slab_chunk<T>(min(kSlabChunk, n - c), ii, vv, xs, acc, hsum, hkey, hend, tkey);
+        }
+        k = kn;
+    }
+    __syncthreads();  // every row's sum is complete (also for a panel without entries)
+    for (int i = t; i < nrows; i += kSlabThreads) {
+        const int64_t r = r0 + i;
+        T out = acc[i];
+        if (Add != nullptr) out = fma(add_scale, Add[r * ld_add], out);
+        Y[r * ldy] = out;
+    }
+    stamp_end(stamp);
+}
+
 struct VariantInfo {
     int p, lc, cpl;
     const char* name;
@@ -308,6 +542,33 @@ void launch_stream(hipStream_t s, const CsrView& A, const T* X, int64_t ldx, T* 
 }
 
 template <typename V, typename T>
+void launch_slab_typed(hipStream_t s, const CsrView& A, const T* X, int64_t ldx, T* Y, int64_t ldy, const T* Add,
+                       int64_t ld_add, double add_scale, hipEvent_t e0, hipEvent_t e1, ProductStamp st) {
+    const dim3 grid(static_cast<unsigned>(A.slab_panel_count));
+    const V* val = static_cast<const V*>(A.slab_val);
+    if (e0 || e1)
+        hipExtLaunchKernelGGL((csr_slab_kernel<V, T>), grid, dim3(kSlabThreads), 0, s, e0, e1, 0, A.cols,
+                              A.slab_count, A.slab_panels, A.slab_tiles, A.slab_idx, val, X, ldx, Y, ldy, Add,
+                              ld_add, static_cast<T>(add_scale), st);
+    else
+        hipLaunchKernelGGL((csr_slab_kernel<V, T>), grid, dim3(kSlabThreads), 0, s, A.cols, A.slab_count,
+                           A.slab_panels, A.slab_tiles, A.slab_idx, val, X, ldx, Y, ldy, Add, ld_add,
+                           static_cast<T>(add_scale), st);
+    AMFLS_LAUNCH_CHECK();
+}
+
+template <typename T>
+void launch_slab(hipStream_t s, const CsrView& A, const T* X, int64_t ldx, T* Y, int64_t ldy, const T* Add,
+                 int64_t ld_add, double add_scale, hipEvent_t e0, hipEvent_t e1, ProductStamp st) {
+    if (A.type == ValueType::f32)
+        launch_slab_typed<float, T>(s, A, X, ldx, Y, ldy, Add, ld_add, add_scale, e0, e1, st);
+    else if (A.type == ValueType::f64)
+        launch_slab_typed<double, T>(s, A, X, ldx, Y, ldy, Add, ld_add, add_scale, e0, e1, st);
+    else
+        throw std::invalid_argument("CSR values must be float32 or float64");
+}
+
+template <typename V, typename T>
 void launch_by_index(int variant, hipStream_t s, const CsrView& A, int ncols, const T* X, int64_t ldx,
                      T* Y, int64_t ldy, const T* Add, int64_t ld_add, double add_scale, hipEvent_t e0,
                      hipEvent_t e1, ProductStamp st) {
@@ -359,6 +620,40 @@ __global__ void transpose_gather_kernel(int64_t nnz, const int32_t* order, const
     const int32_t e = order[k];
     t_ci[k] = row_of[e];
     t_val[k] = val[e];
+}
+
+// ------------------------------------------------------------ slab copy
+// Sort key (panel * slabs + slab) and packed index of every entry; a thread
+// per row finds its panel by bisection of the panel boundaries.
+__global__ void slab_key_kernel(int64_t rows, const int64_t* rp, const int32_t* ci, const int64_t* panels,
+                                int64_t npanels, int nslab, int32_t* key, uint32_t* packed) {
+    const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    int64_t lo = 0, hi = npanels;  // panels[lo] <= r < panels[hi]
+    while (hi - lo > 1) {
+        const int64_t mid = (lo + hi) / 2;
+        if (panels[mid] <= r)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const int32_t kb = static_cast<int32_t>(lo * nslab);
+    const uint32_t rl = static_cast<uint32_t>(r - panels[lo]) << 16;
+    for (int64_t e = rp[r]; e < rp[r + 1]; ++e) {
+        const int32_t c = ci[e];
+        key[e] = kb + c / kSlabCols;
+        packed[e] = rl | static_cast<uint32_t>(c % kSlabCols);
+    }
+}
+
+template <typename V>
+__global__ void slab_gather_kernel(int64_t nnz, const int32_t* order, const uint32_t* packed, const V* val,
+                                   uint32_t* out_idx, V* out_val) {
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= nnz) return;
+    const int32_t e = order[k];
+    out_idx[k] = packed[e];
+    out_val[k] = val[e];
 }
 
 // ------------------------------------------------------------ stats
@@ -442,13 +737,90 @@ __global__ void csr_validate_kernel(int64_t rows, int64_t cols, int64_t nnz, con
 
 }  // namespace
 
-// The stream kernel follows the table's variants.
+// The stream and slab kernels follow the table's variants.
 constexpr int kStreamVariant = kVariantCount;
 constexpr int kStreamWideVariant = kVariantCount + 1;
+constexpr int kSlabVariant = kVariantCount + 2;
 
-int csr_spmm_variant_count() { return kVariantCount + 2; }
+int csr_spmm_variant_count() { return kVariantCount + 3; }
 int csr_spmm_stream_variant() { return kStreamVariant; }
 int csr_stream_budget() { return kStreamBudget; }
+int csr_spmm_slab_variant() { return kSlabVariant; }
+int csr_slab_cols() { return kSlabCols; }
+
+int csr_slab_count(int64_t cols) {
+    const int64_t n = ceil_div(cols, static_cast<int64_t>(kSlabCols));
+    return n <= kSlabMaxCount ? static_cast<int>(n) : -1;
+}
+
+int64_t csr_slab_panel_entries(int64_t nnz) {
+    // about kSlabPanels panels, each staging every slab of x it touches
+    return std::max<int64_t>(4096, ceil_div(nnz, static_cast<int64_t>(kSlabPanels)));
+}
+
+std::vector<int64_t> csr_slab_partition(const std::vector<int64_t>& rp, int64_t panel_entries) {
+    const int64_t rows = static_cast<int64_t>(rp.size()) - 1;
+    std::vector<int64_t> br{0};
+    int64_t r = 0;
+    while (r < rows) {
+        const int64_t e0 = rp[r];
+        int64_t end = r + 1;
+        while (end < rows && rp[end + 1] - e0 <= panel_entries && end - r < kSlabRows) ++end;
+        br.push_back(end);
+        r = end;
+    }
+    return br;
+}
+
+void csr_slab_build(hipStream_t s, const CsrView& A, const int64_t* panels, int64_t npanels, int slab_count,
+                    int64_t* tiles, uint32_t* idx, void* val, Scratch& scratch) {
+    const int64_t nnz = A.nnz;
+    const int64_t ntiles = npanels * slab_count;
+    if (slab_count < 1 || npanels < 1 || nnz >= (int64_t(1) << 31) || ntiles >= (int64_t(1) << 31) - 1)
+        throw std::invalid_argument("CSR slab copy supports nnz and tiles below 2^31");
+    if (nnz == 0) {
+        AMFLS_HIP_CHECK(hipMemsetAsync(tiles, 0, static_cast<size_t>(ntiles + 1) * sizeof(int64_t), s));
+        return;
+    }
+    int end_bit = 1;
+    while ((int64_t(1) << end_bit) < ntiles) ++end_bit;
+    size_t temp_bytes = 0;
+    AMFLS_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, static_cast<const int32_t*>(nullptr),
+                                                       static_cast<int32_t*>(nullptr),
+                                                       static_cast<const int32_t*>(nullptr),
+                                                       static_cast<int32_t*>(nullptr), static_cast<int>(nnz), 0,
+                                                       end_bit, s));
+    const size_t arr = static_cast<size_t>(nnz) * sizeof(int32_t);
+    const size_t align = 256;
+    const size_t arr_al = (arr + align - 1) / align * align;
+    scratch.ensure(5 * arr_al + temp_bytes + align);
+    char* base = static_cast<char*>(scratch.ptr);
+    int32_t* keys_in = reinterpret_cast<int32_t*>(base);
+    int32_t* keys_out = reinterpret_cast<int32_t*>(base + arr_al);
+    int32_t* vals_in = reinterpret_cast<int32_t*>(base + 2 * arr_al);
+    int32_t* vals_out = reinterpret_cast<int32_t*>(base + 3 * arr_al);
+    uint32_t* packed = reinterpret_cast<uint32_t*>(base + 4 * arr_al);
+    void* temp = base + 5 * arr_al;
+    hipLaunchKernelGGL(slab_key_kernel, dim3(static_cast<unsigned>(ceil_div(A.rows, kThreads))), dim3(kThreads), 0,
+                       s, A.rows, A.row_ptr, A.col_idx, panels, npanels, slab_count, keys_in, packed);
+    AMFLS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(iota_kernel, dim3(static_cast<unsigned>(ceil_div(nnz, kThreads))), dim3(kThreads), 0, s,
+                       nnz, vals_in);
+    AMFLS_LAUNCH_CHECK();
+    AMFLS_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys_in, keys_out, vals_in, vals_out,
+                                                       static_cast<int>(nnz), 0, end_bit, s));
+    hipLaunchKernelGGL(offsets_from_sorted_kernel, dim3(static_cast<unsigned>(ceil_div(nnz + 1, kThreads))),
+                       dim3(kThreads), 0, s, nnz, ntiles, keys_out, tiles);
+    AMFLS_LAUNCH_CHECK();
+    const dim3 g(static_cast<unsigned>(ceil_div(nnz, kThreads)));
+    if (A.type == ValueType::f32)
+        hipLaunchKernelGGL(slab_gather_kernel<float>, g, dim3(kThreads), 0, s, nnz, vals_out, packed,
+                           static_cast<const float*>(A.values), idx, static_cast<float*>(val));
+    else
+        hipLaunchKernelGGL(slab_gather_kernel<double>, g, dim3(kThreads), 0, s, nnz, vals_out, packed,
+                           static_cast<const double*>(A.values), idx, static_cast<double*>(val));
+    AMFLS_LAUNCH_CHECK();
+}
 
 std::vector<int64_t> csr_stream_partition(const std::vector<int64_t>& rp, bool wide) {
     const int64_t rows = static_cast<int64_t>(rp.size()) - 1;
@@ -467,7 +839,7 @@ std::vector<int64_t> csr_stream_partition(const std::vector<int64_t>& rp, bool w
 }
 
 bool csr_spmm_variant_supports(int variant, int ncols) {
-    if (variant == kStreamVariant || variant == kStreamWideVariant) return ncols == 1;
+    if (variant == kStreamVariant || variant == kStreamWideVariant || variant == kSlabVariant) return ncols == 1;
     if (variant < 0 || variant >= kVariantCount) return false;
     return ncols <= kVariants[variant].lc * kVariants[variant].cpl;
 }
@@ -475,11 +847,12 @@ bool csr_spmm_variant_supports(int variant, int ncols) {
 const char* csr_spmm_variant_name(int variant) {
     if (variant == kStreamVariant) return "stream";
     if (variant == kStreamWideVariant) return "stream8k";
+    if (variant == kSlabVariant) return "slab";
     return (variant >= 0 && variant < kVariantCount) ? kVariants[variant].name : "invalid";
 }
 
 bool csr_spmm_variant_shape(int variant, int* p, int* lc, int* cpl) {
-    if (variant == kStreamVariant || variant == kStreamWideVariant) {
+    if (variant == kStreamVariant || variant == kStreamWideVariant || variant == kSlabVariant) {
         *p = 0;
         *lc = 1;
         *cpl = 1;
@@ -529,7 +902,8 @@ void csr_spmm(hipStream_t s, const CsrView& A, int ncols, const T* X, int64_t ld
         const int nc = std::min(256, ncols - c0);
         int v = variant;
         if ((v == kStreamVariant && A.stream_rows == nullptr) ||
-            (v == kStreamWideVariant && A.stream_rows_wide == nullptr))
+            (v == kStreamWideVariant && A.stream_rows_wide == nullptr) ||
+            (v == kSlabVariant && A.slab_panels == nullptr))
             v = -1;
         if (v < 0 || !csr_spmm_variant_supports(v, nc))
             v = csr_spmm_default_variant(nc, A.rows > 0 ? static_cast<double>(A.nnz) / A.rows : 0.0);
@@ -543,6 +917,10 @@ void csr_spmm(hipStream_t s, const CsrView& A, int ncols, const T* X, int64_t ld
         }
         if (v == kStreamWideVariant) {
             launch_stream<T, kStreamWideBudget>(s, A, X, ldx, Y, ldy, add, ld_add, add_scale, e0, e1, st);
+            continue;
+        }
+        if (v == kSlabVariant) {
+            launch_slab<T>(s, A, X, ldx, Y, ldy, add, ld_add, add_scale, e0, e1, st);
             continue;
         }
         switch (A.type) {

@@ -212,7 +212,56 @@ int CsrOperator::variant_for(int s, bool transpose) const {
     return it == variants_.end() ? -1 : it->second;
 }
 
-void CsrOperator::set_variant(int s, bool transpose, int variant) { variants_[{s, transpose}] = variant; }
+void CsrOperator::set_variant(int s, bool transpose, int variant) {
+    if (s == 1 && variant == csr_spmm_slab_variant() && !ensure_slab(transpose, true))
+        throw std::length_error("the slab kernel takes at most 1024 slabs of 8192 columns and fewer than 2^31 entries");
+    variants_[{s, transpose}] = variant;
+}
+
+bool CsrOperator::ensure_slab(bool transpose, bool force) {
+    CsrView& v = transpose ? at_ : a_;
+    if (v.slab_panels != nullptr) return true;
+    const int nslab = csr_slab_count(v.cols);
+    if (nslab < 0 || v.rows <= 0 || v.nnz >= (int64_t(1) << 31)) return false;
+    std::vector<int64_t> hrp(static_cast<size_t>(v.rows + 1));
+    AMFLS_HIP_CHECK(hipMemcpy(hrp.data(), v.row_ptr, hrp.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+    const std::vector<int64_t> br = csr_slab_partition(hrp, csr_slab_panel_entries(v.nnz));
+    const int64_t np = static_cast<int64_t>(br.size()) - 1;
+    if (np * nslab >= (int64_t(1) << 31) - 1) return false;
+    if (!force) {
+        const double staging = static_cast<double>(np) * static_cast<double>(v.cols) * 8.0;
+        const double entries = static_cast<double>(v.nnz) * (4.0 + static_cast<double>(value_size(v.type)));
+        if (staging > 8.0 * entries) return false;
+    }
+    SlabCopy& c = transpose ? slab_at_ : slab_a_;
+    c.panels.reset(static_cast<size_t>(np + 1));
+    c.tiles.reset(static_cast<size_t>(np * nslab + 1));
+    c.idx.reset(static_cast<size_t>(std::max<int64_t>(v.nnz, 1)));
+    c.val.reset(static_cast<size_t>(std::max<int64_t>(v.nnz, 1)) * value_size(v.type));
+    // a private stream and scratch: this runs outside any engine call
+    hipStream_t s = nullptr;
+    AMFLS_HIP_CHECK(hipStreamCreate(&s));
+    Scratch scratch;
+    struct Cleanup {
+        hipStream_t s;
+        Scratch& scratch;
+        ~Cleanup() {
+            (void)hipStreamSynchronize(s);
+            scratch.release();
+            (void)hipStreamDestroy(s);
+        }
+    } cleanup{s, scratch};
+    AMFLS_HIP_CHECK(hipMemcpyAsync(c.panels.get(), br.data(), br.size() * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    csr_slab_build(s, v, c.panels.get(), np, nslab, c.tiles.get(), c.idx.get(), c.val.get(), scratch);
+    AMFLS_HIP_CHECK(hipStreamSynchronize(s));
+    v.slab_panels = c.panels.get();
+    v.slab_tiles = c.tiles.get();
+    v.slab_idx = c.idx.get();
+    v.slab_val = c.val.get();
+    v.slab_panel_count = np;
+    v.slab_count = nslab;
+    return true;
+}
 
 namespace {
 
@@ -270,6 +319,10 @@ std::vector<int> tune_widths(Context& ctx, const CsrView& a, const CsrView& at, 
 
 std::vector<int> CsrOperator::autotune(Context& ctx, const std::vector<int>& widths, bool fp64_blocks, int reps) {
     if (reps < 1) throw std::invalid_argument("autotune needs at least one timed launch");
+    if (std::find(widths.begin(), widths.end(), 1) != widths.end()) {
+        ensure_slab(false, false);
+        ensure_slab(true, false);
+    }
     ctx.sync();
     return fp64_blocks ? tune_widths<double>(ctx, a_, at_, rows_, cols_, widths, reps, variants_)
                        : tune_widths<float>(ctx, a_, at_, rows_, cols_, widths, reps, variants_);

@@ -153,6 +153,11 @@ public:
     // blocks, and keeps the fastest for A and A^T (median kernel time of
     // `reps` launches).  Returns the chosen variants, (A, A^T) per width.
     std::vector<int> autotune(Context& ctx, const std::vector<int>& widths, bool fp64_blocks, int reps);
+    // Builds the slab-tiled copy of A (or of A^T) that the width-one slab
+    // kernel reads, unless it exists.  Without `force` it is built only where
+    // staging x once per panel moves at most 8 times the entries' bytes (x is
+    // reused enough within a panel).  Returns whether the copy exists.
+    bool ensure_slab(bool transpose, bool force);
 
     // Augmented [A; a I] products: Y rows [rows, rows+cols) = a X; the
     // transpose adds a * Y[rows:] to A^T Y[0:rows].
@@ -171,6 +176,12 @@ private:
     CsrView a_, at_;
     int64_t nnz_ = 0;
     DevBuf<int64_t> t_rp_, stream_a_, stream_at_, stream_a_wide_, stream_at_wide_;
+    struct SlabCopy {
+        DevBuf<int64_t> panels, tiles;
+        DevBuf<uint32_t> idx;
+        DevBuf<char> val;
+    };
+    SlabCopy slab_a_, slab_at_;
     DevBuf<int32_t> t_ci_;
     DevBuf<char> t_val_;
     ValidationModel model_;

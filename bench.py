@@ -82,8 +82,8 @@ def spmm_sweep(torch, np, am, P, widths=(1, 4, 16, 64), reps=20):
         row = {}
         for tr, (src, dst, ldi, ldo) in ((0, (X, Y, s, s)), (1, (Y, Z, s, s))):
             names = [lib.amfls_spmm_variant_name(v).decode() for v in range(lib.amfls_spmm_variant_count())]
-            kinds = [("", -1)] + ([("_stream", names.index("stream")), ("_stream8k", names.index("stream8k"))]
-                                  if s == 1 else [])
+            kinds = [("", -1)] + ([("_stream", names.index("stream")), ("_stream8k", names.index("stream8k")),
+                                   ("_slab", names.index("slab"))] if s == 1 else [])
             for tag, var in kinds:
                 M.set_variant(s, bool(tr), var)
                 torch.cuda.synchronize()

@@ -192,7 +192,10 @@ int amfls_operator_apply_rows_noalloc(amfls_context* ctx, const amfls_operator* 
 double amfls_operator_product_bytes(const amfls_operator* op, int block_cols, int transpose, int dtype);
 double amfls_operator_product_flops(const amfls_operator* op, int block_cols);
 /* Kernel variant per (width, direction); -1 restores the default.  CSR
- * operators: an SpMM variant index.  Dense row-major operators: 0 the
+ * operators: an SpMM variant index; choosing the width-one "slab" variant
+ * builds the operator's slab-tiled copy of A (or A^T) first, synchronously,
+ * and fails with AMFLS_ERR_LENGTH beyond its limits (more than 1024 slabs
+ * of 8192 columns, 2^31 entries).  Dense row-major operators: 0 the
  * vector-ALU kernel (widths <= 8; the default for one column), 1 the matrix
  * cores (the default for wider blocks). */
 int amfls_operator_set_variant(amfls_operator* op, int block_cols, int transpose, int variant);
@@ -200,8 +203,11 @@ int amfls_operator_set_variant(amfls_operator* op, int block_cols, int transpose
  * loop, src/algorithms/operators/sparse_csr_operator.cpp:203-269): for each
  * of `count` block widths, times the kernel variants whose column lanes fit
  * the width on this matrix (median of `reps` launches, blocks of dtype
- * AMFLS_F64 or AMFLS_F32) for A and A^T and keeps the fastest.  chosen (may
- * be NULL) receives 2 * count variant indices, (A, A^T) per width. */
+ * AMFLS_F64 or AMFLS_F32) for A and A^T and keeps the fastest.  With width
+ * one among the widths it first builds the slab-tiled copies where x is
+ * reused enough within a panel (staging it moves at most 8 times the entry
+ * bytes), so the slab kernel is a candidate there.  chosen (may be NULL)
+ * receives 2 * count variant indices, (A, A^T) per width. */
 int amfls_operator_autotune(amfls_context* ctx, amfls_operator* op, const int32_t* widths, int32_t count,
                             int dtype, int32_t reps, int32_t* chosen);
 /* Per-product kernel timing (off by default): CSR products are timed by the

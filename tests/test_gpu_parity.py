@@ -76,6 +76,83 @@ def test_csr_products_all_variants(torch_mod, fx, vt):
         assert np.abs(y32 - Y).max() <= 1e-5 * np.abs(Y).max()
 
 
+def _slab_matrix(rng, rows, cols, mean):
+    """Random CSR with empty rows (every 97th) and one full row."""
+    lens = rng.integers(0, 2 * mean, size=rows)
+    lens[::97] = 0
+    full = rows // 3
+    cis = []
+    for i, L in enumerate(lens):
+        cis.append(np.arange(cols) if i == full else np.unique(rng.integers(0, cols, size=L)))
+    rp = np.concatenate([[0], np.cumsum([len(c) for c in cis])]).astype(np.int64)
+    return rp, np.concatenate(cis).astype(np.int32)
+
+
+def _variant_index(lib, name):
+    return [lib.amfls_spmm_variant_name(v).decode() for v in range(lib.amfls_spmm_variant_count())].index(name)
+
+
+@pytest.mark.parametrize("vt", ["float32", "float64"])
+def test_slab_kernel_products(torch_mod, vt):
+    """The width-one slab kernel (x staged in LDS one column slab at a time)
+    against scipy: three and four slabs of 8192 columns (A, A^T), many
+    panels, empty rows, and a full row whose tiles span several chunks
+    (more than the two loaded ahead); FP64 and FP32 blocks; the same bits
+    on a second launch."""
+    import scipy.sparse as sp
+    import amfls_solver_amd as am
+    torch = torch_mod
+    rng = np.random.default_rng(7)
+    rows, cols = 30000, 20000
+    rp, ci = _slab_matrix(rng, rows, cols, 12)
+    val = rng.standard_normal(len(ci)).astype(vt)
+    A = sp.csr_matrix((val.astype(np.float64), ci, rp), shape=(rows, cols))
+    M = am.DeviceMatrix(csr_tensor(torch, rp, ci, val, (rows, cols)), value_dtype=getattr(torch, vt))
+    slab = _variant_index(M.context._lib, "slab")
+    M.set_variant(1, False, slab)
+    M.set_variant(1, True, slab)
+    x = rng.standard_normal(cols)
+    y_ref = A @ x
+    z_ref = A.T @ y_ref
+    y = M.apply(torch.from_numpy(x)).cpu().numpy()[:, 0]
+    z = M.apply(torch.from_numpy(y_ref), transpose=True).cpu().numpy()[:, 0]
+    assert np.abs(y - y_ref).max() <= 1e-13 * np.abs(y_ref).max()
+    assert np.abs(z - z_ref).max() <= 1e-13 * np.abs(z_ref).max()
+    assert np.array_equal(y, M.apply(torch.from_numpy(x)).cpu().numpy()[:, 0])
+    assert np.array_equal(z, M.apply(torch.from_numpy(y_ref), transpose=True).cpu().numpy()[:, 0])
+    assert np.all(y[::97] == 0.0)  # empty rows
+    y32 = M.apply(torch.from_numpy(x).float()).cpu().numpy()[:, 0]
+    z32 = M.apply(torch.from_numpy(y_ref).float(), transpose=True).cpu().numpy()[:, 0]
+    assert np.abs(y32 - y_ref).max() <= 1e-5 * np.abs(y_ref).max()
+    assert np.abs(z32 - z_ref).max() <= 1e-5 * np.abs(z_ref).max()
+
+
+def test_slab_kernel_in_ridge_solves(torch_mod):
+    """Ridge and minimum-norm solves with the slab kernel for every width-one
+    product (A^T with the augmented add) certify and agree with the default
+    kernels' solves."""
+    import amfls_solver_amd as am
+    torch = torch_mod
+    rng = np.random.default_rng(8)
+    rows, cols = 20000, 9000
+    rp, ci = _slab_matrix(rng, rows, cols, 10)
+    val = rng.standard_normal(len(ci))
+    b = torch.from_numpy(rng.standard_normal(rows)).cuda()
+    M = am.DeviceMatrix(csr_tensor(torch, rp, ci, val, (rows, cols)))
+    slab = _variant_index(M.context._lib, "slab")
+    for lam in (1e-2, 1e-4, 0.0):
+        ref = am.run_amfls(M, b, regularization=lam, tolerance=1e-8, clock="synthetic")
+        M.set_variant(1, False, slab)
+        M.set_variant(1, True, slab)
+        r = am.run_amfls(M, b, regularization=lam, tolerance=1e-8, clock="synthetic")
+        M.set_variant(1, False, -1)
+        M.set_variant(1, True, -1)
+        assert r.status == ref.status, (lam, r.status, ref.status)
+        if lam > 0:
+            assert r.status == 0 and r.relative_energy_error_upper_bound <= 1e-8, lam
+        assert rel(r.solution.cpu(), ref.solution.cpu()) <= 1e-6, lam
+
+
 @pytest.mark.parametrize("layout", ["col", "row"])
 @pytest.mark.parametrize("vt", ["float64", "float32", "bfloat16"])
 def test_dense_products(torch_mod, fx, layout, vt):

@@ -753,6 +753,223 @@ __global__ __launch_bounds__(kThreads, 2) void tn_split3_kernel(int64_t M, int64
     }
 }
 
+// The same partial Z = A^T Y for FP32 Y tiles of at most 32 columns, with
+// every tile moved by direct global-to-LDS loads (CDNA4 global_load_lds): the
+// same block-to-(strip, chunk) mapping and partials as tn_split3_kernel and
+// the same matrix-core operands in the same order (with the same chunks, the
+// partials are bitwise equal; the dispatch gives it one round of 256 blocks,
+// one per CU, since its 126 KB of LDS hold one block per CU).  The loads
+// of trip t + 2 (its A tile, 16 bytes per lane, and its raw FP32 Y tile, 4
+// bytes per lane) are issued while trip t runs on the matrix cores: three A
+// stages, two raw Y stages, no registers and no register transposes.  Each
+// A row's 16-byte pieces are XOR-swizzled (piece ^ 2 (row % 4)) so that the
+// transpose reads spread over the LDS banks.  The A operand (eight
+// consecutive rows of one column per lane) comes from the LDS transpose load
+// of gfx950, ds_read_b64_tr_b16: when the 16 lanes of a group read the four
+// rows of a 4 x 16 block (lane l: row l / 4, columns 4 (l % 4) .. + 3),
+// lane o receives column o of the four rows.  Each trip splits its raw Y
+// tile into the three bf16 planes once per block (rows outside the chunk
+// and columns past s become zero).  Needs N % 8 == 0 (whole pieces); rows
+// past the matrix load its last row and are zeroed in LDS before use, and
+// pieces past column N load column 0 of their row (their outputs are not
+// written).  Every stage is its own LDS array, so the compiler sees that a
+// read of one stage does not wait for the loads into another.
+typedef short shortx4 __attribute__((ext_vector_type(4)));
+typedef short shortx8 __attribute__((ext_vector_type(8)));
+typedef __attribute__((address_space(3))) void lds_void;
+typedef __attribute__((address_space(3))) shortx4 lds_shortx4;
+
+constexpr int kTnStage = kTnRows * kTnCols;  // bf16 per A stage
+
+__device__ __forceinline__ int tn_swz(int row) { return 2 * (row & 3); }
+
+// Waits until at most N vector memory operations of this wavefront are outstanding.
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+    static_assert(N >= 0 && N < 64, "vmcnt field");
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
+}
+
+// Barrier for LDS traffic only (outstanding global loads stay in flight).
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+template <typename TX, int NT>
+__global__ __launch_bounds__(kThreads, 1) void tn_lds_kernel(int64_t M, int64_t N, int s,
+                                                               const __hip_bfloat16* __restrict__ A, int64_t lda,
+                                                               const TX* __restrict__ Yb, int64_t ldy,
+                                                               int64_t rows_per_chunk, int strips, int chunks,
+                                                               double* __restrict__ P) {
+    static_assert(NT <= 2 && sizeof(TX) == 4, "FP32 Y tiles of at most 32 columns");
+    constexpr int KP = kTnRows + kTnPad;
+    constexpr int SW = 16 * NT;
+    constexpr int kWaves = kThreads / kWave;
+    constexpr int kALoads = kTnStage * 2 / (kWave * 16) / kWaves;  // 16-byte loads per lane per trip (8)
+    constexpr int kYLoads = kTnRows * SW / kWave / kWaves;          // 4-byte loads per lane per trip
+    constexpr int kLoads = kALoads + kYLoads;
+    constexpr int YPAIRS = (SW * (kTnRows / 8) + kThreads - 1) / kThreads;
+    __shared__ __align__(16) __bf16 As0[kTnStage];
+    __shared__ __align__(16) __bf16 As1[kTnStage];
+    __shared__ __align__(16) __bf16 As2[kTnStage];
+    __shared__ __align__(16) float Yr0[kTnRows * SW];
+    __shared__ __align__(16) float Yr1[kTnRows * SW];
+    __shared__ __align__(16) __bf16 Yp[3][SW][KP];
+    const int b = blockIdx.x;
+    int strip, chunk;
+    if (chunks % 8 == 0) {
+        const int x = b % 8, qq = b / 8;
+        strip = qq % strips;
+        chunk = (qq / strips) * 8 + x;
+    } else {
+        strip = b % strips;
+        chunk = b / strips;
+    }
+    const int64_t j0 = static_cast<int64_t>(strip) * kTnCols;
+    const int64_t r0 = static_cast<int64_t>(chunk) * rows_per_chunk;
+    const int64_t r1 = std::min<int64_t>(M, r0 + rows_per_chunk);
+    const int64_t trips = r0 < r1 ? (r1 - r0 + kTnRows - 1) / kTnRows : 0;
+    const int t = threadIdx.x, lane = t % kWave;
+    const int wave = __builtin_amdgcn_readfirstlane(t / kWave);
+    const int c16 = lane % 16, q = lane / 16;
+    floatx4 acc[4][NT];
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+#pragma unroll
+        for (int j = 0; j < NT; ++j) acc[e][j] = floatx4{0, 0, 0, 0};
+    // the loads of trip `tr` into one A stage and one raw Y stage: wavefront w
+    // moves 1 KB pieces w * kALoads + u of the A tile (tile rows 2 c, 2 c + 1)
+    // and 256-byte pieces w * kYLoads + u of the Y tile (64 / SW rows each)
+    auto issue = [&](int64_t tr, __bf16* as, float* yr) {
+        const int64_t i0 = r0 + tr * kTnRows;
+#pragma unroll
+        for (int u = 0; u < kALoads; ++u) {
+            const int ck = wave * kALoads + u;
+            const int row = 2 * ck + lane / 32;
+            const int piece = (lane % 32) ^ tn_swz(row);
+            const int64_t grow = std::min<int64_t>(i0 + row, M - 1);
+            const int64_t gcol = j0 + 8 * piece;
+            __builtin_amdgcn_global_load_lds(A + grow * lda + (gcol < N ? gcol : 0), (lds_void*)(as + ck * 512), 16,
+                                             0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < kYLoads; ++u) {
+            const int ck = wave * kYLoads + u;
+            const int e = ck * kWave + lane;  // element of the raw tile, row-major SW wide
+            const int row = e / SW, col = e % SW;
+            const int64_t grow = std::min<int64_t>(i0 + row, M - 1);
+            __builtin_amdgcn_global_load_lds(Yb + grow * ldy + (col < s ? col : 0), (lds_void*)(yr + ck * kWave), 4,
+                                             0, 0);
+        }
+    };
+    // split of the raw Y tile into the three bf16 planes (once per block)
+    auto split_y = [&](int64_t tr, const float* yr) {
+        const int64_t i0 = r0 + tr * kTnRows;
+#pragma unroll
+        for (int pp = 0; pp < YPAIRS; ++pp) {
+            const int pair = t + pp * kThreads;
+            const int col = pair % SW, kg = pair / SW;
+            if (kg >= kTnRows / 8) continue;
+            bf16x8 h, m, l;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const bool in = col < s && i0 + 8 * kg + u < r1;
+                const float v = yr[(8 * kg + u) * SW + col];
+                __bf16 x0, x1, x2;
+                split3(static_cast<double>(in ? v : 0.0f), x0, x1, x2);
+                h[u] = x0;
+                m[u] = x1;
+                l[u] = x2;
+            }
+            *reinterpret_cast<bf16x8*>(&Yp[0][col][8 * kg]) = h;
+            *reinterpret_cast<bf16x8*>(&Yp[1][col][8 * kg]) = m;
+            *reinterpret_cast<bf16x8*>(&Yp[2][col][8 * kg]) = l;
+        }
+    };
+    // rows of the A tile past the matrix (the last trip of the last chunk)
+    auto zero_tail = [&](int64_t tr, __bf16* as) {
+        const int64_t i0 = r0 + tr * kTnRows;
+        if (i0 + kTnRows <= M) return;
+        const int first = static_cast<int>(M - i0);
+        for (int idx = t; idx < (kTnRows - first) * (kTnCols / 8); idx += kThreads)
+            *reinterpret_cast<uint4*>(as + (first + idx / (kTnCols / 8)) * kTnCols + 8 * (idx % (kTnCols / 8))) =
+                uint4{0, 0, 0, 0};
+    };
+    // one trip: its tile has landed in (as, yr); the loads of trip tr + 2 go
+    // into (as_next2, yr) once yr is split
+    auto trip = [&](int64_t tr, __bf16* as, float* yr, __bf16* as_next2) {
+        if (tr + 1 < trips)
+            wait_vmcnt<kLoads>();  // this wavefront's share of trip tr has landed (trip tr + 1's may not)
+        else
+            wait_vmcnt<0>();
+        lds_barrier();  // every wavefront's share; every wavefront is done with trip tr - 1
+        zero_tail(tr, as);
+        split_y(tr, yr);
+        lds_barrier();
+        bf16x8 a[kTnRows / 32][4];
+        bf16x8 yp[kTnRows / 32][NT][3];
+#pragma unroll
+        for (int h = 0; h < kTnRows / 32; ++h) {
+            const int k = 32 * h + 8 * q;
+            const int r = k + (c16 >> 2);
+            const int off = 4 * (c16 & 1);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int piece = 2 * (wave + 4 * e) + ((c16 & 3) >> 1);
+                const __bf16* p1 = as + r * kTnCols + 8 * (piece ^ tn_swz(r)) + off;
+                const __bf16* p2 = as + (r + 4) * kTnCols + 8 * (piece ^ tn_swz(r + 4)) + off;
+                const shortx4 lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_shortx4*)(p1));
+                const shortx4 hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_shortx4*)(p2));
+                a[h][e] = __builtin_bit_cast(bf16x8, __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7));
+            }
+#pragma unroll
+            for (int j = 0; j < NT; ++j)
+#pragma unroll
+                for (int pl = 0; pl < 3; ++pl) yp[h][j][pl] = *reinterpret_cast<const bf16x8*>(&Yp[pl][16 * j + c16][k]);
+        }
+        if (tr + 2 < trips) issue(tr + 2, as_next2, yr);
+#pragma unroll
+        for (int h = 0; h < kTnRows / 32; ++h)
+#pragma unroll
+            for (int j = 0; j < NT; ++j)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    acc[e][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[h][e], yp[h][j][2], acc[e][j], 0, 0, 0);
+                    acc[e][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[h][e], yp[h][j][1], acc[e][j], 0, 0, 0);
+                    acc[e][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[h][e], yp[h][j][0], acc[e][j], 0, 0, 0);
+                }
+    };
+    // The compiler inserts waits for direct-to-LDS loads before LDS accesses
+    // it cannot tell apart from their destinations; the loads into the first
+    // destination it sees are tracked without that distinction, so every read
+    // of that array would wait for every load in flight.  A first load into
+    // an array that is never read takes that place, and the waits above
+    // (explicit, per trip) are the only ones.
+    __shared__ __align__(16) float sink[kWave];
+    __builtin_amdgcn_global_load_lds(Yb, (lds_void*)sink, 4, 0, 0);
+    if (trips > 0) issue(0, As0, Yr0);
+    if (trips > 1) issue(1, As1, Yr1);
+    // trip tr uses A stage tr % 3 and Y stage tr % 2
+    for (int64_t tr = 0; tr < trips; tr += 6) {
+        trip(tr, As0, Yr0, As2);
+        if (tr + 1 < trips) trip(tr + 1, As1, Yr1, As0);
+        if (tr + 2 < trips) trip(tr + 2, As2, Yr0, As1);
+        if (tr + 3 < trips) trip(tr + 3, As0, Yr1, As2);
+        if (tr + 4 < trips) trip(tr + 4, As1, Yr0, As0);
+        if (tr + 5 < trips) trip(tr + 5, As2, Yr1, As1);
+    }
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+        const int col = 16 * j + c16;
+        if (col >= s) continue;
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) {
+                const int64_t jj = j0 + 16 * (wave + 4 * e) + 4 * q + rr;
+                if (jj < N) P[(static_cast<int64_t>(chunk) * N + jj) * s + col] = static_cast<double>(acc[e][j][rr]);
+            }
+    }
+}
+
 // Z[j][col] = sum over chunks of P[chunk][j][col]: a wavefront per output,
 // lanes over chunks in order, then the butterfly (a fixed order).
 template <typename TX>
@@ -837,7 +1054,10 @@ void tn_split3_dispatch(hipStream_t st, int64_t M, int64_t N, int s, const __hip
     // one round of three resident blocks per CU over 256 CUs (768 blocks:
     // a second, partial round measured 10% slower), whole 64-row trips per
     // chunk, a multiple of 8 chunks for the XCD mapping
-    int64_t chunks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(768, strips), ceil_div(M, 4 * kTnRows)));
+    // the direct-to-LDS kernel holds one block per CU: one round of 256 blocks
+    const bool use_lds = sizeof(TX) == 4 && N % 8 == 0 && s <= 32;
+    int64_t chunks =
+        std::max<int64_t>(1, std::min<int64_t>(ceil_div(use_lds ? 256 : 768, strips), ceil_div(M, 4 * kTnRows)));
     chunks = fit_chunks(scratch, static_cast<size_t>(N * s) * sizeof(double), chunks);
     if (chunks >= 8) chunks -= chunks % 8;
     const int64_t rows_per_chunk = ceil_div(ceil_div(M, chunks), kTnRows) * kTnRows;
@@ -849,11 +1069,24 @@ void tn_split3_dispatch(hipStream_t st, int64_t M, int64_t N, int s, const __hip
 #define AMFLS_TN3(NT)                                                                                             \
     hipLaunchKernelGGL((tn_split3_kernel<TX, NT>), grid, dim3(kThreads), 0, st, M, N, s, A, lda, Y, ldy,          \
                        rows_per_chunk, strips, ch, P)
-    if (s <= 16) { AMFLS_TN3(1); }
+#define AMFLS_TNL(NT)                                                                                             \
+    hipLaunchKernelGGL((tn_lds_kernel<TX, NT>), grid, dim3(kThreads), 0, st, M, N, s, A, lda, Y, ldy,             \
+                       rows_per_chunk, strips, ch, P)
+    bool lds = false;
+    if constexpr (sizeof(TX) == 4) {
+        if (use_lds) {
+            lds = true;
+            if (s <= 16) { AMFLS_TNL(1); }
+            else { AMFLS_TNL(2); }
+        }
+    }
+    if (lds) {}
+    else if (s <= 16) { AMFLS_TN3(1); }
     else if (s <= 32) { AMFLS_TN3(2); }
     else if (s <= 48) { AMFLS_TN3(3); }
     else { AMFLS_TN3(4); }
 #undef AMFLS_TN3
+#undef AMFLS_TNL
     AMFLS_LAUNCH_CHECK();
     hipLaunchKernelGGL(dense_tn_reduce_kernel<TX>, dim3(static_cast<unsigned>(ceil_div(N * s, kThreads / kWave))),
                        dim3(kThreads), 0, st, N, s, chunks, P, Z, ldz);

@@ -244,9 +244,9 @@ constexpr int kSlabStage = kSlabCols / kSlabThreads;
 constexpr int kSlabPanels = 256;      // panels per matrix, about one per CU
 static_assert(kSlabCols <= 65536 && kSlabRows <= 65536, "packed 16-bit offsets");
 
-// Barrier for LDS traffic only: waits for this wave's LDS operations, not
-// for its global loads, so the next slab's loads stay in flight across it
-// (__syncthreads() also waits for every outstanding global load).
+// Barrier for LDS traffic only: waits for this wave's LDS operations and
+// never for its global loads, so the next slab's loads stay in flight
+// across it whatever fences the compiler would attach to __syncthreads().
 __device__ __forceinline__ void slab_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
 template <typename T>

@@ -422,15 +422,19 @@ def test_dense_matrix_core_products(torch_mod, vt):
     import amfls_solver_amd as am
     torch = torch_mod
     rng = np.random.default_rng(7)
-    # the last case is a view with a padded leading dimension (208), so the
+    # (1531, 203, 5) is a view with a padded leading dimension (208), so the
     # 16-byte-aligned A^T kernels run on a column count that is not a
-    # multiple of 8 and a last row chunk that is partial
-    for m, n, pad in ((1000, 203, 0), (600, 517, 0), (37, 64, 0), (4099, 1, 0), (1531, 203, 5)):
+    # multiple of 8 and a last row chunk that is partial; (3001, 520) and
+    # (100003, 264) take the direct-to-LDS A^T kernel (FP32 blocks, columns a
+    # multiple of 8) with a last strip of one or a few 8-column pieces, a
+    # last 64-row trip past the matrix, and chunks of more than six trips
+    for m, n, pad in ((1000, 203, 0), (600, 517, 0), (37, 64, 0), (4099, 1, 0), (1531, 203, 5), (3001, 520, 0),
+                      (100003, 264, 0)):
         full = torch.from_numpy(rng.standard_normal((m, n + pad))).to(getattr(torch, vt))
         a = full.cuda()[:, :n] if pad else full
         stored = a.double().cpu().numpy()
         M = am.DeviceMatrix(a if pad else a.contiguous(), value_dtype=getattr(torch, vt))
-        for s in (1, 5, 16, 17, 40, 64, 70):
+        for s in (1, 5, 16, 17, 32, 40, 64, 70):
             X = rng.standard_normal((n, s))
             Y = stored @ X
             Yg = rng.standard_normal((m, s))

@@ -248,6 +248,22 @@ __global__ __launch_bounds__(kThreads) void dense_nn_kernel(int64_t M, int64_t N
     }
 }
 
+// Direct-to-LDS loads (global_load_lds) and the LDS transpose load of gfx950.
+typedef short shortx4 __attribute__((ext_vector_type(4)));
+typedef short shortx8 __attribute__((ext_vector_type(8)));
+typedef __attribute__((address_space(3))) void lds_void;
+typedef __attribute__((address_space(3))) shortx4 lds_shortx4;
+
+// Waits until at most N vector memory operations of this wavefront are outstanding.
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+    static_assert(N >= 0 && N < 64, "vmcnt field");
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
+}
+
+// Barrier for LDS traffic only (outstanding global loads stay in flight).
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 // Split-bf16 Y = A X for bf16 A (16-byte aligned rows): a block of W
 // wavefronts owns 16 RT W rows, a wavefront RT 16-row tiles that share each
 // X fragment it reads from LDS.  Each 64-wide K
@@ -365,6 +381,141 @@ __global__ __launch_bounds__(64 * W) void nn_split3_kernel(int64_t M, int64_t N,
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const int64_t row = rb0 + 16 * (RT * wave + r) + 4 * q + e;
+                if (row < M) Y[row * ldy + col] = static_cast<TX>(acc[r][j][e]);
+            }
+    }
+}
+
+// Split-bf16 Y = A X for FP32 blocks of at most 32 columns and N a multiple
+// of 8, with every tile moved by direct global-to-LDS loads: a block of four
+// wavefronts owns 256 rows, each wavefront four 16-row tiles that share
+// every X fragment it reads from LDS (a quarter of the X reads per
+// matrix-core operation of nn_split3_kernel with one tile, whose register
+// pipeline leaves no room for more tiles).  The A tile (256 rows x 64 K,
+// 32 KB) and the X planes' slice (3 planes x 16 NT columns x 64 K) of trip
+// t + 2 load into one of three LDS stages while trip t runs, with no
+// registers; their 16-byte pieces are swizzled (A: piece ^ row % 8, X:
+// piece ^ column % 8) so that the operand reads of 16 lanes spread over the
+// banks.  The matrix-core operations and their order are those of
+// nn_split3_kernel, so the results are bitwise equal.  Rows past the matrix
+// load its last row (their outputs are not written); pieces past column N
+// load eight zeros from the X planes' padding (A may hold anything past its
+// last column, and NaN times the zero of X is NaN).
+constexpr int kNlRows = 256, kNlK = 64, kNlRT = 4;
+
+template <typename TX, int NT>
+__global__ __launch_bounds__(kThreads, 1) void nn_lds_kernel(int64_t M, int64_t N, int s,
+                                                               const __hip_bfloat16* __restrict__ A, int64_t lda,
+                                                               const __bf16* __restrict__ XT, int64_t nk, int64_t plane,
+                                                               TX* __restrict__ Y, int64_t ldy) {
+    static_assert(NT <= 2 && sizeof(TX) == 4, "FP32 blocks of at most 32 columns");
+    constexpr int SW = 16 * NT;
+    constexpr int kWaves = kThreads / kWave;
+    constexpr int kPieceRow = kNlK / 8;                     // 16-byte pieces per row of a K trip
+    constexpr int APIECES = kNlRows * kPieceRow;            // 2048
+    constexpr int XPIECES = 3 * SW * kPieceRow;
+    constexpr int kALoads = APIECES / kWave / kWaves;       // 1 KB loads per wavefront per trip
+    constexpr int kXLoads = (XPIECES / kWave + kWaves - 1) / kWaves;
+    constexpr int XSLOTS = kXLoads * kWaves * kWave;        // pieces of an X stage (the tail past XPIECES unused)
+    constexpr int kLoads = kALoads + kXLoads;
+    static_assert(kALoads * kWaves * kWave == APIECES && XPIECES % kWave == 0, "whole 1 KB loads");
+    __shared__ __align__(16) uint4 As0[APIECES];
+    __shared__ __align__(16) uint4 As1[APIECES];
+    __shared__ __align__(16) uint4 As2[APIECES];
+    __shared__ __align__(16) uint4 Xs0[XSLOTS];
+    __shared__ __align__(16) uint4 Xs1[XSLOTS];
+    __shared__ __align__(16) uint4 Xs2[XSLOTS];
+    const int t = threadIdx.x, lane = t % kWave;
+    const int wave = __builtin_amdgcn_readfirstlane(t / kWave);
+    const int c = lane % 16, q = lane / 16;
+    const int64_t rb0 = static_cast<int64_t>(blockIdx.x) * kNlRows;
+    const int64_t trips = (N + kNlK - 1) / kNlK;
+    floatx4 acc[kNlRT][NT];
+#pragma unroll
+    for (int r = 0; r < kNlRT; ++r)
+#pragma unroll
+        for (int j = 0; j < NT; ++j) acc[r][j] = floatx4{0, 0, 0, 0};
+    // wavefront w moves 1 KB pieces w * kALoads + u of the A tile (rows 8 c ..
+    // 8 c + 7) and w * kXLoads + u of the X slice
+    auto issue = [&](int64_t tr, uint4* as, uint4* xs) {
+        const int64_t k0 = tr * kNlK;
+#pragma unroll
+        for (int u = 0; u < kALoads; ++u) {
+            const int ck = wave * kALoads + u;
+            const int row = 8 * ck + lane / 8;
+            const int piece = (lane % 8) ^ (row & 7);
+            const int64_t grow = std::min<int64_t>(rb0 + row, M - 1);
+            const int64_t kk = k0 + 8 * piece;
+            // past column N: eight zeros of the X planes' padding (N <= nk - 8 then)
+            const void* src = kk < N ? static_cast<const void*>(A + grow * lda + kk)
+                                     : static_cast<const void*>(XT + (nk - 8));
+            __builtin_amdgcn_global_load_lds(src, (lds_void*)(as + ck * kWave), 16, 0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < kXLoads; ++u) {
+            const int ck = wave * kXLoads + u;
+            const int x = ck * kWave + lane;
+            const int col = (x / kPieceRow) % SW, pl = (x / kPieceRow) / SW;
+            const int piece = (x % kPieceRow) ^ (col & 7);
+            const __bf16* src = x < XPIECES ? XT + pl * plane + static_cast<int64_t>(col) * nk + k0 + 8 * piece : XT;
+            __builtin_amdgcn_global_load_lds(src, (lds_void*)(xs + ck * kWave), 16, 0, 0);
+        }
+    };
+    auto trip = [&](int64_t tr, uint4* as, uint4* xs, uint4* as_next2, uint4* xs_next2) {
+        if (tr + 1 < trips)
+            wait_vmcnt<kLoads>();  // this wavefront's share of trip tr has landed (trip tr + 1's may not)
+        else
+            wait_vmcnt<0>();
+        lds_barrier();  // every wavefront's share; every wavefront is done with trip tr - 1
+        bf16x8 av[2][kNlRT];
+        bf16x8 xv[2][NT][3];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int sl = 4 * h + q;
+#pragma unroll
+            for (int r = 0; r < kNlRT; ++r) {
+                const int row = 16 * (kNlRT * wave + r) + c;
+                av[h][r] = __builtin_bit_cast(bf16x8, as[row * kPieceRow + (sl ^ (row & 7))]);
+            }
+#pragma unroll
+            for (int j = 0; j < NT; ++j) {
+                const int col = 16 * j + c;
+#pragma unroll
+                for (int pl = 0; pl < 3; ++pl)
+                    xv[h][j][pl] = __builtin_bit_cast(bf16x8, xs[(pl * SW + col) * kPieceRow + (sl ^ (col & 7))]);
+            }
+        }
+        if (tr + 2 < trips) issue(tr + 2, as_next2, xs_next2);
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int j = 0; j < NT; ++j)
+#pragma unroll
+                for (int r = 0; r < kNlRT; ++r) {
+                    acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[h][r], xv[h][j][2], acc[r][j], 0, 0, 0);
+                    acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[h][r], xv[h][j][1], acc[r][j], 0, 0, 0);
+                    acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[h][r], xv[h][j][0], acc[r][j], 0, 0, 0);
+                }
+    };
+    // a first direct-to-LDS load into an array that is never read (see tn_lds_kernel)
+    __shared__ __align__(16) uint4 sink[kWave];
+    __builtin_amdgcn_global_load_lds(XT, (lds_void*)sink, 16, 0, 0);
+    if (trips > 0) issue(0, As0, Xs0);
+    if (trips > 1) issue(1, As1, Xs1);
+    for (int64_t tr = 0; tr < trips; tr += 3) {
+        trip(tr, As0, Xs0, As2, Xs2);
+        if (tr + 1 < trips) trip(tr + 1, As1, Xs1, As0, Xs0);
+        if (tr + 2 < trips) trip(tr + 2, As2, Xs2, As1, Xs1);
+    }
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+        const int col = 16 * j + c;
+        if (col >= s) continue;
+#pragma unroll
+        for (int r = 0; r < kNlRT; ++r)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int64_t row = rb0 + 16 * (kNlRT * wave + r) + 4 * q + e;
                 if (row < M) Y[row * ldy + col] = static_cast<TX>(acc[r][j][e]);
             }
     }
@@ -774,24 +925,9 @@ __global__ __launch_bounds__(kThreads, 2) void tn_split3_kernel(int64_t M, int64
 // pieces past column N load column 0 of their row (their outputs are not
 // written).  Every stage is its own LDS array, so the compiler sees that a
 // read of one stage does not wait for the loads into another.
-typedef short shortx4 __attribute__((ext_vector_type(4)));
-typedef short shortx8 __attribute__((ext_vector_type(8)));
-typedef __attribute__((address_space(3))) void lds_void;
-typedef __attribute__((address_space(3))) shortx4 lds_shortx4;
-
 constexpr int kTnStage = kTnRows * kTnCols;  // bf16 per A stage
 
 __device__ __forceinline__ int tn_swz(int row) { return 2 * (row & 3); }
-
-// Waits until at most N vector memory operations of this wavefront are outstanding.
-template <int N>
-__device__ __forceinline__ void wait_vmcnt() {
-    static_assert(N >= 0 && N < 64, "vmcnt field");
-    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
-}
-
-// Barrier for LDS traffic only (outstanding global loads stay in flight).
-__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
 template <typename TX, int NT>
 __global__ __launch_bounds__(kThreads, 1) void tn_lds_kernel(int64_t M, int64_t N, int s,
@@ -1018,6 +1154,19 @@ void nn_dispatch(hipStream_t st, int64_t M, int64_t N, int s, const TA* A, int64
         }
     }
     if constexpr (MODE == kSplit3 && std::is_same_v<TA, __hip_bfloat16>) {
+        if constexpr (sizeof(TX) == 4) {
+            if ((reinterpret_cast<uintptr_t>(A) & 15) == 0 && lda % 8 == 0 && N % 8 == 0 && NTt <= 2) {
+                const dim3 gl(static_cast<unsigned>(ceil_div(M, kNlRows)));
+                if (NTt == 1)
+                    hipLaunchKernelGGL((nn_lds_kernel<TX, 1>), gl, dim3(kThreads), 0, st, M, N, s, A, lda,
+                                       static_cast<const __bf16*>(XT), nk, plane, Y, ldy);
+                else
+                    hipLaunchKernelGGL((nn_lds_kernel<TX, 2>), gl, dim3(kThreads), 0, st, M, N, s, A, lda,
+                                       static_cast<const __bf16*>(XT), nk, plane, Y, ldy);
+                AMFLS_LAUNCH_CHECK();
+                return;
+            }
+        }
         if ((reinterpret_cast<uintptr_t>(A) & 15) == 0 && lda % 8 == 0) {
             // two row tiles per wavefront where the block still fits the CU as
             // often (16 and 64 columns: A X 7-9% and 20% faster), one where the

@@ -157,9 +157,9 @@ def dense_kernel_name(c):
     nt = (hi + 15) // 16
     if c["block"] == "f32":
         if c["transpose"]:
-            # A^T Y of at most 32 columns on the 2000-column matrix: the direct-to-LDS kernel
+            # products of at most 32 columns on the 2000-column matrix: the direct-to-LDS kernels
             return f"tn_lds_kernel<float, {nt}>" if nt <= 2 else f"tn_split3_kernel<float, {nt}>"
-        return f"nn_split3_kernel<float, {nt}>"
+        return f"nn_lds_kernel<float, {nt}>" if nt <= 2 else f"nn_split3_kernel<float, {nt}>"
     return f"dense_{'tn' if c['transpose'] else 'nn_f64'}_kernel<bf16, double, ..., {nt}>"
 
 

@@ -965,16 +965,24 @@ __device__ __forceinline__ double gaussian_at(uint64_t key, uint64_t index) {
     return sqrt(-2.0 * log(u1)) * cos(2.0 * 3.141592653589793 * u2);
 }
 
+// A block owns kGaussRows consecutive rows; its threads take their entries
+// in row-major order, so each wavefront's stores cover whole row segments of
+// the row-major block (entry (i, j) is still the column-major counter
+// index j * rows + i of the generator).
+constexpr int kGaussRows = 64;
+
 template <typename T>
 __global__ void gaussian_block_kernel(int64_t rows, int cols, uint64_t key, uint64_t first_col, T* Y,
                                       int64_t ldy) {
-    const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (idx >= rows * cols) return;
-    const int j = static_cast<int>(idx / rows);
-    const int64_t i = idx % rows;
-    const uint64_t index = (first_col + static_cast<uint64_t>(j)) * static_cast<uint64_t>(rows) +
-                           static_cast<uint64_t>(i);
-    Y[i * ldy + j] = static_cast<T>(gaussian_at(key, index));
+    const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kGaussRows;
+    const int nr = static_cast<int>(std::min<int64_t>(kGaussRows, rows - r0));
+    for (int e = threadIdx.x; e < nr * cols; e += blockDim.x) {
+        const int64_t i = r0 + e / cols;
+        const int j = e % cols;
+        const uint64_t index = (first_col + static_cast<uint64_t>(j)) * static_cast<uint64_t>(rows) +
+                               static_cast<uint64_t>(i);
+        Y[i * ldy + j] = static_cast<T>(gaussian_at(key, index));
+    }
 }
 
 uint64_t host_splitmix64(uint64_t v) {
@@ -1235,7 +1243,7 @@ void gaussian_block(hipStream_t s, int64_t rows, int cols, uint64_t seed, uint64
     const int64_t total = rows * cols;
     if (total <= 0) return;
     const uint64_t key = host_splitmix64(seed) ^ host_splitmix64(stream + 0xd2b74407b1ce6e93ULL);
-    hipLaunchKernelGGL(gaussian_block_kernel<T>, dim3(static_cast<unsigned>(ceil_div(total, kThreads))),
+    hipLaunchKernelGGL(gaussian_block_kernel<T>, dim3(static_cast<unsigned>(ceil_div(rows, kGaussRows))),
                        dim3(kThreads), 0, s, rows, cols, key, first_col, Y, ldy);
     AMFLS_LAUNCH_CHECK();
 }

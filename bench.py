@@ -437,6 +437,8 @@ def main() -> int:
                     help="host threads of the batch, a stream each (six fill the GPU best on the batched workload)")
     ap.add_argument("--pmc-calibrate", action="store_true",
                     help="start with a 1 GiB device copy (reference kernel for PMC byte counters)")
+    ap.add_argument("--headline-solves-only", action="store_true",
+                    help="profiling runs: only the headline engine's dense solves (warmup + steps), one summary line")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {a.gpus}); multi-GPU runs are not part of this build",
@@ -456,6 +458,9 @@ def main() -> int:
         torch.cuda.synchronize()
         log("pmc calibration copy: 1 GiB read + 1 GiB written")
         del src, dst
+
+    if a.headline_solves_only:
+        return headline_solves_only(torch, np, am, P, a.steps, a.warmup)
 
     c2 = None
     if not a.no_dense:
@@ -506,6 +511,28 @@ def main() -> int:
         "batched_csr": batch,
     }
     print(json.dumps(line), flush=True)
+    return 0
+
+
+def headline_solves_only(torch, np, am, P, steps, warmup):
+    """The headline engine's solves of the dense workload alone (mixed
+    precision, learned width controller), for a kernel trace of just them."""
+    a, b, _ = P.config2_device()
+    M = am.DeviceMatrix(a, value_dtype=torch.bfloat16)
+    from amfls_solver_amd.width_model import WidthModel
+    wm = WidthModel.load(context=M.context)
+    wm.attach()
+    kw = dict(precision="mixed", forward_tolerance=1e-9, width_policy=1, context=M.context, tolerance=1e-8)
+    for _ in range(warmup):
+        am.run_amfls(M, b, **kw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r = am.run_amfls(M, b, **kw)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    print(json.dumps({"headline_solves": steps, "warmup": warmup, "seconds_per_solve": dt, "status": int(r.status),
+                      "depth": r.depth, "refinement_iterations": r.refinement_steps}), flush=True)
     return 0
 
 

@@ -52,6 +52,13 @@ __device__ __forceinline__ void load_row_piece(const T* __restrict__ A, int64_t 
             return;
         }
     }
+    if constexpr (KL * sizeof(T) == 8) {
+        if (row < M && k0 + KL <= K && (reinterpret_cast<uintptr_t>(src) & 7) == 0) {
+            const uint2 raw = *reinterpret_cast<const uint2*>(src);
+            __builtin_memcpy(v, &raw, sizeof(v));
+            return;
+        }
+    }
 #pragma unroll
     for (int t = 0; t < KL; ++t) v[t] = (row < M && k0 + t < K) ? src[t] : T(0);
 }
@@ -313,7 +320,7 @@ void tall_tn_wide(hipStream_t s, int64_t M, int K, int N, const T* A, int64_t rs
     if (K <= 0 || N <= 0) return;
     const int kt = static_cast<int>(std::min<int64_t>(4, ceil_div(K, 16))), nt = static_cast<int>(ceil_div(N, 16));
     const int slabs = static_cast<int>(ceil_div(K, 64));  // K > 64: slabs of 64 rows of C on grid.y
-    constexpr int U = 2;
+    constexpr int U = 4;
     constexpr int64_t trip = 4 * 4 * U;  // four wavefronts of U row quads
     const int64_t KN = static_cast<int64_t>(K) * N;
     int64_t nb = std::max<int64_t>(1, std::min<int64_t>(ceil_div(1024, slabs), ceil_div(std::max<int64_t>(M, 1), 256)));

@@ -3,6 +3,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <functional>
 #include <limits>
 #include <stdexcept>
 
@@ -554,64 +556,73 @@ bool DenseOperator::supports_refinement() const {
     return static_cast<double>(rows_) * static_cast<double>(cols_) <= 6.4e7;
 }
 
-void DenseOperator::ensure_host_copy() const {
-    std::call_once(host_once_, [this] {
-        h_val_.assign(static_cast<size_t>(rows_ * cols_), 0.0);
-        const size_t es = value_size(a_.type);
-        std::vector<char> raw;
-        // copy the strided matrix row by row (row-major) or column by column
-        if (a_.cs == 1) {
-            raw.resize(static_cast<size_t>(cols_) * es);
-            for (int64_t i = 0; i < rows_; ++i) {
-                AMFLS_HIP_CHECK(hipMemcpy(raw.data(), static_cast<const char*>(a_.ptr) + i * a_.rs * es,
-                                          cols_ * es, hipMemcpyDeviceToHost));
-                for (int64_t j = 0; j < cols_; ++j) h_val_[i * cols_ + j] = 0.0;
-                for (int64_t j = 0; j < cols_; ++j) {
-                    double v = 0.0;
-                    if (a_.type == ValueType::f64) std::memcpy(&v, raw.data() + j * es, 8);
-                    else if (a_.type == ValueType::f32) { float f; std::memcpy(&f, raw.data() + j * es, 4); v = f; }
-                    else { uint16_t b; std::memcpy(&b, raw.data() + j * es, 2); uint32_t u = uint32_t(b) << 16; float f; std::memcpy(&f, &u, 4); v = f; }
-                    h_val_[i * cols_ + j] = v;
-                }
-            }
-        } else {
-            raw.resize(static_cast<size_t>(rows_) * es);
+bool DenseOperator::stream_rows(const std::function<bool(int64_t, int64_t, const double*)>& use) const {
+    const size_t es = value_size(a_.type);
+    const int64_t chunk = std::min<int64_t>(rows_, std::max<int64_t>(1, (int64_t(1) << 22) / cols_));
+    std::vector<char> raw(static_cast<size_t>(chunk * cols_) * es);
+    std::vector<double> val(static_cast<size_t>(chunk * cols_));
+    const char* base = static_cast<const char*>(a_.ptr);
+    const bool row_major = a_.cs == 1;
+    for (int64_t i0 = 0; i0 < rows_; i0 += chunk) {
+        const int64_t nr = std::min(chunk, rows_ - i0);
+        // one 2-D copy per chunk: nr rows of pitch rs (row-major) or cols
+        // column segments of nr entries and pitch cs (column-major)
+        if (row_major)
+            AMFLS_HIP_CHECK(hipMemcpy2D(raw.data(), cols_ * es, base + i0 * a_.rs * es, a_.rs * es, cols_ * es, nr,
+                                        hipMemcpyDeviceToHost));
+        else
+            AMFLS_HIP_CHECK(hipMemcpy2D(raw.data(), nr * es, base + i0 * es, a_.cs * es, nr * es, cols_,
+                                        hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < nr; ++i)
             for (int64_t j = 0; j < cols_; ++j) {
-                AMFLS_HIP_CHECK(hipMemcpy(raw.data(), static_cast<const char*>(a_.ptr) + j * a_.cs * es,
-                                          rows_ * es, hipMemcpyDeviceToHost));
-                for (int64_t i = 0; i < rows_; ++i) {
-                    double v = 0.0;
-                    if (a_.type == ValueType::f64) std::memcpy(&v, raw.data() + i * es, 8);
-                    else if (a_.type == ValueType::f32) { float f; std::memcpy(&f, raw.data() + i * es, 4); v = f; }
-                    else { uint16_t b; std::memcpy(&b, raw.data() + i * es, 2); uint32_t u = uint32_t(b) << 16; float f; std::memcpy(&f, &u, 4); v = f; }
-                    h_val_[i * cols_ + j] = v;
+                const char* e = raw.data() + (row_major ? i * cols_ + j : j * nr + i) * es;
+                double v;
+                if (a_.type == ValueType::f64) {
+                    std::memcpy(&v, e, 8);
+                } else if (a_.type == ValueType::f32) {
+                    float f;
+                    std::memcpy(&f, e, 4);
+                    v = f;
+                } else {  // bf16: the high half of an fp32
+                    uint16_t h;
+                    std::memcpy(&h, e, 2);
+                    const uint32_t u = uint32_t(h) << 16;
+                    float f;
+                    std::memcpy(&f, &u, 4);
+                    v = f;
                 }
+                val[i * cols_ + j] = v;
             }
-        }
-    });
-}
-
-bool DenseOperator::refine_apply(const long double* x, long double* y) const {
-    ensure_host_copy();
-    for (int64_t i = 0; i < rows_; ++i) {
-        long double sum = 0.0L;
-        const double* row = h_val_.data() + i * cols_;
-        for (int64_t j = 0; j < cols_; ++j) sum += static_cast<long double>(row[j]) * x[j];
-        if (!std::isfinite(sum)) return false;
-        y[i] = sum;
+        if (!use(i0, nr, val.data())) return false;
     }
     return true;
 }
 
+bool DenseOperator::refine_apply(const long double* x, long double* y) const {
+    return stream_rows([&](int64_t i0, int64_t nr, const double* v) {
+        for (int64_t i = 0; i < nr; ++i) {
+            long double sum = 0.0L;
+            const double* row = v + i * cols_;
+            for (int64_t j = 0; j < cols_; ++j) sum += static_cast<long double>(row[j]) * x[j];
+            if (!std::isfinite(sum)) return false;
+            y[i0 + i] = sum;
+        }
+        return true;
+    });
+}
+
 bool DenseOperator::refine_apply_t(const long double* y, long double* x) const {
-    ensure_host_copy();
     std::fill(x, x + cols_, 0.0L);
-    for (int64_t i = 0; i < rows_; ++i) {
-        const long double in = y[i];
-        if (!std::isfinite(in)) return false;
-        const double* row = h_val_.data() + i * cols_;
-        for (int64_t j = 0; j < cols_; ++j) x[j] += static_cast<long double>(row[j]) * in;
-    }
+    const bool ok = stream_rows([&](int64_t i0, int64_t nr, const double* v) {
+        for (int64_t i = 0; i < nr; ++i) {
+            const long double in = y[i0 + i];
+            if (!std::isfinite(in)) return false;
+            const double* row = v + i * cols_;
+            for (int64_t j = 0; j < cols_; ++j) x[j] += static_cast<long double>(row[j]) * in;
+        }
+        return true;
+    });
+    if (!ok) return false;
     for (int64_t j = 0; j < cols_; ++j)
         if (!std::isfinite(x[j])) return false;
     return true;

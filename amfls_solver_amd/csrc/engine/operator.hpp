@@ -10,6 +10,7 @@
 // row_major_dense_operator.cpp:23-101).
 #pragma once
 
+#include <functional>
 #include <map>
 #include <memory>
 #include <atomic>
@@ -238,13 +239,14 @@ private:
     template <typename T>
     void do_apply_t(hipStream_t s, const T* Y, int n, int64_t ldy, T* X, int64_t ldx, Scratch& scratch,
                     bool profile) const;
-    void ensure_host_copy() const;
+    // Streams A to the host in chunks of whole rows (at most 2^22 entries,
+    // widened to double, row-major) and hands each to use(first_row, rows,
+    // values); stops early when use returns false.  No copy of A is kept.
+    bool stream_rows(const std::function<bool(int64_t, int64_t, const double*)>& use) const;
 
     Strided a_;
     ValidationModel model_;
     Context& ctx_;
-    mutable std::once_flag host_once_;
-    mutable std::vector<double> h_val_;  // row-major copy for refinement
 };
 
 // A caller's operator that lives on the host (the reference's custom

@@ -130,6 +130,37 @@ def test_forced_long_double_refinement_matches_oracle(torch_mod):
                               "backward_error_upper_bound", "operator_norm_lower_bound"])
 
 
+@pytest.mark.parametrize("layout", ["column_major_f64", "padded_row_major_f32"])
+def test_forced_refinement_streams_large_dense_matrix(torch_mod, layout):
+    """The long-double refinement streams A to the host in chunks of 2^22
+    entries: a 1.5e6 x 3 integer problem spans two chunks (1398101 rows and
+    the rest).  Column-major storage (a transposed view) and row-major
+    storage with a padded leading dimension take the two copy shapes; both
+    validators must take the refinement branch and agree with the oracle."""
+    import amfls_solver_amd as am
+    from oracle import operators as O
+    torch = torch_mod
+    rng = np.random.default_rng(7)
+    m, n = 1_500_000, 3
+    A = rng.integers(-4, 5, size=(m, n)).astype(np.float64)
+    b = rng.integers(-4, 5, size=m).astype(np.float64)
+    x = np.linalg.lstsq(A, b, rcond=None)[0]
+    if layout == "column_major_f64":
+        At = torch.from_numpy(np.ascontiguousarray(A.T)).to("cuda").t()
+        assert At.stride() == (1, m)
+    else:
+        At = torch.zeros(m, n + 5, dtype=torch.float32, device="cuda")
+        At[:, :n] = torch.from_numpy(A.astype(np.float32)).to("cuda")
+        At = At[:, :n]
+        assert At.stride() == (n + 5, 1)
+    eng = am.validate_candidate(At, torch.from_numpy(b), torch.from_numpy(x), tolerance=1e-30)
+    ora = _oracle(O.DenseOperator(A), b, x, 0.0, 1e-30)
+    assert eng.base_validation_a_columns == ora.statistics.base_validation_a_columns == 2
+    assert eng.base_validation_at_columns == ora.statistics.base_validation_at_columns == 2
+    _compare(eng, ora, 1e-9, ["compatible_backward_error_upper_bound", "least_squares_backward_error_upper_bound",
+                              "backward_error_upper_bound", "operator_norm_lower_bound"])
+
+
 def test_forced_ridge_correction_matches_oracle(torch_mod):
     """A = diag(10, 1), lambda = 1e-2, an error along the first eigenvector:
     both validators attempt the one-step ridge correction, improve the

@@ -909,10 +909,11 @@ __global__ __launch_bounds__(kThreads, 2) void tn_split3_kernel(int64_t M, int64
 // same block-to-(strip, chunk) mapping and partials as tn_split3_kernel and
 // the same matrix-core operands in the same order (with the same chunks, the
 // partials are bitwise equal; the dispatch gives it one round of 256 blocks,
-// one per CU, since its 126 KB of LDS hold one block per CU).  The loads
-// of trip t + 2 (its A tile, 16 bytes per lane, and its raw FP32 Y tile, 4
-// bytes per lane) are issued while trip t runs on the matrix cores: three A
-// stages, two raw Y stages, no registers and no register transposes.  Each
+// one per CU, since its 158 KB of LDS hold one block per CU).  The A tile
+// of trip t + 3 (16 bytes per lane) and the raw FP32 Y tile of trip t + 2
+// (4 bytes per lane) are issued while trip t runs on the matrix cores: four
+// A stages, so that up to 96 KB of A per CU are in flight, two raw Y
+// stages, no registers and no register transposes.  Each
 // A row's 16-byte pieces are XOR-swizzled (piece ^ 2 (row % 4)) so that the
 // transpose reads spread over the LDS banks.  The A operand (eight
 // consecutive rows of one column per lane) comes from the LDS transpose load
@@ -941,11 +942,11 @@ __global__ __launch_bounds__(kThreads, 1) void tn_lds_kernel(int64_t M, int64_t 
     constexpr int kWaves = kThreads / kWave;
     constexpr int kALoads = kTnStage * 2 / (kWave * 16) / kWaves;  // 16-byte loads per lane per trip (8)
     constexpr int kYLoads = kTnRows * SW / kWave / kWaves;          // 4-byte loads per lane per trip
-    constexpr int kLoads = kALoads + kYLoads;
     constexpr int YPAIRS = (SW * (kTnRows / 8) + kThreads - 1) / kThreads;
     __shared__ __align__(16) __bf16 As0[kTnStage];
     __shared__ __align__(16) __bf16 As1[kTnStage];
     __shared__ __align__(16) __bf16 As2[kTnStage];
+    __shared__ __align__(16) __bf16 As3[kTnStage];
     __shared__ __align__(16) float Yr0[kTnRows * SW];
     __shared__ __align__(16) float Yr1[kTnRows * SW];
     __shared__ __align__(16) __bf16 Yp[3][SW][KP];
@@ -971,10 +972,10 @@ __global__ __launch_bounds__(kThreads, 1) void tn_lds_kernel(int64_t M, int64_t 
     for (int e = 0; e < 4; ++e)
 #pragma unroll
         for (int j = 0; j < NT; ++j) acc[e][j] = floatx4{0, 0, 0, 0};
-    // the loads of trip `tr` into one A stage and one raw Y stage: wavefront w
-    // moves 1 KB pieces w * kALoads + u of the A tile (tile rows 2 c, 2 c + 1)
-    // and 256-byte pieces w * kYLoads + u of the Y tile (64 / SW rows each)
-    auto issue = [&](int64_t tr, __bf16* as, float* yr) {
+    // the loads of trip `tr` into one A stage (wavefront w moves 1 KB pieces
+    // w * kALoads + u of the A tile, tile rows 2 c, 2 c + 1) and into one raw
+    // Y stage (256-byte pieces w * kYLoads + u, 64 / SW rows each)
+    auto issue_a = [&](int64_t tr, __bf16* as) {
         const int64_t i0 = r0 + tr * kTnRows;
 #pragma unroll
         for (int u = 0; u < kALoads; ++u) {
@@ -986,6 +987,9 @@ __global__ __launch_bounds__(kThreads, 1) void tn_lds_kernel(int64_t M, int64_t 
             __builtin_amdgcn_global_load_lds(A + grow * lda + (gcol < N ? gcol : 0), (lds_void*)(as + ck * 512), 16,
                                              0, 0);
         }
+    };
+    auto issue_y = [&](int64_t tr, float* yr) {
+        const int64_t i0 = r0 + tr * kTnRows;
 #pragma unroll
         for (int u = 0; u < kYLoads; ++u) {
             const int ck = wave * kYLoads + u;
@@ -1029,11 +1033,17 @@ __global__ __launch_bounds__(kThreads, 1) void tn_lds_kernel(int64_t M, int64_t 
             *reinterpret_cast<uint4*>(as + (first + idx / (kTnCols / 8)) * kTnCols + 8 * (idx % (kTnCols / 8))) =
                 uint4{0, 0, 0, 0};
     };
-    // one trip: its tile has landed in (as, yr); the loads of trip tr + 2 go
-    // into (as_next2, yr) once yr is split
-    auto trip = [&](int64_t tr, __bf16* as, float* yr, __bf16* as_next2) {
-        if (tr + 1 < trips)
-            wait_vmcnt<kLoads>();  // this wavefront's share of trip tr has landed (trip tr + 1's may not)
+    // One trip: its tiles have landed in (as, yr).  Each trip issues, in this
+    // order, the Y loads of trip tr + 2 (into yr, once split) and the A loads
+    // of trip tr + 3 (into as_next3, the stage trip tr - 1 used), so three
+    // A stages are in flight while a trip computes.  Issued after Y(tr), in
+    // order: A(tr + 1), then Y(tr + 1) and A(tr + 2) as they exist; those may
+    // still be in flight.
+    auto trip = [&](int64_t tr, __bf16* as, float* yr, __bf16* as_next3) {
+        if (tr + 2 < trips)
+            wait_vmcnt<2 * kALoads + kYLoads>();
+        else if (tr + 1 < trips)
+            wait_vmcnt<kALoads + kYLoads>();
         else
             wait_vmcnt<0>();
         lds_barrier();  // every wavefront's share; every wavefront is done with trip tr - 1
@@ -1061,7 +1071,8 @@ __global__ __launch_bounds__(kThreads, 1) void tn_lds_kernel(int64_t M, int64_t 
 #pragma unroll
                 for (int pl = 0; pl < 3; ++pl) yp[h][j][pl] = *reinterpret_cast<const bf16x8*>(&Yp[pl][16 * j + c16][k]);
         }
-        if (tr + 2 < trips) issue(tr + 2, as_next2, yr);
+        if (tr + 2 < trips) issue_y(tr + 2, yr);
+        if (tr + 3 < trips) issue_a(tr + 3, as_next3);
 #pragma unroll
         for (int h = 0; h < kTnRows / 32; ++h)
 #pragma unroll
@@ -1081,16 +1092,19 @@ __global__ __launch_bounds__(kThreads, 1) void tn_lds_kernel(int64_t M, int64_t 
     // (explicit, per trip) are the only ones.
     __shared__ __align__(16) float sink[kWave];
     __builtin_amdgcn_global_load_lds(Yb, (lds_void*)sink, 4, 0, 0);
-    if (trips > 0) issue(0, As0, Yr0);
-    if (trips > 1) issue(1, As1, Yr1);
-    // trip tr uses A stage tr % 3 and Y stage tr % 2
-    for (int64_t tr = 0; tr < trips; tr += 6) {
-        trip(tr, As0, Yr0, As2);
+    // the loads before trip 0, in the order the trips keep: A(0), Y(0), A(1),
+    // Y(1), A(2)
+    if (trips > 0) issue_a(0, As0);
+    if (trips > 0) issue_y(0, Yr0);
+    if (trips > 1) issue_a(1, As1);
+    if (trips > 1) issue_y(1, Yr1);
+    if (trips > 2) issue_a(2, As2);
+    // trip tr uses A stage tr % 4 and Y stage tr % 2
+    for (int64_t tr = 0; tr < trips; tr += 4) {
+        trip(tr, As0, Yr0, As3);
         if (tr + 1 < trips) trip(tr + 1, As1, Yr1, As0);
         if (tr + 2 < trips) trip(tr + 2, As2, Yr0, As1);
-        if (tr + 3 < trips) trip(tr + 3, As0, Yr1, As2);
-        if (tr + 4 < trips) trip(tr + 4, As1, Yr0, As0);
-        if (tr + 5 < trips) trip(tr + 5, As2, Yr1, As1);
+        if (tr + 3 < trips) trip(tr + 3, As3, Yr1, As2);
     }
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
